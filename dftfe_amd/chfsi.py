@@ -1,0 +1,248 @@
+"""Host-side mirror of the reference's device operator and ChFSI solver classes.
+
+    KohnShamOperatorDevice   ~ operatorDFTDeviceClass / kohnShamDFTOperatorDeviceClass
+                               (include/operatorDevice.h:43-420)
+    ChebyshevSolverDevice    ~ chebyshevOrthogonalizedSubspaceIterationSolverDevice
+                               (include/chebyshevOrthogonalizedSubspaceIterationSolverDevice.h:37-166)
+    free functions           ~ linearAlgebraOperationsDevice::* (include/linearAlgebraOperationsDevice.h)
+
+Device memory and streams come from PyTorch (plumbing only); every compute call
+goes through libchfsi.so.  Multivectors are node-major torch tensors of shape
+(M, N) (X[node, vec]) on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+from typing import Optional
+
+import numpy as np
+import torch
+
+from . import lib as L
+
+
+def _stream() -> ct.c_void_p:
+    return ct.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t: torch.Tensor) -> ct.c_void_p:
+    return ct.c_void_p(t.data_ptr())
+
+
+def _as(arr, dtype):
+    return np.ascontiguousarray(arr, dtype=dtype)
+
+
+class KohnShamOperatorDevice:
+    """Device FE Hamiltonian H~ = M^{-1/2} H M^{-1/2} of one rank's full domain."""
+
+    def __init__(self, prob, device: str = "cuda:0"):
+        self.lib = L.load()
+        self.prob = prob
+        self.device = torch.device(device)
+        torch.cuda.set_device(self.device)
+        keep = {}
+
+        def arr(name, a, dtype, ctype):
+            keep[name] = _as(a, dtype)
+            return keep[name].ctypes.data_as(ct.POINTER(ctype))
+
+        d = L.OperatorDesc()
+        d.n_dofs = prob.M
+        d.n_cells = prob.C
+        d.nodes_per_cell = prob.ne
+        d.quad_per_cell = prob.nqc
+        d.cell_map = arr("map", prob.cell_map, np.int64, ct.c_int64)
+        d.cell_color = arr("col", prob.cell_color, np.int8, ct.c_int8)
+        d.sqrt_mass = arr("sq", prob.sqrt_mass, np.float64, ct.c_double)
+        d.inv_sqrt_mass = arr("isq", prob.inv_sqrt_mass, np.float64, ct.c_double)
+        d.shape_values = arr("N", prob.Nq, np.float64, ct.c_double)
+        d.stiffness = arr("K", prob.K_e, np.float64, ct.c_double)
+        d.n_proj = prob.n_proj
+        d.n_pairs = len(prob.pair_cell)
+        d.proj_per_pair = 4
+        if len(prob.pair_cell):
+            d.pair_cell = arr("pc", prob.pair_cell, np.int32, ct.c_int32)
+            d.pair_proj0 = arr("pp", 4 * prob.pair_atom, np.int32, ct.c_int32)
+            d.pair_C = arr("pC", prob.pair_C, np.float64, ct.c_double)
+            d.proj_V = arr("pV", prob.proj_V, np.float64, ct.c_double)
+        if len(prob.slave_rows):
+            ns = len(prob.slave_rows)
+            d.n_constrained = ns
+            d.constraint_rows = arr("cr", prob.slave_rows, np.int64, ct.c_int64)
+            d.constraint_ptr = arr("cp", np.arange(ns + 1), np.int64, ct.c_int64)
+            d.constraint_cols = arr("cc", prob.master_rows, np.int64, ct.c_int64)
+            d.constraint_vals = arr("cv", np.ones(ns), np.float64, ct.c_double)
+        h = ct.c_void_p()
+        L.check(self.lib.chfsi_operator_create(ct.byref(d), ct.byref(h)))
+        self.handle = h
+        self.M = prob.M
+        self.set_potential(prob.vJxW)
+
+    def __del__(self):
+        if getattr(self, "handle", None) and self.handle.value:
+            self.lib.chfsi_operator_destroy(self.handle)
+            self.handle = None
+
+    # -- setup --------------------------------------------------------------
+    def set_potential(self, vJxW: np.ndarray) -> None:
+        """computeHamiltonianMatricesAllkpt: cell matrices from V_eff JxW on device."""
+        v = torch.as_tensor(_as(vJxW, np.float64), device=self.device)
+        L.check(self.lib.chfsi_operator_set_potential(self.handle, _ptr(v), _stream()))
+        torch.cuda.current_stream().synchronize()
+
+    def set_cell_matrices(self, H: torch.Tensor) -> None:
+        L.check(self.lib.chfsi_operator_set_cell_matrices(self.handle, _ptr(H), _stream()))
+
+    def cell_matrices(self) -> torch.Tensor:
+        p = self.prob
+        H = torch.empty((p.C, p.ne, p.ne), dtype=torch.float64, device=self.device)
+        L.check(self.lib.chfsi_operator_get_cell_matrices(self.handle, _ptr(H), _stream()))
+        return H
+
+    def getSqrtMassVec(self) -> int:
+        return self.lib.chfsi_operator_sqrt_mass(self.handle)
+
+    def getInvSqrtMassVec(self) -> int:
+        return self.lib.chfsi_operator_inv_sqrt_mass(self.handle)
+
+    # -- applies ------------------------------------------------------------
+    def HX(self, src: torch.Tensor, scale_flag: bool, scalar: float, dst: torch.Tensor) -> None:
+        """dst <- (scale_flag ? dst : M^{-1/2} dst) + scalar H~ src (in place on dst)."""
+        B = src.shape[1]
+        if src.dtype == torch.float64:
+            L.check(self.lib.chfsi_hx(self.handle, _ptr(src), src.stride(0), _ptr(dst),
+                                      dst.stride(0), B, int(scale_flag), float(scalar), _stream()))
+        elif src.dtype == torch.float32:
+            L.check(self.lib.chfsi_hx_fp32(self.handle, _ptr(src), src.stride(0), _ptr(dst),
+                                           dst.stride(0), B, int(scale_flag), float(scalar),
+                                           _stream()))
+        else:
+            raise TypeError("HX supports float64 and float32 blocks")
+
+    def HXCheby(self, src: torch.Tensor, dst: torch.Tensor) -> None:
+        """dst <- dst + H src (unscaled FE H; src already M^{-1/2}-scaled)."""
+        L.check(self.lib.chfsi_hx_cheby(self.handle, _ptr(src), src.stride(0), _ptr(dst),
+                                        dst.stride(0), src.shape[1], _stream()))
+
+    def apply(self, X: torch.Tensor) -> torch.Tensor:
+        """H~ X as a new tensor."""
+        Y = torch.zeros_like(X)
+        self.HX(X, False, 1.0, Y)
+        return Y
+
+    def distribute(self, X: torch.Tensor) -> None:
+        L.check(self.lib.chfsi_constraints_distribute(self.handle, _ptr(X), X.stride(0),
+                                                      X.shape[1], _stream()))
+
+    def distribute_slave_to_master(self, X: torch.Tensor) -> None:
+        L.check(self.lib.chfsi_constraints_slave_to_master(self.handle, _ptr(X), X.stride(0),
+                                                           X.shape[1], _stream()))
+
+    def set_zero(self, X: torch.Tensor) -> None:
+        L.check(self.lib.chfsi_constraints_set_zero(self.handle, _ptr(X), X.stride(0),
+                                                    X.shape[1], _stream()))
+
+
+# -- linearAlgebraOperationsDevice mirrors ------------------------------------
+
+
+def chebyshev_filter(op: KohnShamOperatorDevice, X: torch.Tensor, m: int, a: float, b: float,
+                     a0: float, precision: int = L.FP64) -> None:
+    """In-place degree-m filter of a (M x B) block view of a node-major array."""
+    L.check(op.lib.chfsi_chebyshev_filter(op.handle, _ptr(X), X.stride(0), X.shape[1], m, a, b,
+                                          a0, precision, _stream()))
+
+
+def lanczos_bounds(op: KohnShamOperatorDevice, v0: torch.Tensor, steps: int,
+                   reproducible: bool = False):
+    lo, up = ct.c_double(), ct.c_double()
+    L.check(op.lib.chfsi_lanczos_bounds(op.handle, _ptr(v0), steps, int(reproducible),
+                                        ct.byref(lo), ct.byref(up), _stream()))
+    return lo.value, up.value
+
+
+def cholgs(op: KohnShamOperatorDevice, X: torch.Tensor) -> None:
+    L.check(op.lib.chfsi_cholgs(op.handle, _ptr(X), X.shape[0], X.shape[1], _stream()))
+
+
+def rayleigh_ritz(op: KohnShamOperatorDevice, X: torch.Tensor, block: int) -> np.ndarray:
+    N = X.shape[1]
+    ev = np.zeros(N)
+    L.check(op.lib.chfsi_rayleigh_ritz(op.handle, _ptr(X), X.shape[0], N, block,
+                                       ev.ctypes.data_as(L.c_double_p), _stream()))
+    return ev
+
+
+def residual_norms(op: KohnShamOperatorDevice, X: torch.Tensor, lam: np.ndarray,
+                   block: int) -> np.ndarray:
+    N = X.shape[1]
+    lam = _as(lam, np.float64)
+    r = np.zeros(N)
+    L.check(op.lib.chfsi_residual_norms(op.handle, _ptr(X), X.shape[0], N, block,
+                                        lam.ctypes.data_as(L.c_double_p),
+                                        r.ctypes.data_as(L.c_double_p), _stream()))
+    return r
+
+
+class ChebyshevSolverDevice:
+    """chebyshevOrthogonalizedSubspaceIterationSolverDevice: one ChFSI pass per solve()."""
+
+    def __init__(self, a0: float = 0.0, bLow: float = 0.0, bUp: float = 0.0, *,
+                 cheby_block_size: int = 400, cheby_degree: int = 0, lanczos_steps: int = 20,
+                 reproducible: bool = False, filter_precision: int = L.FP32,
+                 first_scf_degree_factor: float = 1.34, reuse_lanczos_upper: bool = False):
+        self.lib = L.load()
+        p = L.Params()
+        self.lib.chfsi_default_params(ct.byref(p))
+        p.cheby_block_size = cheby_block_size
+        p.cheby_degree = cheby_degree
+        p.lanczos_steps = lanczos_steps
+        p.reproducible = int(reproducible)
+        p.filter_precision = filter_precision
+        p.first_scf_degree_factor = first_scf_degree_factor
+        p.reuse_lanczos_upper = int(reuse_lanczos_upper)
+        self.params = p
+        h = ct.c_void_p()
+        L.check(self.lib.chfsi_solver_create(ct.byref(p), a0, bLow, bUp, ct.byref(h)))
+        self.handle = h
+        self.last_timings_ms = None
+
+    def __del__(self):
+        if getattr(self, "handle", None) and self.handle.value:
+            self.lib.chfsi_solver_destroy(self.handle)
+            self.handle = None
+
+    def reinitSpectrumBounds(self, a0: float, bLow: float, bUp: float) -> None:
+        L.check(self.lib.chfsi_solver_reinit_bounds(self.handle, a0, bLow, bUp))
+
+    def bounds(self):
+        a, b, c = ct.c_double(), ct.c_double(), ct.c_double()
+        L.check(self.lib.chfsi_solver_get_bounds(self.handle, ct.byref(a), ct.byref(b),
+                                                 ct.byref(c)))
+        return a.value, b.value, c.value
+
+    def set_lanczos_start(self, v0: Optional[torch.Tensor]) -> None:
+        if v0 is None:
+            L.check(self.lib.chfsi_solver_set_lanczos_start(self.handle, None, 0))
+        else:
+            L.check(self.lib.chfsi_solver_set_lanczos_start(self.handle, _ptr(v0), v0.numel()))
+
+    def solve(self, op: KohnShamOperatorDevice, X: torch.Tensor, *,
+              is_first_filtering_call: bool = False, compute_residual: bool = True,
+              use_mixed_prec_overall: bool = False, is_first_scf: bool = False):
+        """One pass.  Returns (eigenvalues, residual norms, bUp); X is overwritten."""
+        M, N = X.shape
+        ev = np.zeros(N)
+        res = np.zeros(N)
+        bup = ct.c_double()
+        t = np.zeros(6)
+        L.check(self.lib.chfsi_solve(self.handle, op.handle, _ptr(X), M, N,
+                                     ev.ctypes.data_as(L.c_double_p),
+                                     res.ctypes.data_as(L.c_double_p),
+                                     int(is_first_filtering_call), int(compute_residual),
+                                     int(use_mixed_prec_overall), int(is_first_scf),
+                                     ct.byref(bup), t.ctypes.data_as(L.c_double_p), _stream()))
+        self.last_timings_ms = dict(zip(["lanczos", "filter", "cholgs", "rayleigh_ritz",
+                                         "residual", "total"], t.tolist()))
+        return ev, res, bup.value

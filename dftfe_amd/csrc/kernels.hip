@@ -1,0 +1,715 @@
+// gfx950 kernels of the ChFSI hot path.
+//
+// The FE Hamiltonian apply is one fused kernel per colour class of cells:
+//   1. gather the cell's nodal rows of the source block through the cell->DOF
+//      map into LDS (scaled by M^{-1/2}), tile by tile along K;
+//   2. MFMA (16x16x4, FP64 or FP32) of the dense cell Hamiltonian against the
+//      gathered block, accumulated in registers; the nonlocal term C V C^T is
+//      the same GEMM continued over extra K tiles (the cell's projector
+//      columns against the V-scaled projections W);
+//   3. epilogue: scaled scatter into the destination with the Chebyshev
+//      three-term combine applied exactly once per DOF (first touch in colour
+//      order).  Cells of one colour share no DOF, so the scatter is a plain
+//      read-modify-write: no atomics, deterministic results.
+// Reference structure being replaced (three kernels and several HBM sweeps per
+// HX): src/dftOperator/matrixVectorProductImplementationsDevice.cc:27-117,
+// computeNonLocalHamiltonianTimesXMemoryOptBatchGEMMDevice.cc:27-283,
+// src/linAlg/linearAlgebraOperationsDevice.cc:37-104 (combinedDeviceKernel).
+
+#include "kernels.hpp"
+
+namespace chfsi {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct V4;
+template <>
+struct V4<float> {
+  using t = f32x4;
+};
+template <>
+struct V4<double> {
+  using t = f64x4;
+};
+
+__device__ __forceinline__ f32x4 mfma16(float a, float b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f64x4 mfma16(double a, double b, f64x4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Row of the 16x16 MFMA result held by (lane, register r).  The FP32
+// instruction keeps four consecutive rows per lane group; the FP64 one strides
+// them by four (checked against the hardware with a single-wave probe).
+template <typename T>
+__device__ __forceinline__ int acc_row(int lane, int r) {
+  return sizeof(T) == 8 ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r;
+}
+
+// LDS row stride (elements) of a 16-wide K slice: padded so that the 16-byte
+// operand reads of eight consecutive lanes hit distinct banks.
+template <typename T>
+constexpr int row_stride() {
+  return sizeof(T) == 4 ? 20 : 18;
+}
+
+// ---------------------------------------------------------------------------
+// Fused HX over one colour class
+// ---------------------------------------------------------------------------
+// Work item = (cell, column tile).  The 1-D grid is laid out so that the
+// column tiles of one cell are consecutive items inside one XCD's slice
+// (workgroup w runs on XCD w % 8): the cell matrix then stays in that XCD's L2
+// while its tiles are processed.
+//
+// 8 waves; wave = (wr, wc): wc selects CBW column blocks of 16, wr selects the
+// row blocks rb = wr + WR*j.  MFMA operand mapping (16x16x4): lane l supplies
+// A[l%16][k] and B[k][l%16] with k = 4*(l/16) + r for the r-th of four MFMAs
+// over a 16-wide K slice; result lane l holds column l%16, rows acc_row().
+template <typename T, int TN, int WC, int RBW>
+__global__ __launch_bounds__(512) void hx_kernel(HxLaunch<T> a) {
+  using v4 = typename V4<T>::t;
+  constexpr int NW = 8;
+  constexpr int WR = NW / WC;
+  constexpr int RS = row_stride<T>();
+  constexpr int ROWS_MAX = RBW * WR * 16;
+  constexpr int CBW = TN / 16 / WC;
+  constexpr int KPT = kTile * TN / 512;
+  static_assert(KPT >= 1, "column tile too narrow");
+  static_assert(CBW >= 1 && TN % (16 * WC) == 0, "bad column split");
+  constexpr int HCH = (ROWS_MAX * 4 + 511) / 512;
+
+  __shared__ __align__(16) T Hs[ROWS_MAX * RS];
+  __shared__ __align__(16) T Xs[TN * RS];
+
+  const int wid = blockIdx.x;
+  const int local = wid >> 3;
+  if (local >= a.per_xcd) return;
+  const int item = (wid & 7) * a.per_xcd + local;
+  if (item >= a.ncells * a.ntiles) return;
+  const int cell = a.cells[item / a.ntiles];
+  const int v0 = (item % a.ntiles) * TN;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wc = wave % WC;
+  const int wr = wave / WC;
+  const int nrb = a.ne_pad >> 4;
+  const int nkt = a.ne_pad >> 4;
+
+  int nl_begin = 0, nkt_nl = 0;
+  if (a.nl_off) {
+    nl_begin = a.nl_off[cell];
+    nkt_nl = (a.nl_off[cell + 1] - nl_begin) >> 4;
+  }
+  const int ntot = nkt + nkt_nl;
+
+  const uint32_t *cmap = a.map + (int64_t)cell * a.ne;
+  const T *Hcell = a.Ht + (int64_t)cell * a.ne_pad * a.ne_pad;
+  const T *NLcell = a.nl_C ? a.nl_C + (int64_t)nl_begin * a.ne_pad : nullptr;
+
+  const int gv = tid % TN;
+  const int gk = (tid / TN) * KPT;
+  const bool vok = (v0 + gv) < a.B;
+  const int nchunk = a.ne_pad * 4;
+
+  v4 acc[RBW][CBW];
+#pragma unroll
+  for (int j = 0; j < RBW; ++j)
+#pragma unroll
+    for (int cb = 0; cb < CBW; ++cb) acc[j][cb] = v4{0, 0, 0, 0};
+
+  v4 hreg[HCH];
+  T xreg[KPT];
+
+  auto load_tile = [&](int t) {
+    const T *tile_src = (t < nkt) ? Hcell + (int64_t)t * a.ne_pad * 16
+                                  : NLcell + (int64_t)(t - nkt) * a.ne_pad * 16;
+#pragma unroll
+    for (int j = 0; j < HCH; ++j) {
+      const int chunk = tid + 512 * j;
+      if (chunk < nchunk) hreg[j] = reinterpret_cast<const v4 *>(tile_src)[chunk];
+    }
+    if (t < nkt) {
+#pragma unroll
+      for (int kk = 0; kk < KPT; ++kk) {
+        const int k = t * 16 + gk + kk;
+        T val = T(0);
+        if (k < a.ne && vok) {
+          const uint32_t g = cmap[k] & 0x7fffffffu;
+          const T sc = a.s_in ? a.s_in[g] : T(1);
+          val = sc * a.src[(int64_t)g * a.ld_src + v0 + gv];
+        }
+        xreg[kk] = val;
+      }
+    } else {
+#pragma unroll
+      for (int kk = 0; kk < KPT; ++kk) {
+        const int s = nl_begin + (t - nkt) * 16 + gk + kk;
+        const int pj = a.slot_proj[s];
+        xreg[kk] = (pj >= 0 && vok) ? a.W[(int64_t)pj * a.ldW + v0 + gv] : T(0);
+      }
+    }
+  };
+  auto store_tile = [&]() {
+#pragma unroll
+    for (int j = 0; j < HCH; ++j) {
+      const int chunk = tid + 512 * j;
+      if (chunk < nchunk)
+        *reinterpret_cast<v4 *>(&Hs[(chunk >> 2) * RS + (chunk & 3) * 4]) = hreg[j];
+    }
+#pragma unroll
+    for (int kk = 0; kk < KPT; ++kk) Xs[gv * RS + gk + kk] = xreg[kk];
+  };
+
+  if (ntot > 0) load_tile(0);
+  for (int t = 0; t < ntot; ++t) {
+    __syncthreads();
+    store_tile();
+    __syncthreads();
+    if (t + 1 < ntot) load_tile(t + 1);
+    v4 b4[CBW];
+#pragma unroll
+    for (int cb = 0; cb < CBW; ++cb) {
+      const int col = (wc * CBW + cb) * 16 + (lane & 15);
+      b4[cb] = *reinterpret_cast<const v4 *>(&Xs[col * RS + (lane >> 4) * 4]);
+    }
+#pragma unroll
+    for (int j = 0; j < RBW; ++j) {
+      const int rb = wr + WR * j;
+      if (rb < nrb) {
+        const v4 a4 =
+            *reinterpret_cast<const v4 *>(&Hs[(rb * 16 + (lane & 15)) * RS + (lane >> 4) * 4]);
+#pragma unroll
+        for (int cb = 0; cb < CBW; ++cb) {
+          acc[j][cb] = mfma16(a4[0], b4[cb][0], acc[j][cb]);
+          acc[j][cb] = mfma16(a4[1], b4[cb][1], acc[j][cb]);
+          acc[j][cb] = mfma16(a4[2], b4[cb][2], acc[j][cb]);
+          acc[j][cb] = mfma16(a4[3], b4[cb][3], acc[j][cb]);
+        }
+      }
+    }
+  }
+
+  // epilogue: scaled scatter with the first-touch combine
+#pragma unroll
+  for (int j = 0; j < RBW; ++j) {
+    const int rb = wr + WR * j;
+    if (rb >= nrb) continue;
+#pragma unroll
+    for (int cb = 0; cb < CBW; ++cb) {
+      const int v = v0 + (wc * CBW + cb) * 16 + (lane & 15);
+      if (v >= a.B) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = rb * 16 + acc_row<T>(lane, r);
+        if (row >= a.ne) continue;
+        const uint32_t w = cmap[row];
+        const int64_t g = w & 0x7fffffffu;
+        const T so = a.s_out ? a.s_out[g] : T(1);
+        const int64_t addr = g * a.ld_dst + v;
+        T base;
+        if (w >> 31) {
+          base = T(0);
+          if (a.beta_mode == 0)
+            base = a.beta * a.dst[addr];
+          else if (a.beta_mode == 2)
+            base = a.beta * so * a.dst[addr];
+          if (a.gamma != T(0)) base += a.gamma * a.src[g * a.ld_src + v];
+        } else {
+          base = a.dst[addr];
+        }
+        a.dst[addr] = base + a.alpha * so * acc[j][cb][r];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Projection partials  P_slot[v] = sum_i C_e[i, slot] * s_in[g_i] src[g_i, v]
+// ---------------------------------------------------------------------------
+template <typename T, int TN, int MAXSRB>
+__global__ __launch_bounds__(512) void proj_kernel(ProjLaunch<T> a) {
+  using v4 = typename V4<T>::t;
+  constexpr int RS = row_stride<T>();
+  constexpr int KPT = kTile * TN / 512;
+  constexpr int NCB = TN / 16;
+  constexpr int PPW = (MAXSRB * NCB + 7) / 8;
+  constexpr int ACH = (MAXSRB * 16 * 4 + 511) / 512;
+  __shared__ __align__(16) T As[MAXSRB * 16 * RS];
+  __shared__ __align__(16) T Xs[TN * RS];
+
+  const int item = blockIdx.x;
+  if (item >= a.ncells * a.ntiles) return;
+  const int cell = a.cells[item / a.ntiles];
+  const int v0 = (item % a.ntiles) * TN;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int s0 = a.nl_off[cell];
+  const int nslot = a.nl_off[cell + 1] - s0;
+  const int nsrb = nslot >> 4;
+  const int npairs = nsrb * NCB;
+  const int nkt = a.ne_pad >> 4;
+  const uint32_t *cmap = a.map + (int64_t)cell * a.ne;
+  const T *Acell = a.nl_Ct + (int64_t)s0 * a.ne_pad;
+
+  const int gv = tid % TN;
+  const int gk = (tid / TN) * KPT;
+  const bool vok = (v0 + gv) < a.B;
+  const int nchunk = nslot * 4;
+
+  v4 acc[PPW];
+#pragma unroll
+  for (int j = 0; j < PPW; ++j) acc[j] = v4{0, 0, 0, 0};
+
+  for (int t = 0; t < nkt; ++t) {
+    const T *tile_src = Acell + (int64_t)t * nslot * 16;
+    v4 areg[ACH];
+    T xreg[KPT];
+#pragma unroll
+    for (int j = 0; j < ACH; ++j) {
+      const int chunk = tid + 512 * j;
+      if (chunk < nchunk) areg[j] = reinterpret_cast<const v4 *>(tile_src)[chunk];
+    }
+#pragma unroll
+    for (int kk = 0; kk < KPT; ++kk) {
+      const int k = t * 16 + gk + kk;
+      T val = T(0);
+      if (k < a.ne && vok) {
+        const uint32_t g = cmap[k] & 0x7fffffffu;
+        const T sc = a.s_in ? a.s_in[g] : T(1);
+        val = sc * a.src[(int64_t)g * a.ld_src + v0 + gv];
+      }
+      xreg[kk] = val;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < ACH; ++j) {
+      const int chunk = tid + 512 * j;
+      if (chunk < nchunk)
+        *reinterpret_cast<v4 *>(&As[(chunk >> 2) * RS + (chunk & 3) * 4]) = areg[j];
+    }
+#pragma unroll
+    for (int kk = 0; kk < KPT; ++kk) Xs[gv * RS + gk + kk] = xreg[kk];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < PPW; ++j) {
+      const int p = wave + 8 * j;
+      if (p < npairs) {
+        const int srb = p / NCB, cb = p % NCB;
+        const v4 a4 =
+            *reinterpret_cast<const v4 *>(&As[(srb * 16 + (lane & 15)) * RS + (lane >> 4) * 4]);
+        const v4 b4 =
+            *reinterpret_cast<const v4 *>(&Xs[(cb * 16 + (lane & 15)) * RS + (lane >> 4) * 4]);
+        acc[j] = mfma16(a4[0], b4[0], acc[j]);
+        acc[j] = mfma16(a4[1], b4[1], acc[j]);
+        acc[j] = mfma16(a4[2], b4[2], acc[j]);
+        acc[j] = mfma16(a4[3], b4[3], acc[j]);
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < PPW; ++j) {
+    const int p = wave + 8 * j;
+    if (p >= npairs) continue;
+    const int srb = p / NCB, cb = p % NCB;
+    const int v = v0 + cb * 16 + (lane & 15);
+    if (v >= a.B) continue;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int srow = srb * 16 + acc_row<T>(lane, r);
+      a.partial[(int64_t)(s0 + srow) * a.ldP + v] = acc[j][r];
+    }
+  }
+}
+
+template <typename T>
+__global__ void proj_reduce_kernel(const int *proj_ptr, const int *proj_slots,
+                                   const double *proj_V, int n_proj, const T *partial,
+                                   int64_t ldP, T *W, int64_t ldW, int B) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n_proj * B) return;
+  const int p = (int)(idx / B);
+  const int v = (int)(idx % B);
+  T s = T(0);
+  for (int j = proj_ptr[p]; j < proj_ptr[p + 1]; ++j) s += partial[(int64_t)proj_slots[j] * ldP + v];
+  W[(int64_t)p * ldW + v] = T(proj_V[p]) * s;
+}
+
+// ---------------------------------------------------------------------------
+// Cell Hamiltonian build  H_e = 1/2 K + N^T diag(vJxW_c) N
+// (hamMatrixKernelLDA, hamiltonianMatrixCalculatorFlattenedDevice.cc:61-116)
+// Output written in the padded K-tiled layout [kt][row][16] of the HX kernel.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void build_h_kernel(const double *K, const double *Nq,
+                                                      const double *vJxW, int ne, int ne_pad,
+                                                      int nq, double *Ht64, float *Ht32) {
+  constexpr int TB = 32, TQ = 32;
+  __shared__ double Ni[TQ][TB + 1];
+  __shared__ double Nj[TQ][TB + 1];
+  const int ntile = (ne_pad + TB - 1) / TB;
+  const int ti = blockIdx.x / ntile, tj = blockIdx.x % ntile;
+  const int cell = blockIdx.y;
+  const int tid = threadIdx.x;
+  const int tx = tid % 16, ty = tid / 16;
+  const double *v = vJxW + (int64_t)cell * nq;
+  double acc[2][2] = {{0, 0}, {0, 0}};
+  for (int q0 = 0; q0 < nq; q0 += TQ) {
+    for (int e = tid; e < TQ * TB; e += 256) {
+      const int qq = e / TB, ii = e % TB;
+      const int q = q0 + qq;
+      const int i = ti * TB + ii, j = tj * TB + ii;
+      double vi = 0, vj = 0;
+      if (q < nq) {
+        if (i < ne) vi = Nq[(int64_t)q * ne + i] * v[q];
+        if (j < ne) vj = Nq[(int64_t)q * ne + j];
+      }
+      Ni[qq][ii] = vi;
+      Nj[qq][ii] = vj;
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int qq = 0; qq < TQ; ++qq) {
+      const double a0 = Ni[qq][2 * ty], a1 = Ni[qq][2 * ty + 1];
+      const double b0 = Nj[qq][2 * tx], b1 = Nj[qq][2 * tx + 1];
+      acc[0][0] += a0 * b0;
+      acc[0][1] += a0 * b1;
+      acc[1][0] += a1 * b0;
+      acc[1][1] += a1 * b1;
+    }
+    __syncthreads();
+  }
+  const int64_t base = (int64_t)cell * ne_pad * ne_pad;
+#pragma unroll
+  for (int di = 0; di < 2; ++di)
+#pragma unroll
+    for (int dj = 0; dj < 2; ++dj) {
+      const int i = ti * TB + 2 * ty + di, j = tj * TB + 2 * tx + dj;
+      if (i >= ne_pad || j >= ne_pad) continue;
+      double h = 0.0;
+      if (i < ne && j < ne) h = 0.5 * K[(int64_t)i * ne + j] + acc[di][dj];
+      const int64_t o = base + (int64_t)(j >> 4) * ne_pad * 16 + (int64_t)i * 16 + (j & 15);
+      Ht64[o] = h;
+      if (Ht32) Ht32[o] = (float)h;
+    }
+}
+
+__global__ void untile_h_kernel(const double *Ht, int C, int ne, int ne_pad, double *H) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = (int64_t)ne * ne;
+  if (idx >= per * C) return;
+  const int64_t c = idx / per;
+  const int i = (int)((idx % per) / ne), j = (int)(idx % ne);
+  H[idx] = Ht[c * ne_pad * ne_pad + (int64_t)(j >> 4) * ne_pad * 16 + (int64_t)i * 16 + (j & 15)];
+}
+
+__global__ void tile_h_kernel(const double *H, int C, int ne, int ne_pad, double *Ht64,
+                              float *Ht32) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = (int64_t)ne_pad * ne_pad;
+  if (idx >= per * C) return;
+  const int64_t c = idx / per;
+  const int64_t r = idx % per;
+  const int kt = (int)(r / (ne_pad * 16));
+  const int i = (int)((r / 16) % ne_pad);
+  const int j = kt * 16 + (int)(r % 16);
+  const double h = (i < ne && j < ne) ? H[c * ne * ne + (int64_t)i * ne + j] : 0.0;
+  Ht64[idx] = h;
+  if (Ht32) Ht32[idx] = (float)h;
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise / reduction kernels
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void scale_rows_kernel(T *x, int64_t ld, int64_t M, int B, double a, const double *s) {
+  const int64_t n = M * B;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t g = idx / B;
+    const int v = (int)(idx % B);
+    x[g * ld + v] = (T)((double)x[g * ld + v] * a * s[g]);
+  }
+}
+
+template <typename TS, typename TD>
+__global__ void copy_block_kernel(const TS *src, int64_t ld_src, TD *dst, int64_t ld_dst,
+                                  int64_t M, int B) {
+  const int64_t n = M * B;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t g = idx / B;
+    const int v = (int)(idx % B);
+    dst[g * ld_dst + v] = (TD)src[g * ld_src + v];
+  }
+}
+
+template <typename T>
+__global__ void axpby_kernel(int64_t n, T a, const T *x, T b, T *y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = a * x[i] + b * y[i];
+}
+
+__global__ __launch_bounds__(256) void residual_partial_kernel(const double *HX, int64_t ldH,
+                                                               const double *X, int64_t ldX,
+                                                               const double *lam, int64_t M,
+                                                               int B, double *partial,
+                                                               int nchunks) {
+  __shared__ double red[4][64];
+  const int tv = threadIdx.x & 63, tg = threadIdx.x >> 6;
+  const int v = blockIdx.y * 64 + tv;
+  const int64_t rows = (M + nchunks - 1) / nchunks;
+  const int64_t g0 = (int64_t)blockIdx.x * rows;
+  const int64_t g1 = g0 + rows < M ? g0 + rows : M;
+  double s = 0.0;
+  if (v < B) {
+    const double l = lam[v];
+    for (int64_t g = g0 + tg; g < g1; g += 4) {
+      const double r = HX[g * ldH + v] - l * X[g * ldX + v];
+      s += r * r;
+    }
+  }
+  red[tg][tv] = s;
+  __syncthreads();
+  if (tg == 0 && v < B)
+    partial[(int64_t)blockIdx.x * B + v] = red[0][tv] + red[1][tv] + red[2][tv] + red[3][tv];
+}
+
+__global__ __launch_bounds__(256) void dot_partial_kernel(const double *x, const double *y,
+                                                          int64_t n, double *partial,
+                                                          int nchunks) {
+  __shared__ double red[256];
+  const int64_t per = (n + nchunks - 1) / nchunks;
+  const int64_t i0 = (int64_t)blockIdx.x * per;
+  const int64_t i1 = i0 + per < n ? i0 + per : n;
+  double s = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) s += x[i] * y[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void distribute_kernel(const int64_t *rows, const int64_t *ptr, const int64_t *cols,
+                                  const double *vals, int64_t nrows, double *X, int64_t ld,
+                                  int B) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nrows * B) return;
+  const int64_t r = idx / B;
+  const int v = (int)(idx % B);
+  double s = 0.0;
+  for (int64_t j = ptr[r]; j < ptr[r + 1]; ++j) s += vals[j] * X[cols[j] * ld + v];
+  X[rows[r] * ld + v] = s;
+}
+
+__global__ void slave_to_master_kernel(const int64_t *m_rows, const int64_t *m_ptr,
+                                       const int64_t *m_slaves, const double *m_vals, int64_t nm,
+                                       double *X, int64_t ld, int B) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nm * B) return;
+  const int64_t r = idx / B;
+  const int v = (int)(idx % B);
+  double s = 0.0;
+  for (int64_t j = m_ptr[r]; j < m_ptr[r + 1]; ++j) s += m_vals[j] * X[m_slaves[j] * ld + v];
+  X[m_rows[r] * ld + v] += s;
+}
+
+__global__ void zero_rows_kernel(const int64_t *rows, int64_t nrows, double *X, int64_t ld,
+                                 int B) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nrows * B) return;
+  X[rows[idx / B] * ld + idx % B] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+static inline unsigned grid_for(int64_t n, int block, unsigned cap = 1u << 20) {
+  int64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
+template <typename T, int TN, int WC, int RBW>
+static hipError_t hx_go(HxLaunch<T> a, hipStream_t s) {
+  const int items = a.ncells * a.ntiles;
+  if (items == 0) return hipSuccess;
+  a.per_xcd = (items + 7) / 8;
+  hipLaunchKernelGGL((hx_kernel<T, TN, WC, RBW>), dim3(8 * a.per_xcd), dim3(512), 0, s, a);
+  return hipGetLastError();
+}
+
+// Tile choices: FP32 uses 128-column tiles with one column block per wave
+// (8 x 16 = 128) and all row blocks; FP64 uses 64-column tiles, two row groups.
+template <>
+hipError_t launch_hx<float>(const HxLaunch<float> &a, hipStream_t s) {
+  HxLaunch<float> b = a;
+  if (a.B > 64) {
+    b.ntiles = (a.B + 127) / 128;
+    if (a.ne_pad <= 128) return hx_go<float, 128, 8, 8>(b, s);
+    if (a.ne_pad <= 256) return hx_go<float, 128, 8, 16>(b, s);
+    return hx_go<float, 128, 8, 24>(b, s);
+  }
+  b.ntiles = (a.B + 63) / 64;
+  if (a.ne_pad <= 128) return hx_go<float, 64, 4, 4>(b, s);
+  if (a.ne_pad <= 256) return hx_go<float, 64, 4, 8>(b, s);
+  return hx_go<float, 64, 4, 12>(b, s);
+}
+
+template <>
+hipError_t launch_hx<double>(const HxLaunch<double> &a, hipStream_t s) {
+  HxLaunch<double> b = a;
+  b.ntiles = (a.B + 63) / 64;
+  if (a.ne_pad <= 128) return hx_go<double, 64, 4, 4>(b, s);
+  if (a.ne_pad <= 256) return hx_go<double, 64, 4, 8>(b, s);
+  return hx_go<double, 64, 4, 12>(b, s);
+}
+
+template <>
+hipError_t launch_proj<float>(const ProjLaunch<float> &a, hipStream_t s) {
+  ProjLaunch<float> b = a;
+  b.ntiles = (a.B + 63) / 64;
+  const int items = b.ncells * b.ntiles;
+  if (items == 0) return hipSuccess;
+  hipLaunchKernelGGL((proj_kernel<float, 64, 8>), dim3(items), dim3(512), 0, s, b);
+  return hipGetLastError();
+}
+
+template <>
+hipError_t launch_proj<double>(const ProjLaunch<double> &a, hipStream_t s) {
+  ProjLaunch<double> b = a;
+  b.ntiles = (a.B + 63) / 64;
+  const int items = b.ncells * b.ntiles;
+  if (items == 0) return hipSuccess;
+  hipLaunchKernelGGL((proj_kernel<double, 64, 8>), dim3(items), dim3(512), 0, s, b);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_proj_reduce(const int *proj_ptr, const int *proj_slots, const double *proj_V,
+                              int n_proj, const T *partial, int64_t ldP, T *W, int64_t ldW,
+                              int B, hipStream_t s) {
+  const int64_t n = (int64_t)n_proj * B;
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(proj_reduce_kernel<T>, dim3(grid_for(n, 256, 1u << 30)), dim3(256), 0, s,
+                     proj_ptr, proj_slots, proj_V, n_proj, partial, ldP, W, ldW, B);
+  return hipGetLastError();
+}
+template hipError_t launch_proj_reduce<float>(const int *, const int *, const double *, int,
+                                              const float *, int64_t, float *, int64_t, int,
+                                              hipStream_t);
+template hipError_t launch_proj_reduce<double>(const int *, const int *, const double *, int,
+                                               const double *, int64_t, double *, int64_t, int,
+                                               hipStream_t);
+
+hipError_t launch_build_h(const double *K, const double *Nq, const double *vJxW, int C, int ne,
+                          int ne_pad, int nq, double *Ht64, float *Ht32, hipStream_t s) {
+  const int ntile = (ne_pad + 31) / 32;
+  hipLaunchKernelGGL(build_h_kernel, dim3(ntile * ntile, C), dim3(256), 0, s, K, Nq, vJxW, ne,
+                     ne_pad, nq, Ht64, Ht32);
+  return hipGetLastError();
+}
+
+hipError_t launch_untile_h(const double *Ht, int C, int ne, int ne_pad, double *H, hipStream_t s) {
+  const int64_t n = (int64_t)C * ne * ne;
+  hipLaunchKernelGGL(untile_h_kernel, dim3(grid_for(n, 256, 1u << 30)), dim3(256), 0, s, Ht, C,
+                     ne, ne_pad, H);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_h(const double *H, int C, int ne, int ne_pad, double *Ht64, float *Ht32,
+                         hipStream_t s) {
+  const int64_t n = (int64_t)C * ne_pad * ne_pad;
+  hipLaunchKernelGGL(tile_h_kernel, dim3(grid_for(n, 256, 1u << 30)), dim3(256), 0, s, H, C, ne,
+                     ne_pad, Ht64, Ht32);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_scale_rows(T *x, int64_t ld, int64_t M, int B, double a, const double *s,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(scale_rows_kernel<T>, dim3(grid_for(M * B, 256, 65536)), dim3(256), 0, st,
+                     x, ld, M, B, a, s);
+  return hipGetLastError();
+}
+template hipError_t launch_scale_rows<double>(double *, int64_t, int64_t, int, double,
+                                              const double *, hipStream_t);
+template hipError_t launch_scale_rows<float>(float *, int64_t, int64_t, int, double,
+                                             const double *, hipStream_t);
+
+template <typename TS, typename TD>
+hipError_t launch_copy_block(const TS *src, int64_t ld_src, TD *dst, int64_t ld_dst, int64_t M,
+                             int B, hipStream_t st) {
+  hipLaunchKernelGGL((copy_block_kernel<TS, TD>), dim3(grid_for(M * B, 256, 65536)), dim3(256), 0,
+                     st, src, ld_src, dst, ld_dst, M, B);
+  return hipGetLastError();
+}
+template hipError_t launch_copy_block<double, double>(const double *, int64_t, double *, int64_t,
+                                                      int64_t, int, hipStream_t);
+template hipError_t launch_copy_block<double, float>(const double *, int64_t, float *, int64_t,
+                                                     int64_t, int, hipStream_t);
+template hipError_t launch_copy_block<float, double>(const float *, int64_t, double *, int64_t,
+                                                     int64_t, int, hipStream_t);
+
+template <typename T>
+hipError_t launch_axpby(int64_t n, T a, const T *x, T b, T *y, hipStream_t st) {
+  hipLaunchKernelGGL(axpby_kernel<T>, dim3(grid_for(n, 256, 65536)), dim3(256), 0, st, n, a, x, b,
+                     y);
+  return hipGetLastError();
+}
+template hipError_t launch_axpby<double>(int64_t, double, const double *, double, double *,
+                                         hipStream_t);
+template hipError_t launch_axpby<float>(int64_t, float, const float *, float, float *,
+                                        hipStream_t);
+
+hipError_t launch_residual_partial(const double *HX, int64_t ldH, const double *X, int64_t ldX,
+                                   const double *lam, int64_t M, int B, double *partial,
+                                   int nchunks, hipStream_t st) {
+  hipLaunchKernelGGL(residual_partial_kernel, dim3(nchunks, (B + 63) / 64), dim3(256), 0, st, HX,
+                     ldH, X, ldX, lam, M, B, partial, nchunks);
+  return hipGetLastError();
+}
+
+hipError_t launch_dot_partial(const double *x, const double *y, int64_t n, double *partial,
+                              int nchunks, hipStream_t st) {
+  hipLaunchKernelGGL(dot_partial_kernel, dim3(nchunks), dim3(256), 0, st, x, y, n, partial,
+                     nchunks);
+  return hipGetLastError();
+}
+
+hipError_t launch_distribute(const int64_t *rows, const int64_t *ptr, const int64_t *cols,
+                             const double *vals, int64_t nrows, double *X, int64_t ld, int B,
+                             hipStream_t st) {
+  if (nrows == 0) return hipSuccess;
+  hipLaunchKernelGGL(distribute_kernel, dim3(grid_for(nrows * B, 256, 1u << 30)), dim3(256), 0,
+                     st, rows, ptr, cols, vals, nrows, X, ld, B);
+  return hipGetLastError();
+}
+
+hipError_t launch_slave_to_master(const int64_t *m_rows, const int64_t *m_ptr,
+                                  const int64_t *m_slaves, const double *m_vals, int64_t nm,
+                                  double *X, int64_t ld, int B, hipStream_t st) {
+  if (nm == 0) return hipSuccess;
+  hipLaunchKernelGGL(slave_to_master_kernel, dim3(grid_for(nm * B, 256, 1u << 30)), dim3(256), 0,
+                     st, m_rows, m_ptr, m_slaves, m_vals, nm, X, ld, B);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_rows(const int64_t *rows, int64_t nrows, double *X, int64_t ld, int B,
+                            hipStream_t st) {
+  if (nrows == 0) return hipSuccess;
+  hipLaunchKernelGGL(zero_rows_kernel, dim3(grid_for(nrows * B, 256, 1u << 30)), dim3(256), 0, st,
+                     rows, nrows, X, ld, B);
+  return hipGetLastError();
+}
+
+}  // namespace chfsi

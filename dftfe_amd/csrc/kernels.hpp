@@ -1,0 +1,113 @@
+// Device kernels of libchfsi and their host-side launchers (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace chfsi {
+
+// Rows of the cell GEMM are padded to a multiple of 16 (one MFMA block) and
+// the K dimension is streamed in tiles of 16.
+constexpr int kTile = 16;
+
+// Arguments of one fused HX launch over one colour class of cells.
+//
+//   dst[g,v] <- base(g,v) + alpha * s_out[g] * sum_k Htile(c)[i,k] * xin(k,v)
+//
+// where for the cell-local node i mapping to DOF g:
+//   base = (first touch of g in colour order)
+//            ? beta' * dst[g,v] + gamma * src[g,v]    (beta' = beta or beta*s_out[g])
+//            : dst[g,v]
+// and xin(k,v) = s_in[g_k] * src[g_k, v] for the n_e nodes, followed by the
+// cell's nonlocal slots whose rows come from W[proj(slot), v] (already scaled
+// by the coupling constants).
+template <typename T>
+struct HxLaunch {
+  const int *cells;        // cell ids of this colour
+  int ncells;
+  int ntiles;              // column tiles of TN
+  int per_xcd;             // work items per XCD slice
+  const uint32_t *map;     // C*n_e: DOF | (first-touch flag << 31)
+  int ne, ne_pad;
+  const T *Ht;             // C * ne_pad * ne_pad, tiled [kt][row][16]
+  const T *src;
+  int64_t ld_src;
+  T *dst;
+  int64_t ld_dst;
+  const T *s_in;           // nullable (== 1)
+  const T *s_out;          // nullable (== 1)
+  T alpha, beta, gamma;
+  int beta_mode;           // 0: beta*dst, 1: dst not read (beta = 0), 2: beta*s_out*dst
+  int B;
+  const int *nl_off;       // C+1 slot offsets (multiples of 16); nullable
+  const T *nl_C;           // per cell [kt][row][16] blocks, slot-major offsets nl_off[c]*ne_pad
+  const int *slot_proj;    // slot -> projector (-1 pad)
+  const T *W;              // n_proj x ldW
+  int64_t ldW;
+};
+
+// Arguments of the projection C_e^T (s_in * src)_e for all cells with slots.
+template <typename T>
+struct ProjLaunch {
+  const int *cells;        // cells that own slots
+  int ncells;
+  int ntiles;
+  const uint32_t *map;
+  int ne, ne_pad;
+  const int *nl_off;
+  const T *nl_Ct;          // per cell [kt][slot][16] over nodes, offset nl_off[c]*ne_pad
+  const T *src;
+  int64_t ld_src;
+  const T *s_in;
+  T *partial;              // n_slots x ldP
+  int64_t ldP;
+  int B;
+};
+
+template <typename T>
+hipError_t launch_hx(const HxLaunch<T> &a, hipStream_t s);
+template <typename T>
+hipError_t launch_proj(const ProjLaunch<T> &a, hipStream_t s);
+template <typename T>
+hipError_t launch_proj_reduce(const int *proj_ptr, const int *proj_slots, const double *proj_V,
+                              int n_proj, const T *partial, int64_t ldP, T *W, int64_t ldW,
+                              int B, hipStream_t s);
+
+// H_e = 1/2 K + N^T diag(vJxW_c) N, written tiled/padded in FP64 and FP32.
+hipError_t launch_build_h(const double *K, const double *Nq, const double *vJxW, int C, int ne,
+                          int ne_pad, int nq, double *Ht64, float *Ht32, hipStream_t s);
+// tiled padded -> dense C x ne x ne (row-major) and back
+hipError_t launch_untile_h(const double *Ht, int C, int ne, int ne_pad, double *H, hipStream_t s);
+hipError_t launch_tile_h(const double *H, int C, int ne, int ne_pad, double *Ht64, float *Ht32,
+                         hipStream_t s);
+
+// x[g, v] *= a * s[g]  (stridedBlockScale, DeviceKernelsGeneric.cc:255-278)
+template <typename T>
+hipError_t launch_scale_rows(T *x, int64_t ld, int64_t M, int B, double a, const double *s,
+                             hipStream_t st);
+// dst[g, v] = (TD) src[g, v]  with independent leading dimensions and types
+template <typename TS, typename TD>
+hipError_t launch_copy_block(const TS *src, int64_t ld_src, TD *dst, int64_t ld_dst, int64_t M,
+                             int B, hipStream_t st);
+// y = a*x + b*y on contiguous arrays
+template <typename T>
+hipError_t launch_axpby(int64_t n, T a, const T *x, T b, T *y, hipStream_t st);
+// column sums of squares of (Y - X*diag(lam)) into partial[chunk][B]
+hipError_t launch_residual_partial(const double *HX, int64_t ldH, const double *X, int64_t ldX,
+                                   const double *lam, int64_t M, int B, double *partial,
+                                   int nchunks, hipStream_t st);
+// dot products x.y into partial[chunk]; then the host sums
+hipError_t launch_dot_partial(const double *x, const double *y, int64_t n, double *partial,
+                              int nchunks, hipStream_t st);
+
+// constraints: CSR over slave rows
+hipError_t launch_distribute(const int64_t *rows, const int64_t *ptr, const int64_t *cols,
+                             const double *vals, int64_t nrows, double *X, int64_t ld, int B,
+                             hipStream_t st);
+hipError_t launch_slave_to_master(const int64_t *m_rows, const int64_t *m_ptr,
+                                  const int64_t *m_slaves, const double *m_vals, int64_t nm,
+                                  double *X, int64_t ld, int B, hipStream_t st);
+hipError_t launch_zero_rows(const int64_t *rows, int64_t nrows, double *X, int64_t ld, int B,
+                            hipStream_t st);
+
+}  // namespace chfsi

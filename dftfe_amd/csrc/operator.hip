@@ -1,0 +1,474 @@
+// Operator side of the C ABI: device-resident FE operator data (cell maps,
+// colour classes, mass vectors, cell Hamiltonians, nonlocal projector slots,
+// constraints) and the HX / HXCheby / constraint entry points.
+//
+// Reference: kohnShamDFTOperatorDeviceClass (include/kohnShamDFTOperatorDevice.h:43-782,
+// src/dftOperator/kohnShamDFTOperatorDevice.cc:492-1057, :3609-3997) and
+// constraintMatrixInfoDevice (utils/constraintMatrixInfoDevice.cc).
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+#include "operator.hpp"
+
+namespace chfsi {
+
+static thread_local std::string g_last_error;
+
+void set_error(const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+}
+
+template <typename T>
+static chfsi_status dev_upload(T **d, const T *h, size_t n) {
+  *d = nullptr;
+  if (n == 0) return CHFSI_OK;
+  if (hipMalloc(reinterpret_cast<void **>(d), n * sizeof(T)) != hipSuccess) {
+    set_error("hipMalloc of %zu bytes failed", n * sizeof(T));
+    return CHFSI_ERR_NOMEM;
+  }
+  if (h) CHFSI_HIP(hipMemcpy(*d, h, n * sizeof(T), hipMemcpyHostToDevice));
+  return CHFSI_OK;
+}
+
+template <typename T>
+static chfsi_status dev_alloc(T **d, size_t n) {
+  return dev_upload<T>(d, nullptr, n);
+}
+
+chfsi_status Scratch::ensure(size_t bytes) {
+  if (bytes <= size) return CHFSI_OK;
+  if (ptr) hipFree(ptr);
+  ptr = nullptr;
+  size = 0;
+  if (hipMalloc(&ptr, bytes) != hipSuccess) {
+    set_error("hipMalloc of %zu bytes failed", bytes);
+    return CHFSI_ERR_NOMEM;
+  }
+  size = bytes;
+  return CHFSI_OK;
+}
+
+void Scratch::release() {
+  if (ptr) hipFree(ptr);
+  ptr = nullptr;
+  size = 0;
+}
+
+}  // namespace chfsi
+
+using namespace chfsi;
+
+chfsi_operator::~chfsi_operator() {
+  void *ptrs[] = {d_map, d_cells_by_color, d_sqrt_mass, d_inv_sqrt_mass, d_sqrt_mass32,
+                  d_inv_sqrt_mass32, d_K, d_Nq, d_Ht64, d_Ht32, d_nl_off, d_nlC64, d_nlC32,
+                  d_nlCt64, d_nlCt32, d_slot_proj, d_proj_ptr, d_proj_slots, d_proj_V,
+                  d_nl_cells, d_c_rows, d_c_ptr, d_c_cols, d_c_vals, d_m_rows, d_m_ptr,
+                  d_m_slaves, d_m_vals};
+  for (void *p : ptrs)
+    if (p) hipFree(p);
+  partial64.release();
+  partial32.release();
+  W64.release();
+  W32.release();
+  tmp.release();
+}
+
+extern "C" const char *chfsi_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" chfsi_status chfsi_operator_create(const chfsi_operator_desc *d, chfsi_operator **out) {
+  CHFSI_ARG(d && out, "null argument");
+  *out = nullptr;
+  CHFSI_ARG(d->n_dofs > 0 && d->n_dofs < (int64_t(1) << 31), "n_dofs out of range");
+  CHFSI_ARG(d->n_cells > 0 && d->nodes_per_cell > 0 && d->quad_per_cell > 0, "bad sizes");
+  CHFSI_ARG(d->cell_map && d->cell_color && d->sqrt_mass && d->inv_sqrt_mass, "missing arrays");
+  const int C = d->n_cells, ne = d->nodes_per_cell;
+  const int ne_pad = (ne + 15) / 16 * 16;
+  CHFSI_ARG(ne_pad <= 384, "nodes_per_cell %d exceeds the supported maximum (FE degree <= 6)", ne);
+  const int64_t M = d->n_dofs;
+
+  auto op = new chfsi_operator();
+  std::unique_ptr<chfsi_operator> guard(op);
+  CHFSI_HIP(hipGetDevice(&op->device));
+  op->M = M;
+  op->C = C;
+  op->ne = ne;
+  op->ne_pad = ne_pad;
+  op->nq = d->quad_per_cell;
+
+  // ---- constraints: periodic identities (single master, coefficient 1) are
+  // folded into the cell map; general rows are kept for the explicit kernels.
+  std::vector<int64_t> fold(M);
+  for (int64_t g = 0; g < M; ++g) fold[g] = g;
+  op->nc = d->n_constrained;
+  bool all_identity = true;
+  if (op->nc > 0) {
+    CHFSI_ARG(d->constraint_rows && d->constraint_ptr && d->constraint_cols && d->constraint_vals,
+              "missing constraint arrays");
+    for (int64_t r = 0; r < op->nc; ++r) {
+      const int64_t s = d->constraint_rows[r];
+      CHFSI_ARG(s >= 0 && s < M, "constraint row out of range");
+      const int64_t b = d->constraint_ptr[r], e = d->constraint_ptr[r + 1];
+      for (int64_t j = b; j < e; ++j)
+        CHFSI_ARG(d->constraint_cols[j] >= 0 && d->constraint_cols[j] < M, "master out of range");
+      if (e - b == 1 && d->constraint_vals[b] == 1.0)
+        fold[s] = d->constraint_cols[b];
+      else
+        all_identity = false;
+    }
+    CHFSI_ARG(all_identity,
+              "only periodic (single-master, unit-coefficient) constraints are supported by HX");
+    const int64_t nnz = d->constraint_ptr[op->nc];
+    std::vector<int64_t> ptr(d->constraint_ptr, d->constraint_ptr + op->nc + 1);
+    CHFSI_TRY(dev_upload(&op->d_c_rows, d->constraint_rows, op->nc));
+    CHFSI_TRY(dev_upload(&op->d_c_ptr, ptr.data(), op->nc + 1));
+    CHFSI_TRY(dev_upload(&op->d_c_cols, d->constraint_cols, nnz));
+    CHFSI_TRY(dev_upload(&op->d_c_vals, d->constraint_vals, nnz));
+    // transpose to master-major CSR for the atomic-free slave->master kernel
+    std::map<int64_t, std::vector<std::pair<int64_t, double>>> bym;
+    for (int64_t r = 0; r < op->nc; ++r)
+      for (int64_t j = d->constraint_ptr[r]; j < d->constraint_ptr[r + 1]; ++j)
+        bym[d->constraint_cols[j]].push_back({d->constraint_rows[r], d->constraint_vals[j]});
+    std::vector<int64_t> mrows, mptr{0}, mslaves;
+    std::vector<double> mvals;
+    for (auto &kv : bym) {
+      mrows.push_back(kv.first);
+      for (auto &sv : kv.second) {
+        mslaves.push_back(sv.first);
+        mvals.push_back(sv.second);
+      }
+      mptr.push_back((int64_t)mslaves.size());
+    }
+    op->nm = (int64_t)mrows.size();
+    CHFSI_TRY(dev_upload(&op->d_m_rows, mrows.data(), mrows.size()));
+    CHFSI_TRY(dev_upload(&op->d_m_ptr, mptr.data(), mptr.size()));
+    CHFSI_TRY(dev_upload(&op->d_m_slaves, mslaves.data(), mslaves.size()));
+    CHFSI_TRY(dev_upload(&op->d_m_vals, mvals.data(), mvals.size()));
+  }
+
+  // ---- colour classes and first-touch flags
+  std::vector<std::vector<int>> by_color(8);
+  for (int c = 0; c < C; ++c) {
+    const int col = d->cell_color[c];
+    CHFSI_ARG(col >= 0 && col < 8, "cell colour must be in [0, 8)");
+    by_color[col].push_back(c);
+  }
+  std::vector<int> cells_sorted;
+  op->color_off[0] = 0;
+  for (int k = 0; k < 8; ++k) {
+    cells_sorted.insert(cells_sorted.end(), by_color[k].begin(), by_color[k].end());
+    op->color_off[k + 1] = (int)cells_sorted.size();
+  }
+  std::vector<uint32_t> map(size_t(C) * ne);
+  std::vector<int> touched_color(M, -1);
+  std::vector<int> touched_cell(M, -1);
+  for (int k = 0; k < 8; ++k) {
+    for (int c : by_color[k]) {
+      for (int i = 0; i < ne; ++i) {
+        const int64_t g0 = d->cell_map[size_t(c) * ne + i];
+        CHFSI_ARG(g0 >= 0 && g0 < M, "cell_map entry out of range");
+        const int64_t g = fold[g0];
+        uint32_t w = (uint32_t)g;
+        if (touched_color[g] == k) {
+          set_error("cells %d and %d of colour %d share DOF %lld: colouring invalid",
+                    touched_cell[g], c, k, (long long)g);
+          return CHFSI_ERR_ARG;
+        }
+        if (touched_color[g] < 0) w |= 0x80000000u;
+        touched_color[g] = k;
+        touched_cell[g] = c;
+        map[size_t(c) * ne + i] = w;
+      }
+    }
+  }
+  CHFSI_TRY(dev_upload(&op->d_map, map.data(), map.size()));
+  CHFSI_TRY(dev_upload(&op->d_cells_by_color, cells_sorted.data(), cells_sorted.size()));
+
+  // ---- mass vectors
+  std::vector<float> f32(M);
+  CHFSI_TRY(dev_upload(&op->d_sqrt_mass, d->sqrt_mass, M));
+  CHFSI_TRY(dev_upload(&op->d_inv_sqrt_mass, d->inv_sqrt_mass, M));
+  for (int64_t g = 0; g < M; ++g) f32[g] = (float)d->sqrt_mass[g];
+  CHFSI_TRY(dev_upload(&op->d_sqrt_mass32, f32.data(), M));
+  for (int64_t g = 0; g < M; ++g) f32[g] = (float)d->inv_sqrt_mass[g];
+  CHFSI_TRY(dev_upload(&op->d_inv_sqrt_mass32, f32.data(), M));
+
+  // ---- shape data for the cell Hamiltonian build
+  if (d->shape_values && d->stiffness) {
+    CHFSI_TRY(dev_upload(&op->d_Nq, d->shape_values, size_t(op->nq) * ne));
+    CHFSI_TRY(dev_upload(&op->d_K, d->stiffness, size_t(ne) * ne));
+  }
+  CHFSI_TRY(dev_alloc(&op->d_Ht64, size_t(C) * ne_pad * ne_pad));
+  CHFSI_TRY(dev_alloc(&op->d_Ht32, size_t(C) * ne_pad * ne_pad));
+
+  // ---- nonlocal slots: per cell the (pair, alpha) columns, padded to 16
+  op->n_proj = d->n_proj;
+  if (d->n_pairs > 0) {
+    const int P = d->proj_per_pair;
+    CHFSI_ARG(P > 0 && d->pair_cell && d->pair_proj0 && d->pair_C && d->proj_V,
+              "missing nonlocal arrays");
+    std::vector<std::vector<int>> pairs_of(C);
+    for (int p = 0; p < d->n_pairs; ++p) {
+      CHFSI_ARG(d->pair_cell[p] >= 0 && d->pair_cell[p] < C, "pair_cell out of range");
+      CHFSI_ARG(d->pair_proj0[p] >= 0 && d->pair_proj0[p] + P <= d->n_proj,
+                "pair projector out of range");
+      pairs_of[d->pair_cell[p]].push_back(p);
+    }
+    std::vector<int> nl_off(C + 1, 0);
+    for (int c = 0; c < C; ++c) {
+      const int cnt = (int)pairs_of[c].size() * P;
+      const int pad = (cnt + 15) / 16 * 16;
+      CHFSI_ARG(pad <= 128, "cell %d overlaps %d projectors; at most 128 supported", c, cnt);
+      nl_off[c + 1] = nl_off[c] + pad;
+    }
+    const int S = nl_off[C];
+    op->n_slots = S;
+    std::vector<int> slot_proj(S, -1);
+    std::vector<double> nlC(size_t(S) * ne_pad, 0.0), nlCt(size_t(S) * ne_pad, 0.0);
+    std::vector<std::vector<int>> slots_of(d->n_proj);
+    std::vector<int> nl_cells;
+    for (int c = 0; c < C; ++c) {
+      const int s0 = nl_off[c], ns = nl_off[c + 1] - s0;
+      if (ns == 0) continue;
+      nl_cells.push_back(c);
+      int local = 0;
+      for (int p : pairs_of[c]) {
+        for (int al = 0; al < P; ++al, ++local) {
+          const int s = s0 + local;
+          slot_proj[s] = d->pair_proj0[p] + al;
+          slots_of[slot_proj[s]].push_back(s);
+          for (int i = 0; i < ne; ++i) {
+            const double v = d->pair_C[(size_t(p) * ne + i) * P + al];
+            // main-kernel layout: block s0*ne_pad, [kts][row i][16 slots]
+            nlC[size_t(s0) * ne_pad + size_t(local / 16) * ne_pad * 16 + size_t(i) * 16 +
+                (local % 16)] = v;
+            // projection layout: [kt over nodes][slot][16 nodes]
+            nlCt[size_t(s0) * ne_pad + size_t(i / 16) * ns * 16 + size_t(local) * 16 + (i % 16)] =
+                v;
+          }
+        }
+      }
+    }
+    std::vector<int> proj_ptr(d->n_proj + 1, 0), proj_slots;
+    for (int a = 0; a < d->n_proj; ++a) {
+      proj_slots.insert(proj_slots.end(), slots_of[a].begin(), slots_of[a].end());
+      proj_ptr[a + 1] = (int)proj_slots.size();
+    }
+    op->n_nl_cells = (int)nl_cells.size();
+    std::vector<float> tmp32(nlC.size());
+    CHFSI_TRY(dev_upload(&op->d_nl_off, nl_off.data(), nl_off.size()));
+    CHFSI_TRY(dev_upload(&op->d_slot_proj, slot_proj.data(), slot_proj.size()));
+    CHFSI_TRY(dev_upload(&op->d_nlC64, nlC.data(), nlC.size()));
+    for (size_t i = 0; i < nlC.size(); ++i) tmp32[i] = (float)nlC[i];
+    CHFSI_TRY(dev_upload(&op->d_nlC32, tmp32.data(), tmp32.size()));
+    CHFSI_TRY(dev_upload(&op->d_nlCt64, nlCt.data(), nlCt.size()));
+    for (size_t i = 0; i < nlCt.size(); ++i) tmp32[i] = (float)nlCt[i];
+    CHFSI_TRY(dev_upload(&op->d_nlCt32, tmp32.data(), tmp32.size()));
+    CHFSI_TRY(dev_upload(&op->d_proj_ptr, proj_ptr.data(), proj_ptr.size()));
+    CHFSI_TRY(dev_upload(&op->d_proj_slots, proj_slots.data(), proj_slots.size()));
+    CHFSI_TRY(dev_upload(&op->d_proj_V, d->proj_V, d->n_proj));
+    CHFSI_TRY(dev_upload(&op->d_nl_cells, nl_cells.data(), nl_cells.size()));
+  }
+  CHFSI_HIP(hipDeviceSynchronize());
+  *out = guard.release();
+  return CHFSI_OK;
+}
+
+extern "C" void chfsi_operator_destroy(chfsi_operator *op) { delete op; }
+
+extern "C" chfsi_status chfsi_operator_set_potential(chfsi_operator *op, const double *vJxW,
+                                                     void *stream) {
+  CHFSI_ARG(op && vJxW, "null argument");
+  CHFSI_ARG(op->d_Nq && op->d_K, "operator was created without shape data");
+  CHFSI_HIP(launch_build_h(op->d_K, op->d_Nq, vJxW, op->C, op->ne, op->ne_pad, op->nq,
+                           op->d_Ht64, op->d_Ht32, as_stream(stream)));
+  op->has_H = true;
+  return CHFSI_OK;
+}
+
+extern "C" chfsi_status chfsi_operator_set_cell_matrices(chfsi_operator *op, const double *H,
+                                                         void *stream) {
+  CHFSI_ARG(op && H, "null argument");
+  CHFSI_HIP(launch_tile_h(H, op->C, op->ne, op->ne_pad, op->d_Ht64, op->d_Ht32,
+                          as_stream(stream)));
+  op->has_H = true;
+  return CHFSI_OK;
+}
+
+extern "C" chfsi_status chfsi_operator_get_cell_matrices(chfsi_operator *op, double *H,
+                                                         void *stream) {
+  CHFSI_ARG(op && H, "null argument");
+  if (!op->has_H) {
+    set_error("cell matrices not set");
+    return CHFSI_ERR_STATE;
+  }
+  CHFSI_HIP(launch_untile_h(op->d_Ht64, op->C, op->ne, op->ne_pad, H, as_stream(stream)));
+  return CHFSI_OK;
+}
+
+extern "C" const double *chfsi_operator_sqrt_mass(const chfsi_operator *op) {
+  return op ? op->d_sqrt_mass : nullptr;
+}
+extern "C" const double *chfsi_operator_inv_sqrt_mass(const chfsi_operator *op) {
+  return op ? op->d_inv_sqrt_mass : nullptr;
+}
+extern "C" int64_t chfsi_operator_n_dofs(const chfsi_operator *op) { return op ? op->M : 0; }
+
+namespace chfsi {
+
+template <typename T>
+struct OpData;
+template <>
+struct OpData<double> {
+  static const double *Ht(chfsi_operator *op) { return op->d_Ht64; }
+  static const double *nlC(chfsi_operator *op) { return op->d_nlC64; }
+  static const double *nlCt(chfsi_operator *op) { return op->d_nlCt64; }
+  static const double *s(chfsi_operator *op) { return op->d_inv_sqrt_mass; }
+  static Scratch &partial(chfsi_operator *op) { return op->partial64; }
+  static Scratch &W(chfsi_operator *op) { return op->W64; }
+};
+template <>
+struct OpData<float> {
+  static const float *Ht(chfsi_operator *op) { return op->d_Ht32; }
+  static const float *nlC(chfsi_operator *op) { return op->d_nlC32; }
+  static const float *nlCt(chfsi_operator *op) { return op->d_nlCt32; }
+  static const float *s(chfsi_operator *op) { return op->d_inv_sqrt_mass32; }
+  static Scratch &partial(chfsi_operator *op) { return op->partial32; }
+  static Scratch &W(chfsi_operator *op) { return op->W32; }
+};
+
+template <typename T>
+chfsi_status hx_fused(chfsi_operator *op, const T *src, int64_t ld_src, T *dst, int64_t ld_dst,
+                      int B, T alpha, T beta, T gamma, int beta_mode, bool scaled,
+                      hipStream_t s) {
+  if (!op->has_H) {
+    set_error("cell matrices not set (call chfsi_operator_set_potential)");
+    return CHFSI_ERR_STATE;
+  }
+  CHFSI_ARG(B > 0 && ld_src >= B && ld_dst >= B, "bad block width / leading dimension");
+  CHFSI_ARG(src != dst, "HX needs distinct source and destination");
+  const T *sc = scaled ? OpData<T>::s(op) : nullptr;
+  T *W = nullptr;
+  if (op->n_slots > 0) {
+    Scratch &P = OpData<T>::partial(op);
+    Scratch &Wb = OpData<T>::W(op);
+    CHFSI_TRY(P.ensure(sizeof(T) * size_t(op->n_slots) * B));
+    CHFSI_TRY(Wb.ensure(sizeof(T) * size_t(op->n_proj) * B));
+    ProjLaunch<T> pl{};
+    pl.cells = op->d_nl_cells;
+    pl.ncells = op->n_nl_cells;
+    pl.map = op->d_map;
+    pl.ne = op->ne;
+    pl.ne_pad = op->ne_pad;
+    pl.nl_off = op->d_nl_off;
+    pl.nl_Ct = OpData<T>::nlCt(op);
+    pl.src = src;
+    pl.ld_src = ld_src;
+    pl.s_in = sc;
+    pl.partial = static_cast<T *>(P.ptr);
+    pl.ldP = B;
+    pl.B = B;
+    CHFSI_HIP(launch_proj<T>(pl, s));
+    W = static_cast<T *>(Wb.ptr);
+    CHFSI_HIP(launch_proj_reduce<T>(op->d_proj_ptr, op->d_proj_slots, op->d_proj_V, op->n_proj,
+                                    static_cast<T *>(P.ptr), B, W, B, B, s));
+  }
+  HxLaunch<T> a{};
+  a.map = op->d_map;
+  a.ne = op->ne;
+  a.ne_pad = op->ne_pad;
+  a.Ht = OpData<T>::Ht(op);
+  a.src = src;
+  a.ld_src = ld_src;
+  a.dst = dst;
+  a.ld_dst = ld_dst;
+  a.s_in = sc;
+  a.s_out = sc;
+  a.alpha = alpha;
+  a.beta = beta;
+  a.gamma = gamma;
+  a.beta_mode = beta_mode;
+  a.B = B;
+  if (op->n_slots > 0) {
+    a.nl_off = op->d_nl_off;
+    a.nl_C = OpData<T>::nlC(op);
+    a.slot_proj = op->d_slot_proj;
+    a.W = W;
+    a.ldW = B;
+  }
+  for (int k = 0; k < 8; ++k) {
+    a.cells = op->d_cells_by_color + op->color_off[k];
+    a.ncells = op->color_off[k + 1] - op->color_off[k];
+    CHFSI_HIP(launch_hx<T>(a, s));
+  }
+  return CHFSI_OK;
+}
+
+template chfsi_status hx_fused<double>(chfsi_operator *, const double *, int64_t, double *,
+                                       int64_t, int, double, double, double, int, bool,
+                                       hipStream_t);
+template chfsi_status hx_fused<float>(chfsi_operator *, const float *, int64_t, float *, int64_t,
+                                      int, float, float, float, int, bool, hipStream_t);
+
+}  // namespace chfsi
+
+extern "C" chfsi_status chfsi_hx(chfsi_operator *op, const double *src, int64_t ld_src,
+                                 double *dst, int64_t ld_dst, int32_t B, int32_t scale_flag,
+                                 double scalar, void *stream) {
+  CHFSI_ARG(op && src && dst, "null argument");
+  // dst <- (scale_flag ? dst : S dst) + scalar * S H S src
+  CHFSI_TRY(hx_fused<double>(op, src, ld_src, dst, ld_dst, B, scalar, 1.0, 0.0,
+                             scale_flag ? 0 : 2, true, as_stream(stream)));
+  if (op->nc > 0) CHFSI_HIP(launch_zero_rows(op->d_c_rows, op->nc, dst, ld_dst, B, as_stream(stream)));
+  return CHFSI_OK;
+}
+
+extern "C" chfsi_status chfsi_hx_fp32(chfsi_operator *op, const float *src, int64_t ld_src,
+                                      float *dst, int64_t ld_dst, int32_t B, int32_t scale_flag,
+                                      float scalar, void *stream) {
+  CHFSI_ARG(op && src && dst, "null argument");
+  CHFSI_ARG(op->nc == 0, "FP32 HX supports unconstrained (periodic-folded) layouts only");
+  return hx_fused<float>(op, src, ld_src, dst, ld_dst, B, scalar, 1.0f, 0.0f,
+                         scale_flag ? 0 : 2, true, as_stream(stream));
+}
+
+extern "C" chfsi_status chfsi_hx_cheby(chfsi_operator *op, const double *src, int64_t ld_src,
+                                       double *dst, int64_t ld_dst, int32_t B, void *stream) {
+  CHFSI_ARG(op && src && dst, "null argument");
+  return hx_fused<double>(op, src, ld_src, dst, ld_dst, B, 1.0, 1.0, 0.0, 0, false,
+                          as_stream(stream));
+}
+
+extern "C" chfsi_status chfsi_constraints_distribute(chfsi_operator *op, double *X, int64_t ld,
+                                                     int32_t B, void *stream) {
+  CHFSI_ARG(op && X && ld >= B, "bad argument");
+  CHFSI_HIP(launch_distribute(op->d_c_rows, op->d_c_ptr, op->d_c_cols, op->d_c_vals, op->nc, X,
+                              ld, B, as_stream(stream)));
+  return CHFSI_OK;
+}
+
+extern "C" chfsi_status chfsi_constraints_slave_to_master(chfsi_operator *op, double *X,
+                                                          int64_t ld, int32_t B, void *stream) {
+  CHFSI_ARG(op && X && ld >= B, "bad argument");
+  CHFSI_HIP(launch_slave_to_master(op->d_m_rows, op->d_m_ptr, op->d_m_slaves, op->d_m_vals,
+                                   op->nm, X, ld, B, as_stream(stream)));
+  CHFSI_HIP(launch_zero_rows(op->d_c_rows, op->nc, X, ld, B, as_stream(stream)));
+  return CHFSI_OK;
+}
+
+extern "C" chfsi_status chfsi_constraints_set_zero(chfsi_operator *op, double *X, int64_t ld,
+                                                   int32_t B, void *stream) {
+  CHFSI_ARG(op && X && ld >= B, "bad argument");
+  CHFSI_HIP(launch_zero_rows(op->d_c_rows, op->nc, X, ld, B, as_stream(stream)));
+  return CHFSI_OK;
+}

@@ -1,0 +1,58 @@
+// Device-resident operator state shared by the operator and solver sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <memory>
+
+#include "common.hpp"
+
+namespace chfsi {
+
+// Grow-only device buffer.
+struct Scratch {
+  void *ptr = nullptr;
+  size_t size = 0;
+  chfsi_status ensure(size_t bytes);
+  void release();
+};
+
+template <typename T>
+chfsi_status hx_fused(chfsi_operator *op, const T *src, int64_t ld_src, T *dst, int64_t ld_dst,
+                      int B, T alpha, T beta, T gamma, int beta_mode, bool scaled, hipStream_t s);
+
+}  // namespace chfsi
+
+struct chfsi_operator {
+  int device = 0;
+  int64_t M = 0;
+  int C = 0, ne = 0, ne_pad = 0, nq = 0;
+  // cell -> DOF map with first-touch flag (bit 31), cells sorted by colour
+  uint32_t *d_map = nullptr;
+  int *d_cells_by_color = nullptr;
+  int color_off[9] = {0};
+  // Loewdin scalings
+  double *d_sqrt_mass = nullptr, *d_inv_sqrt_mass = nullptr;
+  float *d_sqrt_mass32 = nullptr, *d_inv_sqrt_mass32 = nullptr;
+  // shape data and cell Hamiltonians (padded K-tiled layout)
+  double *d_K = nullptr, *d_Nq = nullptr;
+  double *d_Ht64 = nullptr;
+  float *d_Ht32 = nullptr;
+  bool has_H = false;
+  // nonlocal projector slots
+  int n_proj = 0, n_slots = 0, n_nl_cells = 0;
+  int *d_nl_off = nullptr, *d_slot_proj = nullptr, *d_proj_ptr = nullptr,
+      *d_proj_slots = nullptr, *d_nl_cells = nullptr;
+  double *d_nlC64 = nullptr, *d_nlCt64 = nullptr, *d_proj_V = nullptr;
+  float *d_nlC32 = nullptr, *d_nlCt32 = nullptr;
+  chfsi::Scratch partial64, partial32, W64, W32, tmp;
+  // constraints (slave-major and master-major CSR)
+  int64_t nc = 0, nm = 0;
+  int64_t *d_c_rows = nullptr, *d_c_ptr = nullptr, *d_c_cols = nullptr;
+  double *d_c_vals = nullptr;
+  int64_t *d_m_rows = nullptr, *d_m_ptr = nullptr, *d_m_slaves = nullptr;
+  double *d_m_vals = nullptr;
+
+  ~chfsi_operator();
+};

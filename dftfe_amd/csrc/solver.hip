@@ -1,0 +1,523 @@
+// Solver side of the C ABI: Chebyshev filter, Lanczos bounds, CholGS,
+// Rayleigh-Ritz, residual norms and the one-pass driver.
+//
+// Reference: chebyshevOrthogonalizedSubspaceIterationSolverDevice
+// (src/solvers/eigenSolvers/chebyshevOrthogonalizedSubspaceIterationSolverDevice.cc:155-736),
+// linearAlgebraOperationsDevice (src/linAlg/linearAlgebraOperationsDevice.cc:340-727,
+// :1832-2242, :3078-3240, :4610-4766), pseudoGSDevice.cc:81-463,
+// rayleighRitzDevice.cc:80-353.  The dense N x N Cholesky and eigensolve run on
+// the GPU (rocSOLVER) instead of ELPA/ScaLAPACK on the host.
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+#include "operator.hpp"
+
+using namespace chfsi;
+
+struct chfsi_solver {
+  chfsi_params params;
+  double a0 = 0, bLow = 0, bUp = 0;
+  rocblas_handle blas = nullptr;
+  Scratch Y64, X32, Y32, HXb, Hp, rot, evals, info, part, lam, v0;
+  bool have_v0 = false;
+  ~chfsi_solver() {
+    if (blas) rocblas_destroy_handle(blas);
+    for (Scratch *s : {&Y64, &X32, &Y32, &HXb, &Hp, &rot, &evals, &info, &part, &lam, &v0})
+      s->release();
+  }
+};
+
+namespace {
+
+#define CHFSI_RB(...)                                                                    \
+  do {                                                                                   \
+    rocblas_status rs_ = (__VA_ARGS__);                                                       \
+    if (rs_ != rocblas_status_success) {                                                 \
+      set_error("%s:%d: %s: %s", __FILE__, __LINE__, #__VA_ARGS__, rocblas_status_to_string(rs_)); \
+      return CHFSI_ERR_BLAS;                                                             \
+    }                                                                                    \
+  } while (0)
+
+// Degree lookup on the upper bound (…SolverDevice.cc:29-46, :96-105).
+unsigned cheby_order_lookup(double upper) {
+  static const unsigned table[][2] = {{500, 24},    {750, 30},    {1000, 39},   {1500, 50},
+                                      {2000, 53},   {3000, 57},   {4000, 62},   {5000, 69},
+                                      {9000, 77},   {14000, 104}, {20000, 119}, {30000, 162},
+                                      {50000, 300}, {80000, 450}, {100000, 550}, {200000, 700},
+                                      {500000, 1000}};
+  const unsigned u = (unsigned)upper;
+  for (auto &row : table)
+    if (u <= row[0]) return row[1];
+  return 1250;
+}
+
+chfsi_status ensure_blas(chfsi_solver *s, hipStream_t st) {
+  if (!s->blas) CHFSI_RB(rocblas_create_handle(&s->blas));
+  CHFSI_RB(rocblas_set_stream(s->blas, st));
+  return CHFSI_OK;
+}
+
+// Sum of `n` doubles of a device array (synchronises the stream).
+chfsi_status host_sum(const double *d, int n, double *out, hipStream_t st) {
+  std::vector<double> h(n);
+  CHFSI_HIP(hipMemcpyAsync(h.data(), d, n * sizeof(double), hipMemcpyDeviceToHost, st));
+  CHFSI_HIP(hipStreamSynchronize(st));
+  double s = 0;
+  for (double x : h) s += x;
+  *out = s;
+  return CHFSI_OK;
+}
+
+chfsi_status dot(chfsi_solver *s, const double *x, const double *y, int64_t n, double *out,
+                 hipStream_t st) {
+  const int nch = 256;
+  CHFSI_TRY(s->part.ensure(nch * sizeof(double)));
+  CHFSI_HIP(launch_dot_partial(x, y, n, static_cast<double *>(s->part.ptr), nch, st));
+  return host_sum(static_cast<double *>(s->part.ptr), nch, out, st);
+}
+
+// Eigenvalues of a small symmetric matrix (cyclic Jacobi), ascending.
+std::vector<double> small_sym_eigvals(std::vector<double> A, int n) {
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0;
+    for (int i = 0; i < n; ++i)
+      for (int j = i + 1; j < n; ++j) off += A[i * n + j] * A[i * n + j];
+    if (off < 1e-30) break;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = A[p * n + q];
+        if (std::fabs(apq) < 1e-300) continue;
+        const double theta = (A[q * n + q] - A[p * n + p]) / (2 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1));
+        const double c = 1 / std::sqrt(t * t + 1), sn = t * c;
+        for (int k = 0; k < n; ++k) {
+          const double akp = A[k * n + p], akq = A[k * n + q];
+          A[k * n + p] = c * akp - sn * akq;
+          A[k * n + q] = sn * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double apk = A[p * n + k], aqk = A[q * n + k];
+          A[p * n + k] = c * apk - sn * aqk;
+          A[q * n + k] = sn * apk + c * aqk;
+        }
+      }
+  }
+  std::vector<double> ev(n);
+  for (int i = 0; i < n; ++i) ev[i] = A[i * n + i];
+  std::sort(ev.begin(), ev.end());
+  return ev;
+}
+
+template <typename T>
+T *sptr(Scratch &s) {
+  return static_cast<T *>(s.ptr);
+}
+
+// Scaled Chebyshev recurrence on [a, b] with a0 scaling, fused into the HX
+// launches:  X_new = alpha1 (H~ Y - c Y) + alpha2 X  is one hx_fused call
+// (alpha = alpha1, gamma = -c alpha1, beta = alpha2).
+// Coefficients: linearAlgebraOperationsDevice.cc:547-726 / Opt.cc:286-351.
+template <typename T>
+chfsi_status filter_impl(chfsi_operator *op, T *X, int64_t ldX, T *Y, int64_t ldY, int B, int m,
+                         double a, double b, double a0, hipStream_t st, T **result,
+                         int64_t *ld_result) {
+  const double e = (b - a) / 2.0, c = (b + a) / 2.0;
+  double sigma = e / (a0 - c);
+  const double sigma1 = sigma, gamma = 2.0 / sigma1;
+  // degree 1: Y = (sigma1/e) (H~ X - c X)
+  CHFSI_TRY(hx_fused<T>(op, X, ldX, Y, ldY, B, T(sigma1 / e), T(0), T(-c * sigma1 / e), 1, true,
+                        st));
+  T *cur = Y, *prev = X;
+  int64_t ldc = ldY, ldp = ldX;
+  for (int degree = 2; degree <= m; ++degree) {
+    const double sigma2 = 1.0 / (gamma - sigma);
+    const double alpha1 = 2.0 * sigma2 / e, alpha2 = -(sigma * sigma2);
+    CHFSI_TRY(hx_fused<T>(op, cur, ldc, prev, ldp, B, T(alpha1), T(alpha2), T(-c * alpha1), 0,
+                          true, st));
+    std::swap(cur, prev);
+    std::swap(ldc, ldp);
+    sigma = sigma2;
+  }
+  *result = cur;
+  *ld_result = ldc;
+  return CHFSI_OK;
+}
+
+chfsi_status filter_block(chfsi_solver *s, chfsi_operator *op, double *X, int64_t ld, int B, int m,
+                          double a, double b, double a0, int precision, hipStream_t st) {
+  const int64_t M = op->M;
+  if (precision == CHFSI_FP64) {
+    CHFSI_TRY(s->Y64.ensure(sizeof(double) * M * B));
+    double *Y = sptr<double>(s->Y64);
+    CHFSI_HIP(hipMemsetAsync(Y, 0, sizeof(double) * M * B, st));
+    double *res;
+    int64_t ldr;
+    CHFSI_TRY(filter_impl<double>(op, X, ld, Y, B, B, m, a, b, a0, st, &res, &ldr));
+    if (res != X) CHFSI_HIP(launch_copy_block<double, double>(res, ldr, X, ld, M, B, st));
+    return CHFSI_OK;
+  }
+  if (precision == CHFSI_FP32) {
+    CHFSI_TRY(s->X32.ensure(sizeof(float) * M * B));
+    CHFSI_TRY(s->Y32.ensure(sizeof(float) * M * B));
+    float *X32 = sptr<float>(s->X32), *Y32 = sptr<float>(s->Y32);
+    CHFSI_HIP(launch_copy_block<double, float>(X, ld, X32, B, M, B, st));
+    CHFSI_HIP(hipMemsetAsync(Y32, 0, sizeof(float) * M * B, st));
+    float *res;
+    int64_t ldr;
+    CHFSI_TRY(filter_impl<float>(op, X32, B, Y32, B, B, m, a, b, a0, st, &res, &ldr));
+    CHFSI_HIP(launch_copy_block<float, double>(res, ldr, X, ld, M, B, st));
+    return CHFSI_OK;
+  }
+  set_error("unsupported filter precision %d", precision);
+  return CHFSI_ERR_ARG;
+}
+
+chfsi_status lanczos_impl(chfsi_solver *s, chfsi_operator *op, const double *v0, int steps,
+                          int reproducible, double *lower, double *upper, hipStream_t st) {
+  const int64_t M = op->M;
+  CHFSI_ARG(steps >= 1, "lanczos steps must be >= 1");
+  CHFSI_TRY(s->HXb.ensure(sizeof(double) * M * 3));
+  double *v = sptr<double>(s->HXb), *f = v + M, *vold = v + 2 * M;
+  CHFSI_HIP(hipMemsetAsync(v, 0, sizeof(double) * M * 3, st));
+  CHFSI_HIP(hipMemcpyAsync(v, v0, sizeof(double) * M, hipMemcpyDeviceToDevice, st));
+  if (op->nc) CHFSI_HIP(launch_zero_rows(op->d_c_rows, op->nc, v, 1, 1, st));
+  double nrm2;
+  CHFSI_TRY(dot(s, v, v, M, &nrm2, st));
+  CHFSI_ARG(nrm2 > 0, "lanczos start vector is zero");
+  CHFSI_HIP(launch_axpby<double>(M, 0.0, v, 1.0 / std::sqrt(nrm2), v, st));
+  std::vector<double> T(size_t(steps) * steps, 0.0);
+  auto apply = [&](const double *x, double *y) -> chfsi_status {
+    CHFSI_TRY(hx_fused<double>(op, x, 1, y, 1, 1, 1.0, 0.0, 0.0, 1, true, st));
+    if (op->nc) CHFSI_HIP(launch_zero_rows(op->d_c_rows, op->nc, y, 1, 1, st));
+    return CHFSI_OK;
+  };
+  CHFSI_TRY(apply(v, f));
+  double alpha;
+  CHFSI_TRY(dot(s, f, v, M, &alpha, st));
+  CHFSI_HIP(launch_axpby<double>(M, -alpha, v, 1.0, f, st));
+  T[0] = alpha;
+  for (int j = 1; j < steps; ++j) {
+    double fn2;
+    CHFSI_TRY(dot(s, f, f, M, &fn2, st));
+    const double beta = std::sqrt(fn2);
+    std::swap(v, vold);                                        // vold <- v
+    CHFSI_HIP(launch_axpby<double>(M, 1.0 / beta, f, 0.0, v, st));  // v <- f / beta
+    CHFSI_TRY(apply(v, f));
+    CHFSI_HIP(launch_axpby<double>(M, -beta, vold, 1.0, f, st));
+    CHFSI_TRY(dot(s, f, v, M, &alpha, st));
+    CHFSI_HIP(launch_axpby<double>(M, -alpha, v, 1.0, f, st));
+    T[size_t(j) * steps + j - 1] = beta;
+    T[size_t(j - 1) * steps + j] = beta;
+    T[size_t(j) * steps + j] = alpha;
+  }
+  std::vector<double> theta = small_sym_eigvals(T, steps);
+  double fn2;
+  CHFSI_TRY(dot(s, f, f, M, &fn2, st));
+  const double fnorm = std::sqrt(fn2);
+  *lower = std::floor(theta.front());
+  *upper = std::ceil(theta.back() + (reproducible ? fnorm : fnorm / 10.0));
+  return CHFSI_OK;
+}
+
+// S = X^T X (lower), S = L L^T, X <- X L^{-T}.  Column-major view: Xc = X^T is
+// N x M with leading dimension N, so S = Xc Xc^T and Xc <- L^{-1} Xc.
+chfsi_status cholgs_impl(chfsi_solver *s, double *X, int64_t M, int N, hipStream_t st) {
+  CHFSI_TRY(ensure_blas(s, st));
+  CHFSI_TRY(s->Hp.ensure(sizeof(double) * N * N));
+  CHFSI_TRY(s->info.ensure(sizeof(rocblas_int)));
+  double *S = sptr<double>(s->Hp);
+  const double one = 1.0, zero = 0.0;
+  CHFSI_RB(rocblas_dsyrk(s->blas, rocblas_fill_lower, rocblas_operation_none, N, (rocblas_int)M,
+                         &one, X, N, &zero, S, N));
+  rocblas_int *dinfo = sptr<rocblas_int>(s->info);
+  CHFSI_RB(rocsolver_dpotrf(s->blas, rocblas_fill_lower, N, S, N, dinfo));
+  rocblas_int hinfo = 0;
+  CHFSI_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(rocblas_int), hipMemcpyDeviceToHost, st));
+  CHFSI_HIP(hipStreamSynchronize(st));
+  if (hinfo != 0) {
+    set_error("overlap matrix is not positive definite (potrf info %d)", (int)hinfo);
+    return CHFSI_ERR_NOT_SPD;
+  }
+  CHFSI_RB(rocblas_dtrsm(s->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+                         rocblas_diagonal_non_unit, N, (rocblas_int)M, &one, S, N, X, N));
+  return CHFSI_OK;
+}
+
+// X <- X Q, i.e. Xc <- Q^T Xc, blocked over DOFs through a scratch panel
+// (subspaceRotationScalapack, linearAlgebraOperationsDevice.cc:1832-2242).
+chfsi_status rotate(chfsi_solver *s, double *X, int64_t M, int N, const double *Q,
+                    hipStream_t st) {
+  const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(M, (int64_t(1) << 27) / N));
+  CHFSI_TRY(s->rot.ensure(sizeof(double) * chunk * N));
+  double *tmp = sptr<double>(s->rot);
+  const double one = 1.0, zero = 0.0;
+  for (int64_t n0 = 0; n0 < M; n0 += chunk) {
+    const int64_t nc = std::min(chunk, M - n0);
+    CHFSI_RB(rocblas_dgemm(s->blas, rocblas_operation_transpose, rocblas_operation_none, N,
+                           (rocblas_int)nc, N, &one, Q, N, X + n0 * N, N, &zero, tmp, N));
+    CHFSI_HIP(hipMemcpyAsync(X + n0 * N, tmp, sizeof(double) * nc * N, hipMemcpyDeviceToDevice,
+                             st));
+  }
+  return CHFSI_OK;
+}
+
+// Hp = X^T H~ X (lower triangle by column blocks), eig, X <- X Q.
+chfsi_status rr_impl(chfsi_solver *s, chfsi_operator *op, double *X, int64_t M, int N, int B,
+                     double *eig_host, hipStream_t st) {
+  CHFSI_TRY(ensure_blas(s, st));
+  CHFSI_TRY(s->Hp.ensure(sizeof(double) * N * N));
+  CHFSI_TRY(s->HXb.ensure(sizeof(double) * M * B));
+  CHFSI_TRY(s->evals.ensure(sizeof(double) * 2 * N));
+  CHFSI_TRY(s->info.ensure(sizeof(rocblas_int)));
+  double *Hp = sptr<double>(s->Hp), *HX = sptr<double>(s->HXb);
+  const double one = 1.0, zero = 0.0;
+  for (int j = 0; j < N; j += B) {
+    const int bw = std::min(B, N - j);
+    CHFSI_TRY(hx_fused<double>(op, X + j, N, HX, bw, bw, 1.0, 0.0, 0.0, 1, true, st));
+    if (op->nc) CHFSI_HIP(launch_zero_rows(op->d_c_rows, op->nc, HX, bw, bw, st));
+    // Hp[j:N, j:j+bw] = Xc[j:N, :] * HXc^T
+    CHFSI_RB(rocblas_dgemm(s->blas, rocblas_operation_none, rocblas_operation_transpose, N - j, bw,
+                           (rocblas_int)M, &one, X + j, N, HX, bw, &zero, Hp + j + (int64_t)j * N,
+                           N));
+  }
+  double *D = sptr<double>(s->evals), *E = D + N;
+  rocblas_int *dinfo = sptr<rocblas_int>(s->info);
+  CHFSI_RB(rocsolver_dsyevd(s->blas, rocblas_evect_original, rocblas_fill_lower, N, Hp, N, D, E,
+                            dinfo));
+  rocblas_int hinfo = 0;
+  CHFSI_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(rocblas_int), hipMemcpyDeviceToHost, st));
+  CHFSI_HIP(hipMemcpyAsync(eig_host, D, sizeof(double) * N, hipMemcpyDeviceToHost, st));
+  CHFSI_HIP(hipStreamSynchronize(st));
+  if (hinfo != 0) {
+    set_error("projected eigensolve failed (syevd info %d)", (int)hinfo);
+    return CHFSI_ERR_EIG;
+  }
+  return rotate(s, X, M, N, Hp, st);
+}
+
+chfsi_status residual_impl(chfsi_solver *s, chfsi_operator *op, const double *X, int64_t M, int N,
+                           int B, const double *eig_host, double *res_host, hipStream_t st) {
+  CHFSI_TRY(s->HXb.ensure(sizeof(double) * M * B));
+  CHFSI_TRY(s->lam.ensure(sizeof(double) * N));
+  const int nch = 512;
+  CHFSI_TRY(s->part.ensure(sizeof(double) * nch * B));
+  double *HX = sptr<double>(s->HXb), *lam = sptr<double>(s->lam), *part = sptr<double>(s->part);
+  CHFSI_HIP(hipMemcpyAsync(lam, eig_host, sizeof(double) * N, hipMemcpyHostToDevice, st));
+  std::vector<double> h(size_t(nch) * B);
+  for (int j = 0; j < N; j += B) {
+    const int bw = std::min(B, N - j);
+    CHFSI_TRY(hx_fused<double>(op, X + j, N, HX, bw, bw, 1.0, 0.0, 0.0, 1, true, st));
+    if (op->nc) CHFSI_HIP(launch_zero_rows(op->d_c_rows, op->nc, HX, bw, bw, st));
+    CHFSI_HIP(launch_residual_partial(HX, bw, X + j, N, lam + j, M, bw, part, nch, st));
+    CHFSI_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * nch * bw, hipMemcpyDeviceToHost, st));
+    CHFSI_HIP(hipStreamSynchronize(st));
+    for (int v = 0; v < bw; ++v) {
+      double acc = 0;
+      for (int k = 0; k < nch; ++k) acc += h[size_t(k) * bw + v];
+      res_host[j + v] = std::sqrt(acc);
+    }
+  }
+  return CHFSI_OK;
+}
+
+}  // namespace
+
+extern "C" void chfsi_default_params(chfsi_params *p) {
+  if (!p) return;
+  p->cheby_block_size = 400;            // utils/dftParameters.cc:1232
+  p->cheby_degree = 0;
+  p->lanczos_steps = 20;
+  p->reproducible = 0;
+  p->reuse_lanczos_upper = 0;
+  p->first_scf_degree_factor = 1.34;    // utils/dftParameters.cc:872-876
+  p->filter_precision = CHFSI_FP32;
+  p->use_shep = 1;                      // utils/dftParameters.cc:934-938
+}
+
+extern "C" chfsi_status chfsi_solver_create(const chfsi_params *params, double a0, double bLow,
+                                            double bUp, chfsi_solver **out) {
+  CHFSI_ARG(out, "null argument");
+  auto s = new chfsi_solver();
+  if (params)
+    s->params = *params;
+  else
+    chfsi_default_params(&s->params);
+  s->a0 = a0;
+  s->bLow = bLow;
+  s->bUp = bUp;
+  *out = s;
+  return CHFSI_OK;
+}
+
+extern "C" void chfsi_solver_destroy(chfsi_solver *s) { delete s; }
+
+extern "C" chfsi_status chfsi_solver_reinit_bounds(chfsi_solver *s, double a0, double bLow,
+                                                   double bUp) {
+  CHFSI_ARG(s, "null solver");
+  s->a0 = a0;
+  s->bLow = bLow;
+  s->bUp = bUp;
+  return CHFSI_OK;
+}
+
+extern "C" chfsi_status chfsi_solver_get_bounds(const chfsi_solver *s, double *a0, double *bLow,
+                                                double *bUp) {
+  CHFSI_ARG(s && a0 && bLow && bUp, "null argument");
+  *a0 = s->a0;
+  *bLow = s->bLow;
+  *bUp = s->bUp;
+  return CHFSI_OK;
+}
+
+extern "C" chfsi_status chfsi_solver_set_lanczos_start(chfsi_solver *s, const double *v0_dev,
+                                                       int64_t M) {
+  CHFSI_ARG(s, "null solver");
+  s->have_v0 = false;
+  s->v0.release();
+  if (!v0_dev) return CHFSI_OK;
+  CHFSI_ARG(M > 0, "bad length");
+  CHFSI_TRY(s->v0.ensure(sizeof(double) * M));
+  CHFSI_HIP(hipMemcpy(s->v0.ptr, v0_dev, sizeof(double) * M, hipMemcpyDeviceToDevice));
+  s->have_v0 = true;
+  return CHFSI_OK;
+}
+
+extern "C" chfsi_status chfsi_chebyshev_filter(chfsi_operator *op, double *X, int64_t ld,
+                                               int32_t B, int32_t m, double a, double b,
+                                               double a0, int32_t precision, void *stream) {
+  CHFSI_ARG(op && X && B > 0 && ld >= B && m >= 1, "bad argument");
+  chfsi_solver tmp;
+  return filter_block(&tmp, op, X, ld, B, m, a, b, a0, precision, as_stream(stream));
+}
+
+extern "C" chfsi_status chfsi_lanczos_bounds(chfsi_operator *op, const double *v0, int32_t steps,
+                                             int32_t reproducible, double *lower, double *upper,
+                                             void *stream) {
+  CHFSI_ARG(op && v0 && lower && upper, "null argument");
+  chfsi_solver tmp;
+  return lanczos_impl(&tmp, op, v0, steps, reproducible, lower, upper, as_stream(stream));
+}
+
+extern "C" chfsi_status chfsi_cholgs(chfsi_operator *op, double *X, int64_t M, int32_t N,
+                                     void *stream) {
+  CHFSI_ARG(op && X && M == op->M && N > 0, "bad argument");
+  chfsi_solver tmp;
+  return cholgs_impl(&tmp, X, M, N, as_stream(stream));
+}
+
+extern "C" chfsi_status chfsi_rayleigh_ritz(chfsi_operator *op, double *X, int64_t M, int32_t N,
+                                            int32_t block, double *eig_host, void *stream) {
+  CHFSI_ARG(op && X && eig_host && M == op->M && N > 0 && block > 0, "bad argument");
+  chfsi_solver tmp;
+  return rr_impl(&tmp, op, X, M, N, std::min<int>(block, N), eig_host, as_stream(stream));
+}
+
+extern "C" chfsi_status chfsi_residual_norms(chfsi_operator *op, const double *X, int64_t M,
+                                             int32_t N, int32_t block, const double *eig_host,
+                                             double *res_host, void *stream) {
+  CHFSI_ARG(op && X && eig_host && res_host && M == op->M && N > 0 && block > 0, "bad argument");
+  chfsi_solver tmp;
+  return residual_impl(&tmp, op, X, M, N, std::min<int>(block, N), eig_host, res_host,
+                       as_stream(stream));
+}
+
+extern "C" chfsi_status chfsi_solve(chfsi_solver *s, chfsi_operator *op, double *X, int64_t M,
+                                    int32_t N, double *eig_host, double *res_host,
+                                    int32_t is_first_filtering_call, int32_t compute_residual,
+                                    int32_t use_mixed_prec_overall, int32_t is_first_scf,
+                                    double *bUp_out, double *timings_ms, void *stream) {
+  CHFSI_ARG(s && op && X && eig_host, "null argument");
+  CHFSI_ARG(M == op->M && N > 0, "X must be n_dofs x N");
+  CHFSI_ARG(!compute_residual || res_host, "res_host required when computing residuals");
+  hipStream_t st = as_stream(stream);
+  const chfsi_params &P = s->params;
+  hipEvent_t ev[6];
+  for (auto &e : ev) CHFSI_HIP(hipEventCreate(&e));
+  struct EvGuard {
+    hipEvent_t *e;
+    ~EvGuard() {
+      for (int i = 0; i < 6; ++i) hipEventDestroy(e[i]);
+    }
+  } guard{ev};
+  CHFSI_HIP(hipEventRecord(ev[0], st));
+
+  // Lanczos bounds (…SolverDevice.cc:241-297)
+  if (is_first_filtering_call || !P.reuse_lanczos_upper) {
+    double lo, up;
+    const double *v0 = s->have_v0 ? sptr<double>(s->v0) : nullptr;
+    std::vector<double> hv;
+    Scratch gen;
+    if (!v0) {
+      // seeded uniform [0,1) start (the reference seeds rand() with the rank)
+      hv.resize(M);
+      uint64_t state = 0x9E3779B97F4A7C15ull;
+      for (int64_t i = 0; i < M; ++i) {
+        state ^= state << 13;
+        state ^= state >> 7;
+        state ^= state << 17;
+        hv[i] = double(state >> 11) * (1.0 / 9007199254740992.0);
+      }
+      CHFSI_TRY(gen.ensure(sizeof(double) * M));
+      CHFSI_HIP(hipMemcpy(gen.ptr, hv.data(), sizeof(double) * M, hipMemcpyHostToDevice));
+      v0 = sptr<double>(gen);
+    }
+    chfsi_status stl = lanczos_impl(s, op, v0, P.lanczos_steps, P.reproducible, &lo, &up, st);
+    gen.release();
+    CHFSI_TRY(stl);
+    if (is_first_filtering_call) {
+      s->a0 = lo;
+      s->bUp = up;
+      s->bLow = lo + (up - lo) * N / double(M) * (P.reproducible ? 10.0 : 200.0);
+    } else {
+      s->bUp = up;
+    }
+  }
+  CHFSI_HIP(hipEventRecord(ev[1], st));
+
+  // degree (…SolverDevice.cc:307-326)
+  unsigned m = P.cheby_degree > 0 ? (unsigned)P.cheby_degree : cheby_order_lookup(s->bUp);
+  if (is_first_scf) m = (unsigned)(m * P.first_scf_degree_factor);
+
+  // M^{1/2} scaling into the Loewdin basis, then blocked filtering
+  CHFSI_HIP(launch_scale_rows<double>(X, N, M, N, 1.0, op->d_sqrt_mass, st));
+  const int B = std::min<int>(P.cheby_block_size > 0 ? P.cheby_block_size : N, N);
+  const int prec = use_mixed_prec_overall ? P.filter_precision : CHFSI_FP64;
+  for (int j = 0; j < N; j += B) {
+    const int bw = std::min(B, N - j);
+    CHFSI_TRY(filter_block(s, op, X + j, N, bw, (int)m, s->bLow, s->bUp, s->a0, prec, st));
+  }
+  CHFSI_HIP(hipEventRecord(ev[2], st));
+
+  // CholGS + Rayleigh-Ritz (SHEP route, …SolverDevice.cc:609-643)
+  CHFSI_TRY(cholgs_impl(s, X, M, N, st));
+  CHFSI_HIP(hipEventRecord(ev[3], st));
+  CHFSI_TRY(rr_impl(s, op, X, M, N, B, eig_host, st));
+  CHFSI_HIP(hipEventRecord(ev[4], st));
+
+  if (compute_residual) CHFSI_TRY(residual_impl(s, op, X, M, N, B, eig_host, res_host, st));
+
+  // back to the FE basis
+  CHFSI_HIP(launch_scale_rows<double>(X, N, M, N, 1.0, op->d_inv_sqrt_mass, st));
+  CHFSI_HIP(hipEventRecord(ev[5], st));
+  CHFSI_HIP(hipEventSynchronize(ev[5]));
+  if (timings_ms) {
+    float t;
+    for (int k = 0; k < 4; ++k) {
+      CHFSI_HIP(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+      timings_ms[k] = t;
+    }
+    CHFSI_HIP(hipEventElapsedTime(&t, ev[4], ev[5]));
+    timings_ms[4] = t;
+    CHFSI_HIP(hipEventElapsedTime(&t, ev[0], ev[5]));
+    timings_ms[5] = t;
+  }
+  if (bUp_out) *bUp_out = s->bUp;
+  return CHFSI_OK;
+}

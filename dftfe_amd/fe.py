@@ -1,0 +1,340 @@
+"""Structured periodic hexahedral finite-element problems for the ChFSI path.
+
+This module plays the role deal.II plays for the reference solver: it builds
+the mesh, the Gauss-Lobatto-Legendre (GLL) Lagrange basis, the Gauss
+quadrature, the lumped (GLL) mass vector, the synthetic local potential at the
+quadrature points, the Gaussian s+p nonlocal projectors and the cell->DOF
+index maps that the device operator consumes.
+
+Reference counterparts (behaviour restated, deal.II-free):
+  * FE_Q on GLL support points, QGauss quadrature:       src/dft/dft.cc:86-92,
+                                                           include/constants.h:37-42
+  * lumped mass on QGaussLobatto(p+1), sqrt / inverse sqrt only on
+    unconstrained rows, zero on constrained rows:        src/dftOperator/kohnShamDFTOperatorDevice.cc:938-1031
+  * cell -> local DOF map:                               utils/vectorTools/vectorUtilities.cc:398-500
+  * projector element matrices C_e[i,a] = sum_q N_i beta_a JxW:
+                                                           src/dft/initPseudo-OV.cc:344-720
+  * periodic constraints (slave on the x=L face tied to its x=0 master):
+                                                           src/dft/initUnmovedTriangulation.cc:136-211
+
+Everything is seeded (numpy PCG64) so every machine regenerates identical inputs.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import Optional
+
+import numpy as np
+
+# ----------------------------------------------------------------------------
+# 1-D rules
+# ----------------------------------------------------------------------------
+
+
+def gll_rule(p: int):
+    """GLL nodes and weights on [0, 1] for polynomial degree p (p+1 points).
+
+    Nodes: 0, 1 and the roots of P_p'(x) mapped from [-1, 1]; weights
+    2 / (p (p+1) P_p(x_i)^2), halved for the unit interval.
+    """
+    if p < 1:
+        raise ValueError("degree must be >= 1")
+    leg = np.polynomial.legendre.Legendre.basis(p)
+    inner = np.sort(leg.deriv().roots().real) if p > 1 else np.zeros(0)
+    x = np.concatenate([[-1.0], inner, [1.0]])
+    w = 2.0 / (p * (p + 1) * leg(x) ** 2)
+    return 0.5 * (x + 1.0), 0.5 * w
+
+
+def gauss_rule(nq: int):
+    """Gauss-Legendre nodes and weights on [0, 1]."""
+    x, w = np.polynomial.legendre.leggauss(nq)
+    return 0.5 * (x + 1.0), 0.5 * w
+
+
+def lagrange_tables(nodes: np.ndarray, x: np.ndarray):
+    """Values and first derivatives of the Lagrange polynomials on `nodes`,
+    evaluated at `x`.  Returns (phi, dphi) with shape (len(x), len(nodes))."""
+    nn = len(nodes)
+    phi = np.ones((len(x), nn))
+    dphi = np.zeros((len(x), nn))
+    for a in range(nn):
+        others = [b for b in range(nn) if b != a]
+        denom = np.prod([nodes[a] - nodes[b] for b in others])
+        for b in others:
+            phi[:, a] *= x - nodes[b]
+        for skip in others:
+            term = np.ones(len(x))
+            for b in others:
+                if b != skip:
+                    term *= x - nodes[b]
+            dphi[:, a] += term
+        phi[:, a] /= denom
+        dphi[:, a] /= denom
+    return phi, dphi
+
+
+# ----------------------------------------------------------------------------
+# Workload descriptions
+# ----------------------------------------------------------------------------
+
+
+@dataclasses.dataclass(frozen=True)
+class Workload:
+    name: str
+    L: float              # cube side (bohr)
+    n: int                # cells per dimension
+    p: int                # FE degree
+    nq: int               # Gauss points per dimension
+    n_atoms: int
+    N: int                # wavefunctions
+    B: int                # Chebyshev block width
+    m: int                # Chebyshev degree
+    lanczos_steps: int = 10
+    passes: int = 1
+    seed: int = 20240601
+    depth: tuple = (2.0, 4.0)     # Gaussian well depth range (Ha)
+    width: tuple = (0.8, 1.2)     # Gaussian well width range (bohr)
+    vnl: tuple = (0.5, 1.5)       # projector coupling range (Ha)
+    rcut_factor: float = 4.0      # projector truncation radius / r_c
+
+
+WORKLOADS = {
+    # periodic 20 bohr cube, 8^3 cells, p=4 (32,768 DOFs), Gauss 5^3, 8 wells,
+    # 64 wavefunctions, degree 20, 10 Lanczos steps, one pass
+    "cube20_p4_n64": Workload("cube20_p4_n64", 20.0, 8, 4, 5, 8, 64, 64, 20,
+                              seed=1001),
+    # periodic 30 bohr cube, 16^3 cells, p=6 (884,736 DOFs), Gauss 8^3,
+    # 108 wells, 1024 wavefunctions in blocks of 256, degree 30, 5 passes
+    "cube30_p6_n1024": Workload("cube30_p6_n1024", 30.0, 16, 6, 8, 108, 1024,
+                                256, 30, passes=5, seed=1002),
+    # periodic 24 bohr cube, 12^3 cells, p=5, 32 wells, 256 wavefunctions
+    "cube24_p5_n256": Workload("cube24_p5_n256", 24.0, 12, 5, 6, 32, 256, 256,
+                               20, seed=1004),
+    # tiny case for brute-force checks
+    "tiny": Workload("tiny", 8.0, 2, 2, 3, 2, 8, 8, 8, seed=7),
+}
+
+
+# ----------------------------------------------------------------------------
+# Problem
+# ----------------------------------------------------------------------------
+
+
+@dataclasses.dataclass
+class FEProblem:
+    wl: Workload
+    mode: str                 # "unique" (periodicity folded into the map) or "dealii"
+    h: float
+    ne: int                   # nodes per cell (p+1)^3
+    nqc: int                  # quadrature points per cell nq^3
+    C: int                    # cells
+    M: int                    # DOFs (rows of the node-major multivector)
+    cell_coords: np.ndarray   # (C, 3) int
+    cell_map: np.ndarray      # (C, ne) int64 global DOF of local node i
+    cell_color: np.ndarray    # (C,) int8 parity colour (cx%2 + 2 cy%2 + 4 cz%2)
+    phi: np.ndarray           # (nq, p+1) 1-D basis at Gauss points
+    dphi: np.ndarray          # (nq, p+1) 1-D derivatives (reference interval)
+    wq: np.ndarray            # (nq,) Gauss weights on [0,1]
+    gll_w: np.ndarray         # (p+1,) GLL weights on [0,1]
+    gll_x: np.ndarray         # (p+1,) GLL nodes on [0,1]
+    gauss_x: np.ndarray       # (nq,) Gauss nodes on [0,1]
+    Nq: np.ndarray            # (nqc, ne) 3-D shape values
+    JxW: np.ndarray           # (nqc,) same for every cell (Cartesian, uniform)
+    K_e: np.ndarray           # (ne, ne) cell stiffness  int grad Ni . grad Nj
+    V_q: np.ndarray           # (C, nqc) local potential at quadrature points
+    mass: np.ndarray          # (M,) lumped mass (0 on constrained rows)
+    sqrt_mass: np.ndarray     # (M,)
+    inv_sqrt_mass: np.ndarray # (M,)
+    constrained: np.ndarray   # (M,) bool
+    slave_rows: np.ndarray    # (S,) constrained rows (dealii mode)
+    master_rows: np.ndarray   # (S,) their master (coefficient 1)
+    atom_pos: np.ndarray      # (na, 3)
+    atom_depth: np.ndarray
+    atom_width: np.ndarray
+    n_proj: int
+    proj_V: np.ndarray        # (n_proj,) coupling constants
+    pair_atom: np.ndarray     # (n_pairs,) atom of each (atom, cell) pair
+    pair_cell: np.ndarray     # (n_pairs,) cell of each pair
+    pair_C: np.ndarray        # (n_pairs, ne, 4) C_e[i, alpha]
+
+    @property
+    def vJxW(self) -> np.ndarray:
+        return self.V_q * self.JxW[None, :]
+
+    @property
+    def lanczos_start(self) -> np.ndarray:
+        """Seeded uniform [0,1) start vector, constrained entries zero (the
+        reference uses rand()/RAND_MAX then set_zero and normalisation:
+        src/linAlg/linearAlgebraOperationsOpt.cc:2914-2931)."""
+        rng = np.random.default_rng(self.wl.seed + 17)
+        v = rng.random(self.M)
+        v[self.constrained] = 0.0
+        return v / np.linalg.norm(v)
+
+    def initial_guess(self, N: Optional[int] = None, seed_offset: int = 0) -> np.ndarray:
+        """Seeded U[-1,1] node-major start X[node*N + vec]; constrained rows 0."""
+        N = self.wl.N if N is None else N
+        rng = np.random.default_rng(self.wl.seed + 31 + seed_offset)
+        X = rng.uniform(-1.0, 1.0, size=(self.M, N))
+        X[self.constrained, :] = 0.0
+        return X
+
+    def quad_points(self, cells: Optional[np.ndarray] = None) -> np.ndarray:
+        """Physical coordinates of quadrature points, shape (len(cells), nqc, 3)."""
+        if cells is None:
+            cells = np.arange(self.C)
+        nq = self.wl.nq
+        g = self.gauss_x * self.h
+        qx = np.tile(g, nq * nq)
+        qy = np.tile(np.repeat(g, nq), nq)
+        qz = np.repeat(g, nq * nq)
+        local = np.stack([qx, qy, qz], axis=1)                  # (nqc, 3)
+        origin = self.cell_coords[cells].astype(np.float64) * self.h
+        return origin[:, None, :] + local[None, :, :]
+
+
+def _min_image(d: np.ndarray, L: float) -> np.ndarray:
+    return d - L * np.round(d / L)
+
+
+def build_problem(wl: Workload, mode: str = "unique") -> FEProblem:
+    if mode not in ("unique", "dealii"):
+        raise ValueError("mode must be 'unique' or 'dealii'")
+    n, p, nq, L = wl.n, wl.p, wl.nq, wl.L
+    if n % 2:
+        raise ValueError("parity colouring needs an even number of cells per dimension")
+    h = L / n
+    pp = p + 1
+    ne = pp ** 3
+    nqc = nq ** 3
+    C = n ** 3
+
+    gll_x, gll_w = gll_rule(p)
+    gx, gw = gauss_rule(nq)
+    phi, dphi = lagrange_tables(gll_x, gx)
+
+    # ---- cells and index maps (local node i = a + pp*(b + pp*c), x fastest)
+    cz, cy, cx = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    cell_coords = np.stack([cx.ravel(), cy.ravel(), cz.ravel()], axis=1)
+    lc, lb, la = np.meshgrid(np.arange(pp), np.arange(pp), np.arange(pp), indexing="ij")
+    la, lb, lc = la.ravel(), lb.ravel(), lc.ravel()
+    gxi = cell_coords[:, 0:1] * p + la[None, :]
+    gyi = cell_coords[:, 1:2] * p + lb[None, :]
+    gzi = cell_coords[:, 2:3] * p + lc[None, :]
+    nu = n * p
+    if mode == "unique":
+        M = nu ** 3
+        cell_map = (gxi % nu) + nu * ((gyi % nu) + nu * (gzi % nu))
+        constrained = np.zeros(M, dtype=bool)
+        slave_rows = np.zeros(0, dtype=np.int64)
+        master_rows = np.zeros(0, dtype=np.int64)
+    else:
+        nd = nu + 1
+        M = nd ** 3
+        cell_map = gxi + nd * (gyi + nd * gzi)
+        iz, iy, ix = np.meshgrid(np.arange(nd), np.arange(nd), np.arange(nd), indexing="ij")
+        ix, iy, iz = ix.ravel(), iy.ravel(), iz.ravel()
+        constrained = (ix == nu) | (iy == nu) | (iz == nu)
+        slave_rows = np.nonzero(constrained)[0].astype(np.int64)
+        mx, my, mz = ix % nu, iy % nu, iz % nu
+        master_rows = (mx + nd * (my + nd * mz))[slave_rows].astype(np.int64)
+    cell_map = cell_map.astype(np.int64)
+    cell_color = ((cell_coords[:, 0] % 2) + 2 * (cell_coords[:, 1] % 2)
+                  + 4 * (cell_coords[:, 2] % 2)).astype(np.int8)
+
+    # ---- 3-D shape data at Gauss points (q = qx + nq*(qy + nq*qz))
+    Nq = np.empty((nqc, ne))
+    wq3 = np.empty(nqc)
+    for qz in range(nq):
+        for qy in range(nq):
+            for qx in range(nq):
+                q = qx + nq * (qy + nq * qz)
+                Nq[q, :] = phi[qx, la] * phi[qy, lb] * phi[qz, lc]
+                wq3[q] = gw[qx] * gw[qy] * gw[qz]
+    JxW = wq3 * h ** 3
+
+    # ---- cell stiffness via 1-D tensor factors (exact Gauss integration)
+    M1 = (phi * gw[:, None]).T @ phi          # int phi_a phi_b on [0,1]
+    D1 = (dphi * gw[:, None]).T @ dphi        # int phi_a' phi_b' on [0,1]
+    K_e = h * (D1[la][:, la] * M1[lb][:, lb] * M1[lc][:, lc]
+               + M1[la][:, la] * D1[lb][:, lb] * M1[lc][:, lc]
+               + M1[la][:, la] * M1[lb][:, lb] * D1[lc][:, lc])
+
+    # ---- atoms, wells
+    rng = np.random.default_rng(wl.seed)
+    atom_pos = rng.uniform(0.0, L, size=(wl.n_atoms, 3))
+    atom_depth = rng.uniform(*wl.depth, size=wl.n_atoms)
+    atom_width = rng.uniform(*wl.width, size=wl.n_atoms)
+    proj_V = rng.uniform(*wl.vnl, size=4 * wl.n_atoms)
+
+    prob = FEProblem(
+        wl=wl, mode=mode, h=h, ne=ne, nqc=nqc, C=C, M=M,
+        cell_coords=cell_coords, cell_map=cell_map, cell_color=cell_color,
+        phi=phi, dphi=dphi, wq=gw, gll_w=gll_w, gll_x=gll_x, gauss_x=gx,
+        Nq=Nq, JxW=JxW, K_e=K_e, V_q=np.zeros((C, nqc)),
+        mass=np.zeros(M), sqrt_mass=np.zeros(M), inv_sqrt_mass=np.zeros(M),
+        constrained=constrained, slave_rows=slave_rows, master_rows=master_rows,
+        atom_pos=atom_pos, atom_depth=atom_depth, atom_width=atom_width,
+        n_proj=4 * wl.n_atoms, proj_V=proj_V,
+        pair_atom=np.zeros(0, np.int64), pair_cell=np.zeros(0, np.int64),
+        pair_C=np.zeros((0, ne, 4)),
+    )
+
+    # ---- local potential V(x) = -sum_a D_a exp(-|x-R_a|^2 / w_a^2)
+    V = np.zeros((C, nqc))
+    chunk = max(1, (1 << 22) // nqc)
+    for c0 in range(0, C, chunk):
+        cells = np.arange(c0, min(C, c0 + chunk))
+        xq = prob.quad_points(cells)
+        acc = np.zeros((len(cells), nqc))
+        for a in range(wl.n_atoms):
+            d = _min_image(xq - atom_pos[a], L)
+            r2 = np.einsum("cqk,cqk->cq", d, d)
+            acc -= atom_depth[a] * np.exp(-r2 / atom_width[a] ** 2)
+        V[c0:c0 + len(cells)] = acc
+    prob.V_q = V
+
+    # ---- lumped mass on GLL points: m_i(cell) = h^3 w_a w_b w_c
+    m_loc = h ** 3 * gll_w[la] * gll_w[lb] * gll_w[lc]
+    mass = np.zeros(M)
+    np.add.at(mass, cell_map.ravel(), np.tile(m_loc, C))
+    if mode == "dealii":
+        # distribute_local_to_global: slave contributions are condensed into masters
+        np.add.at(mass, master_rows, mass[slave_rows])
+        mass[slave_rows] = 0.0
+    prob.mass = mass
+    ok = ~constrained
+    prob.sqrt_mass = np.where(ok, np.sqrt(np.maximum(mass, 0.0)), 0.0)
+    prob.inv_sqrt_mass = np.where(ok & (mass > 1e-15), 1.0 / np.sqrt(np.where(mass > 0, mass, 1.0)), 0.0)
+
+    # ---- Gaussian s + p projectors, truncated at r <= rcut_factor * r_c
+    centres = (cell_coords.astype(np.float64) + 0.5) * h
+    pair_atom, pair_cell, pair_C = [], [], []
+    for a in range(wl.n_atoms):
+        rc = atom_width[a]
+        rcut = wl.rcut_factor * rc
+        dc = _min_image(centres - atom_pos[a], L)
+        sel = np.nonzero(np.sqrt((dc ** 2).sum(1)) <= rcut + h * math.sqrt(3.0) / 2.0)[0]
+        if len(sel) == 0:
+            continue
+        xq = prob.quad_points(sel)                       # (s, nqc, 3)
+        d = _min_image(xq - atom_pos[a], L)
+        r2 = (d ** 2).sum(-1)
+        g = np.exp(-r2 / (2.0 * rc * rc)) * (r2 <= rcut * rcut)
+        beta = np.stack([g, d[..., 0] * g, d[..., 1] * g, d[..., 2] * g], axis=-1)  # (s, nqc, 4)
+        Ce = np.einsum("qi,sqa->sia", Nq * JxW[:, None], beta)
+        keep = np.abs(Ce).reshape(len(sel), -1).max(axis=1) > 0.0
+        pair_atom.append(np.full(int(keep.sum()), a, dtype=np.int64))
+        pair_cell.append(sel[keep].astype(np.int64))
+        pair_C.append(Ce[keep])
+    if pair_atom:
+        prob.pair_atom = np.concatenate(pair_atom)
+        prob.pair_cell = np.concatenate(pair_cell)
+        prob.pair_C = np.concatenate(pair_C, axis=0)
+    return prob
+
+
+def build_workload(name: str, mode: str = "unique") -> FEProblem:
+    return build_problem(WORKLOADS[name], mode)

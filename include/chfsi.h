@@ -1,0 +1,209 @@
+/*
+ * chfsi.h -- C ABI of the MI355X (gfx950) Chebyshev-filtered subspace
+ * iteration library (libchfsi.so).
+ *
+ * Drop-in boundary for one MPI rank's full-domain ChFSI pass of DFT-FE.
+ * Every entry point below replaces one member of the reference's operator /
+ * solver classes; the citation next to each declaration names it.
+ *
+ *   operatorDFTDeviceClass              include/operatorDevice.h:43-420
+ *   kohnShamDFTOperatorDeviceClass      include/kohnShamDFTOperatorDevice.h:43-782
+ *   chebyshevOrthogonalizedSubspaceIterationSolverDevice
+ *                                       include/chebyshevOrthogonalizedSubspaceIterationSolverDevice.h:37-166
+ *   linearAlgebraOperationsDevice::*    include/linearAlgebraOperationsDevice.h:68-402
+ *   dftUtils::constraintMatrixInfoDevice include/constraintMatrixInfoDevice.h:36-211
+ *
+ * Conventions
+ *   - Multivectors are node-major, X[node * ld + vec] (the reference's
+ *     flattened layout), resident in device memory owned by the caller.
+ *   - All calls are asynchronous on the given stream (hipStream_t passed as
+ *     void*; NULL = the null stream).  Calls that return host values
+ *     (eigenvalues, residual norms, bounds) synchronise that stream first.
+ *   - Errors are returned as chfsi_status; chfsi_last_error() gives a
+ *     thread-local message.  Nothing calls exit().
+ *   - Handles are not thread-safe; use one handle per host thread.
+ */
+#ifndef CHFSI_H
+#define CHFSI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  CHFSI_OK = 0,
+  CHFSI_ERR_ARG = 1,      /* invalid argument or size */
+  CHFSI_ERR_HIP = 2,      /* HIP runtime error */
+  CHFSI_ERR_BLAS = 3,     /* rocBLAS / rocSOLVER error */
+  CHFSI_ERR_NOT_SPD = 4,  /* overlap matrix not positive definite (pseudoGSDevice.cc:297-301) */
+  CHFSI_ERR_EIG = 5,      /* dense eigensolver failed */
+  CHFSI_ERR_STATE = 6,    /* call out of order (e.g. no potential set) */
+  CHFSI_ERR_NOMEM = 7,    /* device allocation failed */
+  CHFSI_ERR_NAN = 8       /* non-finite value detected */
+} chfsi_status;
+
+typedef enum {
+  CHFSI_FP64 = 0, /* filter in FP64 (the drop-in path) */
+  CHFSI_FP32 = 1, /* filter bulk in FP32 on MFMA, FP32 accumulation */
+  CHFSI_BF16 = 2  /* BF16 cell matrices and inputs, FP32 accumulation */
+} chfsi_precision;
+
+typedef struct chfsi_operator chfsi_operator;
+typedef struct chfsi_solver chfsi_solver;
+
+/* Description of the discretised Kohn-Sham operator on one rank.  Host
+ * pointers; copied to the device by chfsi_operator_create.  Mirrors what
+ * kohnShamDFTOperatorDeviceClass::init/reinit gather from deal.II
+ * (kohnShamDFTOperatorDevice.cc:492-933, :938-1031). */
+typedef struct {
+  int64_t n_dofs;            /* M: rows of the node-major multivector          */
+  int32_t n_cells;           /* C                                              */
+  int32_t nodes_per_cell;    /* n_e = (p+1)^3                                   */
+  int32_t quad_per_cell;     /* n_q                                            */
+  const int64_t *cell_map;   /* C*n_e: DOF of local node i of cell c
+                                (vectorUtilities.cc:398-500)                   */
+  const int8_t *cell_color;  /* C: colour 0..7; cells of one colour share no DOF */
+  const double *sqrt_mass;   /* M: M^{1/2}, 0 on constrained rows              */
+  const double *inv_sqrt_mass; /* M: M^{-1/2}, 0 on constrained rows           */
+  const double *shape_values;  /* n_q*n_e: N_i(q), row q                        */
+  const double *stiffness;     /* n_e*n_e: int grad N_i . grad N_j (all cells)  */
+  /* nonlocal projectors (computeNonLocalHamiltonianTimesXMemoryOptBatchGEMMDevice.cc) */
+  int32_t n_proj;            /* total projectors                               */
+  int32_t n_pairs;           /* (atom, cell) pairs in projector support         */
+  int32_t proj_per_pair;     /* projectors per atom (4: s + 3 p)               */
+  const int32_t *pair_cell;  /* n_pairs                                        */
+  const int32_t *pair_proj0; /* n_pairs: first projector id of the pair's atom */
+  const double *pair_C;      /* n_pairs*n_e*proj_per_pair: C_e[i][alpha]        */
+  const double *proj_V;      /* n_proj: coupling constants                     */
+  /* affine constraints, CSR over constrained rows (constraintMatrixInfoDevice.cc:446-542).
+   * n_constrained == 0 means none. */
+  int64_t n_constrained;
+  const int64_t *constraint_rows;  /* n_constrained slave rows                  */
+  const int64_t *constraint_ptr;   /* n_constrained+1                           */
+  const int64_t *constraint_cols;  /* masters                                   */
+  const double *constraint_vals;   /* coefficients                              */
+} chfsi_operator_desc;
+
+/* Solver parameters: the ChFSI knobs of dftParameters
+ * (utils/dftParameters.cc:840-1021). */
+typedef struct {
+  int32_t cheby_block_size;        /* CHEBY WFC BLOCK SIZE                      */
+  int32_t cheby_degree;            /* CHEBYSHEV POLYNOMIAL DEGREE (0: table)    */
+  int32_t lanczos_steps;           /* reference: 20, or 40 when reproducible    */
+  int32_t reproducible;            /* REPRODUCIBLE OUTPUT                       */
+  int32_t reuse_lanczos_upper;     /* REUSE LANCZOS UPPER BOUND                 */
+  double first_scf_degree_factor;  /* CHEBYSHEV FILTER POLYNOMIAL DEGREE SCALING
+                                      FACTOR FIRST SCF (1.34)                   */
+  int32_t filter_precision;        /* chfsi_precision for the filter bulk when
+                                      use_mixed_prec_overall is set             */
+  int32_t use_shep;                /* SUBSPACE PROJ SHEP GPU: 1 = CholGS + RR   */
+} chfsi_params;
+
+const char *chfsi_last_error(void);
+void chfsi_default_params(chfsi_params *params);
+
+/* ---------------- operator (operatorDFTDeviceClass) ---------------- */
+
+/* init + reinit + computeMassVector (operatorDevice.h:58-59, :139-140, :160-164) */
+chfsi_status chfsi_operator_create(const chfsi_operator_desc *desc, chfsi_operator **op);
+void chfsi_operator_destroy(chfsi_operator *op);
+
+/* Cell Hamiltonian build from V_eff*JxW at quadrature points (device, C*n_q):
+ * H_e = 1/2 K_e + N^T diag(vJxW_c) N.  Replaces computeHamiltonianMatricesAllkpt
+ * / hamMatrixKernelLDA (hamiltonianMatrixCalculatorFlattenedDevice.cc:61-116, :1073-1375). */
+chfsi_status chfsi_operator_set_potential(chfsi_operator *op, const double *vJxW_dev, void *stream);
+
+/* Direct upload of FP64 cell matrices (device, C*n_e*n_e, row i contiguous). */
+chfsi_status chfsi_operator_set_cell_matrices(chfsi_operator *op, const double *H_dev, void *stream);
+
+/* Copy the FP64 cell matrices out (device, C*n_e*n_e) -- for checks. */
+chfsi_status chfsi_operator_get_cell_matrices(chfsi_operator *op, double *H_dev, void *stream);
+
+/* getSqrtMassVec / getInvSqrtMassVec (operatorDevice.h:68-72): device pointers. */
+const double *chfsi_operator_sqrt_mass(const chfsi_operator *op);
+const double *chfsi_operator_inv_sqrt_mass(const chfsi_operator *op);
+int64_t chfsi_operator_n_dofs(const chfsi_operator *op);
+
+/* HX (operatorDevice.h:176-185; kohnShamDFTOperatorDevice.cc:3765-3860):
+ *   dst <- (scale_flag ? dst : M^{-1/2} dst) + scalar * M^{-1/2} H M^{-1/2} src
+ * src, dst: node-major FP64 blocks of B columns with leading dimensions ld_src, ld_dst.
+ * src is not modified (the reference restores it when doUnscalingSrc). */
+chfsi_status chfsi_hx(chfsi_operator *op, const double *src, int64_t ld_src, double *dst,
+                      int64_t ld_dst, int32_t B, int32_t scale_flag, double scalar, void *stream);
+
+/* Same operator with FP32 multivectors, FP32 cell matrices, FP32 MFMA. */
+chfsi_status chfsi_hx_fp32(chfsi_operator *op, const float *src, int64_t ld_src, float *dst,
+                           int64_t ld_dst, int32_t B, int32_t scale_flag, float scalar,
+                           void *stream);
+
+/* HXCheby (operatorDevice.h:188-197; kohnShamDFTOperatorDevice.cc:3874-3997):
+ *   dst <- dst + H src   (unscaled FE H; src already carries its M^{-1/2} scaling). */
+chfsi_status chfsi_hx_cheby(chfsi_operator *op, const double *src, int64_t ld_src, double *dst,
+                            int64_t ld_dst, int32_t B, void *stream);
+
+/* Constraint kernels (constraintMatrixInfoDevice.cc:544-588, :596-816, :818-855). */
+chfsi_status chfsi_constraints_distribute(chfsi_operator *op, double *X, int64_t ld, int32_t B, void *stream);
+chfsi_status chfsi_constraints_slave_to_master(chfsi_operator *op, double *X, int64_t ld, int32_t B, void *stream);
+chfsi_status chfsi_constraints_set_zero(chfsi_operator *op, double *X, int64_t ld, int32_t B, void *stream);
+
+/* ---------------- linear algebra (linearAlgebraOperationsDevice) ---------------- */
+
+/* chebyshevFilter (linearAlgebraOperationsDevice.cc:531-727): in-place degree-m
+ * scaled Chebyshev filter of the B-column block X (already M^{1/2}-scaled).
+ * precision selects the arithmetic of the recurrence. */
+chfsi_status chfsi_chebyshev_filter(chfsi_operator *op, double *X, int64_t ld, int32_t B, int32_t m,
+                                    double a, double b, double a0, int32_t precision, void *stream);
+
+/* lanczosLowerUpperBoundEigenSpectrum (linearAlgebraOperationsDevice.cc:340-527).
+ * v0: device start vector of length M (normalised and constrained rows zeroed
+ * internally).  Returns floor(theta_min), ceil(theta_max + |f|/10 or |f|). */
+chfsi_status chfsi_lanczos_bounds(chfsi_operator *op, const double *v0_dev, int32_t steps,
+                                  int32_t reproducible, double *lower, double *upper, void *stream);
+
+/* pseudoGramSchmidtOrthogonalization (pseudoGSDevice.cc:81-463): X <- X L^{-T}, S = X^T X = L L^T. */
+chfsi_status chfsi_cholgs(chfsi_operator *op, double *X, int64_t M, int32_t N, void *stream);
+
+/* rayleighRitz (rayleighRitzDevice.cc:80-353): Hp = X^T H~ X, eig, X <- X Q.
+ * eig_host: N ascending eigenvalues. */
+chfsi_status chfsi_rayleigh_ritz(chfsi_operator *op, double *X, int64_t M, int32_t N, int32_t block,
+                                 double *eig_host, void *stream);
+
+/* computeEigenResidualNorm (linearAlgebraOperationsDevice.cc:4610-4766). */
+chfsi_status chfsi_residual_norms(chfsi_operator *op, const double *X, int64_t M, int32_t N,
+                                  int32_t block, const double *eig_host, double *res_host,
+                                  void *stream);
+
+/* ---------------- solver (chebyshevOrthogonalizedSubspaceIterationSolverDevice) ---------------- */
+
+/* constructor (…SolverDevice.h:48-54) */
+chfsi_status chfsi_solver_create(const chfsi_params *params, double a0, double bLow, double bUp,
+                                 chfsi_solver **solver);
+void chfsi_solver_destroy(chfsi_solver *solver);
+/* reinitSpectrumBounds (…SolverDevice.h:121-124) */
+chfsi_status chfsi_solver_reinit_bounds(chfsi_solver *solver, double a0, double bLow, double bUp);
+chfsi_status chfsi_solver_get_bounds(const chfsi_solver *solver, double *a0, double *bLow, double *bUp);
+/* Supplies the Lanczos start vector used on a first filtering call (device,
+ * length M); without it a seeded uniform [0,1) vector is generated. */
+chfsi_status chfsi_solver_set_lanczos_start(chfsi_solver *solver, const double *v0_dev,
+                                            int64_t M);
+
+/* solve (…SolverDevice.h:61-77; .cc:155-736): one ChFSI pass on X (device,
+ * node-major M x N, FE basis).  Overwrites X with the Ritz vectors, writes N
+ * ascending eigenvalues and residual norms (host), returns the upper bound of
+ * the unwanted spectrum in *bUp_out.  Per-phase device times (ms) go to
+ * timings_ms[0..5] = {lanczos, filter, cholgs, rayleigh_ritz, residual, total}
+ * when timings_ms is not NULL. */
+chfsi_status chfsi_solve(chfsi_solver *solver, chfsi_operator *op, double *X, int64_t M,
+                         int32_t N, double *eig_host, double *res_host,
+                         int32_t is_first_filtering_call, int32_t compute_residual,
+                         int32_t use_mixed_prec_overall, int32_t is_first_scf, double *bUp_out,
+                         double *timings_ms, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CHFSI_H */

@@ -1,0 +1,380 @@
+"""ORACLE -- CPU FP64 restatement of the reference ChFSI pass.
+
+TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py may import this module, and only as the checker.
+The product path (dftfe_amd) never imports it and has no CPU fallback.
+
+Every function cites the reference file:line whose behaviour it restates.
+The reference itself cannot be built here (deal.II, p4est, ELPA, libxc are
+absent), so each function is pinned by the mathematics instead (closed forms,
+brute force, invariants); see tests/test_oracle.py.
+
+Layout: node-major multivectors, X[node, vec] (numpy row-major == the
+reference's X[node*N + vec]).
+"""
+from __future__ import annotations
+
+import math
+from typing import Optional
+
+import numpy as np
+import scipy.linalg as sla
+import scipy.sparse as sp
+
+# ----------------------------------------------------------------------------
+# Cell Hamiltonians
+# ----------------------------------------------------------------------------
+
+
+def shape_gradients(prob) -> np.ndarray:
+    """3-D shape gradients at Gauss points, (nqc, ne, 3), physical units.
+
+    Built point by point from the 1-D tables (not from the tensor-factored
+    stiffness the product uses) so that the stiffness below is an independent
+    computation.  Reference: utils/FEBasisOperations.cc:1135-1330 (stiffness
+    from shape gradients and JxW)."""
+    nq, pp = prob.wl.nq, prob.wl.p + 1
+    phi, dphi, h = prob.phi, prob.dphi, prob.h
+    G = np.zeros((prob.nqc, prob.ne, 3))
+    for i in range(prob.ne):
+        a, b, c = i % pp, (i // pp) % pp, i // (pp * pp)
+        for q in range(prob.nqc):
+            qx, qy, qz = q % nq, (q // nq) % nq, q // (nq * nq)
+            G[q, i, 0] = dphi[qx, a] * phi[qy, b] * phi[qz, c] / h
+            G[q, i, 1] = phi[qx, a] * dphi[qy, b] * phi[qz, c] / h
+            G[q, i, 2] = phi[qx, a] * phi[qy, b] * dphi[qz, c] / h
+    return G
+
+
+def cell_hamiltonians(prob, V_q: Optional[np.ndarray] = None, cells=None) -> np.ndarray:
+    """H_e[i,j] = 1/2 int grad N_i . grad N_j + sum_q vJxW_q N_i(q) N_j(q).
+
+    Reference: src/dftOperator/hamiltonianMatrixCalculatorFlattenedDevice.cc:61-116
+    (hamMatrixKernelLDA, real case); CPU hamiltonianMatrixCalculator.cc:21-985.
+    """
+    V_q = prob.V_q if V_q is None else V_q
+    cells = np.arange(prob.C) if cells is None else np.asarray(cells)
+    G = shape_gradients(prob)
+    K = np.einsum("q,qid,qjd->ij", prob.JxW, G, G)
+    vJxW = V_q[cells] * prob.JxW[None, :]
+    Nt = prob.Nq.T
+    H = np.empty((len(cells), prob.ne, prob.ne))
+    for k in range(len(cells)):
+        H[k] = 0.5 * K + (Nt * vJxW[k][None, :]) @ prob.Nq
+    return H
+
+
+# ----------------------------------------------------------------------------
+# Operator
+# ----------------------------------------------------------------------------
+
+
+class OracleOperator:
+    """FP64 H~ = M^{-1/2} H M^{-1/2} on node-major blocks.
+
+    HX follows src/dftOperator/kohnShamDFTOperator.cc:950-1044 (CPU) /
+    kohnShamDFTOperatorDevice.cc:3765-3860 (device): scale src by M^{-1/2},
+    distribute constraints, per-cell dense GEMM (matrixVectorProductImplementations.cc:96-169),
+    nonlocal C V C^T (computeNonLocalHamiltonianTimesXMemoryOpt.cc:269-503),
+    distribute_slave_to_master, scale dst by M^{-1/2}.
+    """
+
+    def __init__(self, prob, H_e: Optional[np.ndarray] = None, cell_chunk: int = 256):
+        self.prob = prob
+        self.H_e = cell_hamiltonians(prob) if H_e is None else H_e
+        self.cell_chunk = cell_chunk
+        C, ne, M = prob.C, prob.ne, prob.M
+        rows = np.arange(C * ne)
+        # gather matrix A: (C*ne) x M with A[c*ne+i, map[c,i]] = 1
+        self.A = sp.csr_matrix((np.ones(C * ne), (rows, prob.cell_map.ravel())), shape=(C * ne, M))
+        self.At = self.A.T.tocsr()
+        self.s = prob.inv_sqrt_mass
+        self.sq = prob.sqrt_mass
+        # nonlocal: pair p -> projector ids 4*atom + alpha
+        self.pair_proj = (4 * prob.pair_atom[:, None] + np.arange(4)[None, :])
+
+    # -- constraints (utils/constraintMatrixInfo.cc; device
+    #    utils/constraintMatrixInfoDevice.cc:544-588 distribute, :596-816 slave->master,
+    #    :818-855 set_zero).  Periodic constraints: one master, coefficient 1.
+    def distribute(self, X):
+        p = self.prob
+        if len(p.slave_rows):
+            X[p.slave_rows] = X[p.master_rows]
+
+    def distribute_slave_to_master(self, Y):
+        p = self.prob
+        if len(p.slave_rows):
+            np.add.at(Y, p.master_rows, Y[p.slave_rows])
+            Y[p.slave_rows] = 0.0
+
+    def set_zero(self, X):
+        p = self.prob
+        if len(p.slave_rows):
+            X[p.slave_rows] = 0.0
+
+    def h_fe(self, Xs: np.ndarray) -> np.ndarray:
+        """Unscaled FE H (local + nonlocal) on an already-distributed block."""
+        p = self.prob
+        B = Xs.shape[1]
+        Xc = (self.A @ Xs).reshape(p.C, p.ne, B)
+        Yc = np.empty_like(Xc)
+        for c0 in range(0, p.C, self.cell_chunk):
+            c1 = min(p.C, c0 + self.cell_chunk)
+            Yc[c0:c1] = np.matmul(self.H_e[c0:c1], Xc[c0:c1])
+        if len(p.pair_cell):
+            # C^T X per (atom, cell) pair, reduced per projector
+            part = np.einsum("pia,piv->pav", p.pair_C, Xc[p.pair_cell])
+            P = np.zeros((p.n_proj, B))
+            np.add.at(P, self.pair_proj.ravel(), part.reshape(-1, B))
+            W = p.proj_V[:, None] * P
+            contrib = np.einsum("pia,pav->piv", p.pair_C, W[self.pair_proj])
+            np.add.at(Yc, p.pair_cell, contrib)
+        return self.At @ Yc.reshape(p.C * p.ne, B)
+
+    def HX(self, src: np.ndarray, scale_flag: bool, scalar: float, dst: np.ndarray) -> np.ndarray:
+        """dst <- (scale_flag ? dst : M^{-1/2} dst) + scalar * H~ src
+        (kohnShamDFTOperatorDevice.cc:3790-3859; returns the new dst)."""
+        Xs = (scalar * self.s)[:, None] * src
+        self.distribute(Xs)
+        base = dst * self.sq[:, None] if scale_flag else dst.copy()
+        Y = base + self.h_fe(Xs)
+        self.distribute_slave_to_master(Y)
+        return self.s[:, None] * Y
+
+    def apply(self, X: np.ndarray) -> np.ndarray:
+        """H~ X."""
+        Xs = self.s[:, None] * X
+        self.distribute(Xs)
+        Y = self.h_fe(Xs)
+        self.distribute_slave_to_master(Y)
+        return self.s[:, None] * Y
+
+    def sparse_matrix(self) -> sp.csr_matrix:
+        """Assembled H~ as a sparse matrix (for brute-force checks)."""
+        p = self.prob
+        C, ne = p.C, p.ne
+        r = np.repeat(p.cell_map, ne, axis=1).ravel()
+        c = np.tile(p.cell_map, (1, ne)).ravel()
+        H = sp.coo_matrix((self.H_e.reshape(C, ne * ne).ravel(), (r, c)), shape=(p.M, p.M)).tocsr()
+        if len(p.pair_cell):
+            # assembled projector vectors c_alpha (M x n_proj)
+            rows = np.repeat(p.cell_map[p.pair_cell], 4, axis=1).ravel()
+            cols = np.repeat(self.pair_proj[:, None, :], ne, axis=1).ravel()
+            Cg = sp.coo_matrix((p.pair_C.ravel(), (rows, cols)), shape=(p.M, p.n_proj)).tocsr()
+            H = H + Cg @ sp.diags(p.proj_V) @ Cg.T
+        if len(p.slave_rows):
+            # P^T H P with P the periodic prolongation (slaves copy masters)
+            keep = np.nonzero(~p.constrained)[0]
+            src = np.arange(p.M).copy()
+            src[p.slave_rows] = p.master_rows
+            P = sp.csr_matrix((np.ones(p.M), (np.arange(p.M), src)), shape=(p.M, p.M))
+            H = (P.T @ H @ P).tocsr()
+            mask = np.zeros(p.M)
+            mask[keep] = 1.0
+            H = sp.diags(mask) @ H @ sp.diags(mask)
+        S = sp.diags(self.s)
+        return (S @ H @ S).tocsr()
+
+
+# ----------------------------------------------------------------------------
+# Lanczos bounds
+# ----------------------------------------------------------------------------
+
+
+def lanczos_bounds(op, v0: np.ndarray, steps: int, reproducible: bool = False):
+    """k-step Lanczos on H~: returns (floor(theta_min), ceil(theta_max + |f|/10)),
+    or + |f| when reproducible.  Reference: src/linAlg/linearAlgebraOperationsOpt.cc:2891-3023
+    (device twin linearAlgebraOperationsDevice.cc:340-527)."""
+    v = v0.copy()
+    op.set_zero(v[:, None]) if v.ndim == 1 else None
+    v = v / np.linalg.norm(v)
+    f = op.apply(v[:, None])[:, 0]
+    alpha = f @ v
+    f = f - alpha * v
+    T = np.zeros((steps, steps))
+    T[0, 0] = alpha
+    for j in range(1, steps):
+        beta = np.linalg.norm(f)
+        v_old = v
+        v = f / beta
+        f = op.apply(v[:, None])[:, 0]
+        f = f - beta * v_old
+        alpha = f @ v
+        f = f - alpha * v
+        T[j, j - 1] = beta
+        T[j - 1, j] = beta
+        T[j, j] = alpha
+    theta = np.linalg.eigvalsh(T)
+    fnorm = np.linalg.norm(f)
+    lower = math.floor(theta[0])
+    upper = math.ceil(theta[-1] + (fnorm if reproducible else fnorm / 10.0))
+    return float(lower), float(upper), theta, fnorm
+
+
+def first_call_bounds(lower: float, upper: float, N: int, M_global: int, reproducible: bool = False):
+    """bLow = a0 + (bUp - a0) * N / M * (10 or 200).  Reference:
+    src/solvers/eigenSolvers/chebyshevOrthogonalizedSubspaceIterationSolverDevice.cc:264-271."""
+    a0, bUp = lower, upper
+    bLow = a0 + (bUp - a0) * N / M_global * (10.0 if reproducible else 200.0)
+    return a0, bLow, bUp
+
+
+# ----------------------------------------------------------------------------
+# Chebyshev filter
+# ----------------------------------------------------------------------------
+
+
+def chebyshev_filter(apply_hx, X: np.ndarray, m: int, a: float, b: float, a0: float) -> np.ndarray:
+    """Scaled three-term Chebyshev recurrence of degree m on [a, b], a0-scaled.
+
+    Reference: src/linAlg/linearAlgebraOperationsOpt.cc:276-352.  `apply_hx(src,
+    scale_flag, scalar, dst)` has the HX(src, scaleFlag, scalar, dst) semantics."""
+    e = (b - a) / 2.0
+    c = (b + a) / 2.0
+    sigma = e / (a0 - c)
+    sigma1 = sigma
+    gamma = 2.0 / sigma1
+    Xa = X.copy()
+    Y = apply_hx(Xa, False, 1.0, np.zeros_like(Xa))
+    alpha1, alpha2 = sigma1 / e, -c
+    Y = alpha1 * (Y + alpha2 * Xa)
+    for _degree in range(2, m + 1):
+        sigma2 = 1.0 / (gamma - sigma)
+        alpha1, alpha2 = 2.0 * sigma2 / e, -(sigma * sigma2)
+        Xa = alpha2 * Xa + (-c * alpha1) * Y
+        Xa = apply_hx(Y, True, alpha1, Xa)
+        Xa, Y = Y, Xa
+        sigma = sigma2
+    return Y
+
+
+def chebyshev_scalar(lam: np.ndarray, m: int, a: float, b: float, a0: float) -> np.ndarray:
+    """Closed form of the filter on an eigen-component: T_m((l-c)/e) / T_m((a0-c)/e)."""
+    e = (b - a) / 2.0
+    c = (b + a) / 2.0
+
+    def T(m, t):
+        t = np.asarray(t, dtype=np.float64)
+        out = np.empty_like(t)
+        inside = np.abs(t) <= 1
+        out[inside] = np.cos(m * np.arccos(t[inside]))
+        ti = t[~inside]
+        out[~inside] = np.sign(ti) ** m * np.cosh(m * np.arccosh(np.abs(ti)))
+        return out
+
+    return T(m, (lam - c) / e) / T(m, np.array([(a0 - c) / e]))[0]
+
+
+# ----------------------------------------------------------------------------
+# CholGS, Rayleigh-Ritz, residuals
+# ----------------------------------------------------------------------------
+
+
+def cholgs(X: np.ndarray) -> np.ndarray:
+    """S = X^T X, S = L L^T, X <- X L^{-T}.  Reference: src/linAlg/pseudoGS.cc:29-262;
+    device src/linAlg/pseudoGSDevice.cc:121-368."""
+    S = X.T @ X
+    L = np.linalg.cholesky(S)
+    return sla.solve_triangular(L, X.T, lower=True).T
+
+
+def rayleigh_ritz(op, X: np.ndarray, block: int):
+    """Hp = X^T H~ X (blocked HX), symmetrised; Hp = Q diag(l) Q^T; X <- X Q.
+    Reference: src/linAlg/rayleighRitzDevice.cc:80-353 (SHEP) and the XtHX
+    blocks of kohnShamDFTOperatorDevice.cc:4000-4157."""
+    N = X.shape[1]
+    Hp = np.zeros((N, N))
+    for j in range(0, N, block):
+        HXb = op.apply(X[:, j:j + block])
+        Hp[:, j:j + block] = X.T @ HXb
+    Hp = 0.5 * (Hp + Hp.T)
+    lam, Q = np.linalg.eigh(Hp)
+    return lam, X @ Q
+
+
+def rayleigh_ritz_gep(op, X: np.ndarray, block: int):
+    """CPU-default generalized route: S = L L^T, Hhat = L^{-1} Hp L^{-T}, eig,
+    X <- X L^{-T} Q.  Reference: src/linAlg/linearAlgebraOperationsOpt.cc:613-969."""
+    N = X.shape[1]
+    S = X.T @ X
+    Hp = np.zeros((N, N))
+    for j in range(0, N, block):
+        Hp[:, j:j + block] = X.T @ op.apply(X[:, j:j + block])
+    Hp = 0.5 * (Hp + Hp.T)
+    L = np.linalg.cholesky(S)
+    Linv = sla.solve_triangular(L, np.eye(N), lower=True)
+    lam, Q = np.linalg.eigh(Linv @ Hp @ Linv.T)
+    return lam, X @ (Linv.T @ Q)
+
+
+def residual_norms(op, X: np.ndarray, lam: np.ndarray, block: int) -> np.ndarray:
+    """r_j = || H~ x_j - lambda_j x_j ||_2.  Reference:
+    src/linAlg/linearAlgebraOperationsOpt.cc:2394-2503; device :4610-4766."""
+    N = X.shape[1]
+    r = np.zeros(N)
+    for j in range(0, N, block):
+        R = op.apply(X[:, j:j + block]) - X[:, j:j + block] * lam[None, j:j + block]
+        r[j:j + block] = np.sqrt((R * R).sum(axis=0))
+    return r
+
+
+# ----------------------------------------------------------------------------
+# One ChFSI pass
+# ----------------------------------------------------------------------------
+
+
+def chfsi_pass(op, X: np.ndarray, a0: float, bLow: float, bUp: float, m: int, block: int,
+               route: str = "shep"):
+    """One pass in the physical FE basis.  Reference sequence:
+    src/solvers/eigenSolvers/chebyshevOrthogonalizedSubspaceIterationSolverDevice.cc:155-736
+    (M^{1/2} scale :358-363, block filter :376-526, CholGS+RR :609-643,
+    residual :668-713, M^{-1/2} unscale :719-733).
+
+    Returns (eigenvalues, residual norms, X)."""
+    p = op.prob
+    Xt = op.sq[:, None] * X
+    N = X.shape[1]
+
+    def hx(src, scale_flag, scalar, dst):
+        return op.HX(src, scale_flag, scalar, dst)
+
+    for j in range(0, N, block):
+        Xt[:, j:j + block] = chebyshev_filter(hx, Xt[:, j:j + block], m, bLow, bUp, a0)
+    if route == "shep":
+        Xt = cholgs(Xt)
+        lam, Xt = rayleigh_ritz(op, Xt, block)
+    else:
+        lam, Xt = rayleigh_ritz_gep(op, Xt, block)
+    res = residual_norms(op, Xt, lam, block)
+    return lam, res, op.s[:, None] * Xt
+
+
+def next_bounds(lam: np.ndarray):
+    """After a pass: a0 = lambda_0, bLow = lambda_{N-1} (src/dft/kohnShamEigenSolve.cc:489-498)."""
+    return float(lam[0]), float(lam[-1])
+
+
+# ----------------------------------------------------------------------------
+# Density and occupations
+# ----------------------------------------------------------------------------
+
+
+def fermi_dirac(eps: np.ndarray, mu: float, kT: float) -> np.ndarray:
+    """Stable Fermi-Dirac occupation (utils/dftUtils.cc:56-64)."""
+    t = (eps - mu) / kT
+    return np.where(t <= 0, 1.0 / (1.0 + np.exp(np.minimum(t, 0.0))),
+                    np.exp(-np.maximum(t, 0.0)) / (1.0 + np.exp(-np.maximum(t, 0.0))))
+
+
+def density_at_quadrature(prob, X: np.ndarray, occ: np.ndarray, spin_factor: float = 2.0,
+                          kweight: float = 1.0) -> np.ndarray:
+    """rho(q) = sum_i f_i w_k s |psi_i(q)|^2, psi interpolated from the nodes.
+    Reference: src/dft/densityCalculator.cc:212-254, :622-689.  Shape (C, nqc)."""
+    rho = np.zeros((prob.C, prob.nqc))
+    w = spin_factor * kweight * occ
+    for c0 in range(0, prob.C, 256):
+        c1 = min(prob.C, c0 + 256)
+        Xc = X[prob.cell_map[c0:c1]]                      # (k, ne, N)
+        psi = np.einsum("qi,kiv->kqv", prob.Nq, Xc)
+        rho[c0:c1] = np.einsum("kqv,v->kq", psi * psi, w)
+    return rho
