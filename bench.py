@@ -1,0 +1,231 @@
+#!/usr/bin/env python
+"""ChFSI pass benchmark on one MI355X.
+
+Workload (default cube30_p6_n1024): periodic 30 bohr cube, 16^3 cells, FE
+degree p=6 (343x343 cell Hamiltonians, 884,736 DOFs), Gauss 8^3 quadrature,
+108 Gaussian wells with s+p projectors, 1024 FP64 wavefunctions in blocks of
+256, Chebyshev degree 30, Lanczos bounds from 10 steps on the first call.
+
+A step is one ChFSI pass (filter + CholGS + Rayleigh-Ritz + residuals).  The
+timed region runs K passes in the mixed-precision form (FP32 filter on MFMA,
+FP64 CholGS / RR / residuals) and then K passes of the plain FP64 form from
+the same start, with the reference's bound bookkeeping between passes
+(a0 = lambda_0, bLow = lambda_{N-1}, bUp refreshed by Lanczos).
+
+Prints one JSON line: value = mixed-precision pass throughput in algorithmic
+GFLOP/s (filter + dense flops of one pass / pass time), plus pass times, HX
+GFLOP/s, roofline of the fused HX and the CPU baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
+METRIC = "ChFSI pass time + HX GFLOP/s, mixed vs FP64 GPU at 1e-8 Ha eigval error"
+
+
+def hx_flops(prob, B):
+    """Algorithmic flops of one block HX: dense cell GEMMs + C^T X and C (V C^T X)."""
+    return 2.0 * prob.C * prob.ne ** 2 * B + 16.0 * len(prob.pair_cell) * prob.ne * B
+
+
+def hx_bytes(prob, B, s):
+    """Algorithmic HBM bytes of one fused block HX with element size s: read X,
+    read the cell matrices, read+write Y, read the index map."""
+    return prob.M * B * s + prob.C * prob.ne ** 2 * s + 2 * prob.M * B * s + prob.C * prob.ne * 4
+
+
+def pass_flops(prob, N, B, m, lanczos_steps):
+    """Algorithmic flops of one pass: filter, Lanczos, CholGS (syrk + trsm), RR
+    (N/B HX + lower X^T HX + rotation + eig), residual (N/B HX)."""
+    M = prob.M
+    nblk = -(-N // B)
+    f = m * nblk * hx_flops(prob, B)
+    f += lanczos_steps * hx_flops(prob, 1)
+    f += 2.0 * M * N * N                       # syrk (lower) + trsm
+    f += nblk * hx_flops(prob, B) + 1.0 * M * N * N + 2.0 * M * N * N + 9.0 * N ** 3
+    f += nblk * hx_flops(prob, B) + 3.0 * M * N
+    return f
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5, help="timed ChFSI passes per precision")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cube30_p6_n1024")
+    ap.add_argument("--hx-reps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 passes")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
+              "sharding is described in DESIGN.md and not built.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from dftfe_amd import fe
+    from dftfe_amd.chfsi import ChebyshevSolverDevice, KohnShamOperatorDevice
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    wl = fe.WORKLOADS[args.workload]
+    log = lambda *a: print("[bench]", *a, file=sys.stderr, flush=True)  # noqa: E731
+
+    t0 = time.time()
+    prob = fe.build_problem(wl)
+    op = KohnShamOperatorDevice(prob)
+    torch.cuda.synchronize()
+    log(f"setup {time.time() - t0:.1f}s: M={prob.M} cells={prob.C} ne={prob.ne} "
+        f"pairs={len(prob.pair_cell)} N={wl.N} B={wl.B} m={wl.m}")
+
+    N, B, m = wl.N, wl.B, wl.m
+    X0 = torch.tensor(prob.initial_guess(), device=dev)
+    v0 = torch.tensor(prob.lanczos_start, device=dev)
+
+    def run_passes(mixed: bool, npass: int):
+        solver = ChebyshevSolverDevice(cheby_block_size=B, cheby_degree=m,
+                                       lanczos_steps=wl.lanczos_steps)
+        solver.set_lanczos_start(v0)
+        X = X0.clone()
+        lams, ress, phases = [], [], []
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for k in range(npass):
+            lam, res, _ = solver.solve(op, X, is_first_filtering_call=(k == 0),
+                                       use_mixed_prec_overall=mixed)
+            a0, bLow = float(lam[0]), float(lam[-1])
+            _, _, bUp = solver.bounds()
+            solver.reinitSpectrumBounds(a0, bLow, bUp)
+            lams.append(lam)
+            ress.append(res)
+            phases.append(solver.last_timings_ms)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / npass * 1e3, lams, ress, phases
+
+    for _ in range(args.warmup):
+        run_passes(True, 1)
+        if not args.no_fp64:
+            run_passes(False, 1)
+    log("warmup done")
+
+    mixed_ms, lam_mx, res_mx, ph_mx = run_passes(True, args.steps)
+    log(f"mixed: {mixed_ms:.1f} ms/pass; last max residual {res_mx[-1].max():.3e}")
+    fp64_ms, lam_64, res_64, ph_64 = (None, None, None, None)
+    if not args.no_fp64:
+        fp64_ms, lam_64, res_64, ph_64 = run_passes(False, args.steps)
+        log(f"fp64: {fp64_ms:.1f} ms/pass; last max residual {res_64[-1].max():.3e}")
+    del X0
+
+    # ---- fused HX timing with HIP events on the stream the library launches on
+    def time_hx(dtype):
+        Xb = torch.randn(prob.M, B, device=dev, dtype=dtype)
+        Yb = torch.zeros_like(Xb)
+        for _ in range(3):
+            op.HX(Xb, False, 1.0, Yb)
+        st = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(args.hx_reps):
+            op.HX(Xb, True, 1.0, Yb)
+        e1.record(st)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / args.hx_reps
+
+    hx32_ms = time_hx(torch.float32)
+    hx64_ms = time_hx(torch.float64)
+    F = hx_flops(prob, B)
+    hx32_gf = F / hx32_ms / 1e6
+    hx64_gf = F / hx64_ms / 1e6
+    log(f"HX B={B}: fp32 {hx32_ms:.3f} ms = {hx32_gf:.0f} GFLOP/s; "
+        f"fp64 {hx64_ms:.3f} ms = {hx64_gf:.0f} GFLOP/s")
+
+    PF = pass_flops(prob, N, B, m, wl.lanczos_steps)
+    value = PF / mixed_ms / 1e6
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed_ms, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 filter / fp64 CholGS,RR,residual",
+        "data": "synthetic (seeded FE problem and start vectors)",
+        "config": {"workload": args.workload, "dofs": prob.M, "cells": prob.C,
+                   "fe_degree": wl.p, "wavefunctions": N, "block": B, "cheby_degree": m,
+                   "atoms": wl.n_atoms, "passes": args.steps, "parallelism": "single GPU"},
+        "mixed_pass_ms": round(mixed_ms, 2),
+        "fp64_pass_ms": round(fp64_ms, 2) if fp64_ms else None,
+        "mixed_over_fp64_speedup": round(fp64_ms / mixed_ms, 3) if fp64_ms else None,
+        "hx_fp32_ms": round(hx32_ms, 4),
+        "hx_fp32_gflops": round(hx32_gf, 1),
+        "hx_fp64_ms": round(hx64_ms, 4),
+        "hx_fp64_gflops": round(hx64_gf, 1),
+        "pass_gflop": round(PF / 1e9, 1),
+        "phase_ms_mixed_last": ph_mx[-1],
+        "phase_ms_fp64_last": ph_64[-1] if ph_64 else None,
+        "max_residual_mixed": [float(r.max()) for r in res_mx],
+    }
+    if lam_64 is not None:
+        out["max_eig_err_mixed_vs_fp64_Ha"] = [float(np.abs(a - b).max())
+                                              for a, b in zip(lam_mx, lam_64)]
+        out["max_residual_fp64"] = [float(r.max()) for r in res_64]
+    out["roofline"] = {
+        "kernel": "fused HX (8 colour launches of hx_kernel<float> + projection)",
+        "bound": "mfma",
+        "achieved": round(F / hx32_ms / 1e9, 2),
+        "peak": PEAK_TFLOPS["fp32"],
+        "unit": "TFLOP/s",
+        "frac": round(F / hx32_ms / 1e9 / PEAK_TFLOPS["fp32"], 4),
+        "traffic": None,
+        "algorithmic_bytes": hx_bytes(prob, B, 4),
+        "fp64": {"achieved": round(F / hx64_ms / 1e9, 2), "peak": PEAK_TFLOPS["fp64"],
+                 "frac": round(F / hx64_ms / 1e9 / PEAK_TFLOPS["fp64"], 4)},
+    }
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(prob, B, log)
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(prob, B, log):
+    """The FP64 oracle's HX (numpy / OpenBLAS, per-cell batched GEMM) on the
+    same workload, one B-column block, timed on this host's cores."""
+    import threadpoolctl
+    from oracle import chfsi_oracle as O
+
+    t = time.time()
+    ref = O.OracleOperator(prob)
+    log(f"cpu oracle setup {time.time() - t:.1f}s")
+    X = prob.initial_guess(N=B, seed_offset=99)
+    reps = 2
+    t = time.perf_counter()
+    for _ in range(reps):
+        ref.h_fe(X)
+    dt = (time.perf_counter() - t) / reps
+    threads = max((i.get("num_threads", 1) for i in threadpoolctl.threadpool_info()), default=1)
+    gf = hx_flops(prob, B) / dt / 1e9
+    log(f"cpu HX {dt:.2f}s = {gf:.1f} GFLOP/s on {threads} threads")
+    return {"value": round(gf, 2), "unit": "GFLOP/s (HX)", "cores": threads, "kind": "port",
+            "hx_seconds": round(dt, 3),
+            "sample": f"{reps} full HX applications (B={B}) of the same workload through the "
+                      "numpy/OpenBLAS FP64 oracle (per-cell batched GEMM, gather/scatter, "
+                      "projectors)"}
+
+
+if __name__ == "__main__":
+    main()

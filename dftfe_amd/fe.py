@@ -283,18 +283,18 @@ def build_problem(wl: Workload, mode: str = "unique") -> FEProblem:
     )
 
     # ---- local potential V(x) = -sum_a D_a exp(-|x-R_a|^2 / w_a^2)
-    V = np.zeros((C, nqc))
-    chunk = max(1, (1 << 22) // nqc)
-    for c0 in range(0, C, chunk):
-        cells = np.arange(c0, min(C, c0 + chunk))
-        xq = prob.quad_points(cells)
-        acc = np.zeros((len(cells), nqc))
-        for a in range(wl.n_atoms):
-            d = _min_image(xq - atom_pos[a], L)
-            r2 = np.einsum("cqk,cqk->cq", d, d)
-            acc -= atom_depth[a] * np.exp(-r2 / atom_width[a] ** 2)
-        V[c0:c0 + len(cells)] = acc
-    prob.V_q = V
+    # The quadrature points form a tensor grid (n*nq values per axis) and the
+    # minimum-image Gaussian factorises over the axes, so each well is an
+    # outer product of three 1-D factors.
+    axis = (np.arange(n)[:, None] * h + gx[None, :] * h).ravel()        # (n*nq,)
+    Vg = np.zeros((n * nq, n * nq, n * nq))                              # [z, y, x]
+    for a in range(wl.n_atoms):
+        f = [np.exp(-_min_image(axis - atom_pos[a, k], L) ** 2 / atom_width[a] ** 2)
+             for k in range(3)]
+        Vg -= atom_depth[a] * (f[2][:, None, None] * f[1][None, :, None] * f[0][None, None, :])
+    # [cz, qz, cy, qy, cx, qx] -> [cz, cy, cx, qz, qy, qx] == (cell, q)
+    prob.V_q = np.ascontiguousarray(
+        Vg.reshape(n, nq, n, nq, n, nq).transpose(0, 2, 4, 1, 3, 5)).reshape(C, nqc)
 
     # ---- lumped mass on GLL points: m_i(cell) = h^3 w_a w_b w_c
     m_loc = h ** 3 * gll_w[la] * gll_w[lb] * gll_w[lc]
@@ -324,7 +324,8 @@ def build_problem(wl: Workload, mode: str = "unique") -> FEProblem:
         r2 = (d ** 2).sum(-1)
         g = np.exp(-r2 / (2.0 * rc * rc)) * (r2 <= rcut * rcut)
         beta = np.stack([g, d[..., 0] * g, d[..., 1] * g, d[..., 2] * g], axis=-1)  # (s, nqc, 4)
-        Ce = np.einsum("qi,sqa->sia", Nq * JxW[:, None], beta)
+        B2 = beta.transpose(1, 0, 2).reshape(nqc, -1)                   # (nqc, s*4)
+        Ce = ((Nq * JxW[:, None]).T @ B2).reshape(ne, len(sel), 4).transpose(1, 0, 2)
         keep = np.abs(Ce).reshape(len(sel), -1).max(axis=1) > 0.0
         pair_atom.append(np.full(int(keep.sum()), a, dtype=np.int64))
         pair_cell.append(sel[keep].astype(np.int64))

@@ -87,11 +87,30 @@ class OracleOperator:
         rows = np.arange(C * ne)
         # gather matrix A: (C*ne) x M with A[c*ne+i, map[c,i]] = 1
         self.A = sp.csr_matrix((np.ones(C * ne), (rows, prob.cell_map.ravel())), shape=(C * ne, M))
-        self.At = self.A.T.tocsr()
+        for i in range(ne):
+            if len(np.unique(prob.cell_map[:, i])) != C:
+                raise ValueError("local node index maps two cells to one DOF")
         self.s = prob.inv_sqrt_mass
         self.sq = prob.sqrt_mass
         # nonlocal: pair p -> projector ids 4*atom + alpha
         self.pair_proj = (4 * prob.pair_atom[:, None] + np.arange(4)[None, :])
+        # projector columns grouped per cell: Cc[c, i, s], slot s -> projector
+        self.Cc = None
+        if len(prob.pair_cell):
+            cnt = np.bincount(prob.pair_cell, minlength=C)
+            S = 4 * int(cnt.max())
+            self.Cc = np.zeros((C, ne, S))
+            self.slot_proj = np.full((C, S), prob.n_proj, dtype=np.int64)
+            fill = np.zeros(C, dtype=np.int64)
+            for q, c in enumerate(prob.pair_cell):
+                s0 = 4 * fill[c]
+                self.Cc[c, :, s0:s0 + 4] = prob.pair_C[q]
+                self.slot_proj[c, s0:s0 + 4] = self.pair_proj[q]
+                fill[c] += 1
+            valid = self.slot_proj.ravel() < prob.n_proj
+            cols = np.nonzero(valid)[0]
+            self.R = sp.csr_matrix((np.ones(len(cols)), (self.slot_proj.ravel()[valid], cols)),
+                                   shape=(prob.n_proj, C * S))
 
     # -- constraints (utils/constraintMatrixInfo.cc; device
     #    utils/constraintMatrixInfoDevice.cc:544-588 distribute, :596-816 slave->master,
@@ -116,20 +135,26 @@ class OracleOperator:
         """Unscaled FE H (local + nonlocal) on an already-distributed block."""
         p = self.prob
         B = Xs.shape[1]
-        Xc = (self.A @ Xs).reshape(p.C, p.ne, B)
+        Xc = Xs[p.cell_map]                                          # (C, ne, B) gather
         Yc = np.empty_like(Xc)
         for c0 in range(0, p.C, self.cell_chunk):
             c1 = min(p.C, c0 + self.cell_chunk)
             Yc[c0:c1] = np.matmul(self.H_e[c0:c1], Xc[c0:c1])
-        if len(p.pair_cell):
-            # C^T X per (atom, cell) pair, reduced per projector
-            part = np.einsum("pia,piv->pav", p.pair_C, Xc[p.pair_cell])
-            P = np.zeros((p.n_proj, B))
-            np.add.at(P, self.pair_proj.ravel(), part.reshape(-1, B))
-            W = p.proj_V[:, None] * P
-            contrib = np.einsum("pia,pav->piv", p.pair_C, W[self.pair_proj])
-            np.add.at(Yc, p.pair_cell, contrib)
-        return self.At @ Yc.reshape(p.C * p.ne, B)
+        if self.Cc is not None:
+            # P_alpha = sum over the atom's cells of C_e[:, alpha]^T X_e, then
+            # Y_e += C_e V P for every cell in the support.
+            S = self.Cc.shape[2]
+            Pc = np.matmul(self.Cc.transpose(0, 2, 1), Xc)           # (C, S, B)
+            W = p.proj_V[:, None] * (self.R @ Pc.reshape(p.C * S, B))
+            W = np.vstack([W, np.zeros((1, B))])                      # row for empty slots
+            Yc += np.matmul(self.Cc, W[self.slot_proj])               # (C, ne, B)
+        # scatter-add (matrixVectorProductImplementations.cc:127-168 adds each
+        # cell's rows into the global vector); looping over the local node index
+        # keeps each vectorised add free of repeated DOFs.
+        Y = np.zeros((p.M, B))
+        for i in range(p.ne):
+            Y[p.cell_map[:, i]] += Yc[:, i, :]
+        return Y
 
     def HX(self, src: np.ndarray, scale_flag: bool, scalar: float, dst: np.ndarray) -> np.ndarray:
         """dst <- (scale_flag ? dst : M^{-1/2} dst) + scalar * H~ src
