@@ -112,8 +112,9 @@ WORKLOADS = {
     # periodic 24 bohr cube, 12^3 cells, p=5, 32 wells, 256 wavefunctions
     "cube24_p5_n256": Workload("cube24_p5_n256", 24.0, 12, 5, 6, 32, 256, 256,
                                20, seed=1004),
-    # tiny case for brute-force checks
+    # tiny cases for brute-force checks (dense eigendecomposition)
     "tiny": Workload("tiny", 8.0, 2, 2, 3, 2, 8, 8, 8, seed=7),
+    "brute": Workload("brute", 10.0, 4, 3, 4, 3, 16, 16, 12, seed=11),
 }
 
 
