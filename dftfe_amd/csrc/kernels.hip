@@ -80,9 +80,15 @@ __global__ __launch_bounds__(512) void hx_kernel(HxLaunch<T> a) {
   static_assert(KPT >= 1, "column tile too narrow");
   static_assert(CBW >= 1 && TN % (16 * WC) == 0, "bad column split");
   constexpr int HCH = (ROWS_MAX * 4 + 511) / 512;
+  constexpr int NLMAX = 128;  // projector slots per cell
 
-  __shared__ __align__(16) T Hs[ROWS_MAX * RS];
-  __shared__ __align__(16) T Xs[TN * RS];
+  // double-buffered K slices of the cell matrix and of the gathered block
+  __shared__ __align__(16) T Hs[2][ROWS_MAX * RS];
+  __shared__ __align__(16) T Xs[2][TN * RS];
+  // per-cell gather tables: DOF and input scale of each local node, projector of each slot
+  __shared__ uint32_t smap[ROWS_MAX];
+  __shared__ T sscale[ROWS_MAX];
+  __shared__ int sproj[NLMAX];
 
   const int wid = blockIdx.x;
   const int local = wid >> 3;
@@ -100,20 +106,34 @@ __global__ __launch_bounds__(512) void hx_kernel(HxLaunch<T> a) {
   const int nrb = a.ne_pad >> 4;
   const int nkt = a.ne_pad >> 4;
 
-  int nl_begin = 0, nkt_nl = 0;
+  int nl_begin = 0, nslot = 0;
   if (a.nl_off) {
     nl_begin = a.nl_off[cell];
-    nkt_nl = (a.nl_off[cell + 1] - nl_begin) >> 4;
+    nslot = a.nl_off[cell + 1] - nl_begin;
   }
-  const int ntot = nkt + nkt_nl;
+  const int ntot = nkt + (nslot >> 4);
 
   const uint32_t *cmap = a.map + (int64_t)cell * a.ne;
   const T *Hcell = a.Ht + (int64_t)cell * a.ne_pad * a.ne_pad;
-  const T *NLcell = a.nl_C ? a.nl_C + (int64_t)nl_begin * a.ne_pad : nullptr;
+  const T *NLcell = a.nl_C ? a.nl_C + (int64_t)nl_begin * a.ne_pad : Hcell;
+
+  // gather tables (padding rows point at DOF 0 with scale 0)
+  for (int i = tid; i < ROWS_MAX; i += 512) {
+    uint32_t w = 0;
+    T sc = T(0);
+    if (i < a.ne) {
+      w = cmap[i];
+      sc = a.s_in ? a.s_in[w & 0x7fffffffu] : T(1);
+    }
+    smap[i] = w;
+    sscale[i] = sc;
+  }
+  for (int s = tid; s < NLMAX; s += 512) sproj[s] = (s < nslot) ? a.slot_proj[nl_begin + s] : -1;
 
   const int gv = tid % TN;
   const int gk = (tid / TN) * KPT;
   const bool vok = (v0 + gv) < a.B;
+  const int vcl = vok ? v0 + gv : 0;
   const int nchunk = a.ne_pad * 4;
 
   v4 acc[RBW][CBW];
@@ -125,64 +145,66 @@ __global__ __launch_bounds__(512) void hx_kernel(HxLaunch<T> a) {
   v4 hreg[HCH];
   T xreg[KPT];
 
+  // Branch-free global loads into registers: indices are clamped to valid
+  // addresses and out-of-range values are zeroed after the load, so all loads
+  // of a slice are in flight together and overlap the MFMAs of the previous one.
   auto load_tile = [&](int t) {
     const T *tile_src = (t < nkt) ? Hcell + (int64_t)t * a.ne_pad * 16
                                   : NLcell + (int64_t)(t - nkt) * a.ne_pad * 16;
 #pragma unroll
     for (int j = 0; j < HCH; ++j) {
-      const int chunk = tid + 512 * j;
-      if (chunk < nchunk) hreg[j] = reinterpret_cast<const v4 *>(tile_src)[chunk];
+      const int chunk = min(tid + 512 * j, nchunk - 1);
+      hreg[j] = reinterpret_cast<const v4 *>(tile_src)[chunk];
     }
     if (t < nkt) {
 #pragma unroll
       for (int kk = 0; kk < KPT; ++kk) {
         const int k = t * 16 + gk + kk;
-        T val = T(0);
-        if (k < a.ne && vok) {
-          const uint32_t g = cmap[k] & 0x7fffffffu;
-          const T sc = a.s_in ? a.s_in[g] : T(1);
-          val = sc * a.src[(int64_t)g * a.ld_src + v0 + gv];
-        }
-        xreg[kk] = val;
+        const uint32_t g = smap[k] & 0x7fffffffu;
+        const T x = a.src[(int64_t)g * a.ld_src + vcl];
+        xreg[kk] = vok ? sscale[k] * x : T(0);
       }
     } else {
 #pragma unroll
       for (int kk = 0; kk < KPT; ++kk) {
-        const int s = nl_begin + (t - nkt) * 16 + gk + kk;
-        const int pj = a.slot_proj[s];
-        xreg[kk] = (pj >= 0 && vok) ? a.W[(int64_t)pj * a.ldW + v0 + gv] : T(0);
+        const int pj = sproj[(t - nkt) * 16 + gk + kk];
+        const T x = a.W[(int64_t)(pj < 0 ? 0 : pj) * a.ldW + vcl];
+        xreg[kk] = (pj >= 0 && vok) ? x : T(0);
       }
     }
   };
-  auto store_tile = [&]() {
+  auto store_tile = [&](int buf) {
 #pragma unroll
     for (int j = 0; j < HCH; ++j) {
       const int chunk = tid + 512 * j;
       if (chunk < nchunk)
-        *reinterpret_cast<v4 *>(&Hs[(chunk >> 2) * RS + (chunk & 3) * 4]) = hreg[j];
+        *reinterpret_cast<v4 *>(&Hs[buf][(chunk >> 2) * RS + (chunk & 3) * 4]) = hreg[j];
     }
 #pragma unroll
-    for (int kk = 0; kk < KPT; ++kk) Xs[gv * RS + gk + kk] = xreg[kk];
+    for (int kk = 0; kk < KPT; ++kk) Xs[buf][gv * RS + gk + kk] = xreg[kk];
   };
 
-  if (ntot > 0) load_tile(0);
+  __syncthreads();  // gather tables ready
+  if (ntot > 0) {
+    load_tile(0);
+    store_tile(0);
+  }
+  __syncthreads();
   for (int t = 0; t < ntot; ++t) {
-    __syncthreads();
-    store_tile();
-    __syncthreads();
+    const int buf = t & 1;
     if (t + 1 < ntot) load_tile(t + 1);
     v4 b4[CBW];
 #pragma unroll
     for (int cb = 0; cb < CBW; ++cb) {
       const int col = (wc * CBW + cb) * 16 + (lane & 15);
-      b4[cb] = *reinterpret_cast<const v4 *>(&Xs[col * RS + (lane >> 4) * 4]);
+      b4[cb] = *reinterpret_cast<const v4 *>(&Xs[buf][col * RS + (lane >> 4) * 4]);
     }
 #pragma unroll
     for (int j = 0; j < RBW; ++j) {
       const int rb = wr + WR * j;
       if (rb < nrb) {
-        const v4 a4 =
-            *reinterpret_cast<const v4 *>(&Hs[(rb * 16 + (lane & 15)) * RS + (lane >> 4) * 4]);
+        const v4 a4 = *reinterpret_cast<const v4 *>(
+            &Hs[buf][(rb * 16 + (lane & 15)) * RS + (lane >> 4) * 4]);
 #pragma unroll
         for (int cb = 0; cb < CBW; ++cb) {
           acc[j][cb] = mfma16(a4[0], b4[cb][0], acc[j][cb]);
@@ -192,6 +214,8 @@ __global__ __launch_bounds__(512) void hx_kernel(HxLaunch<T> a) {
         }
       }
     }
+    if (t + 1 < ntot) store_tile(buf ^ 1);
+    __syncthreads();
   }
 
   // epilogue: scaled scatter with the first-touch combine
@@ -207,7 +231,7 @@ __global__ __launch_bounds__(512) void hx_kernel(HxLaunch<T> a) {
       for (int r = 0; r < 4; ++r) {
         const int row = rb * 16 + acc_row<T>(lane, r);
         if (row >= a.ne) continue;
-        const uint32_t w = cmap[row];
+        const uint32_t w = smap[row];
         const int64_t g = w & 0x7fffffffu;
         const T so = a.s_out ? a.s_out[g] : T(1);
         const int64_t addr = g * a.ld_dst + v;
@@ -498,6 +522,23 @@ __global__ __launch_bounds__(256) void dot_partial_kernel(const double *x, const
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+__global__ void sum_partials_kernel(const double *P, int nk, int m, int n, double *C,
+                                    int64_t ldc) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t mn = (int64_t)m * n;
+  if (idx >= mn) return;
+  double s = 0.0;
+  for (int k = 0; k < nk; ++k) s += P[k * mn + idx];
+  C[(idx / m) * ldc + idx % m] = s;
+}
+
+__global__ void zero_upper_kernel(double *A, int n, int64_t lda) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n * n) return;
+  const int i = (int)(idx % n), j = (int)(idx / n);
+  if (i < j) A[j * lda + i] = 0.0;
+}
+
 __global__ void distribute_kernel(const int64_t *rows, const int64_t *ptr, const int64_t *cols,
                                   const double *vals, int64_t nrows, double *X, int64_t ld,
                                   int B) {
@@ -683,6 +724,21 @@ hipError_t launch_dot_partial(const double *x, const double *y, int64_t n, doubl
                               int nchunks, hipStream_t st) {
   hipLaunchKernelGGL(dot_partial_kernel, dim3(nchunks), dim3(256), 0, st, x, y, n, partial,
                      nchunks);
+  return hipGetLastError();
+}
+
+hipError_t launch_sum_partials(const double *P, int nk, int m, int n, double *C, int64_t ldc,
+                               hipStream_t st) {
+  const int64_t mn = (int64_t)m * n;
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(grid_for(mn, 256, 1u << 30)), dim3(256), 0, st, P,
+                     nk, m, n, C, ldc);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_upper(double *A, int n, int64_t lda, hipStream_t st) {
+  const int64_t nn = (int64_t)n * n;
+  hipLaunchKernelGGL(zero_upper_kernel, dim3(grid_for(nn, 256, 1u << 30)), dim3(256), 0, st, A, n,
+                     lda);
   return hipGetLastError();
 }
 

@@ -100,6 +100,12 @@ hipError_t launch_residual_partial(const double *HX, int64_t ldH, const double *
 hipError_t launch_dot_partial(const double *x, const double *y, int64_t n, double *partial,
                               int nchunks, hipStream_t st);
 
+// C[i + j*ldc] = sum_k P[k*m*n + i + j*m]  (split-K reduction, column-major m x n)
+hipError_t launch_sum_partials(const double *P, int nk, int m, int n, double *C, int64_t ldc,
+                               hipStream_t st);
+// A[i + j*lda] = 0 for i < j (clear the strict upper triangle of an n x n matrix)
+hipError_t launch_zero_upper(double *A, int n, int64_t lda, hipStream_t st);
+
 // constraints: CSR over slave rows
 hipError_t launch_distribute(const int64_t *rows, const int64_t *ptr, const int64_t *cols,
                              const double *vals, int64_t nrows, double *X, int64_t ld, int B,

@@ -25,11 +25,11 @@ struct chfsi_solver {
   chfsi_params params;
   double a0 = 0, bLow = 0, bUp = 0;
   rocblas_handle blas = nullptr;
-  Scratch Y64, X32, Y32, HXb, Hp, rot, evals, info, part, lam, v0;
+  Scratch Y64, X32, Y32, HXb, Hp, rot, evals, info, part, lam, v0, split;
   bool have_v0 = false;
   ~chfsi_solver() {
     if (blas) rocblas_destroy_handle(blas);
-    for (Scratch *s : {&Y64, &X32, &Y32, &HXb, &Hp, &rot, &evals, &info, &part, &lam, &v0})
+    for (Scratch *s : {&Y64, &X32, &Y32, &HXb, &Hp, &rot, &evals, &info, &part, &lam, &v0, &split})
       s->release();
   }
 };
@@ -226,16 +226,61 @@ chfsi_status lanczos_impl(chfsi_solver *s, chfsi_operator *op, const double *v0,
   return CHFSI_OK;
 }
 
-// S = X^T X (lower), S = L L^T, X <- X L^{-T}.  Column-major view: Xc = X^T is
-// N x M with leading dimension N, so S = Xc Xc^T and Xc <- L^{-1} Xc.
+// C (m x n, column-major, ldc) = A B^T with A m x K (lda) and B n x K (ldb).
+// The contraction length K (the DOF count) is split into chunks computed as
+// one strided-batched GEMM, so that the small m x n output still spreads over
+// the whole GPU, then the partial products are summed.
+chfsi_status gemm_nt_splitk(chfsi_solver *s, const double *A, int64_t lda, const double *B,
+                            int64_t ldb, int m, int n, int64_t K, double *C, int64_t ldc,
+                            hipStream_t st) {
+  const int nk = (int)std::max<int64_t>(1, std::min<int64_t>(64, K / 8192));
+  const int64_t Kc = K / nk;
+  const int64_t rem = K - Kc * nk;
+  const int slots = nk + (rem ? 1 : 0);
+  CHFSI_TRY(s->split.ensure(sizeof(double) * slots * m * n));
+  double *P = sptr<double>(s->split);
+  const double one = 1.0, zero = 0.0;
+  CHFSI_RB(rocblas_dgemm_strided_batched(
+      s->blas, rocblas_operation_none, rocblas_operation_transpose, m, n, (rocblas_int)Kc, &one,
+      A, (rocblas_int)lda, Kc * lda, B, (rocblas_int)ldb, Kc * ldb, &zero, P, m, (int64_t)m * n,
+      nk));
+  if (rem)
+    CHFSI_RB(rocblas_dgemm(s->blas, rocblas_operation_none, rocblas_operation_transpose, m, n,
+                           (rocblas_int)rem, &one, A + nk * Kc * lda, (rocblas_int)lda,
+                           B + nk * Kc * ldb, (rocblas_int)ldb, &zero, P + (int64_t)nk * m * n,
+                           m));
+  CHFSI_HIP(launch_sum_partials(P, slots, m, n, C, ldc, st));
+  return CHFSI_OK;
+}
+
+// X <- X op(Q)^T... precisely Xc <- op(Q) Xc with Xc = X^T (N x M, ld N),
+// blocked over DOFs through a scratch panel (subspaceRotationScalapack,
+// linearAlgebraOperationsDevice.cc:1832-2242).
+chfsi_status rotate(chfsi_solver *s, double *X, int64_t M, int N, const double *Q,
+                    rocblas_operation opQ, hipStream_t st) {
+  const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(M, (int64_t(1) << 27) / N));
+  CHFSI_TRY(s->rot.ensure(sizeof(double) * chunk * N));
+  double *tmp = sptr<double>(s->rot);
+  const double one = 1.0, zero = 0.0;
+  for (int64_t n0 = 0; n0 < M; n0 += chunk) {
+    const int64_t nc = std::min(chunk, M - n0);
+    CHFSI_RB(rocblas_dgemm(s->blas, opQ, rocblas_operation_none, N, (rocblas_int)nc, N, &one, Q,
+                           N, X + n0 * N, N, &zero, tmp, N));
+    CHFSI_HIP(hipMemcpyAsync(X + n0 * N, tmp, sizeof(double) * nc * N, hipMemcpyDeviceToDevice,
+                             st));
+  }
+  return CHFSI_OK;
+}
+
+// S = X^T X, S = L L^T, X <- X L^{-T}.  Column-major view: Xc = X^T is N x M
+// with leading dimension N, so S = Xc Xc^T and Xc <- L^{-1} Xc, applied as a
+// GEMM rotation with the explicit triangular inverse.
 chfsi_status cholgs_impl(chfsi_solver *s, double *X, int64_t M, int N, hipStream_t st) {
   CHFSI_TRY(ensure_blas(s, st));
   CHFSI_TRY(s->Hp.ensure(sizeof(double) * N * N));
   CHFSI_TRY(s->info.ensure(sizeof(rocblas_int)));
   double *S = sptr<double>(s->Hp);
-  const double one = 1.0, zero = 0.0;
-  CHFSI_RB(rocblas_dsyrk(s->blas, rocblas_fill_lower, rocblas_operation_none, N, (rocblas_int)M,
-                         &one, X, N, &zero, S, N));
+  CHFSI_TRY(gemm_nt_splitk(s, X, N, X, N, N, N, M, S, N, st));
   rocblas_int *dinfo = sptr<rocblas_int>(s->info);
   CHFSI_RB(rocsolver_dpotrf(s->blas, rocblas_fill_lower, N, S, N, dinfo));
   rocblas_int hinfo = 0;
@@ -245,27 +290,10 @@ chfsi_status cholgs_impl(chfsi_solver *s, double *X, int64_t M, int N, hipStream
     set_error("overlap matrix is not positive definite (potrf info %d)", (int)hinfo);
     return CHFSI_ERR_NOT_SPD;
   }
-  CHFSI_RB(rocblas_dtrsm(s->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
-                         rocblas_diagonal_non_unit, N, (rocblas_int)M, &one, S, N, X, N));
-  return CHFSI_OK;
-}
-
-// X <- X Q, i.e. Xc <- Q^T Xc, blocked over DOFs through a scratch panel
-// (subspaceRotationScalapack, linearAlgebraOperationsDevice.cc:1832-2242).
-chfsi_status rotate(chfsi_solver *s, double *X, int64_t M, int N, const double *Q,
-                    hipStream_t st) {
-  const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(M, (int64_t(1) << 27) / N));
-  CHFSI_TRY(s->rot.ensure(sizeof(double) * chunk * N));
-  double *tmp = sptr<double>(s->rot);
-  const double one = 1.0, zero = 0.0;
-  for (int64_t n0 = 0; n0 < M; n0 += chunk) {
-    const int64_t nc = std::min(chunk, M - n0);
-    CHFSI_RB(rocblas_dgemm(s->blas, rocblas_operation_transpose, rocblas_operation_none, N,
-                           (rocblas_int)nc, N, &one, Q, N, X + n0 * N, N, &zero, tmp, N));
-    CHFSI_HIP(hipMemcpyAsync(X + n0 * N, tmp, sizeof(double) * nc * N, hipMemcpyDeviceToDevice,
-                             st));
-  }
-  return CHFSI_OK;
+  CHFSI_RB(rocsolver_dtrtri(s->blas, rocblas_fill_lower, rocblas_diagonal_non_unit, N, S, N,
+                            dinfo));
+  CHFSI_HIP(launch_zero_upper(S, N, N, st));
+  return rotate(s, X, M, N, S, rocblas_operation_none, st);
 }
 
 // Hp = X^T H~ X (lower triangle by column blocks), eig, X <- X Q.
@@ -283,9 +311,7 @@ chfsi_status rr_impl(chfsi_solver *s, chfsi_operator *op, double *X, int64_t M, 
     CHFSI_TRY(hx_fused<double>(op, X + j, N, HX, bw, bw, 1.0, 0.0, 0.0, 1, true, st));
     if (op->nc) CHFSI_HIP(launch_zero_rows(op->d_c_rows, op->nc, HX, bw, bw, st));
     // Hp[j:N, j:j+bw] = Xc[j:N, :] * HXc^T
-    CHFSI_RB(rocblas_dgemm(s->blas, rocblas_operation_none, rocblas_operation_transpose, N - j, bw,
-                           (rocblas_int)M, &one, X + j, N, HX, bw, &zero, Hp + j + (int64_t)j * N,
-                           N));
+    CHFSI_TRY(gemm_nt_splitk(s, X + j, N, HX, bw, N - j, bw, M, Hp + j + (int64_t)j * N, N, st));
   }
   double *D = sptr<double>(s->evals), *E = D + N;
   rocblas_int *dinfo = sptr<rocblas_int>(s->info);
@@ -299,7 +325,7 @@ chfsi_status rr_impl(chfsi_solver *s, chfsi_operator *op, double *X, int64_t M, 
     set_error("projected eigensolve failed (syevd info %d)", (int)hinfo);
     return CHFSI_ERR_EIG;
   }
-  return rotate(s, X, M, N, Hp, st);
+  return rotate(s, X, M, N, Hp, rocblas_operation_transpose, st);
 }
 
 chfsi_status residual_impl(chfsi_solver *s, chfsi_operator *op, const double *X, int64_t M, int N,
