@@ -68,13 +68,14 @@ constexpr int row_stride() {
 // row blocks rb = wr + WR*j.  MFMA operand mapping (16x16x4): lane l supplies
 // A[l%16][k] and B[k][l%16] with k = 4*(l/16) + r for the r-th of four MFMAs
 // over a 16-wide K slice; result lane l holds column l%16, rows acc_row().
-template <typename T, int TN, int WC, int RBW>
+template <typename T, int TN, int WC, int NRB>
 __global__ __launch_bounds__(512) void hx_kernel(HxLaunch<T> a) {
   using v4 = typename V4<T>::t;
   constexpr int NW = 8;
   constexpr int WR = NW / WC;
+  constexpr int RBW = (NRB + WR - 1) / WR;
   constexpr int RS = row_stride<T>();
-  constexpr int ROWS_MAX = RBW * WR * 16;
+  constexpr int ROWS_MAX = NRB * 16;
   constexpr int CBW = TN / 16 / WC;
   constexpr int KPT = kTile * TN / 512;
   static_assert(KPT >= 1, "column tile too narrow");
@@ -103,8 +104,8 @@ __global__ __launch_bounds__(512) void hx_kernel(HxLaunch<T> a) {
   const int wave = tid >> 6;
   const int wc = wave % WC;
   const int wr = wave / WC;
-  const int nrb = a.ne_pad >> 4;
-  const int nkt = a.ne_pad >> 4;
+  constexpr int nrb = NRB;
+  constexpr int nkt = NRB;
 
   int nl_begin = 0, nslot = 0;
   if (a.nl_off) {
@@ -134,7 +135,7 @@ __global__ __launch_bounds__(512) void hx_kernel(HxLaunch<T> a) {
   const int gk = (tid / TN) * KPT;
   const bool vok = (v0 + gv) < a.B;
   const int vcl = vok ? v0 + gv : 0;
-  const int nchunk = a.ne_pad * 4;
+  constexpr int nchunk = ROWS_MAX * 4;
 
   v4 acc[RBW][CBW];
 #pragma unroll
@@ -143,17 +144,19 @@ __global__ __launch_bounds__(512) void hx_kernel(HxLaunch<T> a) {
     for (int cb = 0; cb < CBW; ++cb) acc[j][cb] = v4{0, 0, 0, 0};
 
   v4 hreg[HCH];
-  T xreg[KPT];
+  T xreg[KPT], xscl[KPT];
 
   // Branch-free global loads into registers: indices are clamped to valid
-  // addresses and out-of-range values are zeroed after the load, so all loads
-  // of a slice are in flight together and overlap the MFMAs of the previous one.
+  // addresses and out-of-range values are multiplied by zero, so every load
+  // of a slice is unconditional, all are in flight together and they overlap
+  // the MFMAs of the previous slice.
+  const T vmask = vok ? T(1) : T(0);
   auto load_tile = [&](int t) {
-    const T *tile_src = (t < nkt) ? Hcell + (int64_t)t * a.ne_pad * 16
-                                  : NLcell + (int64_t)(t - nkt) * a.ne_pad * 16;
+    const T *tile_src = (t < nkt) ? Hcell + (int64_t)t * ROWS_MAX * 16
+                                  : NLcell + (int64_t)(t - nkt) * ROWS_MAX * 16;
 #pragma unroll
     for (int j = 0; j < HCH; ++j) {
-      const int chunk = min(tid + 512 * j, nchunk - 1);
+      const int chunk = (tid + 512 * j < nchunk) ? tid + 512 * j : nchunk - 1;
       hreg[j] = reinterpret_cast<const v4 *>(tile_src)[chunk];
     }
     if (t < nkt) {
@@ -161,15 +164,15 @@ __global__ __launch_bounds__(512) void hx_kernel(HxLaunch<T> a) {
       for (int kk = 0; kk < KPT; ++kk) {
         const int k = t * 16 + gk + kk;
         const uint32_t g = smap[k] & 0x7fffffffu;
-        const T x = a.src[(int64_t)g * a.ld_src + vcl];
-        xreg[kk] = vok ? sscale[k] * x : T(0);
+        xreg[kk] = a.src[(int64_t)g * a.ld_src + vcl];
+        xscl[kk] = vmask * sscale[k];
       }
     } else {
 #pragma unroll
       for (int kk = 0; kk < KPT; ++kk) {
         const int pj = sproj[(t - nkt) * 16 + gk + kk];
-        const T x = a.W[(int64_t)(pj < 0 ? 0 : pj) * a.ldW + vcl];
-        xreg[kk] = (pj >= 0 && vok) ? x : T(0);
+        xreg[kk] = a.W[(int64_t)(pj < 0 ? 0 : pj) * a.ldW + vcl];
+        xscl[kk] = pj >= 0 ? vmask : T(0);
       }
     }
   };
@@ -181,7 +184,7 @@ __global__ __launch_bounds__(512) void hx_kernel(HxLaunch<T> a) {
         *reinterpret_cast<v4 *>(&Hs[buf][(chunk >> 2) * RS + (chunk & 3) * 4]) = hreg[j];
     }
 #pragma unroll
-    for (int kk = 0; kk < KPT; ++kk) Xs[buf][gv * RS + gk + kk] = xreg[kk];
+    for (int kk = 0; kk < KPT; ++kk) Xs[buf][gv * RS + gk + kk] = xscl[kk] * xreg[kk];
   };
 
   __syncthreads();  // gather tables ready
@@ -199,20 +202,39 @@ __global__ __launch_bounds__(512) void hx_kernel(HxLaunch<T> a) {
       const int col = (wc * CBW + cb) * 16 + (lane & 15);
       b4[cb] = *reinterpret_cast<const v4 *>(&Xs[buf][col * RS + (lane >> 4) * 4]);
     }
+    // Row blocks in groups of G: the G accumulators are independent, so the
+    // MFMAs are issued r-major (no back-to-back dependence on one accumulator)
+    // while the next group's operands are read from LDS.
+    constexpr int G = RBW < 4 ? RBW : 4;
+    const T *hb = &Hs[buf][(lane & 15) * RS + (lane >> 4) * 4];
+    v4 a4[G], a4n[G];
 #pragma unroll
-    for (int j = 0; j < RBW; ++j) {
-      const int rb = wr + WR * j;
-      if (rb < nrb) {
-        const v4 a4 = *reinterpret_cast<const v4 *>(
-            &Hs[buf][(rb * 16 + (lane & 15)) * RS + (lane >> 4) * 4]);
+    for (int g = 0; g < G; ++g) {
+      const int rb = wr + WR * g;
+      a4[g] = *reinterpret_cast<const v4 *>(hb + (rb < nrb ? rb : 0) * 16 * RS);
+    }
 #pragma unroll
-        for (int cb = 0; cb < CBW; ++cb) {
-          acc[j][cb] = mfma16(a4[0], b4[cb][0], acc[j][cb]);
-          acc[j][cb] = mfma16(a4[1], b4[cb][1], acc[j][cb]);
-          acc[j][cb] = mfma16(a4[2], b4[cb][2], acc[j][cb]);
-          acc[j][cb] = mfma16(a4[3], b4[cb][3], acc[j][cb]);
+    for (int j0 = 0; j0 < RBW; j0 += G) {
+      if (j0 + G < RBW) {
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          const int jn = j0 + G + g;
+          const int rb = wr + WR * jn;
+          if (jn < RBW) a4n[g] = *reinterpret_cast<const v4 *>(hb + (rb < nrb ? rb : 0) * 16 * RS);
         }
       }
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          const int j = j0 + g;
+          if (j < RBW && wr + WR * j < nrb) {
+#pragma unroll
+            for (int cb = 0; cb < CBW; ++cb) acc[j][cb] = mfma16(a4[g][r], b4[cb][r], acc[j][cb]);
+          }
+        }
+#pragma unroll
+      for (int g = 0; g < G; ++g) a4[g] = a4n[g];
     }
     if (t + 1 < ntot) store_tile(buf ^ 1);
     __syncthreads();
@@ -580,13 +602,28 @@ static inline unsigned grid_for(int64_t n, int block, unsigned cap = 1u << 20) {
   return (unsigned)g;
 }
 
-template <typename T, int TN, int WC, int RBW>
+template <typename T, int TN, int WC, int NRB>
 static hipError_t hx_go(HxLaunch<T> a, hipStream_t s) {
   const int items = a.ncells * a.ntiles;
   if (items == 0) return hipSuccess;
   a.per_xcd = (items + 7) / 8;
-  hipLaunchKernelGGL((hx_kernel<T, TN, WC, RBW>), dim3(8 * a.per_xcd), dim3(512), 0, s, a);
+  hipLaunchKernelGGL((hx_kernel<T, TN, WC, NRB>), dim3(8 * a.per_xcd), dim3(512), 0, s, a);
   return hipGetLastError();
+}
+
+// Row-block count of the padded cell matrix is a compile-time constant so the
+// MFMA loop has no guards: (p+1)^3 padded to 16 for p = 1..6.
+template <typename T, int TN, int WC>
+static hipError_t hx_dispatch(const HxLaunch<T> &a, hipStream_t s) {
+  switch (a.ne_pad >> 4) {
+    case 1: return hx_go<T, TN, WC, 1>(a, s);
+    case 2: return hx_go<T, TN, WC, 2>(a, s);
+    case 4: return hx_go<T, TN, WC, 4>(a, s);
+    case 8: return hx_go<T, TN, WC, 8>(a, s);
+    case 14: return hx_go<T, TN, WC, 14>(a, s);
+    case 22: return hx_go<T, TN, WC, 22>(a, s);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 // Tile choices: FP32 uses 128-column tiles with one column block per wave
@@ -596,23 +633,17 @@ hipError_t launch_hx<float>(const HxLaunch<float> &a, hipStream_t s) {
   HxLaunch<float> b = a;
   if (a.B > 64) {
     b.ntiles = (a.B + 127) / 128;
-    if (a.ne_pad <= 128) return hx_go<float, 128, 8, 8>(b, s);
-    if (a.ne_pad <= 256) return hx_go<float, 128, 8, 16>(b, s);
-    return hx_go<float, 128, 8, 24>(b, s);
+    return hx_dispatch<float, 128, 8>(b, s);
   }
   b.ntiles = (a.B + 63) / 64;
-  if (a.ne_pad <= 128) return hx_go<float, 64, 4, 4>(b, s);
-  if (a.ne_pad <= 256) return hx_go<float, 64, 4, 8>(b, s);
-  return hx_go<float, 64, 4, 12>(b, s);
+  return hx_dispatch<float, 64, 4>(b, s);
 }
 
 template <>
 hipError_t launch_hx<double>(const HxLaunch<double> &a, hipStream_t s) {
   HxLaunch<double> b = a;
   b.ntiles = (a.B + 63) / 64;
-  if (a.ne_pad <= 128) return hx_go<double, 64, 4, 4>(b, s);
-  if (a.ne_pad <= 256) return hx_go<double, 64, 4, 8>(b, s);
-  return hx_go<double, 64, 4, 12>(b, s);
+  return hx_dispatch<double, 64, 4>(b, s);
 }
 
 template <>

@@ -94,7 +94,11 @@ extern "C" chfsi_status chfsi_operator_create(const chfsi_operator_desc *d, chfs
   CHFSI_ARG(d->cell_map && d->cell_color && d->sqrt_mass && d->inv_sqrt_mass, "missing arrays");
   const int C = d->n_cells, ne = d->nodes_per_cell;
   const int ne_pad = (ne + 15) / 16 * 16;
-  CHFSI_ARG(ne_pad <= 384, "nodes_per_cell %d exceeds the supported maximum (FE degree <= 6)", ne);
+  {
+    const int nrb = ne_pad / 16;
+    CHFSI_ARG(nrb == 1 || nrb == 2 || nrb == 4 || nrb == 8 || nrb == 14 || nrb == 22,
+              "nodes_per_cell %d is not (p+1)^3 for an FE degree p in 1..6", ne);
+  }
   const int64_t M = d->n_dofs;
 
   auto op = new chfsi_operator();
