@@ -89,6 +89,8 @@ def main():
         f"pairs={len(prob.pair_cell)} N={wl.N} B={wl.B} m={wl.m}")
 
     N, B, m = wl.N, wl.B, wl.m
+    tune = op.autotune(B)
+    log(f"autotune (B={B}): {tune}")
     X0 = torch.tensor(prob.initial_guess(), device=dev)
     v0 = torch.tensor(prob.lanczos_start, device=dev)
 
@@ -176,6 +178,7 @@ def main():
         "hx_fp64_ms": round(hx64_ms, 4),
         "hx_fp64_gflops": round(hx64_gf, 1),
         "pass_gflop": round(PF / 1e9, 1),
+        "hx_tile_autotune": tune,
         "phase_ms_mixed_last": ph_mx[-1],
         "phase_ms_fp64_last": ph_64[-1] if ph_64 else None,
         "max_residual_mixed": [float(r.max()) for r in res_mx],

@@ -100,6 +100,18 @@ class KohnShamOperatorDevice:
         L.check(self.lib.chfsi_operator_get_cell_matrices(self.handle, _ptr(H), _stream()))
         return H
 
+    def set_hx_variant(self, precision: int, variant: int) -> None:
+        L.check(self.lib.chfsi_operator_set_hx_variant(self.handle, precision, variant))
+
+    def autotune(self, B: int) -> dict:
+        """Time every HX tile variant on a B-column block and keep the fastest."""
+        b64, b32 = ct.c_int32(), ct.c_int32()
+        t = (ct.c_float * 8)()
+        L.check(self.lib.chfsi_operator_autotune(self.handle, B, ct.byref(b64), ct.byref(b32), t,
+                                                 _stream()))
+        return {"fp64": b64.value, "fp32": b32.value,
+                "fp64_ms": [x for x in t[0:4] if x >= 0], "fp32_ms": [x for x in t[4:8] if x >= 0]}
+
     def getSqrtMassVec(self) -> int:
         return self.lib.chfsi_operator_sqrt_mass(self.handle)
 

@@ -68,19 +68,19 @@ constexpr int row_stride() {
 // row blocks rb = wr + WR*j.  MFMA operand mapping (16x16x4): lane l supplies
 // A[l%16][k] and B[k][l%16] with k = 4*(l/16) + r for the r-th of four MFMAs
 // over a 16-wide K slice; result lane l holds column l%16, rows acc_row().
-template <typename T, int TN, int WC, int NRB>
-__global__ __launch_bounds__(512) void hx_kernel(HxLaunch<T> a) {
+template <typename T, int NW, int TN, int WC, int NRB>
+__global__ __launch_bounds__(64 * NW) void hx_kernel(HxLaunch<T> a) {
   using v4 = typename V4<T>::t;
-  constexpr int NW = 8;
+  constexpr int NT = 64 * NW;
   constexpr int WR = NW / WC;
   constexpr int RBW = (NRB + WR - 1) / WR;
   constexpr int RS = row_stride<T>();
   constexpr int ROWS_MAX = NRB * 16;
   constexpr int CBW = TN / 16 / WC;
-  constexpr int KPT = kTile * TN / 512;
+  constexpr int KPT = kTile * TN / NT;
   static_assert(KPT >= 1, "column tile too narrow");
   static_assert(CBW >= 1 && TN % (16 * WC) == 0, "bad column split");
-  constexpr int HCH = (ROWS_MAX * 4 + 511) / 512;
+  constexpr int HCH = (ROWS_MAX * 4 + NT - 1) / NT;
   constexpr int NLMAX = 128;  // projector slots per cell
 
   // double-buffered K slices of the cell matrix and of the gathered block
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(512) void hx_kernel(HxLaunch<T> a) {
   const T *NLcell = a.nl_C ? a.nl_C + (int64_t)nl_begin * a.ne_pad : Hcell;
 
   // gather tables (padding rows point at DOF 0 with scale 0)
-  for (int i = tid; i < ROWS_MAX; i += 512) {
+  for (int i = tid; i < ROWS_MAX; i += NT) {
     uint32_t w = 0;
     T sc = T(0);
     if (i < a.ne) {
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(512) void hx_kernel(HxLaunch<T> a) {
     smap[i] = w;
     sscale[i] = sc;
   }
-  for (int s = tid; s < NLMAX; s += 512) sproj[s] = (s < nslot) ? a.slot_proj[nl_begin + s] : -1;
+  for (int s = tid; s < NLMAX; s += NT) sproj[s] = (s < nslot) ? a.slot_proj[nl_begin + s] : -1;
 
   const int gv = tid % TN;
   const int gk = (tid / TN) * KPT;
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(512) void hx_kernel(HxLaunch<T> a) {
                                   : NLcell + (int64_t)(t - nkt) * ROWS_MAX * 16;
 #pragma unroll
     for (int j = 0; j < HCH; ++j) {
-      const int chunk = (tid + 512 * j < nchunk) ? tid + 512 * j : nchunk - 1;
+      const int chunk = (tid + NT * j < nchunk) ? tid + NT * j : nchunk - 1;
       hreg[j] = reinterpret_cast<const v4 *>(tile_src)[chunk];
     }
     if (t < nkt) {
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(512) void hx_kernel(HxLaunch<T> a) {
   auto store_tile = [&](int buf) {
 #pragma unroll
     for (int j = 0; j < HCH; ++j) {
-      const int chunk = tid + 512 * j;
+      const int chunk = tid + NT * j;
       if (chunk < nchunk)
         *reinterpret_cast<v4 *>(&Hs[buf][(chunk >> 2) * RS + (chunk & 3) * 4]) = hreg[j];
     }
@@ -602,48 +602,55 @@ static inline unsigned grid_for(int64_t n, int block, unsigned cap = 1u << 20) {
   return (unsigned)g;
 }
 
-template <typename T, int TN, int WC, int NRB>
+template <typename T, int NW, int TN, int WC, int NRB>
 static hipError_t hx_go(HxLaunch<T> a, hipStream_t s) {
+  a.ntiles = (a.B + TN - 1) / TN;
   const int items = a.ncells * a.ntiles;
   if (items == 0) return hipSuccess;
   a.per_xcd = (items + 7) / 8;
-  hipLaunchKernelGGL((hx_kernel<T, TN, WC, NRB>), dim3(8 * a.per_xcd), dim3(512), 0, s, a);
+  hipLaunchKernelGGL((hx_kernel<T, NW, TN, WC, NRB>), dim3(8 * a.per_xcd), dim3(64 * NW), 0, s,
+                     a);
   return hipGetLastError();
 }
 
 // Row-block count of the padded cell matrix is a compile-time constant so the
 // MFMA loop has no guards: (p+1)^3 padded to 16 for p = 1..6.
-template <typename T, int TN, int WC>
+template <typename T, int NW, int TN, int WC>
 static hipError_t hx_dispatch(const HxLaunch<T> &a, hipStream_t s) {
   switch (a.ne_pad >> 4) {
-    case 1: return hx_go<T, TN, WC, 1>(a, s);
-    case 2: return hx_go<T, TN, WC, 2>(a, s);
-    case 4: return hx_go<T, TN, WC, 4>(a, s);
-    case 8: return hx_go<T, TN, WC, 8>(a, s);
-    case 14: return hx_go<T, TN, WC, 14>(a, s);
-    case 22: return hx_go<T, TN, WC, 22>(a, s);
+    case 1: return hx_go<T, NW, TN, WC, 1>(a, s);
+    case 2: return hx_go<T, NW, TN, WC, 2>(a, s);
+    case 4: return hx_go<T, NW, TN, WC, 4>(a, s);
+    case 8: return hx_go<T, NW, TN, WC, 8>(a, s);
+    case 14: return hx_go<T, NW, TN, WC, 14>(a, s);
+    case 22: return hx_go<T, NW, TN, WC, 22>(a, s);
     default: return hipErrorInvalidValue;
   }
 }
 
-// Tile choices: FP32 uses 128-column tiles with one column block per wave
-// (8 x 16 = 128) and all row blocks; FP64 uses 64-column tiles, two row groups.
+// Tile variants (waves per workgroup x column tile x column waves):
+//   FP32  0: 8 x 128 x 8 (one workgroup per CU, all row blocks per wave)
+//         1: 4 x  64 x 4 (two independent workgroups per CU)
+//         2: 8 x  64 x 4 (two row groups)
+//   FP64  0: 8 x  64 x 4 (two row groups)
+//         1: 4 x  32 x 2
+int hx_variants(bool fp64) { return fp64 ? 2 : 3; }
+
 template <>
 hipError_t launch_hx<float>(const HxLaunch<float> &a, hipStream_t s) {
-  HxLaunch<float> b = a;
-  if (a.B > 64) {
-    b.ntiles = (a.B + 127) / 128;
-    return hx_dispatch<float, 128, 8>(b, s);
+  switch (a.variant) {
+    case 1: return hx_dispatch<float, 4, 64, 4>(a, s);
+    case 2: return hx_dispatch<float, 8, 64, 4>(a, s);
+    default: return hx_dispatch<float, 8, 128, 8>(a, s);
   }
-  b.ntiles = (a.B + 63) / 64;
-  return hx_dispatch<float, 64, 4>(b, s);
 }
 
 template <>
 hipError_t launch_hx<double>(const HxLaunch<double> &a, hipStream_t s) {
-  HxLaunch<double> b = a;
-  b.ntiles = (a.B + 63) / 64;
-  return hx_dispatch<double, 64, 4>(b, s);
+  switch (a.variant) {
+    case 1: return hx_dispatch<double, 4, 32, 2>(a, s);
+    default: return hx_dispatch<double, 8, 64, 4>(a, s);
+  }
 }
 
 template <>

@@ -44,6 +44,7 @@ struct HxLaunch {
   const int *slot_proj;    // slot -> projector (-1 pad)
   const T *W;              // n_proj x ldW
   int64_t ldW;
+  int variant;             // tile variant (see hx_variants)
 };
 
 // Arguments of the projection C_e^T (s_in * src)_e for all cells with slots.
@@ -66,6 +67,7 @@ struct ProjLaunch {
 
 template <typename T>
 hipError_t launch_hx(const HxLaunch<T> &a, hipStream_t s);
+int hx_variants(bool fp64);
 template <typename T>
 hipError_t launch_proj(const ProjLaunch<T> &a, hipStream_t s);
 template <typename T>

@@ -403,6 +403,7 @@ chfsi_status hx_fused(chfsi_operator *op, const T *src, int64_t ld_src, T *dst, 
   a.gamma = gamma;
   a.beta_mode = beta_mode;
   a.B = B;
+  a.variant = sizeof(T) == 8 ? op->hx_variant64 : op->hx_variant32;
   if (op->n_slots > 0) {
     a.nl_off = op->d_nl_off;
     a.nl_C = OpData<T>::nlC(op);
@@ -474,5 +475,76 @@ extern "C" chfsi_status chfsi_constraints_set_zero(chfsi_operator *op, double *X
                                                    int32_t B, void *stream) {
   CHFSI_ARG(op && X && ld >= B, "bad argument");
   CHFSI_HIP(launch_zero_rows(op->d_c_rows, op->nc, X, ld, B, as_stream(stream)));
+  return CHFSI_OK;
+}
+
+extern "C" chfsi_status chfsi_operator_set_hx_variant(chfsi_operator *op, int32_t precision,
+                                                     int32_t variant) {
+  CHFSI_ARG(op, "null operator");
+  CHFSI_ARG(precision == CHFSI_FP64 || precision == CHFSI_FP32, "bad precision");
+  CHFSI_ARG(variant >= 0 && variant < hx_variants(precision == CHFSI_FP64), "bad variant");
+  if (precision == CHFSI_FP64)
+    op->hx_variant64 = variant;
+  else
+    op->hx_variant32 = variant;
+  return CHFSI_OK;
+}
+
+namespace {
+template <typename T>
+chfsi_status time_variants(chfsi_operator *op, int B, int *best, float *times, hipStream_t st) {
+  const int nv = hx_variants(sizeof(T) == 8);
+  Scratch x, y;
+  CHFSI_TRY(x.ensure(sizeof(T) * op->M * B));
+  CHFSI_TRY(y.ensure(sizeof(T) * op->M * B));
+  CHFSI_HIP(hipMemsetAsync(x.ptr, 0, sizeof(T) * op->M * B, st));
+  CHFSI_HIP(hipMemsetAsync(y.ptr, 0, sizeof(T) * op->M * B, st));
+  hipEvent_t e0, e1;
+  CHFSI_HIP(hipEventCreate(&e0));
+  CHFSI_HIP(hipEventCreate(&e1));
+  int &var = sizeof(T) == 8 ? op->hx_variant64 : op->hx_variant32;
+  float best_t = 1e30f;
+  for (int v = 0; v < nv; ++v) {
+    var = v;
+    const T *xs = static_cast<const T *>(x.ptr);
+    T *ys = static_cast<T *>(y.ptr);
+    chfsi_status stt = hx_fused<T>(op, xs, B, ys, B, B, T(1), T(1), T(0), 0, true, st);
+    if (stt == CHFSI_OK) {
+      CHFSI_HIP(hipEventRecord(e0, st));
+      for (int r = 0; r < 3 && stt == CHFSI_OK; ++r)
+        stt = hx_fused<T>(op, xs, B, ys, B, B, T(1), T(1), T(0), 0, true, st);
+      CHFSI_HIP(hipEventRecord(e1, st));
+      CHFSI_HIP(hipEventSynchronize(e1));
+    }
+    float ms = 1e30f;
+    if (stt == CHFSI_OK) CHFSI_HIP(hipEventElapsedTime(&ms, e0, e1));
+    times[v] = stt == CHFSI_OK ? ms / 3 : -1.0f;
+    if (stt == CHFSI_OK && ms < best_t) {
+      best_t = ms;
+      *best = v;
+    }
+  }
+  var = *best;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  x.release();
+  y.release();
+  return CHFSI_OK;
+}
+}  // namespace
+
+extern "C" chfsi_status chfsi_operator_autotune(chfsi_operator *op, int32_t B, int32_t *best64,
+                                                int32_t *best32, float *times_ms, void *stream) {
+  CHFSI_ARG(op && B > 0, "bad argument");
+  hipStream_t st = as_stream(stream);
+  float t[8];
+  for (float &v : t) v = -1.0f;
+  int b64 = 0, b32 = 0;
+  CHFSI_TRY(time_variants<double>(op, B, &b64, t, st));
+  CHFSI_TRY(time_variants<float>(op, B, &b32, t + 4, st));
+  if (best64) *best64 = b64;
+  if (best32) *best32 = b32;
+  if (times_ms)
+    for (int i = 0; i < 8; ++i) times_ms[i] = t[i];
   return CHFSI_OK;
 }

@@ -64,6 +64,9 @@ SIGNATURES = {
     "chfsi_operator_set_potential": (ct.c_int, [_VP, _VP, _VP]),
     "chfsi_operator_set_cell_matrices": (ct.c_int, [_VP, _VP, _VP]),
     "chfsi_operator_get_cell_matrices": (ct.c_int, [_VP, _VP, _VP]),
+    "chfsi_operator_set_hx_variant": (ct.c_int, [_VP, ct.c_int32, ct.c_int32]),
+    "chfsi_operator_autotune": (ct.c_int, [_VP, ct.c_int32, c_int32_p, c_int32_p,
+                                           ct.POINTER(ct.c_float), _VP]),
     "chfsi_operator_sqrt_mass": (_VP, [_VP]),
     "chfsi_operator_inv_sqrt_mass": (_VP, [_VP]),
     "chfsi_operator_n_dofs": (ct.c_int64, [_VP]),

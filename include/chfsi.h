@@ -122,6 +122,15 @@ chfsi_status chfsi_operator_set_cell_matrices(chfsi_operator *op, const double *
 /* Copy the FP64 cell matrices out (device, C*n_e*n_e) -- for checks. */
 chfsi_status chfsi_operator_get_cell_matrices(chfsi_operator *op, double *H_dev, void *stream);
 
+/* Tile-shape selection of the fused HX kernel (no reference counterpart: the
+ * reference calls hipBLAS gemmStridedBatched).  chfsi_operator_autotune times
+ * every tile variant of both precisions on a B-column block, keeps the fastest
+ * and reports times_ms[0..3] (FP64 variants) and [4..7] (FP32 variants),
+ * -1 where a variant does not exist. */
+chfsi_status chfsi_operator_set_hx_variant(chfsi_operator *op, int32_t precision, int32_t variant);
+chfsi_status chfsi_operator_autotune(chfsi_operator *op, int32_t B, int32_t *best64,
+                                     int32_t *best32, float *times_ms, void *stream);
+
 /* getSqrtMassVec / getInvSqrtMassVec (operatorDevice.h:68-72): device pointers. */
 const double *chfsi_operator_sqrt_mass(const chfsi_operator *op);
 const double *chfsi_operator_inv_sqrt_mass(const chfsi_operator *op);

@@ -66,6 +66,23 @@ def test_hx_fp64_matches_oracle(small, B):
     assert _rel(Y, ref.apply(X)) < 1e-13
 
 
+@pytest.mark.parametrize("precision,variant", [(0, 0), (0, 1), (1, 0), (1, 1), (1, 2)])
+def test_hx_tile_variants(small, precision, variant):
+    """Every HX tile variant gives the oracle's H~X (FP64: 1e-13, FP32: 2e-6)."""
+    prob, ref, op = small
+    X = prob.initial_guess(N=72, seed_offset=21)
+    dt = torch.float64 if precision == 0 else torch.float32
+    Xd = torch.tensor(X, device="cuda", dtype=dt)
+    Yd = torch.zeros_like(Xd)
+    op.set_hx_variant(precision, variant)
+    try:
+        op.HX(Xd, False, 1.0, Yd)
+    finally:
+        op.set_hx_variant(precision, 0)
+    want = ref.apply(X if precision == 0 else X.astype(np.float32).astype(np.float64))
+    assert _rel(Yd.double().cpu().numpy(), want) < (1e-13 if precision == 0 else 2e-6)
+
+
 def test_hx_strided_block_and_flags(small):
     """HX on a column block of a wider array, with scale_flag / scalar semantics."""
     prob, ref, op = small
