@@ -205,7 +205,9 @@ __global__ __launch_bounds__(64 * NW) void hx_kernel(HxLaunch<T> a) {
     // Row blocks in groups of G: the G accumulators are independent, so the
     // MFMAs are issued r-major (no back-to-back dependence on one accumulator)
     // while the next group's operands are read from LDS.
-    constexpr int G = RBW < 4 ? RBW : 4;
+    // (FP64 MFMA takes twice as long per instruction, so two chains suffice.)
+    constexpr int GMAX = sizeof(T) == 8 ? 2 : 4;
+    constexpr int G = RBW < GMAX ? RBW : GMAX;
     const T *hb = &Hs[buf][(lane & 15) * RS + (lane >> 4) * 4];
     v4 a4[G], a4n[G];
 #pragma unroll
@@ -277,16 +279,20 @@ __global__ __launch_bounds__(64 * NW) void hx_kernel(HxLaunch<T> a) {
 // ---------------------------------------------------------------------------
 // Projection partials  P_slot[v] = sum_i C_e[i, slot] * s_in[g_i] src[g_i, v]
 // ---------------------------------------------------------------------------
-template <typename T, int TN, int MAXSRB>
-__global__ __launch_bounds__(512) void proj_kernel(ProjLaunch<T> a) {
+template <typename T, int NW, int TN, int MAXSRB, int NRB>
+__global__ __launch_bounds__(64 * NW) void proj_kernel(ProjLaunch<T> a) {
   using v4 = typename V4<T>::t;
+  constexpr int NT = 64 * NW;
   constexpr int RS = row_stride<T>();
-  constexpr int KPT = kTile * TN / 512;
-  constexpr int NCB = TN / 16;
-  constexpr int PPW = (MAXSRB * NCB + 7) / 8;
-  constexpr int ACH = (MAXSRB * 16 * 4 + 511) / 512;
-  __shared__ __align__(16) T As[MAXSRB * 16 * RS];
-  __shared__ __align__(16) T Xs[TN * RS];
+  constexpr int KPT = kTile * TN / NT;
+  constexpr int CBW = TN / 16 / NW;
+  constexpr int ROWS = NRB * 16;              // padded nodes per cell
+  constexpr int ACH = (MAXSRB * 16 * 4 + NT - 1) / NT;
+  static_assert(KPT >= 1 && CBW >= 1, "bad projection tile");
+  __shared__ __align__(16) T As[2][MAXSRB * 16 * RS];
+  __shared__ __align__(16) T Xs[2][TN * RS];
+  __shared__ uint32_t smap[ROWS];
+  __shared__ T sscale[ROWS];
 
   const int item = blockIdx.x;
   if (item >= a.ncells * a.ntiles) return;
@@ -298,77 +304,98 @@ __global__ __launch_bounds__(512) void proj_kernel(ProjLaunch<T> a) {
   const int s0 = a.nl_off[cell];
   const int nslot = a.nl_off[cell + 1] - s0;
   const int nsrb = nslot >> 4;
-  const int npairs = nsrb * NCB;
-  const int nkt = a.ne_pad >> 4;
+  const int nchunk = nslot * 4;
+  const T *Acell = a.nl_Ct + (int64_t)s0 * ROWS;
   const uint32_t *cmap = a.map + (int64_t)cell * a.ne;
-  const T *Acell = a.nl_Ct + (int64_t)s0 * a.ne_pad;
+
+  for (int i = tid; i < ROWS; i += NT) {
+    uint32_t w = 0;
+    T sc = T(0);
+    if (i < a.ne) {
+      w = cmap[i];
+      sc = a.s_in ? a.s_in[w & 0x7fffffffu] : T(1);
+    }
+    smap[i] = w;
+    sscale[i] = sc;
+  }
 
   const int gv = tid % TN;
   const int gk = (tid / TN) * KPT;
   const bool vok = (v0 + gv) < a.B;
-  const int nchunk = nslot * 4;
+  const int vcl = vok ? v0 + gv : 0;
+  const T vmask = vok ? T(1) : T(0);
 
-  v4 acc[PPW];
+  v4 acc[MAXSRB][CBW];
 #pragma unroll
-  for (int j = 0; j < PPW; ++j) acc[j] = v4{0, 0, 0, 0};
+  for (int j = 0; j < MAXSRB; ++j)
+#pragma unroll
+    for (int cb = 0; cb < CBW; ++cb) acc[j][cb] = v4{0, 0, 0, 0};
+  v4 areg[ACH];
+  T xreg[KPT], xscl[KPT];
 
-  for (int t = 0; t < nkt; ++t) {
+  auto load_tile = [&](int t) {
     const T *tile_src = Acell + (int64_t)t * nslot * 16;
-    v4 areg[ACH];
-    T xreg[KPT];
 #pragma unroll
     for (int j = 0; j < ACH; ++j) {
-      const int chunk = tid + 512 * j;
-      if (chunk < nchunk) areg[j] = reinterpret_cast<const v4 *>(tile_src)[chunk];
+      const int chunk = (tid + NT * j < nchunk) ? tid + NT * j : nchunk - 1;
+      areg[j] = reinterpret_cast<const v4 *>(tile_src)[chunk];
     }
 #pragma unroll
     for (int kk = 0; kk < KPT; ++kk) {
       const int k = t * 16 + gk + kk;
-      T val = T(0);
-      if (k < a.ne && vok) {
-        const uint32_t g = cmap[k] & 0x7fffffffu;
-        const T sc = a.s_in ? a.s_in[g] : T(1);
-        val = sc * a.src[(int64_t)g * a.ld_src + v0 + gv];
-      }
-      xreg[kk] = val;
+      xreg[kk] = a.src[(int64_t)(smap[k] & 0x7fffffffu) * a.ld_src + vcl];
+      xscl[kk] = vmask * sscale[k];
     }
-    __syncthreads();
+  };
+  auto store_tile = [&](int buf) {
 #pragma unroll
     for (int j = 0; j < ACH; ++j) {
-      const int chunk = tid + 512 * j;
+      const int chunk = tid + NT * j;
       if (chunk < nchunk)
-        *reinterpret_cast<v4 *>(&As[(chunk >> 2) * RS + (chunk & 3) * 4]) = areg[j];
+        *reinterpret_cast<v4 *>(&As[buf][(chunk >> 2) * RS + (chunk & 3) * 4]) = areg[j];
     }
 #pragma unroll
-    for (int kk = 0; kk < KPT; ++kk) Xs[gv * RS + gk + kk] = xreg[kk];
-    __syncthreads();
+    for (int kk = 0; kk < KPT; ++kk) Xs[buf][gv * RS + gk + kk] = xscl[kk] * xreg[kk];
+  };
+
+  __syncthreads();
+  load_tile(0);
+  store_tile(0);
+  __syncthreads();
+  for (int t = 0; t < NRB; ++t) {
+    const int buf = t & 1;
+    if (t + 1 < NRB) load_tile(t + 1);
+    v4 b4[CBW];
 #pragma unroll
-    for (int j = 0; j < PPW; ++j) {
-      const int p = wave + 8 * j;
-      if (p < npairs) {
-        const int srb = p / NCB, cb = p % NCB;
+    for (int cb = 0; cb < CBW; ++cb)
+      b4[cb] = *reinterpret_cast<const v4 *>(
+          &Xs[buf][((wave * CBW + cb) * 16 + (lane & 15)) * RS + (lane >> 4) * 4]);
+#pragma unroll
+    for (int j = 0; j < MAXSRB; ++j) {
+      if (j < nsrb) {
         const v4 a4 =
-            *reinterpret_cast<const v4 *>(&As[(srb * 16 + (lane & 15)) * RS + (lane >> 4) * 4]);
-        const v4 b4 =
-            *reinterpret_cast<const v4 *>(&Xs[(cb * 16 + (lane & 15)) * RS + (lane >> 4) * 4]);
-        acc[j] = mfma16(a4[0], b4[0], acc[j]);
-        acc[j] = mfma16(a4[1], b4[1], acc[j]);
-        acc[j] = mfma16(a4[2], b4[2], acc[j]);
-        acc[j] = mfma16(a4[3], b4[3], acc[j]);
+            *reinterpret_cast<const v4 *>(&As[buf][(j * 16 + (lane & 15)) * RS + (lane >> 4) * 4]);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int cb = 0; cb < CBW; ++cb) acc[j][cb] = mfma16(a4[r], b4[cb][r], acc[j][cb]);
       }
     }
+    if (t + 1 < NRB) store_tile(buf ^ 1);
+    __syncthreads();
   }
 #pragma unroll
-  for (int j = 0; j < PPW; ++j) {
-    const int p = wave + 8 * j;
-    if (p >= npairs) continue;
-    const int srb = p / NCB, cb = p % NCB;
-    const int v = v0 + cb * 16 + (lane & 15);
-    if (v >= a.B) continue;
+  for (int j = 0; j < MAXSRB; ++j) {
+    if (j >= nsrb) continue;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int srow = srb * 16 + acc_row<T>(lane, r);
-      a.partial[(int64_t)(s0 + srow) * a.ldP + v] = acc[j][r];
+    for (int cb = 0; cb < CBW; ++cb) {
+      const int v = v0 + (wave * CBW + cb) * 16 + (lane & 15);
+      if (v >= a.B) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int srow = j * 16 + acc_row<T>(lane, r);
+        a.partial[(int64_t)(s0 + srow) * a.ldP + v] = acc[j][cb][r];
+      }
     }
   }
 }
@@ -653,25 +680,30 @@ hipError_t launch_hx<double>(const HxLaunch<double> &a, hipStream_t s) {
   }
 }
 
-template <>
-hipError_t launch_proj<float>(const ProjLaunch<float> &a, hipStream_t s) {
-  ProjLaunch<float> b = a;
-  b.ntiles = (a.B + 63) / 64;
-  const int items = b.ncells * b.ntiles;
+template <typename T, int NRB>
+static hipError_t proj_go(ProjLaunch<T> a, hipStream_t s) {
+  constexpr int TN = sizeof(T) == 4 ? 128 : 64;
+  a.ntiles = (a.B + TN - 1) / TN;
+  const int items = a.ncells * a.ntiles;
   if (items == 0) return hipSuccess;
-  hipLaunchKernelGGL((proj_kernel<float, 64, 8>), dim3(items), dim3(512), 0, s, b);
+  hipLaunchKernelGGL((proj_kernel<T, 4, TN, 8, NRB>), dim3(items), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
-template <>
-hipError_t launch_proj<double>(const ProjLaunch<double> &a, hipStream_t s) {
-  ProjLaunch<double> b = a;
-  b.ntiles = (a.B + 63) / 64;
-  const int items = b.ncells * b.ntiles;
-  if (items == 0) return hipSuccess;
-  hipLaunchKernelGGL((proj_kernel<double, 64, 8>), dim3(items), dim3(512), 0, s, b);
-  return hipGetLastError();
+template <typename T>
+hipError_t launch_proj(const ProjLaunch<T> &a, hipStream_t s) {
+  switch (a.ne_pad >> 4) {
+    case 1: return proj_go<T, 1>(a, s);
+    case 2: return proj_go<T, 2>(a, s);
+    case 4: return proj_go<T, 4>(a, s);
+    case 8: return proj_go<T, 8>(a, s);
+    case 14: return proj_go<T, 14>(a, s);
+    case 22: return proj_go<T, 22>(a, s);
+    default: return hipErrorInvalidValue;
+  }
 }
+template hipError_t launch_proj<float>(const ProjLaunch<float> &, hipStream_t);
+template hipError_t launch_proj<double>(const ProjLaunch<double> &, hipStream_t);
 
 template <typename T>
 hipError_t launch_proj_reduce(const int *proj_ptr, const int *proj_slots, const double *proj_V,
