@@ -659,15 +659,20 @@ static hipError_t hx_dispatch(const HxLaunch<T> &a, hipStream_t s) {
 //   FP32  0: 8 x 128 x 8 (one workgroup per CU, all row blocks per wave)
 //         1: 4 x  64 x 4 (two independent workgroups per CU)
 //         2: 8 x  64 x 4 (two row groups)
+//         3: 16 x 128 x 8 (two row groups, one cell matrix read per 128 columns)
+//         4: 8 x  32 x 2 (four row groups)
 //   FP64  0: 8 x  64 x 4 (two row groups)
 //         1: 4 x  32 x 2
-int hx_variants(bool fp64) { return fp64 ? 2 : 3; }
+//         2: 16 x 64 x 4 (four row groups, 16 waves per CU)
+int hx_variants(bool fp64) { return fp64 ? 3 : 5; }
 
 template <>
 hipError_t launch_hx<float>(const HxLaunch<float> &a, hipStream_t s) {
   switch (a.variant) {
     case 1: return hx_dispatch<float, 4, 64, 4>(a, s);
     case 2: return hx_dispatch<float, 8, 64, 4>(a, s);
+    case 3: return hx_dispatch<float, 16, 128, 8>(a, s);
+    case 4: return hx_dispatch<float, 8, 32, 2>(a, s);
     default: return hx_dispatch<float, 8, 128, 8>(a, s);
   }
 }
@@ -676,6 +681,7 @@ template <>
 hipError_t launch_hx<double>(const HxLaunch<double> &a, hipStream_t s) {
   switch (a.variant) {
     case 1: return hx_dispatch<double, 4, 32, 2>(a, s);
+    case 2: return hx_dispatch<double, 16, 64, 4>(a, s);
     default: return hx_dispatch<double, 8, 64, 4>(a, s);
   }
 }

@@ -537,14 +537,14 @@ extern "C" chfsi_status chfsi_operator_autotune(chfsi_operator *op, int32_t B, i
                                                 int32_t *best32, float *times_ms, void *stream) {
   CHFSI_ARG(op && B > 0, "bad argument");
   hipStream_t st = as_stream(stream);
-  float t[8];
+  float t[16];
   for (float &v : t) v = -1.0f;
   int b64 = 0, b32 = 0;
   CHFSI_TRY(time_variants<double>(op, B, &b64, t, st));
-  CHFSI_TRY(time_variants<float>(op, B, &b32, t + 4, st));
+  CHFSI_TRY(time_variants<float>(op, B, &b32, t + 8, st));
   if (best64) *best64 = b64;
   if (best32) *best32 = b32;
   if (times_ms)
-    for (int i = 0; i < 8; ++i) times_ms[i] = t[i];
+    for (int i = 0; i < 16; ++i) times_ms[i] = t[i];
   return CHFSI_OK;
 }

@@ -125,7 +125,7 @@ chfsi_status chfsi_operator_get_cell_matrices(chfsi_operator *op, double *H_dev,
 /* Tile-shape selection of the fused HX kernel (no reference counterpart: the
  * reference calls hipBLAS gemmStridedBatched).  chfsi_operator_autotune times
  * every tile variant of both precisions on a B-column block, keeps the fastest
- * and reports times_ms[0..3] (FP64 variants) and [4..7] (FP32 variants),
+ * and reports times_ms[0..7] (FP64 variants) and [8..15] (FP32 variants),
  * -1 where a variant does not exist. */
 chfsi_status chfsi_operator_set_hx_variant(chfsi_operator *op, int32_t precision, int32_t variant);
 chfsi_status chfsi_operator_autotune(chfsi_operator *op, int32_t B, int32_t *best64,

@@ -66,7 +66,7 @@ def test_hx_fp64_matches_oracle(small, B):
     assert _rel(Y, ref.apply(X)) < 1e-13
 
 
-@pytest.mark.parametrize("precision,variant", [(0, 0), (0, 1), (1, 0), (1, 1), (1, 2)])
+@pytest.mark.parametrize("precision,variant", [(0, 0), (0, 1), (0, 2), (1, 0), (1, 1), (1, 2), (1, 3), (1, 4)])
 def test_hx_tile_variants(small, precision, variant):
     """Every HX tile variant gives the oracle's H~X (FP64: 1e-13, FP32: 2e-6)."""
     prob, ref, op = small
