@@ -73,6 +73,7 @@ class KohnShamOperatorDevice:
             d.constraint_ptr = arr("cp", np.arange(ns + 1), np.int64, ct.c_int64)
             d.constraint_cols = arr("cc", prob.master_rows, np.int64, ct.c_int64)
             d.constraint_vals = arr("cv", np.ones(ns), np.float64, ct.c_double)
+        d.jxw = arr("jxw", np.tile(prob.JxW, prob.C), np.float64, ct.c_double)
         h = ct.c_void_p()
         L.check(self.lib.chfsi_operator_create(ct.byref(d), ct.byref(h)))
         self.handle = h
@@ -119,10 +120,19 @@ class KohnShamOperatorDevice:
         return self.lib.chfsi_operator_inv_sqrt_mass(self.handle)
 
     # -- applies ------------------------------------------------------------
-    def HX(self, src: torch.Tensor, scale_flag: bool, scalar: float, dst: torch.Tensor) -> None:
-        """dst <- (scale_flag ? dst : M^{-1/2} dst) + scalar H~ src (in place on dst)."""
+    def HX(self, src: torch.Tensor, scale_flag: bool, scalar: float, dst: torch.Tensor,
+           bf16: bool = False) -> None:
+        """dst <- (scale_flag ? dst : M^{-1/2} dst) + scalar H~ src (in place on dst).
+        float64 blocks run the FP64 apply, float32 blocks the FP32 apply, or with
+        bf16=True the BF16-input / FP32-accumulate apply."""
         B = src.shape[1]
-        if src.dtype == torch.float64:
+        if bf16:
+            if src.dtype != torch.float32:
+                raise TypeError("the BF16 apply takes float32 blocks")
+            L.check(self.lib.chfsi_hx_bf16(self.handle, _ptr(src), src.stride(0), _ptr(dst),
+                                           dst.stride(0), B, int(scale_flag), float(scalar),
+                                           _stream()))
+        elif src.dtype == torch.float64:
             L.check(self.lib.chfsi_hx(self.handle, _ptr(src), src.stride(0), _ptr(dst),
                                       dst.stride(0), B, int(scale_flag), float(scalar), _stream()))
         elif src.dtype == torch.float32:
@@ -154,6 +164,40 @@ class KohnShamOperatorDevice:
     def set_zero(self, X: torch.Tensor) -> None:
         L.check(self.lib.chfsi_constraints_set_zero(self.handle, _ptr(X), X.stride(0),
                                                     X.shape[1], _stream()))
+
+
+    # -- density and derivatives ------------------------------------------------
+    def density(self, X: torch.Tensor, w: torch.Tensor) -> torch.Tensor:
+        """rho(c, q) = sum_v w_v psi_v(q)^2 for FE-basis wavefunctions X (M x N)."""
+        p = self.prob
+        rho = torch.empty((p.C, p.nqc), dtype=torch.float64, device=self.device)
+        w = w.to(device=self.device, dtype=torch.float64).contiguous()
+        L.check(self.lib.chfsi_density(self.handle, _ptr(X), X.stride(0), X.shape[1], _ptr(w),
+                                       _ptr(rho), 0, _stream()))
+        return rho
+
+    def hx_adjoint_v(self, Ybar: torch.Tensor, X: torch.Tensor, scalar: float = 1.0,
+                     g: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """dL/dV(c, q) for Y = scalar H~(V) X given Ybar = dL/dY (accumulates into g)."""
+        p = self.prob
+        acc = g is not None
+        if g is None:
+            g = torch.empty((p.C, p.nqc), dtype=torch.float64, device=self.device)
+        L.check(self.lib.chfsi_hx_adjoint_v(self.handle, _ptr(Ybar), Ybar.stride(0), _ptr(X),
+                                            X.stride(0), X.shape[1], float(scalar), _ptr(g),
+                                            int(acc), _stream()))
+        return g
+
+    def filter_backward(self, X0: torch.Tensor, Ybar: torch.Tensor, m: int, a: float, b: float,
+                        a0: float, checkpoint_every: int = 5):
+        """Reverse pass of the degree-m filter: returns (dL/dX0, dL/dV)."""
+        p = self.prob
+        Xbar = torch.empty_like(X0)
+        g = torch.zeros((p.C, p.nqc), dtype=torch.float64, device=self.device)
+        L.check(self.lib.chfsi_chebyshev_filter_backward(
+            self.handle, _ptr(X0), X0.stride(0), _ptr(Ybar), Ybar.stride(0), X0.shape[1], m, a, b,
+            a0, checkpoint_every, _ptr(Xbar), Xbar.stride(0), _ptr(g), _stream()))
+        return Xbar, g
 
 
 # -- linearAlgebraOperationsDevice mirrors ------------------------------------
@@ -203,7 +247,8 @@ class ChebyshevSolverDevice:
     def __init__(self, a0: float = 0.0, bLow: float = 0.0, bUp: float = 0.0, *,
                  cheby_block_size: int = 400, cheby_degree: int = 0, lanczos_steps: int = 20,
                  reproducible: bool = False, filter_precision: int = L.FP32,
-                 first_scf_degree_factor: float = 1.34, reuse_lanczos_upper: bool = False):
+                 first_scf_degree_factor: float = 1.34, reuse_lanczos_upper: bool = False,
+                 fp64_tail_degrees: int = 0):
         self.lib = L.load()
         p = L.Params()
         self.lib.chfsi_default_params(ct.byref(p))
@@ -214,6 +259,7 @@ class ChebyshevSolverDevice:
         p.filter_precision = filter_precision
         p.first_scf_degree_factor = first_scf_degree_factor
         p.reuse_lanczos_upper = int(reuse_lanczos_upper)
+        p.fp64_tail_degrees = fp64_tail_degrees
         self.params = p
         h = ct.c_void_p()
         L.check(self.lib.chfsi_solver_create(ct.byref(p), a0, bLow, bUp, ct.byref(h)))
@@ -224,6 +270,13 @@ class ChebyshevSolverDevice:
         if getattr(self, "handle", None) and self.handle.value:
             self.lib.chfsi_solver_destroy(self.handle)
             self.handle = None
+
+    def set_filter_precision(self, precision: int, fp64_tail_degrees: int = 0) -> None:
+        """Per-pass precision schedule: filter precision used when solve() is
+        called with use_mixed_prec_overall=True."""
+        self.params.filter_precision = precision
+        self.params.fp64_tail_degrees = fp64_tail_degrees
+        L.check(self.lib.chfsi_solver_set_params(self.handle, ct.byref(self.params)))
 
     def reinitSpectrumBounds(self, a0: float, bLow: float, bUp: float) -> None:
         L.check(self.lib.chfsi_solver_reinit_bounds(self.handle, a0, bLow, bUp))

@@ -277,6 +277,187 @@ __global__ __launch_bounds__(64 * NW) void hx_kernel(HxLaunch<T> a) {
 }
 
 // ---------------------------------------------------------------------------
+// BF16 form of the fused HX: cell matrices and gathered inputs in BF16, FP32
+// accumulation (v_mfma_f32_16x16x16_bf16: one MFMA per 16-wide K slice), FP32
+// multivectors.  Same work decomposition, colouring and epilogue as hx_kernel.
+// ---------------------------------------------------------------------------
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+  return (uint16_t)(u >> 16);
+}
+
+template <int NW, int TN, int WC, int NRB>
+__global__ __launch_bounds__(64 * NW) void hx_bf16_kernel(HxLaunch<float> a, const uint16_t *Ht16,
+                                                          const uint16_t *nlC16) {
+  constexpr int NT = 64 * NW;
+  constexpr int WR = NW / WC;
+  constexpr int RBW = (NRB + WR - 1) / WR;
+  constexpr int RS = 24;  // bf16 elements per LDS row (48 B: 16-B aligned, conflict-free 8-B reads)
+  constexpr int ROWS_MAX = NRB * 16;
+  constexpr int CBW = TN / 16 / WC;
+  constexpr int KPT = kTile * TN / NT;
+  static_assert(KPT >= 1 && CBW >= 1 && TN % (16 * WC) == 0, "bad tile");
+  constexpr int nchunk = ROWS_MAX * 2;  // 16-byte chunks of a K slice
+  constexpr int HCH = (nchunk + NT - 1) / NT;
+  constexpr int NLMAX = 128;
+
+  __shared__ __align__(16) uint16_t Hs[2][ROWS_MAX * RS];
+  __shared__ __align__(16) uint16_t Xs[2][TN * RS];
+  __shared__ uint32_t smap[ROWS_MAX];
+  __shared__ float sscale[ROWS_MAX];
+  __shared__ int sproj[NLMAX];
+
+  const int wid = blockIdx.x;
+  const int local = wid >> 3;
+  if (local >= a.per_xcd) return;
+  const int item = (wid & 7) * a.per_xcd + local;
+  if (item >= a.ncells * a.ntiles) return;
+  const int cell = a.cells[item / a.ntiles];
+  const int v0 = (item % a.ntiles) * TN;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wc = wave % WC;
+  const int wr = wave / WC;
+  constexpr int nkt = NRB;
+
+  int nl_begin = 0, nslot = 0;
+  if (a.nl_off) {
+    nl_begin = a.nl_off[cell];
+    nslot = a.nl_off[cell + 1] - nl_begin;
+  }
+  const int ntot = nkt + (nslot >> 4);
+  const uint32_t *cmap = a.map + (int64_t)cell * a.ne;
+  const uint16_t *Hcell = Ht16 + (int64_t)cell * ROWS_MAX * ROWS_MAX;
+  const uint16_t *NLcell = nlC16 ? nlC16 + (int64_t)nl_begin * ROWS_MAX : Hcell;
+
+  for (int i = tid; i < ROWS_MAX; i += NT) {
+    uint32_t w = 0;
+    float sc = 0.0f;
+    if (i < a.ne) {
+      w = cmap[i];
+      sc = a.s_in ? a.s_in[w & 0x7fffffffu] : 1.0f;
+    }
+    smap[i] = w;
+    sscale[i] = sc;
+  }
+  for (int s = tid; s < NLMAX; s += NT) sproj[s] = (s < nslot) ? a.slot_proj[nl_begin + s] : -1;
+
+  const int gv = tid % TN;
+  const int gk = (tid / TN) * KPT;
+  const bool vok = (v0 + gv) < a.B;
+  const int vcl = vok ? v0 + gv : 0;
+  const float vmask = vok ? 1.0f : 0.0f;
+
+  f32x4 acc[RBW][CBW];
+#pragma unroll
+  for (int j = 0; j < RBW; ++j)
+#pragma unroll
+    for (int cb = 0; cb < CBW; ++cb) acc[j][cb] = f32x4{0, 0, 0, 0};
+  uint4 hreg[HCH];
+  float xreg[KPT], xscl[KPT];
+
+  auto load_tile = [&](int t) {
+    const uint16_t *tile_src = (t < nkt) ? Hcell + (int64_t)t * ROWS_MAX * 16
+                                         : NLcell + (int64_t)(t - nkt) * ROWS_MAX * 16;
+#pragma unroll
+    for (int j = 0; j < HCH; ++j) {
+      const int chunk = (tid + NT * j < nchunk) ? tid + NT * j : nchunk - 1;
+      hreg[j] = reinterpret_cast<const uint4 *>(tile_src)[chunk];
+    }
+    if (t < nkt) {
+#pragma unroll
+      for (int kk = 0; kk < KPT; ++kk) {
+        const int k = t * 16 + gk + kk;
+        xreg[kk] = a.src[(int64_t)(smap[k] & 0x7fffffffu) * a.ld_src + vcl];
+        xscl[kk] = vmask * sscale[k];
+      }
+    } else {
+#pragma unroll
+      for (int kk = 0; kk < KPT; ++kk) {
+        const int pj = sproj[(t - nkt) * 16 + gk + kk];
+        xreg[kk] = a.W[(int64_t)(pj < 0 ? 0 : pj) * a.ldW + vcl];
+        xscl[kk] = pj >= 0 ? vmask : 0.0f;
+      }
+    }
+  };
+  auto store_tile = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < HCH; ++j) {
+      const int chunk = tid + NT * j;
+      if (chunk < nchunk)
+        *reinterpret_cast<uint4 *>(&Hs[buf][(chunk >> 1) * RS + (chunk & 1) * 8]) = hreg[j];
+    }
+#pragma unroll
+    for (int kk = 0; kk < KPT; ++kk) Xs[buf][gv * RS + gk + kk] = to_bf16(xscl[kk] * xreg[kk]);
+  };
+
+  __syncthreads();
+  if (ntot > 0) {
+    load_tile(0);
+    store_tile(0);
+  }
+  __syncthreads();
+  for (int t = 0; t < ntot; ++t) {
+    const int buf = t & 1;
+    if (t + 1 < ntot) load_tile(t + 1);
+    bf16x4 b4[CBW];
+#pragma unroll
+    for (int cb = 0; cb < CBW; ++cb)
+      b4[cb] = *reinterpret_cast<const bf16x4 *>(
+          &Xs[buf][((wc * CBW + cb) * 16 + (lane & 15)) * RS + (lane >> 4) * 4]);
+#pragma unroll
+    for (int j = 0; j < RBW; ++j) {
+      const int rb = wr + WR * j;
+      if (rb < NRB) {
+        const bf16x4 a4 = *reinterpret_cast<const bf16x4 *>(
+            &Hs[buf][(rb * 16 + (lane & 15)) * RS + (lane >> 4) * 4]);
+#pragma unroll
+        for (int cb = 0; cb < CBW; ++cb)
+          acc[j][cb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a4, b4[cb], acc[j][cb], 0, 0, 0);
+      }
+    }
+    if (t + 1 < ntot) store_tile(buf ^ 1);
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int j = 0; j < RBW; ++j) {
+    const int rb = wr + WR * j;
+    if (rb >= NRB) continue;
+#pragma unroll
+    for (int cb = 0; cb < CBW; ++cb) {
+      const int v = v0 + (wc * CBW + cb) * 16 + (lane & 15);
+      if (v >= a.B) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = rb * 16 + 4 * (lane >> 4) + r;
+        if (row >= a.ne) continue;
+        const uint32_t w = smap[row];
+        const int64_t g = w & 0x7fffffffu;
+        const float so = a.s_out ? a.s_out[g] : 1.0f;
+        const int64_t addr = g * a.ld_dst + v;
+        float base;
+        if (w >> 31) {
+          base = 0.0f;
+          if (a.beta_mode == 0)
+            base = a.beta * a.dst[addr];
+          else if (a.beta_mode == 2)
+            base = a.beta * so * a.dst[addr];
+          if (a.gamma != 0.0f) base += a.gamma * a.src[g * a.ld_src + v];
+        } else {
+          base = a.dst[addr];
+        }
+        a.dst[addr] = base + a.alpha * so * acc[j][cb][r];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Projection partials  P_slot[v] = sum_i C_e[i, slot] * s_in[g_i] src[g_i, v]
 // ---------------------------------------------------------------------------
 template <typename T, int NW, int TN, int MAXSRB, int NRB>
@@ -620,6 +801,108 @@ __global__ void zero_rows_kernel(const int64_t *rows, int64_t nrows, double *X, 
 }
 
 // ---------------------------------------------------------------------------
+// Products of two multivectors at the quadrature points
+//   g[c, q] (+)= JxW[q] * sum_v w_v (N(q) . s a_v|_c) (N(q) . s b_v|_c)
+// One workgroup per cell; the cell's nodal rows of a and b are interpolated to
+// its quadrature points in 16-column chunks (FP64 FMA, LDS-staged shape-value
+// slices), multiplied pointwise and summed over the columns in registers, so
+// each g[c, q] is written once (deterministic).
+//   a = b = X, w = spin * occupation        -> density rho(q) JxW (densityCalculator.cc:622-689)
+//   a = Ybar, b = X, w = scalar              -> dL/dV(q) of Y = scalar * S H(V) S X
+// ---------------------------------------------------------------------------
+template <int QPT>
+__global__ __launch_bounds__(256) void quad_product_kernel(
+    const uint32_t *map, int ne, int nqc, const double *Nq, const double *JxW, const double *a,
+    int64_t lda, const double *b, int64_t ldb, int B, const double *s_in, const double *w,
+    double wscale, double *g, int accumulate) {
+  constexpr int VC = 16;  // columns per chunk
+  constexpr int KT = 16;  // nodes per slice
+  __shared__ double Ns[256 * QPT][KT + 1];
+  __shared__ double As[KT][VC];
+  __shared__ double Bs[KT][VC];
+  __shared__ uint32_t smap[384];
+  __shared__ double sscale[384];
+  const int cell = blockIdx.x;
+  const int tid = threadIdx.x;
+  const uint32_t *cmap = map + (int64_t)cell * ne;
+  for (int i = tid; i < ne; i += 256) {
+    const uint32_t gg = cmap[i] & 0x7fffffffu;
+    smap[i] = gg;
+    sscale[i] = s_in ? s_in[gg] : 1.0;
+  }
+  __syncthreads();
+  double gacc[QPT];
+#pragma unroll
+  for (int qi = 0; qi < QPT; ++qi) gacc[qi] = 0.0;
+  for (int v0 = 0; v0 < B; v0 += VC) {
+    double acc_a[QPT][VC], acc_b[QPT][VC];
+#pragma unroll
+    for (int qi = 0; qi < QPT; ++qi)
+#pragma unroll
+      for (int vv = 0; vv < VC; ++vv) acc_a[qi][vv] = acc_b[qi][vv] = 0.0;
+    for (int j0 = 0; j0 < ne; j0 += KT) {
+      for (int e = tid; e < 256 * QPT * KT; e += 256) {
+        const int q = e / KT, jj = e % KT;
+        Ns[q][jj] = (q < nqc && j0 + jj < ne) ? Nq[(int64_t)q * ne + j0 + jj] : 0.0;
+      }
+      {
+        const int jj = tid / VC, vv = tid % VC;  // 256 threads = 16 x 16
+        const int j = j0 + jj, v = v0 + vv;
+        double xa = 0.0, xb = 0.0;
+        if (j < ne && v < B) {
+          const int64_t gg = smap[j];
+          xa = sscale[j] * a[gg * lda + v];
+          xb = sscale[j] * b[gg * ldb + v];
+        }
+        As[jj][vv] = xa;
+        Bs[jj][vv] = xb;
+      }
+      __syncthreads();
+#pragma unroll 4
+      for (int jj = 0; jj < KT; ++jj) {
+#pragma unroll
+        for (int qi = 0; qi < QPT; ++qi) {
+          const double n = Ns[tid + 256 * qi][jj];
+#pragma unroll
+          for (int vv = 0; vv < VC; ++vv) {
+            acc_a[qi][vv] += n * As[jj][vv];
+            acc_b[qi][vv] += n * Bs[jj][vv];
+          }
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int qi = 0; qi < QPT; ++qi)
+#pragma unroll
+      for (int vv = 0; vv < VC; ++vv)
+        if (v0 + vv < B) gacc[qi] += (w ? w[v0 + vv] : 1.0) * acc_a[qi][vv] * acc_b[qi][vv];
+  }
+#pragma unroll
+  for (int qi = 0; qi < QPT; ++qi) {
+    const int q = tid + 256 * qi;
+    if (q < nqc) {
+      const int64_t o = (int64_t)cell * nqc + q;
+      g[o] = (accumulate ? g[o] : 0.0) + (JxW ? JxW[o] : 1.0) * wscale * gacc[qi];
+    }
+  }
+}
+
+hipError_t launch_quad_product(const uint32_t *map, int C, int ne, int nqc, const double *Nq,
+                               const double *JxW, const double *a, int64_t lda, const double *b,
+                               int64_t ldb, int B, const double *s_in, const double *w, double wscale, double *g,
+                               int accumulate, hipStream_t s) {
+  if (ne > 384 || nqc > 512) return hipErrorInvalidValue;
+  if (nqc <= 256)
+    hipLaunchKernelGGL(quad_product_kernel<1>, dim3(C), dim3(256), 0, s, map, ne, nqc, Nq, JxW, a,
+                       lda, b, ldb, B, s_in, w, wscale, g, accumulate);
+  else
+    hipLaunchKernelGGL(quad_product_kernel<2>, dim3(C), dim3(256), 0, s, map, ne, nqc, Nq, JxW, a,
+                       lda, b, ldb, B, s_in, w, wscale, g, accumulate);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 static inline unsigned grid_for(int64_t n, int block, unsigned cap = 1u << 20) {
@@ -684,6 +967,44 @@ hipError_t launch_hx<double>(const HxLaunch<double> &a, hipStream_t s) {
     case 2: return hx_dispatch<double, 16, 64, 4>(a, s);
     default: return hx_dispatch<double, 8, 64, 4>(a, s);
   }
+}
+
+template <int NRB>
+static hipError_t hx_bf16_go(HxLaunch<float> a, const uint16_t *Ht16, const uint16_t *nlC16,
+                             hipStream_t s) {
+  constexpr int NW = 8, TN = 64, WC = 4;
+  a.ntiles = (a.B + TN - 1) / TN;
+  const int items = a.ncells * a.ntiles;
+  if (items == 0) return hipSuccess;
+  a.per_xcd = (items + 7) / 8;
+  hipLaunchKernelGGL((hx_bf16_kernel<NW, TN, WC, NRB>), dim3(8 * a.per_xcd), dim3(64 * NW), 0, s,
+                     a, Ht16, nlC16);
+  return hipGetLastError();
+}
+
+hipError_t launch_hx_bf16(const HxLaunch<float> &a, const uint16_t *Ht16, const uint16_t *nlC16,
+                          hipStream_t s) {
+  switch (a.ne_pad >> 4) {
+    case 1: return hx_bf16_go<1>(a, Ht16, nlC16, s);
+    case 2: return hx_bf16_go<2>(a, Ht16, nlC16, s);
+    case 4: return hx_bf16_go<4>(a, Ht16, nlC16, s);
+    case 8: return hx_bf16_go<8>(a, Ht16, nlC16, s);
+    case 14: return hx_bf16_go<14>(a, Ht16, nlC16, s);
+    case 22: return hx_bf16_go<22>(a, Ht16, nlC16, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+__global__ void f64_to_bf16_kernel(const double *x, int64_t n, uint16_t *y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = to_bf16((float)x[i]);
+}
+
+hipError_t launch_f64_to_bf16(const double *x, int64_t n, uint16_t *y, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(f64_to_bf16_kernel, dim3(grid_for(n, 256, 65536)), dim3(256), 0, s, x, n, y);
+  return hipGetLastError();
 }
 
 template <typename T, int NRB>

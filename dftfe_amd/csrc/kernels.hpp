@@ -68,6 +68,10 @@ struct ProjLaunch {
 template <typename T>
 hipError_t launch_hx(const HxLaunch<T> &a, hipStream_t s);
 int hx_variants(bool fp64);
+// BF16 cell matrices / inputs, FP32 accumulation and multivectors
+hipError_t launch_hx_bf16(const HxLaunch<float> &a, const uint16_t *Ht16, const uint16_t *nlC16,
+                          hipStream_t s);
+hipError_t launch_f64_to_bf16(const double *x, int64_t n, uint16_t *y, hipStream_t s);
 template <typename T>
 hipError_t launch_proj(const ProjLaunch<T> &a, hipStream_t s);
 template <typename T>
@@ -101,6 +105,13 @@ hipError_t launch_residual_partial(const double *HX, int64_t ldH, const double *
 // dot products x.y into partial[chunk]; then the host sums
 hipError_t launch_dot_partial(const double *x, const double *y, int64_t n, double *partial,
                               int nchunks, hipStream_t st);
+
+// g[c*nqc + q] (+)= wscale JxW[c*nqc + q] sum_v w_v (N(q).(s a_v)|_c)(N(q).(s b_v)|_c)
+// s, w and JxW nullable (== 1)
+hipError_t launch_quad_product(const uint32_t *map, int C, int ne, int nqc, const double *Nq,
+                               const double *JxW, const double *a, int64_t lda, const double *b,
+                               int64_t ldb, int B, const double *s_in, const double *w,
+                               double wscale, double *g, int accumulate, hipStream_t s);
 
 // C[i + j*ldc] = sum_k P[k*m*n + i + j*m]  (split-K reduction, column-major m x n)
 hipError_t launch_sum_partials(const double *P, int nk, int m, int n, double *C, int64_t ldc,

@@ -71,7 +71,8 @@ using namespace chfsi;
 
 chfsi_operator::~chfsi_operator() {
   void *ptrs[] = {d_map, d_cells_by_color, d_sqrt_mass, d_inv_sqrt_mass, d_sqrt_mass32,
-                  d_inv_sqrt_mass32, d_K, d_Nq, d_Ht64, d_Ht32, d_nl_off, d_nlC64, d_nlC32,
+                  d_inv_sqrt_mass32, d_K, d_Nq, d_jxw, d_Ht64, d_Ht32, d_Ht16, d_nlC16, d_nl_off,
+                  d_nlC64, d_nlC32,
                   d_nlCt64, d_nlCt32, d_slot_proj, d_proj_ptr, d_proj_slots, d_proj_V,
                   d_nl_cells, d_c_rows, d_c_ptr, d_c_cols, d_c_vals, d_m_rows, d_m_ptr,
                   d_m_slaves, d_m_vals};
@@ -212,8 +213,10 @@ extern "C" chfsi_status chfsi_operator_create(const chfsi_operator_desc *d, chfs
     CHFSI_TRY(dev_upload(&op->d_Nq, d->shape_values, size_t(op->nq) * ne));
     CHFSI_TRY(dev_upload(&op->d_K, d->stiffness, size_t(ne) * ne));
   }
+  if (d->jxw) CHFSI_TRY(dev_upload(&op->d_jxw, d->jxw, size_t(C) * op->nq));
   CHFSI_TRY(dev_alloc(&op->d_Ht64, size_t(C) * ne_pad * ne_pad));
   CHFSI_TRY(dev_alloc(&op->d_Ht32, size_t(C) * ne_pad * ne_pad));
+  CHFSI_TRY(dev_alloc(&op->d_Ht16, size_t(C) * ne_pad * ne_pad));
 
   // ---- nonlocal slots: per cell the (pair, alpha) columns, padded to 16
   op->n_proj = d->n_proj;
@@ -278,6 +281,8 @@ extern "C" chfsi_status chfsi_operator_create(const chfsi_operator_desc *d, chfs
     CHFSI_TRY(dev_upload(&op->d_nlCt64, nlCt.data(), nlCt.size()));
     for (size_t i = 0; i < nlCt.size(); ++i) tmp32[i] = (float)nlCt[i];
     CHFSI_TRY(dev_upload(&op->d_nlCt32, tmp32.data(), tmp32.size()));
+    CHFSI_TRY(dev_alloc(&op->d_nlC16, nlC.size()));
+    CHFSI_HIP(launch_f64_to_bf16(op->d_nlC64, (int64_t)nlC.size(), op->d_nlC16, 0));
     CHFSI_TRY(dev_upload(&op->d_proj_ptr, proj_ptr.data(), proj_ptr.size()));
     CHFSI_TRY(dev_upload(&op->d_proj_slots, proj_slots.data(), proj_slots.size()));
     CHFSI_TRY(dev_upload(&op->d_proj_V, d->proj_V, d->n_proj));
@@ -296,6 +301,8 @@ extern "C" chfsi_status chfsi_operator_set_potential(chfsi_operator *op, const d
   CHFSI_ARG(op->d_Nq && op->d_K, "operator was created without shape data");
   CHFSI_HIP(launch_build_h(op->d_K, op->d_Nq, vJxW, op->C, op->ne, op->ne_pad, op->nq,
                            op->d_Ht64, op->d_Ht32, as_stream(stream)));
+  CHFSI_HIP(launch_f64_to_bf16(op->d_Ht64, int64_t(op->C) * op->ne_pad * op->ne_pad, op->d_Ht16,
+                               as_stream(stream)));
   op->has_H = true;
   return CHFSI_OK;
 }
@@ -305,6 +312,8 @@ extern "C" chfsi_status chfsi_operator_set_cell_matrices(chfsi_operator *op, con
   CHFSI_ARG(op && H, "null argument");
   CHFSI_HIP(launch_tile_h(H, op->C, op->ne, op->ne_pad, op->d_Ht64, op->d_Ht32,
                           as_stream(stream)));
+  CHFSI_HIP(launch_f64_to_bf16(op->d_Ht64, int64_t(op->C) * op->ne_pad * op->ne_pad, op->d_Ht16,
+                               as_stream(stream)));
   op->has_H = true;
   return CHFSI_OK;
 }
@@ -354,7 +363,8 @@ struct OpData<float> {
 template <typename T>
 chfsi_status hx_fused(chfsi_operator *op, const T *src, int64_t ld_src, T *dst, int64_t ld_dst,
                       int B, T alpha, T beta, T gamma, int beta_mode, bool scaled,
-                      hipStream_t s) {
+                      hipStream_t s, bool bf16) {
+  CHFSI_ARG(!bf16 || sizeof(T) == 4, "the BF16 apply works on FP32 multivectors");
   if (!op->has_H) {
     set_error("cell matrices not set (call chfsi_operator_set_potential)");
     return CHFSI_ERR_STATE;
@@ -414,6 +424,12 @@ chfsi_status hx_fused(chfsi_operator *op, const T *src, int64_t ld_src, T *dst, 
   for (int k = 0; k < 8; ++k) {
     a.cells = op->d_cells_by_color + op->color_off[k];
     a.ncells = op->color_off[k + 1] - op->color_off[k];
+    if constexpr (sizeof(T) == 4) {
+      if (bf16) {
+        CHFSI_HIP(launch_hx_bf16(a, op->d_Ht16, op->d_nlC16, s));
+        continue;
+      }
+    }
     CHFSI_HIP(launch_hx<T>(a, s));
   }
   return CHFSI_OK;
@@ -421,9 +437,9 @@ chfsi_status hx_fused(chfsi_operator *op, const T *src, int64_t ld_src, T *dst, 
 
 template chfsi_status hx_fused<double>(chfsi_operator *, const double *, int64_t, double *,
                                        int64_t, int, double, double, double, int, bool,
-                                       hipStream_t);
+                                       hipStream_t, bool);
 template chfsi_status hx_fused<float>(chfsi_operator *, const float *, int64_t, float *, int64_t,
-                                      int, float, float, float, int, bool, hipStream_t);
+                                      int, float, float, float, int, bool, hipStream_t, bool);
 
 }  // namespace chfsi
 
@@ -445,6 +461,15 @@ extern "C" chfsi_status chfsi_hx_fp32(chfsi_operator *op, const float *src, int6
   CHFSI_ARG(op->nc == 0, "FP32 HX supports unconstrained (periodic-folded) layouts only");
   return hx_fused<float>(op, src, ld_src, dst, ld_dst, B, scalar, 1.0f, 0.0f,
                          scale_flag ? 0 : 2, true, as_stream(stream));
+}
+
+extern "C" chfsi_status chfsi_hx_bf16(chfsi_operator *op, const float *src, int64_t ld_src,
+                                      float *dst, int64_t ld_dst, int32_t B, int32_t scale_flag,
+                                      float scalar, void *stream) {
+  CHFSI_ARG(op && src && dst, "null argument");
+  CHFSI_ARG(op->nc == 0, "BF16 HX supports unconstrained (periodic-folded) layouts only");
+  return hx_fused<float>(op, src, ld_src, dst, ld_dst, B, scalar, 1.0f, 0.0f,
+                         scale_flag ? 0 : 2, true, as_stream(stream), true);
 }
 
 extern "C" chfsi_status chfsi_hx_cheby(chfsi_operator *op, const double *src, int64_t ld_src,
@@ -546,5 +571,27 @@ extern "C" chfsi_status chfsi_operator_autotune(chfsi_operator *op, int32_t B, i
   if (best32) *best32 = b32;
   if (times_ms)
     for (int i = 0; i < 16; ++i) times_ms[i] = t[i];
+  return CHFSI_OK;
+}
+
+extern "C" chfsi_status chfsi_density(chfsi_operator *op, const double *X, int64_t ld, int32_t N,
+                                      const double *w_dev, double *rho_dev, int32_t accumulate,
+                                      void *stream) {
+  CHFSI_ARG(op && X && rho_dev && N > 0 && ld >= N, "bad argument");
+  CHFSI_ARG(op->d_Nq, "operator was created without shape data");
+  CHFSI_HIP(launch_quad_product(op->d_map, op->C, op->ne, op->nq, op->d_Nq, nullptr, X, ld, X, ld,
+                                N, nullptr, w_dev, 1.0, rho_dev, accumulate, as_stream(stream)));
+  return CHFSI_OK;
+}
+
+extern "C" chfsi_status chfsi_hx_adjoint_v(chfsi_operator *op, const double *Ybar, int64_t ldy,
+                                           const double *X, int64_t ldx, int32_t B,
+                                           double scalar, double *g_dev, int32_t accumulate,
+                                           void *stream) {
+  CHFSI_ARG(op && Ybar && X && g_dev && B > 0 && ldy >= B && ldx >= B, "bad argument");
+  CHFSI_ARG(op->d_Nq && op->d_jxw, "operator was created without shape data / JxW");
+  CHFSI_HIP(launch_quad_product(op->d_map, op->C, op->ne, op->nq, op->d_Nq, op->d_jxw, Ybar, ldy,
+                                X, ldx, B, op->d_inv_sqrt_mass, nullptr, scalar, g_dev,
+                                accumulate, as_stream(stream)));
   return CHFSI_OK;
 }

@@ -18,9 +18,13 @@ struct Scratch {
   void release();
 };
 
+// One fused apply  dst <- base + alpha * S H S src  over all colour classes
+// (see HxLaunch for base / beta_mode).  bf16 selects the BF16-input kernel
+// (T = float only).
 template <typename T>
 chfsi_status hx_fused(chfsi_operator *op, const T *src, int64_t ld_src, T *dst, int64_t ld_dst,
-                      int B, T alpha, T beta, T gamma, int beta_mode, bool scaled, hipStream_t s);
+                      int B, T alpha, T beta, T gamma, int beta_mode, bool scaled, hipStream_t s,
+                      bool bf16 = false);
 
 }  // namespace chfsi
 
@@ -36,9 +40,10 @@ struct chfsi_operator {
   double *d_sqrt_mass = nullptr, *d_inv_sqrt_mass = nullptr;
   float *d_sqrt_mass32 = nullptr, *d_inv_sqrt_mass32 = nullptr;
   // shape data and cell Hamiltonians (padded K-tiled layout)
-  double *d_K = nullptr, *d_Nq = nullptr;
+  double *d_K = nullptr, *d_Nq = nullptr, *d_jxw = nullptr;
   double *d_Ht64 = nullptr;
   float *d_Ht32 = nullptr;
+  uint16_t *d_Ht16 = nullptr, *d_nlC16 = nullptr;  // BF16 copies
   bool has_H = false;
   // nonlocal projector slots
   int n_proj = 0, n_slots = 0, n_nl_cells = 0;

@@ -120,63 +120,96 @@ T *sptr(Scratch &s) {
   return static_cast<T *>(s.ptr);
 }
 
-// Scaled Chebyshev recurrence on [a, b] with a0 scaling, fused into the HX
-// launches:  X_new = alpha1 (H~ Y - c Y) + alpha2 X  is one hx_fused call
-// (alpha = alpha1, gamma = -c alpha1, beta = alpha2).
+// Scaled Chebyshev recurrence on [a, b] with a0 scaling (Zhou-Saad), fused into
+// the HX launches.  Degree 1:  Y = (sigma1/e)(H~X - cX); degree k >= 2:
+//   X_new = alpha1 (H~ Y - c Y) + alpha2 X
+// which is one hx_fused call (alpha = alpha1, gamma = -c alpha1, beta = alpha2).
 // Coefficients: linearAlgebraOperationsDevice.cc:547-726 / Opt.cc:286-351.
+struct ChebState {
+  double e, c, sigma, gamma;
+};
+
+ChebState cheb_init(double a, double b, double a0) {
+  ChebState s;
+  s.e = (b - a) / 2.0;
+  s.c = (b + a) / 2.0;
+  s.sigma = s.e / (a0 - s.c);
+  s.gamma = 2.0 / s.sigma;
+  return s;
+}
+
+// degree 1 from X into Y
 template <typename T>
-chfsi_status filter_impl(chfsi_operator *op, T *X, int64_t ldX, T *Y, int64_t ldY, int B, int m,
-                         double a, double b, double a0, hipStream_t st, T **result,
-                         int64_t *ld_result) {
-  const double e = (b - a) / 2.0, c = (b + a) / 2.0;
-  double sigma = e / (a0 - c);
-  const double sigma1 = sigma, gamma = 2.0 / sigma1;
-  // degree 1: Y = (sigma1/e) (H~ X - c X)
-  CHFSI_TRY(hx_fused<T>(op, X, ldX, Y, ldY, B, T(sigma1 / e), T(0), T(-c * sigma1 / e), 1, true,
-                        st));
-  T *cur = Y, *prev = X;
-  int64_t ldc = ldY, ldp = ldX;
-  for (int degree = 2; degree <= m; ++degree) {
-    const double sigma2 = 1.0 / (gamma - sigma);
-    const double alpha1 = 2.0 * sigma2 / e, alpha2 = -(sigma * sigma2);
-    CHFSI_TRY(hx_fused<T>(op, cur, ldc, prev, ldp, B, T(alpha1), T(alpha2), T(-c * alpha1), 0,
-                          true, st));
-    std::swap(cur, prev);
-    std::swap(ldc, ldp);
-    sigma = sigma2;
+chfsi_status cheb_first(chfsi_operator *op, const T *X, int64_t ldX, T *Y, int64_t ldY, int B,
+                        const ChebState &cs, hipStream_t st, bool bf16) {
+  return hx_fused<T>(op, X, ldX, Y, ldY, B, T(cs.sigma / cs.e), T(0), T(-cs.c * cs.sigma / cs.e),
+                     1, true, st, bf16);
+}
+
+// degrees d0..d1 (d0 >= 2); *cur holds the latest iterate, *prev the one before
+template <typename T>
+chfsi_status cheb_steps(chfsi_operator *op, T **cur, int64_t *ldc, T **prev, int64_t *ldp, int B,
+                        int d0, int d1, ChebState &cs, hipStream_t st, bool bf16) {
+  for (int degree = d0; degree <= d1; ++degree) {
+    const double sigma2 = 1.0 / (cs.gamma - cs.sigma);
+    const double alpha1 = 2.0 * sigma2 / cs.e, alpha2 = -(cs.sigma * sigma2);
+    CHFSI_TRY(hx_fused<T>(op, *cur, *ldc, *prev, *ldp, B, T(alpha1), T(alpha2),
+                          T(-cs.c * alpha1), 0, true, st, bf16));
+    std::swap(*cur, *prev);
+    std::swap(*ldc, *ldp);
+    cs.sigma = sigma2;
   }
-  *result = cur;
-  *ld_result = ldc;
   return CHFSI_OK;
 }
 
+// In-place filter of the B-column block X (leading dimension ld).  FP32 and
+// BF16 forms cast the block to FP32 once; with tail > 0 the last `tail`
+// degrees run in FP64 from the cast-back pair of iterates, so that the
+// rounding of the low-precision steps is itself filtered.
 chfsi_status filter_block(chfsi_solver *s, chfsi_operator *op, double *X, int64_t ld, int B, int m,
-                          double a, double b, double a0, int precision, hipStream_t st) {
+                          double a, double b, double a0, int precision, int tail,
+                          hipStream_t st) {
   const int64_t M = op->M;
+  ChebState cs = cheb_init(a, b, a0);
+  CHFSI_TRY(s->Y64.ensure(sizeof(double) * M * B));
+  double *Y = sptr<double>(s->Y64);
   if (precision == CHFSI_FP64) {
-    CHFSI_TRY(s->Y64.ensure(sizeof(double) * M * B));
-    double *Y = sptr<double>(s->Y64);
     CHFSI_HIP(hipMemsetAsync(Y, 0, sizeof(double) * M * B, st));
-    double *res;
-    int64_t ldr;
-    CHFSI_TRY(filter_impl<double>(op, X, ld, Y, B, B, m, a, b, a0, st, &res, &ldr));
-    if (res != X) CHFSI_HIP(launch_copy_block<double, double>(res, ldr, X, ld, M, B, st));
+    CHFSI_TRY(cheb_first<double>(op, X, ld, Y, B, B, cs, st, false));
+    double *cur = Y, *prev = X;
+    int64_t ldc = B, ldp = ld;
+    CHFSI_TRY(cheb_steps<double>(op, &cur, &ldc, &prev, &ldp, B, 2, m, cs, st, false));
+    if (cur != X) CHFSI_HIP(launch_copy_block<double, double>(cur, ldc, X, ld, M, B, st));
     return CHFSI_OK;
   }
-  if (precision == CHFSI_FP32) {
-    CHFSI_TRY(s->X32.ensure(sizeof(float) * M * B));
-    CHFSI_TRY(s->Y32.ensure(sizeof(float) * M * B));
-    float *X32 = sptr<float>(s->X32), *Y32 = sptr<float>(s->Y32);
-    CHFSI_HIP(launch_copy_block<double, float>(X, ld, X32, B, M, B, st));
-    CHFSI_HIP(hipMemsetAsync(Y32, 0, sizeof(float) * M * B, st));
-    float *res;
-    int64_t ldr;
-    CHFSI_TRY(filter_impl<float>(op, X32, B, Y32, B, B, m, a, b, a0, st, &res, &ldr));
-    CHFSI_HIP(launch_copy_block<float, double>(res, ldr, X, ld, M, B, st));
+  if (precision != CHFSI_FP32 && precision != CHFSI_BF16) {
+    set_error("unsupported filter precision %d", precision);
+    return CHFSI_ERR_ARG;
+  }
+  const bool bf16 = precision == CHFSI_BF16;
+  tail = std::max(0, std::min(tail, m - 1));
+  const int mlow = m - tail;
+  CHFSI_TRY(s->X32.ensure(sizeof(float) * M * B));
+  CHFSI_TRY(s->Y32.ensure(sizeof(float) * M * B));
+  float *X32 = sptr<float>(s->X32), *Y32 = sptr<float>(s->Y32);
+  CHFSI_HIP(launch_copy_block<double, float>(X, ld, X32, B, M, B, st));
+  CHFSI_HIP(hipMemsetAsync(Y32, 0, sizeof(float) * M * B, st));
+  CHFSI_TRY(cheb_first<float>(op, X32, B, Y32, B, B, cs, st, bf16));
+  float *cur = Y32, *prev = X32;
+  int64_t ldc = B, ldp = B;
+  CHFSI_TRY(cheb_steps<float>(op, &cur, &ldc, &prev, &ldp, B, 2, mlow, cs, st, bf16));
+  if (tail == 0) {
+    CHFSI_HIP(launch_copy_block<float, double>(cur, ldc, X, ld, M, B, st));
     return CHFSI_OK;
   }
-  set_error("unsupported filter precision %d", precision);
-  return CHFSI_ERR_ARG;
+  // FP64 tail: latest iterate -> X, previous -> Y
+  CHFSI_HIP(launch_copy_block<float, double>(cur, ldc, X, ld, M, B, st));
+  CHFSI_HIP(launch_copy_block<float, double>(prev, ldp, Y, B, M, B, st));
+  double *c64 = X, *p64 = Y;
+  int64_t ldc64 = ld, ldp64 = B;
+  CHFSI_TRY(cheb_steps<double>(op, &c64, &ldc64, &p64, &ldp64, B, mlow + 1, m, cs, st, false));
+  if (c64 != X) CHFSI_HIP(launch_copy_block<double, double>(c64, ldc64, X, ld, M, B, st));
+  return CHFSI_OK;
 }
 
 chfsi_status lanczos_impl(chfsi_solver *s, chfsi_operator *op, const double *v0, int steps,
@@ -365,6 +398,13 @@ extern "C" void chfsi_default_params(chfsi_params *p) {
   p->first_scf_degree_factor = 1.34;    // utils/dftParameters.cc:872-876
   p->filter_precision = CHFSI_FP32;
   p->use_shep = 1;                      // utils/dftParameters.cc:934-938
+  p->fp64_tail_degrees = 0;
+}
+
+extern "C" chfsi_status chfsi_solver_set_params(chfsi_solver *s, const chfsi_params *params) {
+  CHFSI_ARG(s && params, "null argument");
+  s->params = *params;
+  return CHFSI_OK;
 }
 
 extern "C" chfsi_status chfsi_solver_create(const chfsi_params *params, double a0, double bLow,
@@ -420,7 +460,110 @@ extern "C" chfsi_status chfsi_chebyshev_filter(chfsi_operator *op, double *X, in
                                                double a0, int32_t precision, void *stream) {
   CHFSI_ARG(op && X && B > 0 && ld >= B && m >= 1, "bad argument");
   chfsi_solver tmp;
-  return filter_block(&tmp, op, X, ld, B, m, a, b, a0, precision, as_stream(stream));
+  return filter_block(&tmp, op, X, ld, B, m, a, b, a0, precision, 0, as_stream(stream));
+}
+
+// Reverse pass of the scaled Chebyshev recurrence
+//   X_1 = k1 (H~ - c) X_0,   X_k = a1_k (H~ - c) X_{k-1} + a2_k X_{k-2}  (k = 2..m),  Y = X_m.
+// With L_k = dL/dX_k, going from k = m down to 1:
+//   g     += a1_k * adjV(L_k, X_{k-1})           (dL/dV through the HX of step k)
+//   L_{k-1} += a1_k (H~ - c) L_k                  (one fused HX, beta = 1)
+//   L_{k-2} += a2_k L_k
+// and X0bar = L_0.  Forward iterates come from checkpoints (pairs X_{k-1}, X_k
+// every `every` degrees) re-run one segment at a time.
+extern "C" chfsi_status chfsi_chebyshev_filter_backward(
+    chfsi_operator *op, const double *X0, int64_t ldx, const double *Ybar, int64_t ldy, int32_t B,
+    int32_t m, double a, double b, double a0, int32_t every, double *Xbar, int64_t ldxb,
+    double *g, void *stream) {
+  CHFSI_ARG(op && X0 && Ybar && Xbar && g && B > 0 && m >= 1, "bad argument");
+  CHFSI_ARG(ldx >= B && ldy >= B && ldxb >= B, "bad leading dimension");
+  CHFSI_ARG(op->d_Nq && op->d_jxw, "operator was created without shape data / JxW");
+  hipStream_t st = as_stream(stream);
+  const int64_t M = op->M;
+  const size_t vec = size_t(M) * B;
+  every = std::max(1, std::min<int>(every, m));
+  // coefficients of every degree
+  ChebState cs = cheb_init(a, b, a0);
+  std::vector<double> a1(m + 1, 0.0), a2(m + 1, 0.0);
+  a1[1] = cs.sigma / cs.e;
+  {
+    ChebState t = cs;
+    for (int k = 2; k <= m; ++k) {
+      const double sigma2 = 1.0 / (t.gamma - t.sigma);
+      a1[k] = 2.0 * sigma2 / t.e;
+      a2[k] = -(t.sigma * sigma2);
+      t.sigma = sigma2;
+    }
+  }
+  // storage: X_0 copy, checkpoint pairs, one segment of iterates, 3 adjoint buffers
+  const int nseg = (m + every - 1) / every;
+  Scratch ck, seg, adj;
+  CHFSI_TRY(ck.ensure(sizeof(double) * vec * (1 + 2 * nseg)));
+  CHFSI_TRY(seg.ensure(sizeof(double) * vec * (every + 2)));
+  CHFSI_TRY(adj.ensure(sizeof(double) * vec * 3));
+  double *ckp = sptr<double>(ck), *segp = sptr<double>(seg), *adjp = sptr<double>(adj);
+  auto ckv = [&](int i) { return ckp + vec * i; };
+  CHFSI_HIP(launch_copy_block<double, double>(X0, ldx, ckv(0), B, M, B, st));
+  CHFSI_HIP(hipMemsetAsync(segp, 0, sizeof(double) * vec * (every + 2), st));
+  // forward sweep storing (X_{k0-1}, X_{k0}) for every segment start k0 = s*every >= 1
+  {
+    double *prev = segp, *cur = segp + vec;
+    CHFSI_HIP(launch_copy_block<double, double>(ckv(0), B, prev, B, M, B, st));
+    int64_t ldc = B, ldp = B;
+    CHFSI_TRY(hx_fused<double>(op, prev, B, cur, B, B, a1[1], 0.0, -cs.c * a1[1], 1, true, st));
+    for (int k = 1; k <= m; ++k) {
+      if (k % every == 0 && k < m) {
+        const int s = k / every;
+        CHFSI_HIP(launch_copy_block<double, double>(prev, ldp, ckv(2 * s - 1), B, M, B, st));
+        CHFSI_HIP(launch_copy_block<double, double>(cur, ldc, ckv(2 * s), B, M, B, st));
+      }
+      if (k == m) break;
+      CHFSI_TRY(hx_fused<double>(op, cur, ldc, prev, ldp, B, a1[k + 1], a2[k + 1],
+                                 -cs.c * a1[k + 1], 0, true, st));
+      std::swap(cur, prev);
+      std::swap(ldc, ldp);
+    }
+  }
+  // adjoint sweep: Lk = L_k, Lm1 = L_{k-1}, Lm2 = L_{k-2}
+  double *Lk = adjp, *Lm1 = adjp + vec, *Lm2 = adjp + 2 * vec;
+  CHFSI_HIP(launch_copy_block<double, double>(Ybar, ldy, Lk, B, M, B, st));
+  CHFSI_HIP(hipMemsetAsync(Lm1, 0, sizeof(double) * vec * 2, st));
+  for (int s = nseg - 1; s >= 0; --s) {
+    const int k0 = s * every, k1 = std::min(m, k0 + every);
+    // iterates X_{k0-1 .. k1-1} into seg[0 .. ]: seg[i] = X_{k0 - 1 + i}
+    auto X = [&](int k) { return segp + vec * (k - k0 + 1); };
+    if (k0 == 0) {
+      CHFSI_HIP(launch_copy_block<double, double>(ckv(0), B, X(0), B, M, B, st));
+    } else {
+      CHFSI_HIP(launch_copy_block<double, double>(ckv(2 * s - 1), B, X(k0 - 1), B, M, B, st));
+      CHFSI_HIP(launch_copy_block<double, double>(ckv(2 * s), B, X(k0), B, M, B, st));
+    }
+    for (int k = k0 + 1; k <= k1 - 1; ++k) {
+      if (k == 1)
+        CHFSI_TRY(hx_fused<double>(op, X(0), B, X(1), B, B, a1[1], 0.0, -cs.c * a1[1], 1, true,
+                                   st));
+      else {
+        CHFSI_HIP(launch_copy_block<double, double>(X(k - 2), B, X(k), B, M, B, st));
+        CHFSI_TRY(hx_fused<double>(op, X(k - 1), B, X(k), B, B, a1[k], a2[k], -cs.c * a1[k], 0,
+                                   true, st));
+      }
+    }
+    for (int k = k1; k >= k0 + 1; --k) {
+      CHFSI_HIP(launch_quad_product(op->d_map, op->C, op->ne, op->nq, op->d_Nq, op->d_jxw, Lk, B,
+                                    X(k - 1), B, B, op->d_inv_sqrt_mass, nullptr, a1[k], g, 1, st));
+      CHFSI_TRY(hx_fused<double>(op, Lk, B, Lm1, B, B, a1[k], 1.0, -cs.c * a1[k], 0, true, st));
+      if (k >= 2) CHFSI_HIP(launch_axpby<double>((int64_t)vec, a2[k], Lk, 1.0, Lm2, st));
+      // shift: (L_k, L_{k-1}, L_{k-2}) <- (L_{k-1}, L_{k-2}, 0)
+      double *t = Lk;
+      Lk = Lm1;
+      Lm1 = Lm2;
+      Lm2 = t;
+      CHFSI_HIP(hipMemsetAsync(Lm2, 0, sizeof(double) * vec, st));
+    }
+  }
+  CHFSI_HIP(launch_copy_block<double, double>(Lk, B, Xbar, ldxb, M, B, st));
+  CHFSI_HIP(hipStreamSynchronize(st));
+  return CHFSI_OK;
 }
 
 extern "C" chfsi_status chfsi_lanczos_bounds(chfsi_operator *op, const double *v0, int32_t steps,
@@ -517,7 +660,8 @@ extern "C" chfsi_status chfsi_solve(chfsi_solver *s, chfsi_operator *op, double 
   const int prec = use_mixed_prec_overall ? P.filter_precision : CHFSI_FP64;
   for (int j = 0; j < N; j += B) {
     const int bw = std::min(B, N - j);
-    CHFSI_TRY(filter_block(s, op, X + j, N, bw, (int)m, s->bLow, s->bUp, s->a0, prec, st));
+    CHFSI_TRY(filter_block(s, op, X + j, N, bw, (int)m, s->bLow, s->bUp, s->a0, prec,
+                           use_mixed_prec_overall ? P.fp64_tail_degrees : 0, st));
   }
   CHFSI_HIP(hipEventRecord(ev[2], st));
 

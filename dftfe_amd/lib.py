@@ -42,6 +42,7 @@ class OperatorDesc(ct.Structure):
         ("n_constrained", ct.c_int64), ("constraint_rows", c_int64_p),
         ("constraint_ptr", c_int64_p), ("constraint_cols", c_int64_p),
         ("constraint_vals", c_double_p),
+        ("jxw", c_double_p),
     ]
 
 
@@ -51,6 +52,7 @@ class Params(ct.Structure):
         ("lanczos_steps", ct.c_int32), ("reproducible", ct.c_int32),
         ("reuse_lanczos_upper", ct.c_int32), ("first_scf_degree_factor", ct.c_double),
         ("filter_precision", ct.c_int32), ("use_shep", ct.c_int32),
+        ("fp64_tail_degrees", ct.c_int32),
     ]
 
 
@@ -74,7 +76,16 @@ SIGNATURES = {
                             ct.c_double, _VP]),
     "chfsi_hx_fp32": (ct.c_int, [_VP, _VP, ct.c_int64, _VP, ct.c_int64, ct.c_int32, ct.c_int32,
                                  ct.c_float, _VP]),
+    "chfsi_hx_bf16": (ct.c_int, [_VP, _VP, ct.c_int64, _VP, ct.c_int64, ct.c_int32, ct.c_int32,
+                                 ct.c_float, _VP]),
     "chfsi_hx_cheby": (ct.c_int, [_VP, _VP, ct.c_int64, _VP, ct.c_int64, ct.c_int32, _VP]),
+    "chfsi_density": (ct.c_int, [_VP, _VP, ct.c_int64, ct.c_int32, _VP, _VP, ct.c_int32, _VP]),
+    "chfsi_hx_adjoint_v": (ct.c_int, [_VP, _VP, ct.c_int64, _VP, ct.c_int64, ct.c_int32,
+                                      ct.c_double, _VP, ct.c_int32, _VP]),
+    "chfsi_chebyshev_filter_backward": (ct.c_int, [_VP, _VP, ct.c_int64, _VP, ct.c_int64,
+                                                   ct.c_int32, ct.c_int32, ct.c_double,
+                                                   ct.c_double, ct.c_double, ct.c_int32, _VP,
+                                                   ct.c_int64, _VP, _VP]),
     "chfsi_constraints_distribute": (ct.c_int, [_VP, _VP, ct.c_int64, ct.c_int32, _VP]),
     "chfsi_constraints_slave_to_master": (ct.c_int, [_VP, _VP, ct.c_int64, ct.c_int32, _VP]),
     "chfsi_constraints_set_zero": (ct.c_int, [_VP, _VP, ct.c_int64, ct.c_int32, _VP]),
@@ -90,6 +101,7 @@ SIGNATURES = {
     "chfsi_solver_create": (ct.c_int, [ct.POINTER(Params), ct.c_double, ct.c_double, ct.c_double,
                                        ct.POINTER(_VP)]),
     "chfsi_solver_destroy": (None, [_VP]),
+    "chfsi_solver_set_params": (ct.c_int, [_VP, ct.POINTER(Params)]),
     "chfsi_solver_reinit_bounds": (ct.c_int, [_VP, ct.c_double, ct.c_double, ct.c_double]),
     "chfsi_solver_get_bounds": (ct.c_int, [_VP, c_double_p, c_double_p, c_double_p]),
     "chfsi_solver_set_lanczos_start": (ct.c_int, [_VP, _VP, ct.c_int64]),

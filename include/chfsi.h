@@ -85,6 +85,9 @@ typedef struct {
   const int64_t *constraint_ptr;   /* n_constrained+1                           */
   const int64_t *constraint_cols;  /* masters                                   */
   const double *constraint_vals;   /* coefficients                              */
+  /* quadrature weights JxW per (cell, quadrature point), C*n_q; nullable when
+   * neither the density nor the potential adjoint is used */
+  const double *jxw;
 } chfsi_operator_desc;
 
 /* Solver parameters: the ChFSI knobs of dftParameters
@@ -100,6 +103,8 @@ typedef struct {
   int32_t filter_precision;        /* chfsi_precision for the filter bulk when
                                       use_mixed_prec_overall is set             */
   int32_t use_shep;                /* SUBSPACE PROJ SHEP GPU: 1 = CholGS + RR   */
+  int32_t fp64_tail_degrees;       /* last degrees of a mixed-precision filter run
+                                      in FP64 (0: whole filter in filter_precision) */
 } chfsi_params;
 
 const char *chfsi_last_error(void);
@@ -148,6 +153,12 @@ chfsi_status chfsi_hx_fp32(chfsi_operator *op, const float *src, int64_t ld_src,
                            int64_t ld_dst, int32_t B, int32_t scale_flag, float scalar,
                            void *stream);
 
+/* Same operator with BF16 cell matrices and BF16-rounded inputs, FP32
+ * accumulation (MFMA) and FP32 multivectors: the loose-tolerance early-pass form. */
+chfsi_status chfsi_hx_bf16(chfsi_operator *op, const float *src, int64_t ld_src, float *dst,
+                           int64_t ld_dst, int32_t B, int32_t scale_flag, float scalar,
+                           void *stream);
+
 /* HXCheby (operatorDevice.h:188-197; kohnShamDFTOperatorDevice.cc:3874-3997):
  *   dst <- dst + H src   (unscaled FE H; src already carries its M^{-1/2} scaling). */
 chfsi_status chfsi_hx_cheby(chfsi_operator *op, const double *src, int64_t ld_src, double *dst,
@@ -157,6 +168,36 @@ chfsi_status chfsi_hx_cheby(chfsi_operator *op, const double *src, int64_t ld_sr
 chfsi_status chfsi_constraints_distribute(chfsi_operator *op, double *X, int64_t ld, int32_t B, void *stream);
 chfsi_status chfsi_constraints_slave_to_master(chfsi_operator *op, double *X, int64_t ld, int32_t B, void *stream);
 chfsi_status chfsi_constraints_set_zero(chfsi_operator *op, double *X, int64_t ld, int32_t B, void *stream);
+
+/* ---------------- density and derivatives ---------------- */
+
+/* Density at quadrature points, the contraction of computeRhoFromPSI
+ * (src/dft/densityCalculator.cc:212-254, :622-689):
+ *   rho[c*n_q + q] (+)= sum_v w[v] (sum_i N_i(q) X[g(c,i), v])^2
+ * X: FE-basis wavefunctions (device, ld >= N); w: device weights
+ * (spin factor * k weight * occupation); rho: device C*n_q. */
+chfsi_status chfsi_density(chfsi_operator *op, const double *X, int64_t ld, int32_t N,
+                           const double *w_dev, double *rho_dev, int32_t accumulate,
+                           void *stream);
+
+/* Adjoint of HX with respect to the local potential at the quadrature points
+ * (no reference counterpart; differentiation harness).  For Y = scalar * H~ X,
+ *   g[c*n_q + q] (+)= scalar * JxW(c,q) * sum_v (N(q).(M^{-1/2} Ybar_v)) (N(q).(M^{-1/2} X_v))
+ * is dL/dV(c,q) given Ybar = dL/dY.  The adjoint with respect to X is HX itself
+ * (H~ is symmetric). */
+chfsi_status chfsi_hx_adjoint_v(chfsi_operator *op, const double *Ybar, int64_t ldy,
+                                const double *X, int64_t ldx, int32_t B, double scalar,
+                                double *g_dev, int32_t accumulate, void *stream);
+
+/* Reverse pass of chfsi_chebyshev_filter (FP64; no reference counterpart).
+ * Given the filter input X0 and Ybar = dL/dY of its output, writes
+ * Xbar = dL/dX0 and accumulates dL/dV(c,q) into g.  Forward iterates are
+ * recomputed from checkpoints stored every `checkpoint_every` degrees. */
+chfsi_status chfsi_chebyshev_filter_backward(chfsi_operator *op, const double *X0, int64_t ldx,
+                                             const double *Ybar, int64_t ldy, int32_t B,
+                                             int32_t m, double a, double b, double a0,
+                                             int32_t checkpoint_every, double *Xbar,
+                                             int64_t ldxb, double *g_dev, void *stream);
 
 /* ---------------- linear algebra (linearAlgebraOperationsDevice) ---------------- */
 
@@ -191,6 +232,8 @@ chfsi_status chfsi_residual_norms(chfsi_operator *op, const double *X, int64_t M
 chfsi_status chfsi_solver_create(const chfsi_params *params, double a0, double bLow, double bUp,
                                  chfsi_solver **solver);
 void chfsi_solver_destroy(chfsi_solver *solver);
+/* Replace the parameters of an existing solver (per-pass precision schedules). */
+chfsi_status chfsi_solver_set_params(chfsi_solver *solver, const chfsi_params *params);
 /* reinitSpectrumBounds (…SolverDevice.h:121-124) */
 chfsi_status chfsi_solver_reinit_bounds(chfsi_solver *solver, double a0, double bLow, double bUp);
 chfsi_status chfsi_solver_get_bounds(const chfsi_solver *solver, double *a0, double *bLow, double *bUp);

@@ -380,6 +380,65 @@ def next_bounds(lam: np.ndarray):
 
 
 # ----------------------------------------------------------------------------
+# Derivatives with respect to the local potential (no reference counterpart;
+# restated from the definitions H_e = 1/2 K + N^T diag(V JxW) N and the
+# three-term recurrence above)
+# ----------------------------------------------------------------------------
+
+
+def quad_values(prob, X: np.ndarray) -> np.ndarray:
+    """Values at the quadrature points, (C, nqc, B): psi(c, q, v) = sum_i N_i(q) X[g(c,i), v]."""
+    return np.matmul(prob.Nq[None, :, :], X[prob.cell_map])
+
+
+def hx_adjoint_v(op, Ybar: np.ndarray, X: np.ndarray, scalar: float = 1.0) -> np.ndarray:
+    """dL/dV(c, q) for Y = scalar * H~(V) X and Ybar = dL/dY:
+    scalar * JxW_q * sum_v (N S Ybar_v)(q) (N S X_v)(q)."""
+    p = op.prob
+    a = quad_values(p, op.s[:, None] * Ybar)
+    b = quad_values(p, op.s[:, None] * X)
+    return scalar * p.JxW[None, :] * (a * b).sum(axis=2)
+
+
+def filter_backward(op, X0: np.ndarray, Ybar: np.ndarray, m: int, a: float, b: float, a0: float):
+    """Reverse pass of chebyshev_filter with all iterates stored:
+    returns (dL/dX0, dL/dV(c, q))."""
+    e = (b - a) / 2.0
+    c = (b + a) / 2.0
+    sigma = e / (a0 - c)
+    gamma = 2.0 / sigma
+    a1 = [0.0, sigma / e]
+    a2 = [0.0, 0.0]
+    for _k in range(2, m + 1):
+        sigma2 = 1.0 / (gamma - sigma)
+        a1.append(2.0 * sigma2 / e)
+        a2.append(-(sigma * sigma2))
+        sigma = sigma2
+    Xs = [X0]
+    Xs.append(a1[1] * (op.apply(X0) - c * X0))
+    for k in range(2, m + 1):
+        Xs.append(a1[k] * (op.apply(Xs[k - 1]) - c * Xs[k - 1]) + a2[k] * Xs[k - 2])
+    Lam = [np.zeros_like(X0) for _ in range(m + 1)]
+    Lam[m] = Ybar.copy()
+    g = np.zeros((op.prob.C, op.prob.nqc))
+    for k in range(m, 0, -1):
+        g += hx_adjoint_v(op, Lam[k], Xs[k - 1], a1[k])
+        Lam[k - 1] += a1[k] * (op.apply(Lam[k]) - c * Lam[k])
+        if k >= 2:
+            Lam[k - 2] += a2[k] * Lam[k]
+    return Lam[0], g
+
+
+def with_potential(prob, V_q: np.ndarray):
+    """Operator of the same problem with another local potential (cell matrices rebuilt)."""
+    import copy
+
+    p2 = copy.copy(prob)
+    p2.V_q = V_q
+    return OracleOperator(p2)
+
+
+# ----------------------------------------------------------------------------
 # Density and occupations
 # ----------------------------------------------------------------------------
 

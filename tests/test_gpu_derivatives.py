@@ -1,0 +1,143 @@
+"""GPU checks of the differentiation form: density, adjoint of HX with
+respect to V(q), checkpointed filter backward, and the autograd gradient of
+the band energy after one pass.
+
+Small workload (20 bohr, 8^3 cells, p=4): device vs oracle (FP64, 1e-10).
+Differentiation workload (24 bohr, 12^3 cells, p=5, 32 wells, 256 bands,
+Fermi-Dirac kT = 0.01 Ha): autograd gradient vs central finite differences at
+64 random quadrature points, and vs the Hellmann-Feynman gradient
+sum_i s f_i |psi_i(q)|^2 JxW(q) after converging passes.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+from dftfe_amd import fe  # noqa: E402
+from oracle import chfsi_oracle as O  # noqa: E402
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def small():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from dftfe_amd.chfsi import KohnShamOperatorDevice
+
+    prob = fe.build_workload("cube20_p4_n64")
+    return prob, O.OracleOperator(prob), KohnShamOperatorDevice(prob)
+
+
+def test_density_matches_oracle(small):
+    prob, ref, op = small
+    X = prob.initial_guess(N=24, seed_offset=50)
+    w = np.linspace(0.5, 2.0, 24)
+    rho = op.density(torch.tensor(X, device="cuda"), torch.tensor(w)).cpu().numpy()
+    want = O.density_at_quadrature(prob, X, w, spin_factor=1.0)
+    assert _rel(rho, want) < 1e-12
+
+
+def test_hx_adjoint_v_matches_oracle(small):
+    prob, ref, op = small
+    X = prob.initial_guess(N=40, seed_offset=51)
+    Yb = prob.initial_guess(N=40, seed_offset=52)
+    g = op.hx_adjoint_v(torch.tensor(Yb, device="cuda"), torch.tensor(X, device="cuda"), 0.3)
+    assert _rel(g.cpu().numpy(), O.hx_adjoint_v(ref, Yb, X, 0.3)) < 1e-12
+
+
+def test_filter_backward_matches_oracle(small):
+    prob, ref, op = small
+    m, a, b, a0 = 7, 16.0, 41.0, 0.0
+    X0 = prob.sqrt_mass[:, None] * prob.initial_guess(N=32, seed_offset=53)
+    Yb = prob.initial_guess(N=32, seed_offset=54)
+    Xbar, g = op.filter_backward(torch.tensor(X0, device="cuda"), torch.tensor(Yb, device="cuda"),
+                                 m, a, b, a0, checkpoint_every=3)
+    Xo, go = O.filter_backward(ref, X0, Yb, m, a, b, a0)
+    assert _rel(Xbar.cpu().numpy(), Xo) < 1e-11
+    assert _rel(g.cpu().numpy(), go) < 1e-11
+
+
+@pytest.fixture(scope="module")
+def diffw():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from dftfe_amd.chfsi import ChebyshevSolverDevice, KohnShamOperatorDevice
+
+    prob = fe.build_workload("cube24_p5_n256")
+    op = KohnShamOperatorDevice(prob)
+    wl = prob.wl
+    solver = ChebyshevSolverDevice(cheby_block_size=wl.B, cheby_degree=wl.m, lanczos_steps=10)
+    solver.set_lanczos_start(torch.tensor(prob.lanczos_start, device="cuda"))
+    X = torch.tensor(prob.initial_guess(), device="cuda")
+    history = []
+    for k in range(8):
+        lam, res, _ = solver.solve(op, X, is_first_filtering_call=(k == 0))
+        _, _, bUp = solver.bounds()
+        solver.reinitSpectrumBounds(float(lam[0]), float(lam[-1]), bUp)
+        history.append(float(res.max()))
+    return prob, op, solver, X, lam, history
+
+
+def _energy_and_grad(prob, op, X0, bounds, occ):
+    from dftfe_amd.autograd import band_energy
+
+    V = torch.tensor(prob.V_q, device="cuda", requires_grad=True)
+    E, eps, Q = band_energy(op, V, X0, bounds, prob.wl.m, occ, prob.wl.B)
+    E.backward()
+    return float(E), V.grad.detach(), eps.detach(), Q.detach()
+
+
+def test_band_energy_gradient_vs_finite_differences(diffw):
+    from dftfe_amd.autograd import band_energy, fermi_occupations
+
+    prob, op, solver, X, lam, history = diffw
+    X0 = X.clone()
+    bounds = solver.bounds()
+    occ, mu = fermi_occupations(torch.tensor(lam, device="cuda"), 0.8 * prob.wl.N, 0.01)
+    E, g, eps, _ = _energy_and_grad(prob, op, X0, bounds, occ)
+    rng = np.random.default_rng(2024)
+    cells = rng.integers(prob.C, size=64)
+    qs = rng.integers(prob.nqc, size=64)
+    results = {}
+    for h in (1e-4, 1e-6):
+        fd = np.zeros(64)
+        for i, (c, q) in enumerate(zip(cells, qs)):
+            vals = []
+            for sgn in (1.0, -1.0):
+                Vp = prob.V_q.copy()
+                Vp[c, q] += sgn * h
+                op.set_potential(Vp * prob.JxW[None, :])
+                with torch.no_grad():
+                    Ep, _, _ = band_energy(op, torch.tensor(Vp, device="cuda"), X0, bounds,
+                                           prob.wl.m, occ, prob.wl.B)
+                vals.append(float(Ep))
+            fd[i] = (vals[0] - vals[1]) / (2 * h)
+        op.set_potential(prob.vJxW)
+        ad = g[torch.tensor(cells), torch.tensor(qs)].cpu().numpy()
+        results[h] = float(np.abs(fd - ad).max() / np.abs(ad).max())
+    print(f"E_band = {E:.10f} Ha, mu = {mu:.6f}; max rel |FD - AD| over 64 points: "
+          f"h=1e-4: {results[1e-4]:.2e}, h=1e-6: {results[1e-6]:.2e}")
+    assert results[1e-4] < 1e-5
+    assert results[1e-6] < 1e-2
+
+
+def test_band_energy_gradient_vs_hellmann_feynman(diffw):
+    """After converging passes the autograd gradient approaches
+    sum_i s f_i |psi_i(q)|^2 JxW(q) (exact only at convergence)."""
+    from dftfe_amd.autograd import fermi_occupations, hellmann_feynman_gradient
+
+    prob, op, solver, X, lam, history = diffw
+    occ, _ = fermi_occupations(torch.tensor(lam, device="cuda"), 0.8 * prob.wl.N, 0.01)
+    E, g, eps, Q = _energy_and_grad(prob, op, X.clone(), solver.bounds(), occ)
+    # Ritz vectors of the differentiated pass, back in the FE basis
+    Hp_vecs = torch.linalg.eigh(Q.T @ op.apply(Q))[1]
+    Xr = torch.as_tensor(prob.inv_sqrt_mass, device="cuda")[:, None] * (Q @ Hp_vecs)
+    ghf = hellmann_feynman_gradient(op, Xr.contiguous(), occ)
+    rel = float((g - ghf).abs().max() / ghf.abs().max())
+    print(f"residual history {['%.1e' % r for r in history]}; max rel |AD - HF| = {rel:.2e}")
+    assert rel < 5e-2

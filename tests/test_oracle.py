@@ -235,3 +235,57 @@ def test_chfsi_converges_to_exact(brute):
         lam, res, X = O.chfsi_pass(ref, X, a0, bLow, bUp, 12, 8)
         a0, bLow = O.next_bounds(lam)
     assert np.abs(lam[:N] - exact).max() < 1e-8
+
+
+# ---------------------------------------------------------------- derivatives
+def test_hx_adjoint_v_finite_differences(brute):
+    """dL/dV of L = <Ybar, H~(V) X> equals central differences (H is linear in V,
+    so the difference quotient is exact up to round-off)."""
+    prob, ref, _ = brute
+    X = prob.initial_guess(N=3, seed_offset=40)
+    Yb = prob.initial_guess(N=3, seed_offset=41)
+    g = O.hx_adjoint_v(ref, Yb, X, 0.7)
+    rng = np.random.default_rng(5)
+    for _ in range(4):
+        c, q = rng.integers(prob.C), rng.integers(prob.nqc)
+        h = 1e-3
+        Vp, Vm = prob.V_q.copy(), prob.V_q.copy()
+        Vp[c, q] += h
+        Vm[c, q] -= h
+        Lp = (Yb * (0.7 * O.with_potential(prob, Vp).apply(X))).sum()
+        Lm = (Yb * (0.7 * O.with_potential(prob, Vm).apply(X))).sum()
+        assert abs((Lp - Lm) / (2 * h) - g[c, q]) < 1e-9 * max(1.0, abs(g[c, q]))
+
+
+def test_filter_backward_finite_differences(brute):
+    prob, ref, _ = brute
+    m, a, b, a0 = 6, 8.0, 60.0, -2.0
+    X0 = prob.sqrt_mass[:, None] * prob.initial_guess(N=2, seed_offset=42)
+    Yb = prob.initial_guess(N=2, seed_offset=43)
+
+    def L(op, X):
+        return (Yb * O.chebyshev_filter(lambda s, f, sc, d: op.HX(s, f, sc, d), X, m, a, b, a0)).sum()
+
+    Xbar, g = O.filter_backward(ref, X0, Yb, m, a, b, a0)
+    rng = np.random.default_rng(6)
+    for _ in range(3):
+        c, q = rng.integers(prob.C), rng.integers(prob.nqc)
+        h = 1e-4
+        Vp, Vm = prob.V_q.copy(), prob.V_q.copy()
+        Vp[c, q] += h
+        Vm[c, q] -= h
+        fd = (L(O.with_potential(prob, Vp), X0) - L(O.with_potential(prob, Vm), X0)) / (2 * h)
+        assert abs(fd - g[c, q]) < 1e-6 * max(1.0, np.abs(g).max())
+    D = prob.initial_guess(N=2, seed_offset=44)
+    h = 1e-6
+    fd = (L(ref, X0 + h * D) - L(ref, X0 - h * D)) / (2 * h)
+    assert abs(fd - (Xbar * D).sum()) < 1e-6 * abs(fd)
+
+
+def test_density_integrates_to_occupation(brute):
+    """With M-orthonormal FE-basis vectors, int rho = sum_i w_i up to the
+    lumped-vs-Gauss quadrature difference; exact for the constant vector."""
+    prob, ref, _ = brute
+    one = np.ones((prob.M, 1))
+    rho = O.density_at_quadrature(prob, one, np.array([1.0]), spin_factor=1.0)
+    assert abs((rho * prob.JxW[None, :]).sum() - prob.wl.L ** 3) < 1e-10
