@@ -66,6 +66,9 @@ def main():
     ap.add_argument("--hx-reps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 passes")
+    ap.add_argument("--no-kpoints", action="store_true",
+                    help="skip the batched k-point filter measurement")
+    ap.add_argument("--kpoint-reps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
@@ -200,9 +203,79 @@ def main():
                  "frac": round(F / hx64_ms / 1e9 / PEAK_TFLOPS["fp64"], 4)},
     }
 
+    if not args.no_kpoints:
+        out["kpoint_batch"] = kpoint_batch(args.kpoint_reps, log)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(prob, B, log)
     print(json.dumps(out), flush=True)
+
+
+def kpoint_batch(reps, log):
+    """Batched k-points: periodic 20 bohr cube, 8^3 cells, p=4 (32,768 DOFs),
+    8 wells, 16 seeded k-points in the first Brillouin zone, complex
+    wavefunctions with band counts N_k uniform in {384..1024}, Chebyshev degree
+    20.  Times one filter pass over every k-point three ways on the same stream:
+    the 16 k-points one after the other with plain launches, all k-points in
+    one batch with plain launches, and the batch replayed as one HIP graph."""
+    import torch
+
+    from dftfe_amd import fe
+    from dftfe_amd import lib as L
+    from dftfe_amd.chfsi import BatchFilter, KohnShamOperatorDevice, lanczos_bounds_complex
+
+    prob = fe.build_workload("cube20_p4_n64")
+    kpts, nvec = fe.kpoint_set(prob, n=16)
+    op = KohnShamOperatorDevice(prob, kpoints=kpts)
+    m = 20
+    dev = "cuda"
+    sq = torch.as_tensor(prob.sqrt_mass, device=dev)
+    gen = torch.Generator(device=dev).manual_seed(5)
+    Xs = [torch.randn(prob.M, int(n), dtype=torch.complex128, device=dev, generator=gen)
+          * sq[:, None] for n in nvec]
+    v0 = torch.as_tensor(prob.lanczos_start.astype(np.complex128), device=dev)
+    bounds = []
+    for k, n in enumerate(nvec):
+        op.reinitkPointSpinIndex(k)
+        lo, up = lanczos_bounds_complex(op, v0, 10)
+        # first-call interval with the REPRODUCIBLE OUTPUT factor (10): the
+        # default factor 200 exceeds bUp when N_k / M > 1/200
+        bounds.append((lo + (up - lo) * n / prob.M * 10.0, up, lo))
+    kidx = list(range(len(nvec)))
+    tune = op.autotune_complex(Xs, kidx, bounds)
+    log(f"complex tile autotune: {tune}")
+    flops = m * sum(4.0 * hx_flops(prob, int(n)) for n in nvec)
+    st = torch.cuda.current_stream()
+
+    def timeit(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            fn()
+        e1.record(st)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    res = {"workload": "cube20_p4, 16 k-points", "dofs": prob.M, "degree": m,
+           "n_vec": [int(n) for n in nvec], "filter_gflop": round(flops / 1e9, 1),
+           "autotune": tune}
+    for prec, name in ((L.FP64, "complex_fp64"), (L.FP32, "complex_fp32")):
+        batch = BatchFilter(op, Xs, kidx, bounds, m, prec)
+        single = [BatchFilter(op, [Xs[k]], [k], [bounds[k]], m, prec) for k in kidx]
+        t_seq = timeit(lambda: [b.launch() for b in single])
+        t_plain = timeit(batch.launch)
+        t_graph = timeit(batch.run)
+        del batch, single
+        log(f"k-point filter {name}: sequential {t_seq:.1f} ms, batched {t_plain:.1f} ms, "
+            f"graph {t_graph:.1f} ms = {flops / t_graph / 1e9:.1f} TFLOP/s")
+        res[name] = {"sequential_ms": round(t_seq, 2), "batched_ms": round(t_plain, 2),
+                     "graph_ms": round(t_graph, 2),
+                     "graph_over_sequential_speedup": round(t_seq / t_graph, 3),
+                     "graph_tflops": round(flops / t_graph / 1e9, 2)}
+    del op, Xs
+    torch.cuda.empty_cache()
+    return res
 
 
 def cpu_baseline(prob, B, log):

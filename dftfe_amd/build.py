@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libchfsi.so")
-SOURCES = ["kernels.hip", "operator.hip", "solver.hip"]
+SOURCES = ["kernels.hip", "kernels_bloch.hip", "operator.hip", "solver.hip", "bloch.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -36,7 +36,10 @@ def _as(arr, dtype):
 class KohnShamOperatorDevice:
     """Device FE Hamiltonian H~ = M^{-1/2} H M^{-1/2} of one rank's full domain."""
 
-    def __init__(self, prob, device: str = "cuda:0"):
+    def __init__(self, prob, device: str = "cuda:0", kpoints=None):
+        """kpoints: optional (n_k, 3) Bloch wave vectors; the operator then also
+        holds the complex cell matrices and projectors of every k-point
+        (d_cellHamiltonianMatrix[kpointSpinIndex] of the reference)."""
         self.lib = L.load()
         self.prob = prob
         self.device = torch.device(device)
@@ -74,6 +77,25 @@ class KohnShamOperatorDevice:
             d.constraint_cols = arr("cc", prob.master_rows, np.int64, ct.c_int64)
             d.constraint_vals = arr("cv", np.ones(ns), np.float64, ct.c_double)
         d.jxw = arr("jxw", np.tile(prob.JxW, prob.C), np.float64, ct.c_double)
+        self.kpoints = None
+        if kpoints is not None:
+            from . import fe
+
+            kp = np.atleast_2d(_as(kpoints, np.float64))
+            self.kpoints = kp
+            d.n_kpoints = len(kp)
+            d.kpoints = arr("kp", kp, np.float64, ct.c_double)
+            d.cell_mass = arr("Mc", prob.M_c, np.float64, ct.c_double)
+            d.cell_grad = arr("Gc", prob.G_c, np.float64, ct.c_double)
+            if len(prob.pair_cell):
+                zc = []
+                for k in kp:
+                    pa, pc, pC = fe.projectors(prob, k)
+                    if not (np.array_equal(pa, prob.pair_atom) and np.array_equal(pc, prob.pair_cell)):
+                        raise ValueError("k-point projector pairs differ from the real ones")
+                    zc.append(pC)
+                zc = np.ascontiguousarray(np.stack(zc), dtype=np.complex128)
+                d.pair_C_kpoints = arr("zC", zc.view(np.float64), np.float64, ct.c_double)
         h = ct.c_void_p()
         L.check(self.lib.chfsi_operator_create(ct.byref(d), ct.byref(h)))
         self.handle = h
@@ -142,6 +164,37 @@ class KohnShamOperatorDevice:
         else:
             raise TypeError("HX supports float64 and float32 blocks")
 
+    # -- complex (Bloch k-point) form -------------------------------------------
+    def reinitkPointSpinIndex(self, k: int, spin: int = 0) -> None:
+        """Selects the k-point the complex calls act on (spin is not modelled)."""
+        L.check(self.lib.chfsi_operator_set_kpoint(self.handle, int(k)))
+
+    def HX_complex(self, src: torch.Tensor, scale_flag: bool, scalar: float,
+                   dst: torch.Tensor) -> None:
+        """Complex HX at the active k-point: complex128 -> FP64, complex64 -> FP32."""
+        if src.dtype == torch.complex128:
+            prec = L.FP64
+        elif src.dtype == torch.complex64:
+            prec = L.FP32
+        else:
+            raise TypeError("complex HX takes complex128 or complex64 blocks")
+        L.check(self.lib.chfsi_hx_complex(self.handle, _ptr(src), src.stride(0), _ptr(dst),
+                                          dst.stride(0), src.shape[1], int(scale_flag),
+                                          float(scalar), prec, _stream()))
+
+    def set_hx_variant_complex(self, precision: int, variant: int) -> None:
+        L.check(self.lib.chfsi_operator_set_hx_variant_complex(self.handle, precision, variant))
+
+    def autotune_complex(self, Xs, kidx, bounds, m: int = 2) -> dict:
+        """Time the complex tile variants on a batch and keep the fastest per precision."""
+        kb, keep = _kbatch(Xs, kidx, bounds, m, L.FP64)
+        b64, b32 = ct.c_int32(), ct.c_int32()
+        t = (ct.c_float * 16)()
+        L.check(self.lib.chfsi_operator_autotune_complex(self.handle, ct.byref(kb), ct.byref(b64),
+                                                         ct.byref(b32), t, _stream()))
+        return {"fp64": b64.value, "fp32": b32.value,
+                "fp64_ms": [x for x in t[0:8] if x >= 0], "fp32_ms": [x for x in t[8:16] if x >= 0]}
+
     def HXCheby(self, src: torch.Tensor, dst: torch.Tensor) -> None:
         """dst <- dst + H src (unscaled FE H; src already M^{-1/2}-scaled)."""
         L.check(self.lib.chfsi_hx_cheby(self.handle, _ptr(src), src.stride(0), _ptr(dst),
@@ -208,6 +261,114 @@ def chebyshev_filter(op: KohnShamOperatorDevice, X: torch.Tensor, m: int, a: flo
     """In-place degree-m filter of a (M x B) block view of a node-major array."""
     L.check(op.lib.chfsi_chebyshev_filter(op.handle, _ptr(X), X.stride(0), X.shape[1], m, a, b,
                                           a0, precision, _stream()))
+
+
+def _kbatch(Xs, kidx, bounds, m: int, precision: int):
+    """chfsi_kbatch for the list of complex128 blocks Xs at k-points kidx;
+    bounds: list of (a, b, a0) per entry.  Returns (struct, arrays to keep alive)."""
+    n = len(Xs)
+    for X in Xs:
+        if X.dtype != torch.complex128 or X.stride(1) != 1:
+            raise TypeError("batched filter blocks must be complex128 row-major views")
+    kk = np.asarray(kidx, dtype=np.int32)
+    ptrs = (ct.c_void_p * n)(*[X.data_ptr() for X in Xs])
+    ld = np.asarray([X.stride(0) for X in Xs], dtype=np.int64)
+    nv = np.asarray([X.shape[1] for X in Xs], dtype=np.int32)
+    bnd = np.asarray(bounds, dtype=np.float64).reshape(n, 3)
+    a, b, a0 = (np.ascontiguousarray(bnd[:, i]) for i in range(3))
+    kb = L.KBatch()
+    kb.n = n
+    kb.kpoint = kk.ctypes.data_as(L.c_int32_p)
+    kb.X = ct.cast(ptrs, ct.POINTER(ct.c_void_p))
+    kb.ld = ld.ctypes.data_as(L.c_int64_p)
+    kb.n_vec = nv.ctypes.data_as(L.c_int32_p)
+    kb.a = a.ctypes.data_as(L.c_double_p)
+    kb.b = b.ctypes.data_as(L.c_double_p)
+    kb.a0 = a0.ctypes.data_as(L.c_double_p)
+    kb.degree = m
+    kb.precision = precision
+    return kb, (kk, ptrs, ld, nv, a, b, a0)
+
+
+def chebyshev_filter_complex(op: KohnShamOperatorDevice, X: torch.Tensor, m: int, a: float,
+                             b: float, a0: float, precision: int = L.FP64) -> None:
+    """In-place degree-m filter of a complex128 block at the active k-point."""
+    L.check(op.lib.chfsi_chebyshev_filter_complex(op.handle, _ptr(X), X.stride(0), X.shape[1], m,
+                                                  a, b, a0, precision, _stream()))
+
+
+def batch_filter(op: KohnShamOperatorDevice, Xs, kidx, bounds, m: int,
+                 precision: int = L.FP64) -> None:
+    """All entries filtered together with plain launches (one launch per colour
+    per degree for the whole batch)."""
+    kb, keep = _kbatch(Xs, kidx, bounds, m, precision)
+    L.check(op.lib.chfsi_batch_filter(op.handle, ct.byref(kb), _stream()))
+
+
+class BatchFilter:
+    """The batched filter captured once into a HIP graph and replayed."""
+
+    def __init__(self, op: KohnShamOperatorDevice, Xs, kidx, bounds, m: int,
+                 precision: int = L.FP64):
+        self.lib = op.lib
+        self.op = op
+        self.n = len(Xs)
+        kb, self._keep = _kbatch(Xs, kidx, bounds, m, precision)
+        h = ct.c_void_p()
+        L.check(self.lib.chfsi_batch_filter_create(op.handle, ct.byref(kb), _stream(),
+                                                   ct.byref(h)))
+        self.handle = h
+
+    def set_bounds(self, bounds) -> None:
+        bnd = np.asarray(bounds, dtype=np.float64).reshape(self.n, 3)
+        a, b, a0 = (np.ascontiguousarray(bnd[:, i]) for i in range(3))
+        L.check(self.lib.chfsi_batch_filter_set_bounds(self.handle, a.ctypes.data_as(L.c_double_p),
+                                                       b.ctypes.data_as(L.c_double_p),
+                                                       a0.ctypes.data_as(L.c_double_p)))
+
+    def run(self) -> None:
+        """Replay the captured graph on the current stream."""
+        L.check(self.lib.chfsi_batch_filter_run(self.handle, _stream()))
+
+    def launch(self) -> None:
+        """Issue the same launches directly (no graph)."""
+        L.check(self.lib.chfsi_batch_filter_launch(self.handle, _stream()))
+
+    def __del__(self):
+        if getattr(self, "handle", None) and self.handle.value:
+            self.lib.chfsi_batch_filter_destroy(self.handle)
+            self.handle = None
+
+
+def lanczos_bounds_complex(op: KohnShamOperatorDevice, v0: torch.Tensor, steps: int,
+                           reproducible: bool = False):
+    lo, up = ct.c_double(), ct.c_double()
+    L.check(op.lib.chfsi_lanczos_bounds_complex(op.handle, _ptr(v0), steps, int(reproducible),
+                                                ct.byref(lo), ct.byref(up), _stream()))
+    return lo.value, up.value
+
+
+def cholgs_complex(op: KohnShamOperatorDevice, X: torch.Tensor) -> None:
+    L.check(op.lib.chfsi_cholgs_complex(op.handle, _ptr(X), X.shape[0], X.shape[1], _stream()))
+
+
+def rayleigh_ritz_complex(op: KohnShamOperatorDevice, X: torch.Tensor, block: int) -> np.ndarray:
+    N = X.shape[1]
+    ev = np.zeros(N)
+    L.check(op.lib.chfsi_rayleigh_ritz_complex(op.handle, _ptr(X), X.shape[0], N, block,
+                                               ev.ctypes.data_as(L.c_double_p), _stream()))
+    return ev
+
+
+def residual_norms_complex(op: KohnShamOperatorDevice, X: torch.Tensor, lam: np.ndarray,
+                           block: int) -> np.ndarray:
+    N = X.shape[1]
+    lam = _as(lam, np.float64)
+    r = np.zeros(N)
+    L.check(op.lib.chfsi_residual_norms_complex(op.handle, _ptr(X), X.shape[0], N, block,
+                                                lam.ctypes.data_as(L.c_double_p),
+                                                r.ctypes.data_as(L.c_double_p), _stream()))
+    return r
 
 
 def lanczos_bounds(op: KohnShamOperatorDevice, v0: torch.Tensor, steps: int,
@@ -308,6 +469,28 @@ class ChebyshevSolverDevice:
                                      int(is_first_filtering_call), int(compute_residual),
                                      int(use_mixed_prec_overall), int(is_first_scf),
                                      ct.byref(bup), t.ctypes.data_as(L.c_double_p), _stream()))
+        self.last_timings_ms = dict(zip(["lanczos", "filter", "cholgs", "rayleigh_ritz",
+                                         "residual", "total"], t.tolist()))
+        return ev, res, bup.value
+
+    def solve_complex(self, op: KohnShamOperatorDevice, X: torch.Tensor, *,
+                      is_first_filtering_call: bool = False, compute_residual: bool = True,
+                      use_mixed_prec_overall: bool = False, is_first_scf: bool = False):
+        """One pass on complex128 X at the operator's active k-point."""
+        if X.dtype != torch.complex128:
+            raise TypeError("solve_complex takes complex128 wavefunctions")
+        M, N = X.shape
+        ev = np.zeros(N)
+        res = np.zeros(N)
+        bup = ct.c_double()
+        t = np.zeros(6)
+        L.check(self.lib.chfsi_solve_complex(self.handle, op.handle, _ptr(X), M, N,
+                                             ev.ctypes.data_as(L.c_double_p),
+                                             res.ctypes.data_as(L.c_double_p),
+                                             int(is_first_filtering_call), int(compute_residual),
+                                             int(use_mixed_prec_overall), int(is_first_scf),
+                                             ct.byref(bup), t.ctypes.data_as(L.c_double_p),
+                                             _stream()))
         self.last_timings_ms = dict(zip(["lanczos", "filter", "cholgs", "rayleigh_ritz",
                                          "residual", "total"], t.tolist()))
         return ev, res, bup.value

@@ -12,6 +12,7 @@
 #include <map>
 #include <vector>
 
+#include "bloch.hpp"
 #include "common.hpp"
 #include "kernels.hpp"
 #include "operator.hpp"
@@ -75,9 +76,13 @@ chfsi_operator::~chfsi_operator() {
                   d_nlC64, d_nlC32,
                   d_nlCt64, d_nlCt32, d_slot_proj, d_proj_ptr, d_proj_slots, d_proj_V,
                   d_nl_cells, d_c_rows, d_c_ptr, d_c_cols, d_c_vals, d_m_rows, d_m_ptr,
-                  d_m_slaves, d_m_vals};
+                  d_m_slaves, d_m_vals, d_Mc, d_Gd, d_Hk64, d_Hk32, d_nlCk64, d_nlCtk64,
+                  d_nlCk32, d_nlCtk32};
   for (void *p : ptrs)
     if (p) hipFree(p);
+  ztab_dev.release();
+  ztab_W.release();
+  ztab_P.release();
   partial64.release();
   partial32.release();
   W64.release();
@@ -214,6 +219,18 @@ extern "C" chfsi_status chfsi_operator_create(const chfsi_operator_desc *d, chfs
     CHFSI_TRY(dev_upload(&op->d_K, d->stiffness, size_t(ne) * ne));
   }
   if (d->jxw) CHFSI_TRY(dev_upload(&op->d_jxw, d->jxw, size_t(C) * op->nq));
+  if (d->n_kpoints > 0) {
+    CHFSI_ARG(d->kpoints && d->cell_mass && d->cell_grad, "k-points need kpoints, cell_mass, cell_grad");
+    CHFSI_ARG(d->n_pairs == 0 || d->pair_C_kpoints, "k-points need pair_C_kpoints");
+    CHFSI_ARG(d->shape_values && d->stiffness, "k-points need shape data for the cell matrices");
+    op->nk = d->n_kpoints;
+    op->kpts.assign(d->kpoints, d->kpoints + 3 * size_t(op->nk));
+    CHFSI_TRY(dev_upload(&op->d_Mc, d->cell_mass, size_t(ne) * ne));
+    CHFSI_TRY(dev_upload(&op->d_Gd, d->cell_grad, size_t(3) * ne * ne));
+    const size_t per_k = size_t(C) * 2 * ne_pad * ne_pad;
+    CHFSI_TRY(dev_alloc(&op->d_Hk64, per_k * op->nk));
+    CHFSI_TRY(dev_alloc(&op->d_Hk32, per_k * op->nk));
+  }
   CHFSI_TRY(dev_alloc(&op->d_Ht64, size_t(C) * ne_pad * ne_pad));
   CHFSI_TRY(dev_alloc(&op->d_Ht32, size_t(C) * ne_pad * ne_pad));
   CHFSI_TRY(dev_alloc(&op->d_Ht16, size_t(C) * ne_pad * ne_pad));
@@ -281,6 +298,39 @@ extern "C" chfsi_status chfsi_operator_create(const chfsi_operator_desc *d, chfs
     CHFSI_TRY(dev_upload(&op->d_nlCt64, nlCt.data(), nlCt.size()));
     for (size_t i = 0; i < nlCt.size(); ++i) tmp32[i] = (float)nlCt[i];
     CHFSI_TRY(dev_upload(&op->d_nlCt32, tmp32.data(), tmp32.size()));
+    if (op->nk > 0) {
+      // complex projector tiles per k-point, paired (Re, Im) layouts
+      const size_t per_k = size_t(2) * S * ne_pad;
+      std::vector<double> zC(per_k * op->nk, 0.0), zCt(per_k * op->nk, 0.0);
+      for (int k = 0; k < op->nk; ++k) {
+        double *zc = zC.data() + per_k * k, *zct = zCt.data() + per_k * k;
+        for (int c = 0; c < C; ++c) {
+          const int s0 = nl_off[c], ns = nl_off[c + 1] - s0;
+          int local = 0;
+          for (int p : pairs_of[c]) {
+            for (int al = 0; al < P; ++al, ++local) {
+              for (int i = 0; i < ne; ++i) {
+                const size_t src = ((size_t(k) * d->n_pairs + p) * ne + i) * P + al;
+                for (int h = 0; h < 2; ++h) {
+                  const double v = d->pair_C_kpoints[2 * src + h];
+                  zc[size_t(s0) * 2 * ne_pad + size_t(2 * (local / 16) + h) * ne_pad * 16 +
+                     size_t(i) * 16 + (local % 16)] = v;
+                  zct[size_t(s0) * 2 * ne_pad + size_t(2 * (i / 16) + h) * ns * 16 +
+                      size_t(local) * 16 + (i % 16)] = v;
+                }
+              }
+            }
+          }
+        }
+      }
+      std::vector<float> z32(zC.size());
+      CHFSI_TRY(dev_upload(&op->d_nlCk64, zC.data(), zC.size()));
+      for (size_t i = 0; i < zC.size(); ++i) z32[i] = (float)zC[i];
+      CHFSI_TRY(dev_upload(&op->d_nlCk32, z32.data(), z32.size()));
+      CHFSI_TRY(dev_upload(&op->d_nlCtk64, zCt.data(), zCt.size()));
+      for (size_t i = 0; i < zCt.size(); ++i) z32[i] = (float)zCt[i];
+      CHFSI_TRY(dev_upload(&op->d_nlCtk32, z32.data(), z32.size()));
+    }
     CHFSI_TRY(dev_alloc(&op->d_nlC16, nlC.size()));
     CHFSI_HIP(launch_f64_to_bf16(op->d_nlC64, (int64_t)nlC.size(), op->d_nlC16, 0));
     CHFSI_TRY(dev_upload(&op->d_proj_ptr, proj_ptr.data(), proj_ptr.size()));
@@ -304,7 +354,7 @@ extern "C" chfsi_status chfsi_operator_set_potential(chfsi_operator *op, const d
   CHFSI_HIP(launch_f64_to_bf16(op->d_Ht64, int64_t(op->C) * op->ne_pad * op->ne_pad, op->d_Ht16,
                                as_stream(stream)));
   op->has_H = true;
-  return CHFSI_OK;
+  return rebuild_kpoint_matrices(op, as_stream(stream));
 }
 
 extern "C" chfsi_status chfsi_operator_set_cell_matrices(chfsi_operator *op, const double *H,
@@ -315,7 +365,7 @@ extern "C" chfsi_status chfsi_operator_set_cell_matrices(chfsi_operator *op, con
   CHFSI_HIP(launch_f64_to_bf16(op->d_Ht64, int64_t(op->C) * op->ne_pad * op->ne_pad, op->d_Ht16,
                                as_stream(stream)));
   op->has_H = true;
-  return CHFSI_OK;
+  return rebuild_kpoint_matrices(op, as_stream(stream));
 }
 
 extern "C" chfsi_status chfsi_operator_get_cell_matrices(chfsi_operator *op, double *H,

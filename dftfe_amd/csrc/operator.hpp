@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <memory>
+#include <vector>
 
 #include "common.hpp"
 
@@ -60,6 +61,22 @@ struct chfsi_operator {
   double *d_c_vals = nullptr;
   int64_t *d_m_rows = nullptr, *d_m_ptr = nullptr, *d_m_slaves = nullptr;
   double *d_m_vals = nullptr;
+  // Bloch k-points: complex cell matrices and projector tiles per k-point in
+  // the paired (Re, Im) tile layout of the complex kernels
+  int nk = 0, active_k = 0;
+  std::vector<double> kpts;  // 3 per k-point
+  double *d_Mc = nullptr, *d_Gd = nullptr;
+  double *d_Hk64 = nullptr;
+  float *d_Hk32 = nullptr;
+  double *d_nlCk64 = nullptr, *d_nlCtk64 = nullptr;
+  float *d_nlCk32 = nullptr, *d_nlCtk32 = nullptr;
+  int zhx_variant64 = 0, zhx_variant32 = 0;
+  chfsi::Scratch ztab_dev, ztab_W, ztab_P;  // tables and scratch of single complex applies
 
   ~chfsi_operator();
 };
+
+namespace chfsi {
+// Rebuilds the complex cell matrices of every k-point from the real tiles.
+chfsi_status rebuild_kpoint_matrices(chfsi_operator *op, hipStream_t s);
+}  // namespace chfsi

@@ -18,34 +18,12 @@
 #include "common.hpp"
 #include "kernels.hpp"
 #include "operator.hpp"
+#include "solver.hpp"
 
 using namespace chfsi;
 
-struct chfsi_solver {
-  chfsi_params params;
-  double a0 = 0, bLow = 0, bUp = 0;
-  rocblas_handle blas = nullptr;
-  Scratch Y64, X32, Y32, HXb, Hp, rot, evals, info, part, lam, v0, split;
-  bool have_v0 = false;
-  ~chfsi_solver() {
-    if (blas) rocblas_destroy_handle(blas);
-    for (Scratch *s : {&Y64, &X32, &Y32, &HXb, &Hp, &rot, &evals, &info, &part, &lam, &v0, &split})
-      s->release();
-  }
-};
+namespace chfsi {
 
-namespace {
-
-#define CHFSI_RB(...)                                                                    \
-  do {                                                                                   \
-    rocblas_status rs_ = (__VA_ARGS__);                                                       \
-    if (rs_ != rocblas_status_success) {                                                 \
-      set_error("%s:%d: %s: %s", __FILE__, __LINE__, #__VA_ARGS__, rocblas_status_to_string(rs_)); \
-      return CHFSI_ERR_BLAS;                                                             \
-    }                                                                                    \
-  } while (0)
-
-// Degree lookup on the upper bound (…SolverDevice.cc:29-46, :96-105).
 unsigned cheby_order_lookup(double upper) {
   static const unsigned table[][2] = {{500, 24},    {750, 30},    {1000, 39},   {1500, 50},
                                       {2000, 53},   {3000, 57},   {4000, 62},   {5000, 69},
@@ -65,7 +43,7 @@ chfsi_status ensure_blas(chfsi_solver *s, hipStream_t st) {
 }
 
 // Sum of `n` doubles of a device array (synchronises the stream).
-chfsi_status host_sum(const double *d, int n, double *out, hipStream_t st) {
+static chfsi_status host_sum(const double *d, int n, double *out, hipStream_t st) {
   std::vector<double> h(n);
   CHFSI_HIP(hipMemcpyAsync(h.data(), d, n * sizeof(double), hipMemcpyDeviceToHost, st));
   CHFSI_HIP(hipStreamSynchronize(st));
@@ -83,7 +61,6 @@ chfsi_status dot(chfsi_solver *s, const double *x, const double *y, int64_t n, d
   return host_sum(static_cast<double *>(s->part.ptr), nch, out, st);
 }
 
-// Eigenvalues of a small symmetric matrix (cyclic Jacobi), ascending.
 std::vector<double> small_sym_eigvals(std::vector<double> A, int n) {
   for (int sweep = 0; sweep < 100; ++sweep) {
     double off = 0;
@@ -114,6 +91,23 @@ std::vector<double> small_sym_eigvals(std::vector<double> A, int n) {
   std::sort(ev.begin(), ev.end());
   return ev;
 }
+
+// seeded uniform [0,1) start (the reference seeds rand() with the rank)
+std::vector<double> default_lanczos_start(int64_t M) {
+  std::vector<double> hv(M);
+  uint64_t state = 0x9E3779B97F4A7C15ull;
+  for (int64_t i = 0; i < M; ++i) {
+    state ^= state << 13;
+    state ^= state >> 7;
+    state ^= state << 17;
+    hv[i] = double(state >> 11) * (1.0 / 9007199254740992.0);
+  }
+  return hv;
+}
+
+}  // namespace chfsi
+
+namespace {
 
 template <typename T>
 T *sptr(Scratch &s) {
@@ -624,15 +618,7 @@ extern "C" chfsi_status chfsi_solve(chfsi_solver *s, chfsi_operator *op, double 
     std::vector<double> hv;
     Scratch gen;
     if (!v0) {
-      // seeded uniform [0,1) start (the reference seeds rand() with the rank)
-      hv.resize(M);
-      uint64_t state = 0x9E3779B97F4A7C15ull;
-      for (int64_t i = 0; i < M; ++i) {
-        state ^= state << 13;
-        state ^= state >> 7;
-        state ^= state << 17;
-        hv[i] = double(state >> 11) * (1.0 / 9007199254740992.0);
-      }
+      hv = default_lanczos_start(M);
       CHFSI_TRY(gen.ensure(sizeof(double) * M));
       CHFSI_HIP(hipMemcpy(gen.ptr, hv.data(), sizeof(double) * M, hipMemcpyHostToDevice));
       v0 = sptr<double>(gen);

@@ -1,0 +1,48 @@
+// Solver state shared by the real and complex solver sources.
+#pragma once
+
+#include <rocblas/rocblas.h>
+
+#include <vector>
+
+#include "common.hpp"
+#include "operator.hpp"
+
+#define CHFSI_RB(...)                                                                    \
+  do {                                                                                   \
+    rocblas_status rs_ = (__VA_ARGS__);                                                  \
+    if (rs_ != rocblas_status_success) {                                                 \
+      ::chfsi::set_error("%s:%d: %s: %s", __FILE__, __LINE__, #__VA_ARGS__,              \
+                         rocblas_status_to_string(rs_));                                 \
+      return CHFSI_ERR_BLAS;                                                             \
+    }                                                                                    \
+  } while (0)
+
+struct chfsi_solver {
+  chfsi_params params;
+  double a0 = 0, bLow = 0, bUp = 0;
+  rocblas_handle blas = nullptr;
+  chfsi::Scratch Y64, X32, Y32, HXb, Hp, rot, evals, info, part, lam, v0, split;
+  bool have_v0 = false;
+  ~chfsi_solver() {
+    if (blas) rocblas_destroy_handle(blas);
+    for (chfsi::Scratch *s : {&Y64, &X32, &Y32, &HXb, &Hp, &rot, &evals, &info, &part, &lam, &v0,
+                              &split})
+      s->release();
+  }
+};
+
+namespace chfsi {
+
+// Degree lookup on the upper bound (…SolverDevice.cc:29-46, :96-105).
+unsigned cheby_order_lookup(double upper);
+chfsi_status ensure_blas(chfsi_solver *s, hipStream_t st);
+// x . y over n doubles (synchronises the stream)
+chfsi_status dot(chfsi_solver *s, const double *x, const double *y, int64_t n, double *out,
+                 hipStream_t st);
+// eigenvalues of a small symmetric matrix (cyclic Jacobi), ascending
+std::vector<double> small_sym_eigvals(std::vector<double> A, int n);
+// seeded uniform [0,1) Lanczos start of length M (host)
+std::vector<double> default_lanczos_start(int64_t M);
+
+}  // namespace chfsi

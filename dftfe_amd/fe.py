@@ -158,7 +158,10 @@ class FEProblem:
     proj_V: np.ndarray        # (n_proj,) coupling constants
     pair_atom: np.ndarray     # (n_pairs,) atom of each (atom, cell) pair
     pair_cell: np.ndarray     # (n_pairs,) cell of each pair
-    pair_C: np.ndarray        # (n_pairs, ne, 4) C_e[i, alpha]
+    pair_C: np.ndarray        # (n_pairs, ne, 4) C_e[i, alpha] (complex when kpoint is set)
+    kpoint: Optional[np.ndarray] = None   # Bloch wave vector (bohr^-1); None = real problem
+    M_c: Optional[np.ndarray] = None      # (ne, ne) consistent cell mass  int N_i N_j
+    G_c: Optional[np.ndarray] = None      # (3, ne, ne) int N_i dN_j/dx_d
 
     @property
     def vJxW(self) -> np.ndarray:
@@ -200,7 +203,13 @@ def _min_image(d: np.ndarray, L: float) -> np.ndarray:
     return d - L * np.round(d / L)
 
 
-def build_problem(wl: Workload, mode: str = "unique") -> FEProblem:
+def build_problem(wl: Workload, mode: str = "unique", kpoint=None) -> FEProblem:
+    """Builds the discretised problem.  With a Bloch wave vector `kpoint` the
+    operator acts on the periodic part u of psi = exp(i k.x) u:
+        H_k[i,j] = int 1/2 grad N_i . grad N_j + (V + |k|^2/2) N_i N_j - i N_i (k . grad N_j)
+    (hermitian after assembly) and the projector element matrices carry the
+    Bloch phase, C_e[i,a] = sum_q N_i beta_a(d_q) exp(-i k.(R_a + d_q)) JxW_q,
+    d_q the minimum-image displacement from the atom."""
     if mode not in ("unique", "dealii"):
         raise ValueError("mode must be 'unique' or 'dealii'")
     n, p, nq, L = wl.n, wl.p, wl.nq, wl.L
@@ -262,6 +271,12 @@ def build_problem(wl: Workload, mode: str = "unique") -> FEProblem:
     K_e = h * (D1[la][:, la] * M1[lb][:, lb] * M1[lc][:, lc]
                + M1[la][:, la] * D1[lb][:, lb] * M1[lc][:, lc]
                + M1[la][:, la] * M1[lb][:, lb] * D1[lc][:, lc])
+    # consistent mass and first-derivative cell matrices (Bloch terms)
+    Q1 = (phi * gw[:, None]).T @ dphi         # int phi_a phi_b' on [0,1]
+    M_c = h ** 3 * M1[la][:, la] * M1[lb][:, lb] * M1[lc][:, lc]
+    G_c = h ** 2 * np.stack([Q1[la][:, la] * M1[lb][:, lb] * M1[lc][:, lc],
+                             M1[la][:, la] * Q1[lb][:, lb] * M1[lc][:, lc],
+                             M1[la][:, la] * M1[lb][:, lb] * Q1[lc][:, lc]])
 
     # ---- atoms, wells
     rng = np.random.default_rng(wl.seed)
@@ -281,6 +296,8 @@ def build_problem(wl: Workload, mode: str = "unique") -> FEProblem:
         n_proj=4 * wl.n_atoms, proj_V=proj_V,
         pair_atom=np.zeros(0, np.int64), pair_cell=np.zeros(0, np.int64),
         pair_C=np.zeros((0, ne, 4)),
+        kpoint=None if kpoint is None else np.asarray(kpoint, dtype=np.float64),
+        M_c=M_c, G_c=G_c,
     )
 
     # ---- local potential V(x) = -sum_a D_a exp(-|x-R_a|^2 / w_a^2)
@@ -310,33 +327,79 @@ def build_problem(wl: Workload, mode: str = "unique") -> FEProblem:
     prob.sqrt_mass = np.where(ok, np.sqrt(np.maximum(mass, 0.0)), 0.0)
     prob.inv_sqrt_mass = np.where(ok & (mass > 1e-15), 1.0 / np.sqrt(np.where(mass > 0, mass, 1.0)), 0.0)
 
-    # ---- Gaussian s + p projectors, truncated at r <= rcut_factor * r_c
-    centres = (cell_coords.astype(np.float64) + 0.5) * h
+    prob.pair_atom, prob.pair_cell, prob.pair_C = projectors(prob, prob.kpoint)
+    return prob
+
+
+def projectors(prob: FEProblem, kpoint=None):
+    """Gaussian s + p projector element matrices, truncated at r <= rcut_factor * r_c.
+
+    Returns (pair_atom, pair_cell, pair_C).  With a Bloch wave vector the
+    element matrices are complex:  C_e[i,a] = sum_q N_i(q) beta_a(d_q)
+    exp(-i k.(R + d_q)) JxW_q, where d_q is the minimum-image displacement of
+    quadrature point q from the atom at R (the Bloch sum over periodic images
+    of a projector whose support is below half the cell side); the nonlocal
+    operator on the periodic part is then C V C^H."""
+    wl, h, L = prob.wl, prob.h, prob.wl.L
+    ne, nqc = prob.ne, prob.nqc
+    kvec = None if kpoint is None else np.asarray(kpoint, dtype=np.float64)
+    NqJ = (prob.Nq * prob.JxW[:, None]).T                               # (ne, nqc)
+    centres = (prob.cell_coords.astype(np.float64) + 0.5) * h
     pair_atom, pair_cell, pair_C = [], [], []
     for a in range(wl.n_atoms):
-        rc = atom_width[a]
+        rc = prob.atom_width[a]
         rcut = wl.rcut_factor * rc
-        dc = _min_image(centres - atom_pos[a], L)
+        dc = _min_image(centres - prob.atom_pos[a], L)
         sel = np.nonzero(np.sqrt((dc ** 2).sum(1)) <= rcut + h * math.sqrt(3.0) / 2.0)[0]
         if len(sel) == 0:
             continue
         xq = prob.quad_points(sel)                       # (s, nqc, 3)
-        d = _min_image(xq - atom_pos[a], L)
+        d = _min_image(xq - prob.atom_pos[a], L)
         r2 = (d ** 2).sum(-1)
         g = np.exp(-r2 / (2.0 * rc * rc)) * (r2 <= rcut * rcut)
         beta = np.stack([g, d[..., 0] * g, d[..., 1] * g, d[..., 2] * g], axis=-1)  # (s, nqc, 4)
+        if kvec is not None:
+            phase = np.exp(-1j * ((prob.atom_pos[a][None, None, :] + d) @ kvec))
+            beta = beta * phase[..., None]
         B2 = beta.transpose(1, 0, 2).reshape(nqc, -1)                   # (nqc, s*4)
-        Ce = ((Nq * JxW[:, None]).T @ B2).reshape(ne, len(sel), 4).transpose(1, 0, 2)
+        Ce = (NqJ @ B2).reshape(ne, len(sel), 4).transpose(1, 0, 2)
         keep = np.abs(Ce).reshape(len(sel), -1).max(axis=1) > 0.0
         pair_atom.append(np.full(int(keep.sum()), a, dtype=np.int64))
         pair_cell.append(sel[keep].astype(np.int64))
         pair_C.append(Ce[keep])
-    if pair_atom:
-        prob.pair_atom = np.concatenate(pair_atom)
-        prob.pair_cell = np.concatenate(pair_cell)
-        prob.pair_C = np.concatenate(pair_C, axis=0)
-    return prob
+    if not pair_atom:
+        dt = np.float64 if kvec is None else np.complex128
+        return np.zeros(0, np.int64), np.zeros(0, np.int64), np.zeros((0, ne, 4), dt)
+    return (np.concatenate(pair_atom), np.concatenate(pair_cell),
+            np.concatenate(pair_C, axis=0))
 
 
-def build_workload(name: str, mode: str = "unique") -> FEProblem:
-    return build_problem(WORKLOADS[name], mode)
+def with_kpoint(prob: FEProblem, kpoint) -> FEProblem:
+    """The same problem at Bloch wave vector `kpoint` (only the projectors
+    change; the k-dependent cell terms are formed from M_c and G_c)."""
+    k = np.asarray(kpoint, dtype=np.float64)
+    pa, pc, pC = projectors(prob, k)
+    return dataclasses.replace(prob, kpoint=k, pair_atom=pa, pair_cell=pc, pair_C=pC)
+
+
+def kpoint_set(prob: FEProblem, n: int = 16, seed: Optional[int] = None,
+               n_min: int = 384, n_max: int = 1024):
+    """Seeded k-points uniform in the first Brillouin zone [-pi/L, pi/L)^3 and
+    band counts N_k uniform in {n_min..n_max}."""
+    rng = np.random.default_rng((prob.wl.seed if seed is None else seed) + 4242)
+    L = prob.wl.L
+    k = rng.uniform(-math.pi / L, math.pi / L, size=(n, 3))
+    nb = rng.integers(n_min, n_max + 1, size=n)
+    return k, nb
+
+
+def complex_guess(prob: FEProblem, N: int, seed_offset: int = 0) -> np.ndarray:
+    """Seeded complex start: real and imaginary parts U[-1,1]; constrained rows 0."""
+    rng = np.random.default_rng(prob.wl.seed + 97 + seed_offset)
+    X = rng.uniform(-1.0, 1.0, size=(prob.M, N)) + 1j * rng.uniform(-1.0, 1.0, size=(prob.M, N))
+    X[prob.constrained, :] = 0.0
+    return X
+
+
+def build_workload(name: str, mode: str = "unique", kpoint=None) -> FEProblem:
+    return build_problem(WORKLOADS[name], mode, kpoint)

@@ -43,6 +43,16 @@ class OperatorDesc(ct.Structure):
         ("constraint_ptr", c_int64_p), ("constraint_cols", c_int64_p),
         ("constraint_vals", c_double_p),
         ("jxw", c_double_p),
+        ("n_kpoints", ct.c_int32), ("kpoints", c_double_p), ("cell_mass", c_double_p),
+        ("cell_grad", c_double_p), ("pair_C_kpoints", c_double_p),
+    ]
+
+
+class KBatch(ct.Structure):
+    _fields_ = [
+        ("n", ct.c_int32), ("kpoint", c_int32_p), ("X", ct.POINTER(ct.c_void_p)),
+        ("ld", c_int64_p), ("n_vec", c_int32_p), ("a", c_double_p), ("b", c_double_p),
+        ("a0", c_double_p), ("degree", ct.c_int32), ("precision", ct.c_int32),
     ]
 
 
@@ -108,6 +118,33 @@ SIGNATURES = {
     "chfsi_solve": (ct.c_int, [_VP, _VP, _VP, ct.c_int64, ct.c_int32, c_double_p, c_double_p,
                                ct.c_int32, ct.c_int32, ct.c_int32, ct.c_int32, c_double_p,
                                c_double_p, _VP]),
+    # complex (Bloch k-point) form
+    "chfsi_operator_set_kpoint": (ct.c_int, [_VP, ct.c_int32]),
+    "chfsi_operator_n_kpoints": (ct.c_int32, [_VP]),
+    "chfsi_hx_complex": (ct.c_int, [_VP, _VP, ct.c_int64, _VP, ct.c_int64, ct.c_int32,
+                                    ct.c_int32, ct.c_double, ct.c_int32, _VP]),
+    "chfsi_chebyshev_filter_complex": (ct.c_int, [_VP, _VP, ct.c_int64, ct.c_int32, ct.c_int32,
+                                                  ct.c_double, ct.c_double, ct.c_double,
+                                                  ct.c_int32, _VP]),
+    "chfsi_batch_filter": (ct.c_int, [_VP, ct.POINTER(KBatch), _VP]),
+    "chfsi_batch_filter_create": (ct.c_int, [_VP, ct.POINTER(KBatch), _VP, ct.POINTER(_VP)]),
+    "chfsi_batch_filter_set_bounds": (ct.c_int, [_VP, c_double_p, c_double_p, c_double_p]),
+    "chfsi_batch_filter_run": (ct.c_int, [_VP, _VP]),
+    "chfsi_batch_filter_launch": (ct.c_int, [_VP, _VP]),
+    "chfsi_batch_filter_destroy": (None, [_VP]),
+    "chfsi_operator_set_hx_variant_complex": (ct.c_int, [_VP, ct.c_int32, ct.c_int32]),
+    "chfsi_operator_autotune_complex": (ct.c_int, [_VP, ct.POINTER(KBatch), c_int32_p, c_int32_p,
+                                                   ct.POINTER(ct.c_float), _VP]),
+    "chfsi_lanczos_bounds_complex": (ct.c_int, [_VP, _VP, ct.c_int32, ct.c_int32, c_double_p,
+                                                c_double_p, _VP]),
+    "chfsi_cholgs_complex": (ct.c_int, [_VP, _VP, ct.c_int64, ct.c_int32, _VP]),
+    "chfsi_rayleigh_ritz_complex": (ct.c_int, [_VP, _VP, ct.c_int64, ct.c_int32, ct.c_int32,
+                                               c_double_p, _VP]),
+    "chfsi_residual_norms_complex": (ct.c_int, [_VP, _VP, ct.c_int64, ct.c_int32, ct.c_int32,
+                                                c_double_p, c_double_p, _VP]),
+    "chfsi_solve_complex": (ct.c_int, [_VP, _VP, _VP, ct.c_int64, ct.c_int32, c_double_p,
+                                       c_double_p, ct.c_int32, ct.c_int32, ct.c_int32,
+                                       ct.c_int32, c_double_p, c_double_p, _VP]),
 }
 
 _lib = None

@@ -88,7 +88,35 @@ typedef struct {
   /* quadrature weights JxW per (cell, quadrature point), C*n_q; nullable when
    * neither the density nor the potential adjoint is used */
   const double *jxw;
+  /* Bloch k-points (complex form; kohnShamDFTOperatorDeviceClass keeps one
+   * set of cell matrices per k-point, d_cellHamiltonianMatrix[kpointSpinIndex],
+   * hamiltonianMatrixCalculator.cc:562-872).  n_kpoints == 0: real operator only.
+   * The operator acts on the periodic part u of psi = exp(i k.x) u:
+   *   H_k,e = H_e + |k|^2/2 Mc - i sum_d k_d Gd,   nonlocal C_k V C_k^H. */
+  int32_t n_kpoints;
+  const double *kpoints;          /* 3*n_kpoints, Cartesian (bohr^-1)               */
+  const double *cell_mass;        /* n_e*n_e: int N_i N_j (all cells)                */
+  const double *cell_grad;        /* 3*n_e*n_e: [d][i][j] = int N_i dN_j/dx_d         */
+  const double *pair_C_kpoints;   /* n_kpoints*n_pairs*n_e*proj_per_pair complex,
+                                     interleaved (re, im): C_e at each k-point       */
 } chfsi_operator_desc;
+
+/* Batch of k-point blocks filtered together (complex form). */
+typedef struct {
+  int32_t n;                /* entries (at most 64)                                   */
+  const int32_t *kpoint;    /* n: operator k-point of each entry                      */
+  void *const *X;           /* n: device pointers, complex double node-major M x ld   */
+  const int64_t *ld;        /* n: leading dimensions (complex elements)               */
+  const int32_t *n_vec;     /* n: columns filtered (band counts N_k)                  */
+  const double *a;          /* n: lower end of the damped interval (bLow)             */
+  const double *b;          /* n: upper bound (bUp)                                   */
+  const double *a0;         /* n: scaling point (lowest Ritz value)                   */
+  int32_t degree;           /* Chebyshev degree m                                     */
+  int32_t precision;        /* CHFSI_FP64: complex double; CHFSI_FP32: complex float
+                               filter between one cast in and one cast out           */
+} chfsi_kbatch;
+
+typedef struct chfsi_batch chfsi_batch;
 
 /* Solver parameters: the ChFSI knobs of dftParameters
  * (utils/dftParameters.cc:840-1021). */
@@ -253,6 +281,84 @@ chfsi_status chfsi_solve(chfsi_solver *solver, chfsi_operator *op, double *X, in
                          int32_t is_first_filtering_call, int32_t compute_residual,
                          int32_t use_mixed_prec_overall, int32_t is_first_scf, double *bUp_out,
                          double *timings_ms, void *stream);
+
+/* ---------------- complex (Bloch k-point) form ----------------
+ * Complex multivectors are std::complex<double> (or <float>) node-major,
+ * X[node * ld + vec], ld and widths counted in complex elements.  The calls
+ * below act on the operator's active k-point (chfsi_operator_set_kpoint),
+ * like the reference's operator after reinitkPointSpinIndex; the solver
+ * routines are the USE_COMPLEX instantiations of the real ones. */
+
+/* reinitkPointSpinIndex (operatorDevice.h:148-149; kohnShamDFTOperatorDevice.cc:1037-1057). */
+chfsi_status chfsi_operator_set_kpoint(chfsi_operator *op, int32_t kpoint);
+int32_t chfsi_operator_n_kpoints(const chfsi_operator *op);
+
+/* HX of the active k-point (kohnShamDFTOperatorDevice.cc:3765-3860, complex):
+ *   dst <- (scale_flag ? dst : M^{-1/2} dst) + scalar * M^{-1/2} H_k M^{-1/2} src.
+ * precision CHFSI_FP64: complex double; CHFSI_FP32: complex float (both multivectors). */
+chfsi_status chfsi_hx_complex(chfsi_operator *op, const void *src, int64_t ld_src, void *dst,
+                              int64_t ld_dst, int32_t B, int32_t scale_flag, double scalar,
+                              int32_t precision, void *stream);
+
+/* chebyshevFilter of a complex double block at the active k-point
+ * (linearAlgebraOperationsDevice.cc:531-727, complex); precision as chfsi_chebyshev_filter. */
+chfsi_status chfsi_chebyshev_filter_complex(chfsi_operator *op, void *X, int64_t ld, int32_t B,
+                                            int32_t m, double a, double b, double a0,
+                                            int32_t precision, void *stream);
+
+/* Batched filter over k-points (no reference counterpart: the reference
+ * filters one k-point after the other, src/dft/dft.cc:3077-3130 calling
+ * kohnShamEigenSolve.cc:335-366).  Every degree is one projection launch,
+ * one reduction launch and eight colour launches covering all entries.
+ *   chfsi_batch_filter         -- plain launches, immediately;
+ *   chfsi_batch_filter_create  -- prepares tables and scratch and captures the
+ *                                 whole filter into one HIP graph on `stream`;
+ *   chfsi_batch_filter_set_bounds -- new (a, b, a0) per entry (graph kept);
+ *   chfsi_batch_filter_run     -- replays the graph on `stream`;
+ *   chfsi_batch_filter_launch  -- issues the prepared launches without the graph.
+ * The X pointers, widths and degree are fixed at creation. */
+chfsi_status chfsi_batch_filter(chfsi_operator *op, const chfsi_kbatch *batch, void *stream);
+chfsi_status chfsi_batch_filter_create(chfsi_operator *op, const chfsi_kbatch *batch,
+                                       void *stream, chfsi_batch **out);
+chfsi_status chfsi_batch_filter_set_bounds(chfsi_batch *bt, const double *a, const double *b,
+                                           const double *a0);
+chfsi_status chfsi_batch_filter_run(chfsi_batch *bt, void *stream);
+/* The same launches issued directly on `stream` (no graph). */
+chfsi_status chfsi_batch_filter_launch(chfsi_batch *bt, void *stream);
+void chfsi_batch_filter_destroy(chfsi_batch *bt);
+
+/* Tile variant of the complex HX kernel (as chfsi_operator_set_hx_variant). */
+chfsi_status chfsi_operator_set_hx_variant_complex(chfsi_operator *op, int32_t precision,
+                                                   int32_t variant);
+
+/* Tile-variant timing of the complex HX on a batch (as chfsi_operator_autotune):
+ * keeps the fastest variant per precision; times_ms[0..7] FP64, [8..15] FP32. */
+chfsi_status chfsi_operator_autotune_complex(chfsi_operator *op, const chfsi_kbatch *batch,
+                                             int32_t *best64, int32_t *best32, float *times_ms,
+                                             void *stream);
+
+/* Complex counterparts of the dense steps at the active k-point
+ * (pseudoGSDevice.cc:81-463, rayleighRitzDevice.cc:80-353,
+ * linearAlgebraOperationsDevice.cc:340-527 and :4610-4766, complex):
+ * S = X^H X = L L^H, X <- X L^{-H};  Hp = X^H H~ X, X <- X Q;  residual norms. */
+chfsi_status chfsi_lanczos_bounds_complex(chfsi_operator *op, const void *v0_dev, int32_t steps,
+                                          int32_t reproducible, double *lower, double *upper,
+                                          void *stream);
+chfsi_status chfsi_cholgs_complex(chfsi_operator *op, void *X, int64_t M, int32_t N,
+                                  void *stream);
+chfsi_status chfsi_rayleigh_ritz_complex(chfsi_operator *op, void *X, int64_t M, int32_t N,
+                                         int32_t block, double *eig_host, void *stream);
+chfsi_status chfsi_residual_norms_complex(chfsi_operator *op, const void *X, int64_t M,
+                                          int32_t N, int32_t block, const double *eig_host,
+                                          double *res_host, void *stream);
+
+/* solve with complex double X at the active k-point (…SolverDevice.cc:155-736,
+ * USE_COMPLEX); arguments as chfsi_solve. */
+chfsi_status chfsi_solve_complex(chfsi_solver *solver, chfsi_operator *op, void *X, int64_t M,
+                                 int32_t N, double *eig_host, double *res_host,
+                                 int32_t is_first_filtering_call, int32_t compute_residual,
+                                 int32_t use_mixed_prec_overall, int32_t is_first_scf,
+                                 double *bUp_out, double *timings_ms, void *stream);
 
 #ifdef __cplusplus
 }

@@ -58,10 +58,31 @@ def cell_hamiltonians(prob, V_q: Optional[np.ndarray] = None, cells=None) -> np.
     K = np.einsum("q,qid,qjd->ij", prob.JxW, G, G)
     vJxW = V_q[cells] * prob.JxW[None, :]
     Nt = prob.Nq.T
-    H = np.empty((len(cells), prob.ne, prob.ne))
+    shift = 0.0
+    if prob.kpoint is not None:
+        shift = bloch_cell_terms(prob, G)
+    H = np.empty((len(cells), prob.ne, prob.ne), dtype=np.result_type(np.float64, shift))
     for k in range(len(cells)):
-        H[k] = 0.5 * K + (Nt * vJxW[k][None, :]) @ prob.Nq
+        H[k] = 0.5 * K + (Nt * vJxW[k][None, :]) @ prob.Nq + shift
     return H
+
+
+def bloch_cell_terms(prob, G: Optional[np.ndarray] = None) -> np.ndarray:
+    """k-dependent part of the cell Hamiltonian acting on the periodic part u of
+    psi = exp(i k.x) u:  1/2 |k|^2 int N_i N_j  -  i int N_i (k . grad N_j).
+
+    Reference: src/dftOperator/hamiltonianMatrixCalculator.cc:562-571 and
+    :790-797 (real part, kSquareTimesHalf * NiNj), :577-735 and :851-872
+    (imaginary part from d_invJacKPointTimesJxW, shapeFunctionDataCalculator.cc:190-224),
+    stored [iNode][jNode] and applied transposed by the cell GEMM
+    (matrixVectorProductImplementations.cc:66-78).  Quadrature sums point by
+    point from the 3-D shape gradients."""
+    G = shape_gradients(prob) if G is None else G
+    k = prob.kpoint
+    Mq = (prob.Nq * prob.JxW[:, None]).T @ prob.Nq                  # int N_i N_j
+    kG = G @ k                                                      # (nqc, ne) k . grad N_j
+    Dq = (prob.Nq * prob.JxW[:, None]).T @ kG                       # int N_i k.grad N_j
+    return 0.5 * float(k @ k) * Mq - 1j * Dq
 
 
 # ----------------------------------------------------------------------------
@@ -99,7 +120,7 @@ class OracleOperator:
         if len(prob.pair_cell):
             cnt = np.bincount(prob.pair_cell, minlength=C)
             S = 4 * int(cnt.max())
-            self.Cc = np.zeros((C, ne, S))
+            self.Cc = np.zeros((C, ne, S), dtype=prob.pair_C.dtype)
             self.slot_proj = np.full((C, S), prob.n_proj, dtype=np.int64)
             fill = np.zeros(C, dtype=np.int64)
             for q, c in enumerate(prob.pair_cell):
@@ -135,23 +156,28 @@ class OracleOperator:
         """Unscaled FE H (local + nonlocal) on an already-distributed block."""
         p = self.prob
         B = Xs.shape[1]
-        Xc = Xs[p.cell_map]                                          # (C, ne, B) gather
+        dt = np.result_type(Xs.dtype, self.H_e.dtype)
+        Xc = Xs[p.cell_map].astype(dt, copy=False)                   # (C, ne, B) gather
         Yc = np.empty_like(Xc)
         for c0 in range(0, p.C, self.cell_chunk):
             c1 = min(p.C, c0 + self.cell_chunk)
             Yc[c0:c1] = np.matmul(self.H_e[c0:c1], Xc[c0:c1])
         if self.Cc is not None:
-            # P_alpha = sum over the atom's cells of C_e[:, alpha]^T X_e, then
-            # Y_e += C_e V P for every cell in the support.
+            # P_alpha = sum over the atom's cells of C_e[:, alpha]^H X_e, then
+            # Y_e += C_e V P for every cell in the support
+            # (complex: computeNonLocalHamiltonianTimesXMemoryOpt.cc:25-264).
             S = self.Cc.shape[2]
-            Pc = np.matmul(self.Cc.transpose(0, 2, 1), Xc)           # (C, S, B)
+            CcH = self.Cc.transpose(0, 2, 1)
+            if np.iscomplexobj(CcH):
+                CcH = CcH.conj()
+            Pc = np.matmul(CcH, Xc)                                  # (C, S, B)
             W = p.proj_V[:, None] * (self.R @ Pc.reshape(p.C * S, B))
-            W = np.vstack([W, np.zeros((1, B))])                      # row for empty slots
+            W = np.vstack([W, np.zeros((1, B), dtype=W.dtype)])       # row for empty slots
             Yc += np.matmul(self.Cc, W[self.slot_proj])               # (C, ne, B)
         # scatter-add (matrixVectorProductImplementations.cc:127-168 adds each
         # cell's rows into the global vector); looping over the local node index
         # keeps each vectorised add free of repeated DOFs.
-        Y = np.zeros((p.M, B))
+        Y = np.zeros((p.M, B), dtype=Yc.dtype)
         for i in range(p.ne):
             Y[p.cell_map[:, i]] += Yc[:, i, :]
         return Y
@@ -186,7 +212,7 @@ class OracleOperator:
             rows = np.repeat(p.cell_map[p.pair_cell], 4, axis=1).ravel()
             cols = np.repeat(self.pair_proj[:, None, :], ne, axis=1).ravel()
             Cg = sp.coo_matrix((p.pair_C.ravel(), (rows, cols)), shape=(p.M, p.n_proj)).tocsr()
-            H = H + Cg @ sp.diags(p.proj_V) @ Cg.T
+            H = H + Cg @ sp.diags(p.proj_V) @ Cg.conj().T
         if len(p.slave_rows):
             # P^T H P with P the periodic prolongation (slaves copy masters)
             keep = np.nonzero(~p.constrained)[0]
@@ -214,7 +240,7 @@ def lanczos_bounds(op, v0: np.ndarray, steps: int, reproducible: bool = False):
     op.set_zero(v[:, None]) if v.ndim == 1 else None
     v = v / np.linalg.norm(v)
     f = op.apply(v[:, None])[:, 0]
-    alpha = f @ v
+    alpha = np.vdot(v, f).real
     f = f - alpha * v
     T = np.zeros((steps, steps))
     T[0, 0] = alpha
@@ -224,7 +250,7 @@ def lanczos_bounds(op, v0: np.ndarray, steps: int, reproducible: bool = False):
         v = f / beta
         f = op.apply(v[:, None])[:, 0]
         f = f - beta * v_old
-        alpha = f @ v
+        alpha = np.vdot(v, f).real
         f = f - alpha * v
         T[j, j - 1] = beta
         T[j - 1, j] = beta
@@ -298,9 +324,10 @@ def chebyshev_scalar(lam: np.ndarray, m: int, a: float, b: float, a0: float) -> 
 def cholgs(X: np.ndarray) -> np.ndarray:
     """S = X^T X, S = L L^T, X <- X L^{-T}.  Reference: src/linAlg/pseudoGS.cc:29-262;
     device src/linAlg/pseudoGSDevice.cc:121-368."""
-    S = X.T @ X
+    XH = X.conj().T
+    S = XH @ X
     L = np.linalg.cholesky(S)
-    return sla.solve_triangular(L, X.T, lower=True).T
+    return sla.solve_triangular(L, XH, lower=True).conj().T
 
 
 def rayleigh_ritz(op, X: np.ndarray, block: int):
@@ -308,11 +335,12 @@ def rayleigh_ritz(op, X: np.ndarray, block: int):
     Reference: src/linAlg/rayleighRitzDevice.cc:80-353 (SHEP) and the XtHX
     blocks of kohnShamDFTOperatorDevice.cc:4000-4157."""
     N = X.shape[1]
-    Hp = np.zeros((N, N))
+    XH = X.conj().T
+    Hp = np.zeros((N, N), dtype=X.dtype)
     for j in range(0, N, block):
         HXb = op.apply(X[:, j:j + block])
-        Hp[:, j:j + block] = X.T @ HXb
-    Hp = 0.5 * (Hp + Hp.T)
+        Hp[:, j:j + block] = XH @ HXb
+    Hp = 0.5 * (Hp + Hp.conj().T)
     lam, Q = np.linalg.eigh(Hp)
     return lam, X @ Q
 
@@ -321,15 +349,16 @@ def rayleigh_ritz_gep(op, X: np.ndarray, block: int):
     """CPU-default generalized route: S = L L^T, Hhat = L^{-1} Hp L^{-T}, eig,
     X <- X L^{-T} Q.  Reference: src/linAlg/linearAlgebraOperationsOpt.cc:613-969."""
     N = X.shape[1]
-    S = X.T @ X
-    Hp = np.zeros((N, N))
+    XH = X.conj().T
+    S = XH @ X
+    Hp = np.zeros((N, N), dtype=X.dtype)
     for j in range(0, N, block):
-        Hp[:, j:j + block] = X.T @ op.apply(X[:, j:j + block])
-    Hp = 0.5 * (Hp + Hp.T)
+        Hp[:, j:j + block] = XH @ op.apply(X[:, j:j + block])
+    Hp = 0.5 * (Hp + Hp.conj().T)
     L = np.linalg.cholesky(S)
     Linv = sla.solve_triangular(L, np.eye(N), lower=True)
-    lam, Q = np.linalg.eigh(Linv @ Hp @ Linv.T)
-    return lam, X @ (Linv.T @ Q)
+    lam, Q = np.linalg.eigh(Linv @ Hp @ Linv.conj().T)
+    return lam, X @ (Linv.conj().T @ Q)
 
 
 def residual_norms(op, X: np.ndarray, lam: np.ndarray, block: int) -> np.ndarray:
@@ -339,7 +368,7 @@ def residual_norms(op, X: np.ndarray, lam: np.ndarray, block: int) -> np.ndarray
     r = np.zeros(N)
     for j in range(0, N, block):
         R = op.apply(X[:, j:j + block]) - X[:, j:j + block] * lam[None, j:j + block]
-        r[j:j + block] = np.sqrt((R * R).sum(axis=0))
+        r[j:j + block] = np.sqrt((R.real ** 2 + R.imag ** 2).sum(axis=0))
     return r
 
 

@@ -1,0 +1,162 @@
+"""Pins of the oracle's complex Bloch (k-point) form (no GPU).
+
+The operator acts on the periodic part u of psi = exp(i k.x) u.  Its pins:
+  * the k-dependent cell terms computed point by point (oracle) agree with the
+    tensor-factored consistent mass and derivative matrices (generator);
+  * k = 0 reduces to the real operator;
+  * the assembled operator is hermitian and equals the cell-batched apply;
+  * free particle: u = const is an exact eigenvector with eigenvalue |k|^2/2,
+    the next levels are the plane waves |k + G|^2 / 2;
+  * time reversal: H(-k) = conj(H(k)), identical spectra;
+  * lattice-vector covariance: spec H(k + G) = spec H(k) up to the
+    discretisation error.  The relative sign of the kinetic Bloch term and the
+    projector phase only shows here: a planted sign error fails it.
+  * complex CholGS / Rayleigh-Ritz / the pass against dense linear algebra.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from dftfe_amd import fe
+from oracle import chfsi_oracle as O
+
+K0 = np.array([0.13, -0.07, 0.05])
+
+
+@pytest.fixture(scope="module")
+def base():
+    return fe.build_workload("brute")
+
+
+@pytest.fixture(scope="module")
+def kprob(base):
+    prob = fe.with_kpoint(base, K0)
+    ref = O.OracleOperator(prob)
+    H = ref.sparse_matrix().toarray()
+    return prob, ref, H
+
+
+def _spectrum(prob, planted=False, n=6):
+    ref = O.OracleOperator(prob)
+    if planted:
+        # planted mistake: kinetic Bloch term with the opposite sign of k
+        ref.H_e = ref.H_e.real - 1j * ref.H_e.imag
+    H = ref.sparse_matrix().toarray()
+    keep = np.nonzero(~prob.constrained)[0]
+    return np.linalg.eigvalsh(H[np.ix_(keep, keep)])[:n]
+
+
+def test_bloch_terms_two_ways(base):
+    """Point-by-point quadrature vs tensor-factored 1-D products."""
+    prob = fe.with_kpoint(base, K0)
+    T = O.bloch_cell_terms(prob)
+    want = 0.5 * (K0 @ K0) * prob.M_c - 1j * np.einsum("d,dij->ij", K0, prob.G_c)
+    assert np.abs(T - want).max() < 1e-13 * np.abs(want).max()
+    # int N_i dN_j / dx_d summed over j vanishes (partition of unity)
+    assert np.abs(prob.G_c.sum(axis=2)).max() < 1e-13
+
+
+def test_k_zero_reduces_to_real(base):
+    prob0 = fe.with_kpoint(base, np.zeros(3))
+    ref0 = O.OracleOperator(prob0)
+    refr = O.OracleOperator(base)
+    assert np.abs(ref0.H_e.imag).max() == 0.0
+    X = base.initial_guess(N=4)
+    assert np.abs(ref0.apply(X.astype(np.complex128)) - refr.apply(X)).max() < 1e-13 * np.abs(
+        refr.apply(X)).max()
+
+
+def test_complex_hx_equals_assembled_and_hermitian(kprob):
+    prob, ref, H = kprob
+    X = fe.complex_guess(prob, 5)
+    Y = ref.apply(X)
+    assert np.abs(Y - H @ X).max() < 1e-13 * np.abs(Y).max()
+    assert np.abs(H - H.conj().T).max() < 1e-13 * np.abs(H).max()
+    assert np.abs(H.imag).max() > 1e-3 * np.abs(H).max()
+    # HX flag semantics carry over
+    D = fe.complex_guess(prob, 5, seed_offset=1)
+    s = prob.inv_sqrt_mass[:, None]
+    assert np.allclose(ref.HX(X, False, 0.7, D), s * D + 0.7 * (H @ X), atol=1e-12)
+    assert np.allclose(ref.HX(X, True, 0.7, D), D + 0.7 * (H @ X), atol=1e-12)
+
+
+def test_free_particle_bloch(base):
+    prob = fe.with_kpoint(base, K0)
+    prob.V_q = np.zeros_like(prob.V_q)
+    prob.pair_cell = prob.pair_cell[:0]
+    prob.pair_atom = prob.pair_atom[:0]
+    prob.pair_C = prob.pair_C[:0]
+    ev = _spectrum(prob, n=7)
+    assert abs(ev[0] - 0.5 * K0 @ K0) < 1e-12
+    g = 2 * math.pi / prob.wl.L
+    rng = range(-2, 3)
+    pw = np.sort([0.5 * (K0 + g * np.array(v)) @ (K0 + g * np.array(v))
+                  for v in [(a, b, c) for a in rng for b in rng for c in rng]])
+    assert abs(pw[0] - ev[0]) < 1e-12
+    assert np.abs(ev[1:7] - pw[1:7]).max() < 5e-4 * pw[6]   # discretisation: 5.1e-5 Ha
+
+
+def test_time_reversal(base):
+    e1 = _spectrum(fe.with_kpoint(base, K0))
+    e2 = _spectrum(fe.with_kpoint(base, -K0))
+    assert np.abs(e1 - e2).max() < 1e-10 * np.abs(e1).max()
+
+
+def test_lattice_vector_covariance_and_planted_sign(base):
+    """spec H(k) vs spec H(k + G), G = 2 pi / L e_x: 5.3e-3 Ha apart on this
+    mesh (discretisation); a sign error in the kinetic Bloch term relative to
+    the projector phase moves them 0.11 Ha apart."""
+    G = np.array([2 * math.pi / base.wl.L, 0.0, 0.0])
+    e1 = _spectrum(fe.with_kpoint(base, K0))
+    e2 = _spectrum(fe.with_kpoint(base, K0 + G))
+    assert np.abs(e1 - e2).max() < 2e-2
+    b1 = _spectrum(fe.with_kpoint(base, K0), planted=True)
+    b2 = _spectrum(fe.with_kpoint(base, K0 + G), planted=True)
+    assert np.abs(b1 - b2).max() > 5e-2
+
+
+def test_complex_cholgs(kprob):
+    prob, _, _ = kprob
+    X = fe.complex_guess(prob, 12)
+    Q = O.cholgs(X)
+    QH = Q.conj().T
+    assert np.abs(QH @ Q - np.eye(12)).max() < 1e-12
+    assert np.abs(Q @ (QH @ X) - X).max() < 1e-10 * np.abs(X).max()
+    assert np.abs(np.tril(QH @ X, -1)).max() < 1e-10
+
+
+def test_complex_rayleigh_ritz_and_residuals(kprob):
+    prob, ref, H = kprob
+    X = O.cholgs(prob.sqrt_mass[:, None] * fe.complex_guess(prob, 12))
+    lam, Xr = O.rayleigh_ritz(ref, X, 5)
+    assert np.allclose(lam, np.linalg.eigvalsh(X.conj().T @ H @ X), atol=1e-12)
+    R = Xr.conj().T @ H @ Xr
+    assert np.abs(R - np.diag(lam)).max() < 1e-11
+    lam2, _ = O.rayleigh_ritz_gep(ref, prob.sqrt_mass[:, None] * fe.complex_guess(prob, 12), 5)
+    assert np.abs(lam2 - lam).max() < 1e-10
+    r = O.residual_norms(ref, Xr, lam, 5)
+    assert np.allclose(r, np.linalg.norm(H @ Xr - Xr * lam, axis=0), atol=1e-12)
+
+
+def test_complex_lanczos_bracket(kprob):
+    prob, ref, H = kprob
+    ev = np.linalg.eigvalsh(H)
+    lo, up, theta, _ = O.lanczos_bounds(ref, prob.lanczos_start.astype(np.complex128), 10)
+    assert up >= ev[-1]
+    assert theta[0] >= ev[0] - 1e-10 and theta[-1] <= ev[-1] + 1e-10
+
+
+def test_complex_chfsi_converges_to_exact(kprob):
+    prob, ref, H = kprob
+    N = 16
+    keep = np.nonzero(~prob.constrained)[0]
+    exact = np.linalg.eigvalsh(H[np.ix_(keep, keep)])[:N]
+    lo, up, _, _ = O.lanczos_bounds(ref, prob.lanczos_start.astype(np.complex128), 10)
+    a0, bLow, bUp = O.first_call_bounds(lo, up, N, prob.M, reproducible=True)
+    X = fe.complex_guess(prob, N + 4)
+    lam = None
+    for _ in range(12):
+        lam, res, X = O.chfsi_pass(ref, X, a0, bLow, bUp, 12, 8)
+        a0, bLow = O.next_bounds(lam)
+    assert np.abs(lam[:N] - exact).max() < 1e-8
