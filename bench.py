@@ -69,6 +69,9 @@ def main():
     ap.add_argument("--no-kpoints", action="store_true",
                     help="skip the batched k-point filter measurement")
     ap.add_argument("--kpoint-reps", type=int, default=3)
+    ap.add_argument("--hx-only", action="store_true",
+                    help="setup, tile autotune and the HX timing only (for counter runs), "
+                         "plus a 1 GiB device copy that calibrates the traffic counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
@@ -96,6 +99,35 @@ def main():
     log(f"autotune (B={B}): {tune}")
     X0 = torch.tensor(prob.initial_guess(), device=dev)
     v0 = torch.tensor(prob.lanczos_start, device=dev)
+
+    # ---- fused HX timing with HIP events on the stream the library launches on
+    def time_hx(dtype):
+        Xb = torch.randn(prob.M, B, device=dev, dtype=dtype)
+        Yb = torch.zeros_like(Xb)
+        for _ in range(3):
+            op.HX(Xb, False, 1.0, Yb)
+        st = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(args.hx_reps):
+            op.HX(Xb, True, 1.0, Yb)
+        e1.record(st)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / args.hx_reps
+
+    if args.hx_only:
+        hx32_ms = time_hx(torch.float32)
+        hx64_ms = time_hx(torch.float64)
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        F = hx_flops(prob, B)
+        print(json.dumps({"hx_fp32_ms": round(hx32_ms, 4), "hx_fp64_ms": round(hx64_ms, 4),
+                          "hx_fp32_tflops": round(F / hx32_ms / 1e9, 2),
+                          "hx_fp64_tflops": round(F / hx64_ms / 1e9, 2),
+                          "calibration_copy_bytes": 4 << 28, "autotune": tune}), flush=True)
+        return
 
     def run_passes(mixed: bool, npass: int):
         solver = ChebyshevSolverDevice(cheby_block_size=B, cheby_degree=m,
@@ -130,21 +162,6 @@ def main():
         fp64_ms, lam_64, res_64, ph_64 = run_passes(False, args.steps)
         log(f"fp64: {fp64_ms:.1f} ms/pass; last max residual {res_64[-1].max():.3e}")
     del X0
-
-    # ---- fused HX timing with HIP events on the stream the library launches on
-    def time_hx(dtype):
-        Xb = torch.randn(prob.M, B, device=dev, dtype=dtype)
-        Yb = torch.zeros_like(Xb)
-        for _ in range(3):
-            op.HX(Xb, False, 1.0, Yb)
-        st = torch.cuda.current_stream()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(st)
-        for _ in range(args.hx_reps):
-            op.HX(Xb, True, 1.0, Yb)
-        e1.record(st)
-        e1.synchronize()
-        return e0.elapsed_time(e1) / args.hx_reps
 
     hx32_ms = time_hx(torch.float32)
     hx64_ms = time_hx(torch.float64)

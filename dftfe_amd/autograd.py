@@ -46,24 +46,27 @@ class HXFunction(torch.autograd.Function):
 
 
 class FilterFunction(torch.autograd.Function):
+    """precision 0 (FP64) or 1 (FP32 recurrence in both directions, FP64
+    accumulation of the potential gradient)."""
+
     @staticmethod
-    def forward(ctx, V, X, op, m, a, b, a0, checkpoint_every):
+    def forward(ctx, V, X, op, m, a, b, a0, checkpoint_every, precision=0):
         from .chfsi import chebyshev_filter
 
         ctx.op = op
-        ctx.args = (m, a, b, a0, checkpoint_every)
+        ctx.args = (m, a, b, a0, checkpoint_every, precision)
         X0 = X.contiguous()
         ctx.save_for_backward(X0)
         Y = X0.clone()
-        chebyshev_filter(op, Y, m, a, b, a0)
+        chebyshev_filter(op, Y, m, a, b, a0, precision)
         return Y
 
     @staticmethod
     def backward(ctx, Ybar):
         (X0,) = ctx.saved_tensors
-        m, a, b, a0, every = ctx.args
-        Xbar, gV = ctx.op.filter_backward(X0, Ybar.contiguous(), m, a, b, a0, every)
-        return gV, Xbar, None, None, None, None, None, None
+        m, a, b, a0, every, precision = ctx.args
+        Xbar, gV = ctx.op.filter_backward(X0, Ybar.contiguous(), m, a, b, a0, every, precision)
+        return gV, Xbar, None, None, None, None, None, None, None
 
 
 def fermi_occupations(eps: torch.Tensor, n_occ: float, kT: float, iters: int = 200):
@@ -88,10 +91,12 @@ def fermi_occupations(eps: torch.Tensor, n_occ: float, kT: float, iters: int = 2
 
 
 def band_energy(op, V: torch.Tensor, X0: torch.Tensor, bounds, m: int, occ: torch.Tensor,
-                block: int, spin: float = 2.0, checkpoint_every: int = 5):
+                block: int, spin: float = 2.0, checkpoint_every: int = 5, precision: int = 0):
     """One ChFSI pass from the FE-basis start X0 with fixed occupations:
     returns (E_band, eigenvalues, Ritz vectors in the Loewdin basis).
-    Differentiable in V (C x n_q) through the filter and Rayleigh-Ritz."""
+    Differentiable in V (C x n_q) through the filter and Rayleigh-Ritz.
+    precision selects the filter arithmetic (0 FP64, 1 FP32 with FP64
+    gradient accumulation); CholGS and Rayleigh-Ritz stay FP64."""
     a0, bLow, bUp = bounds
     sq = torch.as_tensor(op.prob.sqrt_mass, device=X0.device)
     Xt = (sq[:, None] * X0).contiguous()
@@ -99,7 +104,7 @@ def band_energy(op, V: torch.Tensor, X0: torch.Tensor, bounds, m: int, occ: torc
     cols = []
     for j in range(0, N, block):
         cols.append(FilterFunction.apply(V, Xt[:, j:j + block], op, m, bLow, bUp, a0,
-                                         checkpoint_every))
+                                         checkpoint_every, precision))
     Y = torch.cat(cols, dim=1)
     S = Y.T @ Y
     Lc = torch.linalg.cholesky(S)

@@ -126,14 +126,20 @@ class KohnShamOperatorDevice:
     def set_hx_variant(self, precision: int, variant: int) -> None:
         L.check(self.lib.chfsi_operator_set_hx_variant(self.handle, precision, variant))
 
+    def set_proj_variant(self, precision: int, variant: int) -> None:
+        L.check(self.lib.chfsi_operator_set_proj_variant(self.handle, precision, variant))
+
     def autotune(self, B: int) -> dict:
         """Time every HX tile variant on a B-column block and keep the fastest."""
         b64, b32 = ct.c_int32(), ct.c_int32()
-        t = (ct.c_float * 16)()
+        t = (ct.c_float * 32)()
         L.check(self.lib.chfsi_operator_autotune(self.handle, B, ct.byref(b64), ct.byref(b32), t,
                                                  _stream()))
         return {"fp64": b64.value, "fp32": b32.value,
-                "fp64_ms": [x for x in t[0:8] if x >= 0], "fp32_ms": [x for x in t[8:16] if x >= 0]}
+                "fp64_ms": [x for x in t[0:12] if x >= 0],
+                "fp32_ms": [x for x in t[16:28] if x >= 0],
+                "fp64_proj_ms": [x for x in t[12:16] if x >= 0],
+                "fp32_proj_ms": [x for x in t[28:32] if x >= 0]}
 
     def getSqrtMassVec(self) -> int:
         return self.lib.chfsi_operator_sqrt_mass(self.handle)
@@ -242,14 +248,15 @@ class KohnShamOperatorDevice:
         return g
 
     def filter_backward(self, X0: torch.Tensor, Ybar: torch.Tensor, m: int, a: float, b: float,
-                        a0: float, checkpoint_every: int = 5):
-        """Reverse pass of the degree-m filter: returns (dL/dX0, dL/dV)."""
+                        a0: float, checkpoint_every: int = 5, precision: int = L.FP64):
+        """Reverse pass of the degree-m filter: returns (dL/dX0, dL/dV).
+        precision FP32: FP32 iterates and adjoints, FP64 gradient accumulation."""
         p = self.prob
         Xbar = torch.empty_like(X0)
         g = torch.zeros((p.C, p.nqc), dtype=torch.float64, device=self.device)
         L.check(self.lib.chfsi_chebyshev_filter_backward(
             self.handle, _ptr(X0), X0.stride(0), _ptr(Ybar), Ybar.stride(0), X0.shape[1], m, a, b,
-            a0, checkpoint_every, _ptr(Xbar), Xbar.stride(0), _ptr(g), _stream()))
+            a0, checkpoint_every, _ptr(Xbar), Xbar.stride(0), _ptr(g), precision, _stream()))
         return Xbar, g
 
 
@@ -409,7 +416,11 @@ class ChebyshevSolverDevice:
                  cheby_block_size: int = 400, cheby_degree: int = 0, lanczos_steps: int = 20,
                  reproducible: bool = False, filter_precision: int = L.FP32,
                  first_scf_degree_factor: float = 1.34, reuse_lanczos_upper: bool = False,
-                 fp64_tail_degrees: int = 0):
+                 fp64_tail_degrees: int = 0, mixed_dense: bool = False):
+        """mixed_dense: FP32 off-diagonal blocks in the overlap, CholGS rotation,
+        X^T H X and Rayleigh-Ritz rotation when a pass runs with
+        use_mixed_prec_overall (the reference's USE MIXED PREC CGS O / CGS SR /
+        XTHX / RR_SR together)."""
         self.lib = L.load()
         p = L.Params()
         self.lib.chfsi_default_params(ct.byref(p))
@@ -421,6 +432,9 @@ class ChebyshevSolverDevice:
         p.first_scf_degree_factor = first_scf_degree_factor
         p.reuse_lanczos_upper = int(reuse_lanczos_upper)
         p.fp64_tail_degrees = fp64_tail_degrees
+        for f in ("mixed_prec_overlap", "mixed_prec_cgs_rotation", "mixed_prec_xthx",
+                  "mixed_prec_rr_rotation"):
+            setattr(p, f, int(mixed_dense))
         self.params = p
         h = ct.c_void_p()
         L.check(self.lib.chfsi_solver_create(ct.byref(p), a0, bLow, bUp, ct.byref(h)))

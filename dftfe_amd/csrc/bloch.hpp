@@ -59,6 +59,7 @@ struct ZProjLaunch {
   int kbegin[kMaxBatch + 1];
   const int *cells;          // cells that own slots
   int ncells;
+  int per_xcd;
   const uint32_t *map;
   int ne, ne_pad;
   const T *s_in;

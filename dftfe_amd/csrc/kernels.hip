@@ -33,7 +33,7 @@ namespace chfsi {
 // row blocks rb = wr + WR*j.  MFMA operand mapping (16x16x4): lane l supplies
 // A[l%16][k] and B[k][l%16] with k = 4*(l/16) + r for the r-th of four MFMAs
 // over a 16-wide K slice; result lane l holds column l%16, rows acc_row().
-template <typename T, int NW, int TN, int WC, int NRB>
+template <typename T, int NW, int TN, int WC, int NRB, int PD>
 __global__ __launch_bounds__(64 * NW) void hx_kernel(HxLaunch<T> a) {
   using v4 = typename V4<T>::t;
   constexpr int NT = 64 * NW;
@@ -108,15 +108,18 @@ __global__ __launch_bounds__(64 * NW) void hx_kernel(HxLaunch<T> a) {
 #pragma unroll
     for (int cb = 0; cb < CBW; ++cb) acc[j][cb] = v4{0, 0, 0, 0};
 
-  v4 hreg[HCH];
-  T xreg[KPT], xscl[KPT];
+  // register staging of the next slice(s): PD = 1 loads slice t+1 while slice
+  // t is multiplied; PD = 2 keeps two slices in flight (t+1 in one set, t+2
+  // issued into the other) so a gather has two slices of MFMA work to land
+  v4 hregA[HCH], hregB[HCH];
+  T xregA[KPT], xsclA[KPT], xregB[KPT], xsclB[KPT];
 
   // Branch-free global loads into registers: indices are clamped to valid
   // addresses and out-of-range values are multiplied by zero, so every load
   // of a slice is unconditional, all are in flight together and they overlap
   // the MFMAs of the previous slice.
   const T vmask = vok ? T(1) : T(0);
-  auto load_tile = [&](int t) {
+  auto load_tile = [&](int t, v4 (&hreg)[HCH], T (&xreg)[KPT], T (&xscl)[KPT]) {
     const T *tile_src = (t < nkt) ? Hcell + (int64_t)t * ROWS_MAX * 16
                                   : NLcell + (int64_t)(t - nkt) * ROWS_MAX * 16;
 #pragma unroll
@@ -141,7 +144,8 @@ __global__ __launch_bounds__(64 * NW) void hx_kernel(HxLaunch<T> a) {
       }
     }
   };
-  auto store_tile = [&](int buf) {
+  auto store_tile = [&](int buf, const v4 (&hreg)[HCH], const T (&xreg)[KPT],
+                        const T (&xscl)[KPT]) {
 #pragma unroll
     for (int j = 0; j < HCH; ++j) {
       const int chunk = tid + NT * j;
@@ -152,15 +156,8 @@ __global__ __launch_bounds__(64 * NW) void hx_kernel(HxLaunch<T> a) {
     for (int kk = 0; kk < KPT; ++kk) Xs[buf][gv * RS + gk + kk] = xscl[kk] * xreg[kk];
   };
 
-  __syncthreads();  // gather tables ready
-  if (ntot > 0) {
-    load_tile(0);
-    store_tile(0);
-  }
-  __syncthreads();
-  for (int t = 0; t < ntot; ++t) {
-    const int buf = t & 1;
-    if (t + 1 < ntot) load_tile(t + 1);
+  // MFMAs of slice t from LDS buffer t & 1
+  auto compute = [&](int buf) {
     v4 b4[CBW];
 #pragma unroll
     for (int cb = 0; cb < CBW; ++cb) {
@@ -203,8 +200,38 @@ __global__ __launch_bounds__(64 * NW) void hx_kernel(HxLaunch<T> a) {
 #pragma unroll
       for (int g = 0; g < G; ++g) a4[g] = a4n[g];
     }
-    if (t + 1 < ntot) store_tile(buf ^ 1);
+  };
+
+  __syncthreads();  // gather tables ready
+  if (ntot > 0) {
+    load_tile(0, hregA, xregA, xsclA);
+    store_tile(0, hregA, xregA, xsclA);
+  }
+  if constexpr (PD == 1) {
     __syncthreads();
+    for (int t = 0; t < ntot; ++t) {
+      const int buf = t & 1;
+      if (t + 1 < ntot) load_tile(t + 1, hregA, xregA, xsclA);
+      compute(buf);
+      if (t + 1 < ntot) store_tile(buf ^ 1, hregA, xregA, xsclA);
+      __syncthreads();
+    }
+  } else {
+    if (ntot > 1) load_tile(1, hregB, xregB, xsclB);
+    __syncthreads();
+    // step t: issue slice t+2 into the free set, multiply slice t, park slice
+    // t+1 (loaded one step earlier) in the other LDS buffer
+    auto step = [&](int t, v4 (&hl)[HCH], T (&xl)[KPT], T (&sl)[KPT], const v4 (&hs)[HCH],
+                    const T (&xs)[KPT], const T (&ss)[KPT]) {
+      if (t + 2 < ntot) load_tile(t + 2, hl, xl, sl);
+      compute(t & 1);
+      if (t + 1 < ntot) store_tile((t + 1) & 1, hs, xs, ss);
+      __syncthreads();
+    };
+    for (int t = 0; t < ntot; t += 2) {
+      step(t, hregA, xregA, xsclA, hregB, xregB, xsclB);
+      if (t + 1 < ntot) step(t + 1, hregB, xregB, xsclB, hregA, xregA, xsclA);
+    }
   }
 
   // epilogue: scaled scatter with the first-touch combine
@@ -425,7 +452,9 @@ __global__ __launch_bounds__(64 * NW) void hx_bf16_kernel(HxLaunch<float> a, con
 // ---------------------------------------------------------------------------
 // Projection partials  P_slot[v] = sum_i C_e[i, slot] * s_in[g_i] src[g_i, v]
 // ---------------------------------------------------------------------------
-template <typename T, int NW, int TN, int MAXSRB, int NRB>
+// ST node tiles of 16 per stage (one barrier per stage); accumulators for
+// MAXSRB slot blocks of 16 (the largest slot count of any cell, rounded up).
+template <typename T, int NW, int TN, int MAXSRB, int NRB, int ST>
 __global__ __launch_bounds__(64 * NW) void proj_kernel(ProjLaunch<T> a) {
   using v4 = typename V4<T>::t;
   constexpr int NT = 64 * NW;
@@ -434,13 +463,20 @@ __global__ __launch_bounds__(64 * NW) void proj_kernel(ProjLaunch<T> a) {
   constexpr int CBW = TN / 16 / NW;
   constexpr int ROWS = NRB * 16;              // padded nodes per cell
   constexpr int ACH = (MAXSRB * 16 * 4 + NT - 1) / NT;
+  constexpr int NSTAGE = NRB / ST;
   static_assert(KPT >= 1 && CBW >= 1, "bad projection tile");
-  __shared__ __align__(16) T As[2][MAXSRB * 16 * RS];
-  __shared__ __align__(16) T Xs[2][TN * RS];
+  static_assert(NRB % ST == 0, "stage must divide the node tiles");
+  __shared__ __align__(16) T As[2][ST][MAXSRB * 16 * RS];
+  __shared__ __align__(16) T Xs[2][ST][TN * RS];
   __shared__ uint32_t smap[ROWS];
   __shared__ T sscale[ROWS];
 
-  const int item = blockIdx.x;
+  // XCD-aware: consecutive cells (neighbours in the cell order, sharing face
+  // nodes) run on one XCD, so shared rows of the source hit in its L2
+  const int wid = blockIdx.x;
+  const int local = wid >> 3;
+  if (local >= a.per_xcd) return;
+  const int item = (wid & 7) * a.per_xcd + local;
   if (item >= a.ncells * a.ntiles) return;
   const int cell = a.cells[item / a.ntiles];
   const int v0 = (item % a.ntiles) * TN;
@@ -476,58 +512,68 @@ __global__ __launch_bounds__(64 * NW) void proj_kernel(ProjLaunch<T> a) {
   for (int j = 0; j < MAXSRB; ++j)
 #pragma unroll
     for (int cb = 0; cb < CBW; ++cb) acc[j][cb] = v4{0, 0, 0, 0};
-  v4 areg[ACH];
-  T xreg[KPT], xscl[KPT];
+  v4 areg[ST][ACH];
+  T xreg[ST][KPT], xscl[ST][KPT];
 
-  auto load_tile = [&](int t) {
-    const T *tile_src = Acell + (int64_t)t * nslot * 16;
+  auto load_stage = [&](int sg) {
 #pragma unroll
-    for (int j = 0; j < ACH; ++j) {
-      const int chunk = (tid + NT * j < nchunk) ? tid + NT * j : nchunk - 1;
-      areg[j] = reinterpret_cast<const v4 *>(tile_src)[chunk];
-    }
+    for (int u = 0; u < ST; ++u) {
+      const int t = sg * ST + u;
+      const T *tile_src = Acell + (int64_t)t * nslot * 16;
 #pragma unroll
-    for (int kk = 0; kk < KPT; ++kk) {
-      const int k = t * 16 + gk + kk;
-      xreg[kk] = a.src[(int64_t)(smap[k] & 0x7fffffffu) * a.ld_src + vcl];
-      xscl[kk] = vmask * sscale[k];
-    }
-  };
-  auto store_tile = [&](int buf) {
+      for (int j = 0; j < ACH; ++j) {
+        const int chunk = (tid + NT * j < nchunk) ? tid + NT * j : nchunk - 1;
+        areg[u][j] = reinterpret_cast<const v4 *>(tile_src)[chunk];
+      }
 #pragma unroll
-    for (int j = 0; j < ACH; ++j) {
-      const int chunk = tid + NT * j;
-      if (chunk < nchunk)
-        *reinterpret_cast<v4 *>(&As[buf][(chunk >> 2) * RS + (chunk & 3) * 4]) = areg[j];
-    }
-#pragma unroll
-    for (int kk = 0; kk < KPT; ++kk) Xs[buf][gv * RS + gk + kk] = xscl[kk] * xreg[kk];
-  };
-
-  __syncthreads();
-  load_tile(0);
-  store_tile(0);
-  __syncthreads();
-  for (int t = 0; t < NRB; ++t) {
-    const int buf = t & 1;
-    if (t + 1 < NRB) load_tile(t + 1);
-    v4 b4[CBW];
-#pragma unroll
-    for (int cb = 0; cb < CBW; ++cb)
-      b4[cb] = *reinterpret_cast<const v4 *>(
-          &Xs[buf][((wave * CBW + cb) * 16 + (lane & 15)) * RS + (lane >> 4) * 4]);
-#pragma unroll
-    for (int j = 0; j < MAXSRB; ++j) {
-      if (j < nsrb) {
-        const v4 a4 =
-            *reinterpret_cast<const v4 *>(&As[buf][(j * 16 + (lane & 15)) * RS + (lane >> 4) * 4]);
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-#pragma unroll
-          for (int cb = 0; cb < CBW; ++cb) acc[j][cb] = mfma16(a4[r], b4[cb][r], acc[j][cb]);
+      for (int kk = 0; kk < KPT; ++kk) {
+        const int k = t * 16 + gk + kk;
+        xreg[u][kk] = a.src[(int64_t)(smap[k] & 0x7fffffffu) * a.ld_src + vcl];
+        xscl[u][kk] = vmask * sscale[k];
       }
     }
-    if (t + 1 < NRB) store_tile(buf ^ 1);
+  };
+  auto store_stage = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < ST; ++u) {
+#pragma unroll
+      for (int j = 0; j < ACH; ++j) {
+        const int chunk = tid + NT * j;
+        if (chunk < nchunk)
+          *reinterpret_cast<v4 *>(&As[buf][u][(chunk >> 2) * RS + (chunk & 3) * 4]) = areg[u][j];
+      }
+#pragma unroll
+      for (int kk = 0; kk < KPT; ++kk) Xs[buf][u][gv * RS + gk + kk] = xscl[u][kk] * xreg[u][kk];
+    }
+  };
+
+  __syncthreads();
+  load_stage(0);
+  store_stage(0);
+  __syncthreads();
+  for (int sg = 0; sg < NSTAGE; ++sg) {
+    const int buf = sg & 1;
+    if (sg + 1 < NSTAGE) load_stage(sg + 1);
+#pragma unroll
+    for (int u = 0; u < ST; ++u) {
+      v4 b4[CBW];
+#pragma unroll
+      for (int cb = 0; cb < CBW; ++cb)
+        b4[cb] = *reinterpret_cast<const v4 *>(
+            &Xs[buf][u][((wave * CBW + cb) * 16 + (lane & 15)) * RS + (lane >> 4) * 4]);
+#pragma unroll
+      for (int j = 0; j < MAXSRB; ++j) {
+        if (j < nsrb) {
+          const v4 a4 = *reinterpret_cast<const v4 *>(
+              &As[buf][u][(j * 16 + (lane & 15)) * RS + (lane >> 4) * 4]);
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int cb = 0; cb < CBW; ++cb) acc[j][cb] = mfma16(a4[r], b4[cb][r], acc[j][cb]);
+        }
+      }
+    }
+    if (sg + 1 < NSTAGE) store_stage(buf ^ 1);
     __syncthreads();
   }
 #pragma unroll
@@ -727,6 +773,39 @@ __global__ void sum_partials_kernel(const double *P, int nk, int m, int n, doubl
   C[(idx / m) * ldc + idx % m] = s;
 }
 
+// FP32 split-K partials summed in FP64 (mixed-precision off-diagonal blocks)
+__global__ void sum_partials_f32_kernel(const float *P, int nk, int m, int n, double *C,
+                                        int64_t ldc) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t mn = (int64_t)m * n;
+  if (idx >= mn) return;
+  double s = 0.0;
+  for (int k = 0; k < nk; ++k) s += (double)P[k * mn + idx];
+  C[(idx / m) * ldc + idx % m] = s;
+}
+
+// C[i + j*ldc] += A[i + j*lda] (column-major m x n, FP32 into FP64)
+__global__ void add_f32_to_f64_kernel(const float *A, int64_t lda, double *C, int64_t ldc, int m,
+                                      int64_t n) {
+  const int64_t mn = (int64_t)m * n;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < mn;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = idx % m, j = idx / m;
+    C[i + j * ldc] += (double)A[i + j * lda];
+  }
+}
+
+// B = (float) A for an n x n column-major matrix, with the bs x bs diagonal
+// blocks set to zero when zero_diag (their FP64 part is applied separately)
+__global__ void cast_blocks_kernel(const double *A, int n, int64_t lda, int bs, int zero_diag,
+                                   float *B) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n * n) return;
+  const int i = (int)(idx % n), j = (int)(idx / n);
+  const bool diag = zero_diag && (i / bs == j / bs);
+  B[i + (int64_t)j * n] = diag ? 0.0f : (float)A[i + j * lda];
+}
+
 __global__ void zero_upper_kernel(double *A, int n, int64_t lda) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (int64_t)n * n) return;
@@ -775,10 +854,10 @@ __global__ void zero_rows_kernel(const int64_t *rows, int64_t nrows, double *X, 
 //   a = b = X, w = spin * occupation        -> density rho(q) JxW (densityCalculator.cc:622-689)
 //   a = Ybar, b = X, w = scalar              -> dL/dV(q) of Y = scalar * S H(V) S X
 // ---------------------------------------------------------------------------
-template <int QPT>
+template <int QPT, typename TIn>
 __global__ __launch_bounds__(256) void quad_product_kernel(
-    const uint32_t *map, int ne, int nqc, const double *Nq, const double *JxW, const double *a,
-    int64_t lda, const double *b, int64_t ldb, int B, const double *s_in, const double *w,
+    const uint32_t *map, int ne, int nqc, const double *Nq, const double *JxW, const TIn *a,
+    int64_t lda, const TIn *b, int64_t ldb, int B, const double *s_in, const double *w,
     double wscale, double *g, int accumulate) {
   constexpr int VC = 16;  // columns per chunk
   constexpr int KT = 16;  // nodes per slice
@@ -816,8 +895,8 @@ __global__ __launch_bounds__(256) void quad_product_kernel(
         double xa = 0.0, xb = 0.0;
         if (j < ne && v < B) {
           const int64_t gg = smap[j];
-          xa = sscale[j] * a[gg * lda + v];
-          xb = sscale[j] * b[gg * ldb + v];
+          xa = sscale[j] * (double)a[gg * lda + v];
+          xb = sscale[j] * (double)b[gg * ldb + v];
         }
         As[jj][vv] = xa;
         Bs[jj][vv] = xb;
@@ -853,18 +932,35 @@ __global__ __launch_bounds__(256) void quad_product_kernel(
   }
 }
 
-hipError_t launch_quad_product(const uint32_t *map, int C, int ne, int nqc, const double *Nq,
-                               const double *JxW, const double *a, int64_t lda, const double *b,
-                               int64_t ldb, int B, const double *s_in, const double *w, double wscale, double *g,
-                               int accumulate, hipStream_t s) {
+template <typename TIn>
+static hipError_t quad_go(const uint32_t *map, int C, int ne, int nqc, const double *Nq,
+                          const double *JxW, const TIn *a, int64_t lda, const TIn *b, int64_t ldb,
+                          int B, const double *s_in, const double *w, double wscale, double *g,
+                          int accumulate, hipStream_t s) {
   if (ne > 384 || nqc > 512) return hipErrorInvalidValue;
   if (nqc <= 256)
-    hipLaunchKernelGGL(quad_product_kernel<1>, dim3(C), dim3(256), 0, s, map, ne, nqc, Nq, JxW, a,
-                       lda, b, ldb, B, s_in, w, wscale, g, accumulate);
+    hipLaunchKernelGGL((quad_product_kernel<1, TIn>), dim3(C), dim3(256), 0, s, map, ne, nqc, Nq,
+                       JxW, a, lda, b, ldb, B, s_in, w, wscale, g, accumulate);
   else
-    hipLaunchKernelGGL(quad_product_kernel<2>, dim3(C), dim3(256), 0, s, map, ne, nqc, Nq, JxW, a,
-                       lda, b, ldb, B, s_in, w, wscale, g, accumulate);
+    hipLaunchKernelGGL((quad_product_kernel<2, TIn>), dim3(C), dim3(256), 0, s, map, ne, nqc, Nq,
+                       JxW, a, lda, b, ldb, B, s_in, w, wscale, g, accumulate);
   return hipGetLastError();
+}
+
+hipError_t launch_quad_product(const uint32_t *map, int C, int ne, int nqc, const double *Nq,
+                               const double *JxW, const double *a, int64_t lda, const double *b,
+                               int64_t ldb, int B, const double *s_in, const double *w,
+                               double wscale, double *g, int accumulate, hipStream_t s) {
+  return quad_go<double>(map, C, ne, nqc, Nq, JxW, a, lda, b, ldb, B, s_in, w, wscale, g,
+                         accumulate, s);
+}
+
+hipError_t launch_quad_product(const uint32_t *map, int C, int ne, int nqc, const double *Nq,
+                               const double *JxW, const float *a, int64_t lda, const float *b,
+                               int64_t ldb, int B, const double *s_in, const double *w,
+                               double wscale, double *g, int accumulate, hipStream_t s) {
+  return quad_go<float>(map, C, ne, nqc, Nq, JxW, a, lda, b, ldb, B, s_in, w, wscale, g,
+                        accumulate, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -877,28 +973,28 @@ static inline unsigned grid_for(int64_t n, int block, unsigned cap = 1u << 20) {
   return (unsigned)g;
 }
 
-template <typename T, int NW, int TN, int WC, int NRB>
+template <typename T, int NW, int TN, int WC, int NRB, int PD>
 static hipError_t hx_go(HxLaunch<T> a, hipStream_t s) {
   a.ntiles = (a.B + TN - 1) / TN;
   const int items = a.ncells * a.ntiles;
   if (items == 0) return hipSuccess;
   a.per_xcd = (items + 7) / 8;
-  hipLaunchKernelGGL((hx_kernel<T, NW, TN, WC, NRB>), dim3(8 * a.per_xcd), dim3(64 * NW), 0, s,
-                     a);
+  hipLaunchKernelGGL((hx_kernel<T, NW, TN, WC, NRB, PD>), dim3(8 * a.per_xcd), dim3(64 * NW), 0,
+                     s, a);
   return hipGetLastError();
 }
 
 // Row-block count of the padded cell matrix is a compile-time constant so the
 // MFMA loop has no guards: (p+1)^3 padded to 16 for p = 1..6.
-template <typename T, int NW, int TN, int WC>
+template <typename T, int NW, int TN, int WC, int PD = 1>
 static hipError_t hx_dispatch(const HxLaunch<T> &a, hipStream_t s) {
   switch (a.ne_pad >> 4) {
-    case 1: return hx_go<T, NW, TN, WC, 1>(a, s);
-    case 2: return hx_go<T, NW, TN, WC, 2>(a, s);
-    case 4: return hx_go<T, NW, TN, WC, 4>(a, s);
-    case 8: return hx_go<T, NW, TN, WC, 8>(a, s);
-    case 14: return hx_go<T, NW, TN, WC, 14>(a, s);
-    case 22: return hx_go<T, NW, TN, WC, 22>(a, s);
+    case 1: return hx_go<T, NW, TN, WC, 1, PD>(a, s);
+    case 2: return hx_go<T, NW, TN, WC, 2, PD>(a, s);
+    case 4: return hx_go<T, NW, TN, WC, 4, PD>(a, s);
+    case 8: return hx_go<T, NW, TN, WC, 8, PD>(a, s);
+    case 14: return hx_go<T, NW, TN, WC, 14, PD>(a, s);
+    case 22: return hx_go<T, NW, TN, WC, 22, PD>(a, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -912,7 +1008,9 @@ static hipError_t hx_dispatch(const HxLaunch<T> &a, hipStream_t s) {
 //   FP64  0: 8 x  64 x 4 (two row groups)
 //         1: 4 x  32 x 2
 //         2: 16 x 64 x 4 (four row groups, 16 waves per CU)
-int hx_variants(bool fp64) { return fp64 ? 3 : 5; }
+//   prefetch depth two (two slices in flight): FP32 5: 8 x 64 x 4, 6: 16 x 128 x 8;
+//   FP64 3: 16 x 64 x 4, 4: 8 x 64 x 4
+int hx_variants(bool fp64) { return fp64 ? 5 : 7; }
 
 template <>
 hipError_t launch_hx<float>(const HxLaunch<float> &a, hipStream_t s) {
@@ -921,6 +1019,8 @@ hipError_t launch_hx<float>(const HxLaunch<float> &a, hipStream_t s) {
     case 2: return hx_dispatch<float, 8, 64, 4>(a, s);
     case 3: return hx_dispatch<float, 16, 128, 8>(a, s);
     case 4: return hx_dispatch<float, 8, 32, 2>(a, s);
+    case 5: return hx_dispatch<float, 8, 64, 4, 2>(a, s);
+    case 6: return hx_dispatch<float, 16, 128, 8, 2>(a, s);
     default: return hx_dispatch<float, 8, 128, 8>(a, s);
   }
 }
@@ -930,6 +1030,8 @@ hipError_t launch_hx<double>(const HxLaunch<double> &a, hipStream_t s) {
   switch (a.variant) {
     case 1: return hx_dispatch<double, 4, 32, 2>(a, s);
     case 2: return hx_dispatch<double, 16, 64, 4>(a, s);
+    case 3: return hx_dispatch<double, 16, 64, 4, 2>(a, s);
+    case 4: return hx_dispatch<double, 8, 64, 4, 2>(a, s);
     default: return hx_dispatch<double, 8, 64, 4>(a, s);
   }
 }
@@ -972,25 +1074,51 @@ hipError_t launch_f64_to_bf16(const double *x, int64_t n, uint16_t *y, hipStream
   return hipGetLastError();
 }
 
-template <typename T, int NRB>
+template <typename T, int NW, int TN, int MAXSRB, int NRB, int ST>
 static hipError_t proj_go(ProjLaunch<T> a, hipStream_t s) {
-  constexpr int TN = sizeof(T) == 4 ? 128 : 64;
   a.ntiles = (a.B + TN - 1) / TN;
   const int items = a.ncells * a.ntiles;
   if (items == 0) return hipSuccess;
-  hipLaunchKernelGGL((proj_kernel<T, 4, TN, 8, NRB>), dim3(items), dim3(256), 0, s, a);
+  a.per_xcd = (items + 7) / 8;
+  hipLaunchKernelGGL((proj_kernel<T, NW, TN, MAXSRB, NRB, ST>), dim3(8 * a.per_xcd), dim3(64 * NW),
+                     0, s, a);
   return hipGetLastError();
 }
 
+// Projection variants: 0: 4 waves x wide column tile, one node tile per stage,
+// accumulators for 128 slots; 1: same tile, two node tiles per stage,
+// accumulators sized to the largest cell; 2: 2 waves x half-width tile, two
+// node tiles per stage, sized accumulators.
+template <typename T, int NRB, int MAXSRB>
+static hipError_t proj_variant(const ProjLaunch<T> &a, hipStream_t s) {
+  constexpr int TNW = sizeof(T) == 4 ? 128 : 64;
+  constexpr int ST = NRB % 2 == 0 ? 2 : 1;
+  switch (a.variant) {
+    case 1: return proj_go<T, 4, TNW, MAXSRB, NRB, ST>(a, s);
+    case 2: return proj_go<T, 2, TNW / 2, MAXSRB, NRB, ST>(a, s);
+    default: return proj_go<T, 4, TNW, 8, NRB, 1>(a, s);
+  }
+}
+
+template <typename T, int NRB>
+static hipError_t proj_fit(const ProjLaunch<T> &a, hipStream_t s) {
+  if (a.max_srb <= 2) return proj_variant<T, NRB, 2>(a, s);
+  if (a.max_srb <= 4) return proj_variant<T, NRB, 4>(a, s);
+  return proj_variant<T, NRB, 8>(a, s);
+}
+
+int proj_variants() { return 3; }
+
 template <typename T>
 hipError_t launch_proj(const ProjLaunch<T> &a, hipStream_t s) {
+  if (a.max_srb < 1 || a.max_srb > 8) return hipErrorInvalidValue;
   switch (a.ne_pad >> 4) {
-    case 1: return proj_go<T, 1>(a, s);
-    case 2: return proj_go<T, 2>(a, s);
-    case 4: return proj_go<T, 4>(a, s);
-    case 8: return proj_go<T, 8>(a, s);
-    case 14: return proj_go<T, 14>(a, s);
-    case 22: return proj_go<T, 22>(a, s);
+    case 1: return proj_fit<T, 1>(a, s);
+    case 2: return proj_fit<T, 2>(a, s);
+    case 4: return proj_fit<T, 4>(a, s);
+    case 8: return proj_fit<T, 8>(a, s);
+    case 14: return proj_fit<T, 14>(a, s);
+    case 22: return proj_fit<T, 22>(a, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -1060,6 +1188,8 @@ template hipError_t launch_copy_block<double, double>(const double *, int64_t, d
                                                       int64_t, int, hipStream_t);
 template hipError_t launch_copy_block<double, float>(const double *, int64_t, float *, int64_t,
                                                      int64_t, int, hipStream_t);
+template hipError_t launch_copy_block<float, float>(const float *, int64_t, float *, int64_t,
+                                                   int64_t, int, hipStream_t);
 template hipError_t launch_copy_block<float, double>(const float *, int64_t, double *, int64_t,
                                                      int64_t, int, hipStream_t);
 
@@ -1094,6 +1224,32 @@ hipError_t launch_sum_partials(const double *P, int nk, int m, int n, double *C,
   const int64_t mn = (int64_t)m * n;
   hipLaunchKernelGGL(sum_partials_kernel, dim3(grid_for(mn, 256, 1u << 30)), dim3(256), 0, st, P,
                      nk, m, n, C, ldc);
+  return hipGetLastError();
+}
+
+hipError_t launch_sum_partials_f32(const float *P, int nk, int m, int n, double *C, int64_t ldc,
+                                   hipStream_t st) {
+  const int64_t mn = (int64_t)m * n;
+  if (mn == 0) return hipSuccess;
+  hipLaunchKernelGGL(sum_partials_f32_kernel, dim3(grid_for(mn, 256, 1u << 30)), dim3(256), 0, st,
+                     P, nk, m, n, C, ldc);
+  return hipGetLastError();
+}
+
+hipError_t launch_add_f32_to_f64(const float *A, int64_t lda, double *C, int64_t ldc, int m,
+                                 int64_t n, hipStream_t st) {
+  const int64_t mn = (int64_t)m * n;
+  if (mn == 0) return hipSuccess;
+  hipLaunchKernelGGL(add_f32_to_f64_kernel, dim3(grid_for(mn, 256, 65536)), dim3(256), 0, st, A,
+                     lda, C, ldc, m, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_cast_blocks(const double *A, int n, int64_t lda, int bs, int zero_diag,
+                              float *B, hipStream_t st) {
+  const int64_t nn = (int64_t)n * n;
+  hipLaunchKernelGGL(cast_blocks_kernel, dim3(grid_for(nn, 256, 1u << 30)), dim3(256), 0, st, A,
+                     n, lda, bs, zero_diag, B);
   return hipGetLastError();
 }
 

@@ -53,6 +53,7 @@ struct ProjLaunch {
   const int *cells;        // cells that own slots
   int ncells;
   int ntiles;
+  int per_xcd;             // work items per XCD slice
   const uint32_t *map;
   int ne, ne_pad;
   const int *nl_off;
@@ -63,6 +64,8 @@ struct ProjLaunch {
   T *partial;              // n_slots x ldP
   int64_t ldP;
   int B;
+  int max_srb;             // largest slot count of any cell / 16 (1..8)
+  int variant;             // projection variant (see proj_variants)
 };
 
 template <typename T>
@@ -74,6 +77,7 @@ hipError_t launch_hx_bf16(const HxLaunch<float> &a, const uint16_t *Ht16, const 
 hipError_t launch_f64_to_bf16(const double *x, int64_t n, uint16_t *y, hipStream_t s);
 template <typename T>
 hipError_t launch_proj(const ProjLaunch<T> &a, hipStream_t s);
+int proj_variants();
 template <typename T>
 hipError_t launch_proj_reduce(const int *proj_ptr, const int *proj_slots, const double *proj_V,
                               int n_proj, const T *partial, int64_t ldP, T *W, int64_t ldW,
@@ -112,10 +116,24 @@ hipError_t launch_quad_product(const uint32_t *map, int C, int ne, int nqc, cons
                                const double *JxW, const double *a, int64_t lda, const double *b,
                                int64_t ldb, int B, const double *s_in, const double *w,
                                double wscale, double *g, int accumulate, hipStream_t s);
+// the same with FP32 inputs (products and sums in FP64)
+hipError_t launch_quad_product(const uint32_t *map, int C, int ne, int nqc, const double *Nq,
+                               const double *JxW, const float *a, int64_t lda, const float *b,
+                               int64_t ldb, int B, const double *s_in, const double *w,
+                               double wscale, double *g, int accumulate, hipStream_t s);
 
 // C[i + j*ldc] = sum_k P[k*m*n + i + j*m]  (split-K reduction, column-major m x n)
 hipError_t launch_sum_partials(const double *P, int nk, int m, int n, double *C, int64_t ldc,
                                hipStream_t st);
+// the same with FP32 partials (mixed-precision off-diagonal blocks)
+hipError_t launch_sum_partials_f32(const float *P, int nk, int m, int n, double *C, int64_t ldc,
+                                   hipStream_t st);
+// C[i + j*ldc] += A[i + j*lda], column-major m x n, FP32 into FP64
+hipError_t launch_add_f32_to_f64(const float *A, int64_t lda, double *C, int64_t ldc, int m,
+                                 int64_t n, hipStream_t st);
+// B (n x n, ld n) = (float) A, diagonal bs x bs blocks zeroed when zero_diag
+hipError_t launch_cast_blocks(const double *A, int n, int64_t lda, int bs, int zero_diag,
+                              float *B, hipStream_t st);
 // A[i + j*lda] = 0 for i < j (clear the strict upper triangle of an n x n matrix)
 hipError_t launch_zero_upper(double *A, int n, int64_t lda, hipStream_t st);
 

@@ -265,7 +265,10 @@ __global__ __launch_bounds__(64 * NW) void zproj_kernel(ZProjLaunch<T> a) {
   __shared__ uint32_t smap[ROWS];
   __shared__ T sscale[ROWS];
 
-  const int item = blockIdx.x;
+  const int wid = blockIdx.x;
+  const int local = wid >> 3;
+  if (local >= a.per_xcd) return;
+  const int item = (wid & 7) * a.per_xcd + local;
   if (item >= a.kbegin[a.nk]) return;
   const int k = batch_entry(a.kbegin, a.nk, item);
   const ZHxItem<T> it = a.items[k];
@@ -534,7 +537,8 @@ static hipError_t zproj_go(ZProjLaunch<T> a, const ZHxItem<T> *h, hipStream_t s)
   }
   a.kbegin[a.nk] = total;
   if (total == 0) return hipSuccess;
-  hipLaunchKernelGGL((zproj_kernel<T, 4, TN, 8, NRB>), dim3(total), dim3(256), 0, s, a);
+  a.per_xcd = (total + 7) / 8;
+  hipLaunchKernelGGL((zproj_kernel<T, 4, TN, 8, NRB>), dim3(8 * a.per_xcd), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

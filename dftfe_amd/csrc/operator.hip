@@ -257,6 +257,7 @@ extern "C" chfsi_status chfsi_operator_create(const chfsi_operator_desc *d, chfs
     }
     const int S = nl_off[C];
     op->n_slots = S;
+    for (int c = 0; c < C; ++c) op->max_srb = std::max(op->max_srb, (nl_off[c + 1] - nl_off[c]) / 16);
     std::vector<int> slot_proj(S, -1);
     std::vector<double> nlC(size_t(S) * ne_pad, 0.0), nlCt(size_t(S) * ne_pad, 0.0);
     std::vector<std::vector<int>> slots_of(d->n_proj);
@@ -442,6 +443,8 @@ chfsi_status hx_fused(chfsi_operator *op, const T *src, int64_t ld_src, T *dst, 
     pl.partial = static_cast<T *>(P.ptr);
     pl.ldP = B;
     pl.B = B;
+    pl.max_srb = op->max_srb;
+    pl.variant = sizeof(T) == 8 ? op->proj_variant64 : op->proj_variant32;
     CHFSI_HIP(launch_proj<T>(pl, s));
     W = static_cast<T *>(Wb.ptr);
     CHFSI_HIP(launch_proj_reduce<T>(op->d_proj_ptr, op->d_proj_slots, op->d_proj_V, op->n_proj,
@@ -565,6 +568,15 @@ extern "C" chfsi_status chfsi_operator_set_hx_variant(chfsi_operator *op, int32_
   return CHFSI_OK;
 }
 
+extern "C" chfsi_status chfsi_operator_set_proj_variant(chfsi_operator *op, int32_t precision,
+                                                       int32_t variant) {
+  CHFSI_ARG(op, "null operator");
+  CHFSI_ARG(precision == CHFSI_FP64 || precision == CHFSI_FP32, "bad precision");
+  CHFSI_ARG(variant >= 0 && variant < proj_variants(), "bad variant");
+  (precision == CHFSI_FP64 ? op->proj_variant64 : op->proj_variant32) = variant;
+  return CHFSI_OK;
+}
+
 namespace {
 template <typename T>
 chfsi_status time_variants(chfsi_operator *op, int B, int *best, float *times, hipStream_t st) {
@@ -600,6 +612,33 @@ chfsi_status time_variants(chfsi_operator *op, int B, int *best, float *times, h
     }
   }
   var = *best;
+  // projection variants with the chosen tile variant, timed as whole applies
+  if (op->n_slots > 0) {
+    int &pv = sizeof(T) == 8 ? op->proj_variant64 : op->proj_variant32;
+    float best_p = 1e30f;
+    int bp = 0;
+    for (int v = 0; v < proj_variants() && v < 3; ++v) {
+      pv = v;
+      const T *xs = static_cast<const T *>(x.ptr);
+      T *ys = static_cast<T *>(y.ptr);
+      chfsi_status stt = hx_fused<T>(op, xs, B, ys, B, B, T(1), T(1), T(0), 0, true, st);
+      float ms = 1e30f;
+      if (stt == CHFSI_OK) {
+        CHFSI_HIP(hipEventRecord(e0, st));
+        for (int r = 0; r < 3 && stt == CHFSI_OK; ++r)
+          stt = hx_fused<T>(op, xs, B, ys, B, B, T(1), T(1), T(0), 0, true, st);
+        CHFSI_HIP(hipEventRecord(e1, st));
+        CHFSI_HIP(hipEventSynchronize(e1));
+        if (stt == CHFSI_OK) CHFSI_HIP(hipEventElapsedTime(&ms, e0, e1));
+      }
+      times[12 + v] = stt == CHFSI_OK ? ms / 3 : -1.0f;
+      if (stt == CHFSI_OK && ms < best_p) {
+        best_p = ms;
+        bp = v;
+      }
+    }
+    pv = bp;
+  }
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   x.release();
@@ -612,15 +651,15 @@ extern "C" chfsi_status chfsi_operator_autotune(chfsi_operator *op, int32_t B, i
                                                 int32_t *best32, float *times_ms, void *stream) {
   CHFSI_ARG(op && B > 0, "bad argument");
   hipStream_t st = as_stream(stream);
-  float t[16];
+  float t[32];
   for (float &v : t) v = -1.0f;
   int b64 = 0, b32 = 0;
   CHFSI_TRY(time_variants<double>(op, B, &b64, t, st));
-  CHFSI_TRY(time_variants<float>(op, B, &b32, t + 8, st));
+  CHFSI_TRY(time_variants<float>(op, B, &b32, t + 16, st));
   if (best64) *best64 = b64;
   if (best32) *best32 = b32;
   if (times_ms)
-    for (int i = 0; i < 16; ++i) times_ms[i] = t[i];
+    for (int i = 0; i < 32; ++i) times_ms[i] = t[i];
   return CHFSI_OK;
 }
 

@@ -53,8 +53,10 @@ struct chfsi_operator {
   double *d_nlC64 = nullptr, *d_nlCt64 = nullptr, *d_proj_V = nullptr;
   float *d_nlC32 = nullptr, *d_nlCt32 = nullptr;
   chfsi::Scratch partial64, partial32, W64, W32, tmp;
-  // HX tile variant per precision (chosen by chfsi_operator_autotune)
+  // HX tile and projection variants per precision (chosen by chfsi_operator_autotune)
   int hx_variant64 = 0, hx_variant32 = 0;
+  int proj_variant64 = 0, proj_variant32 = 0;
+  int max_srb = 1;  // largest slot count of any cell / 16
   // constraints (slave-major and master-major CSR)
   int64_t nc = 0, nm = 0;
   int64_t *d_c_rows = nullptr, *d_c_ptr = nullptr, *d_c_cols = nullptr;

@@ -280,34 +280,162 @@ chfsi_status gemm_nt_splitk(chfsi_solver *s, const double *A, int64_t lda, const
   return CHFSI_OK;
 }
 
-// X <- X op(Q)^T... precisely Xc <- op(Q) Xc with Xc = X^T (N x M, ld N),
-// blocked over DOFs through a scratch panel (subspaceRotationScalapack,
-// linearAlgebraOperationsDevice.cc:1832-2242).
-chfsi_status rotate(chfsi_solver *s, double *X, int64_t M, int N, const double *Q,
-                    rocblas_operation opQ, hipStream_t st) {
-  const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(M, (int64_t(1) << 27) / N));
+// The same product from FP32 operands (FP32 split-K partials, summed in FP64).
+chfsi_status sgemm_nt_splitk(chfsi_solver *s, const float *A, int64_t lda, const float *B,
+                             int64_t ldb, int m, int n, int64_t K, double *C, int64_t ldc,
+                             hipStream_t st) {
+  const int nk = (int)std::max<int64_t>(1, std::min<int64_t>(64, K / 8192));
+  const int64_t Kc = K / nk;
+  const int64_t rem = K - Kc * nk;
+  const int slots = nk + (rem ? 1 : 0);
+  CHFSI_TRY(s->split32.ensure(sizeof(float) * slots * m * n));
+  float *P = sptr<float>(s->split32);
+  const float one = 1.0f, zero = 0.0f;
+  CHFSI_RB(rocblas_sgemm_strided_batched(
+      s->blas, rocblas_operation_none, rocblas_operation_transpose, m, n, (rocblas_int)Kc, &one,
+      A, (rocblas_int)lda, Kc * lda, B, (rocblas_int)ldb, Kc * ldb, &zero, P, m, (int64_t)m * n,
+      nk));
+  if (rem)
+    CHFSI_RB(rocblas_sgemm(s->blas, rocblas_operation_none, rocblas_operation_transpose, m, n,
+                           (rocblas_int)rem, &one, A + nk * Kc * lda, (rocblas_int)lda,
+                           B + nk * Kc * ldb, (rocblas_int)ldb, &zero, P + (int64_t)nk * m * n,
+                           m));
+  CHFSI_HIP(launch_sum_partials_f32(P, slots, m, n, C, ldc, st));
+  return CHFSI_OK;
+}
+
+// FP32 copy of the node-major M x N block (for the mixed-precision products).
+chfsi_status cast_x(chfsi_solver *s, const double *X, int64_t M, int N, float **out,
+                    hipStream_t st) {
+  CHFSI_TRY(s->Xf.ensure(sizeof(float) * M * N));
+  *out = sptr<float>(s->Xf);
+  CHFSI_HIP(launch_copy_block<double, float>(X, N, *out, N, M, N, st));
+  return CHFSI_OK;
+}
+
+// Rotation panels: rows of DOFs per panel so that one panel is about 128 MB.
+int64_t panel_rows(int64_t M, int N) {
+  return std::max<int64_t>(1, std::min<int64_t>(M, (int64_t(1) << 27) / N));
+}
+
+// X <- X Q, i.e. Xc <- Q^T Xc with Xc = X^T (N x M, ld N), by DOF panels
+// (subspaceRotationScalapack, linearAlgebraOperationsDevice.cc:1832-2242).
+// Mixed (subspaceRotationRRMixedPrecScalapack, :2660-3077): the bs x bs diagonal
+// blocks of Q in FP64, the rest from FP32 copies of Q and X.
+chfsi_status rotate_dense(chfsi_solver *s, double *X, const float *Xf, int64_t M, int N,
+                          const double *Q, int bs, hipStream_t st) {
+  const int64_t chunk = panel_rows(M, N);
   CHFSI_TRY(s->rot.ensure(sizeof(double) * chunk * N));
   double *tmp = sptr<double>(s->rot);
   const double one = 1.0, zero = 0.0;
+  float *Qf = nullptr, *tmpf = nullptr;
+  if (Xf) {
+    CHFSI_TRY(s->Qf.ensure(sizeof(float) * N * N));
+    CHFSI_TRY(s->rot32.ensure(sizeof(float) * chunk * N));
+    Qf = sptr<float>(s->Qf);
+    tmpf = sptr<float>(s->rot32);
+    CHFSI_HIP(launch_cast_blocks(Q, N, N, bs, 1, Qf, st));
+  }
+  const float onef = 1.0f, zerof = 0.0f;
   for (int64_t n0 = 0; n0 < M; n0 += chunk) {
     const int64_t nc = std::min(chunk, M - n0);
-    CHFSI_RB(rocblas_dgemm(s->blas, opQ, rocblas_operation_none, N, (rocblas_int)nc, N, &one, Q,
-                           N, X + n0 * N, N, &zero, tmp, N));
+    if (!Xf) {
+      CHFSI_RB(rocblas_dgemm(s->blas, rocblas_operation_transpose, rocblas_operation_none, N,
+                             (rocblas_int)nc, N, &one, Q, N, X + n0 * N, N, &zero, tmp, N));
+    } else {
+      CHFSI_RB(rocblas_sgemm(s->blas, rocblas_operation_transpose, rocblas_operation_none, N,
+                             (rocblas_int)nc, N, &onef, Qf, N, Xf + n0 * N, N, &zerof, tmpf, N));
+      for (int i0 = 0; i0 < N; i0 += bs) {
+        const int bw = std::min(bs, N - i0);
+        CHFSI_RB(rocblas_dgemm(s->blas, rocblas_operation_transpose, rocblas_operation_none, bw,
+                               (rocblas_int)nc, bw, &one, Q + i0 + (int64_t)i0 * N, N,
+                               X + n0 * N + i0, N, &zero, tmp + i0, N));
+      }
+      CHFSI_HIP(launch_add_f32_to_f64(tmpf, N, tmp, N, N, nc, st));
+    }
     CHFSI_HIP(hipMemcpyAsync(X + n0 * N, tmp, sizeof(double) * nc * N, hipMemcpyDeviceToDevice,
                              st));
   }
   return CHFSI_OK;
 }
 
-// S = X^T X, S = L L^T, X <- X L^{-T}.  Column-major view: Xc = X^T is N x M
-// with leading dimension N, so S = Xc Xc^T and Xc <- L^{-1} Xc, applied as a
-// GEMM rotation with the explicit triangular inverse.
-chfsi_status cholgs_impl(chfsi_solver *s, double *X, int64_t M, int N, hipStream_t st) {
+// Xc <- Linv Xc with Linv lower triangular, by DOF panels and row blocks of bs
+// (only the blocks on and below the diagonal are multiplied).  Mixed
+// (subspaceRotationCGSMixedPrecScalapack, :2243-2659): the diagonal blocks in
+// FP64, the blocks left of them from FP32 copies.
+chfsi_status rotate_lower(chfsi_solver *s, double *X, const float *Xf, int64_t M, int N,
+                          const double *Linv, int bs, hipStream_t st) {
+  const int64_t chunk = panel_rows(M, N);
+  CHFSI_TRY(s->rot.ensure(sizeof(double) * chunk * N));
+  double *tmp = sptr<double>(s->rot);
+  const double one = 1.0, zero = 0.0;
+  const float onef = 1.0f, zerof = 0.0f;
+  float *Lf = nullptr, *tmpf = nullptr;
+  if (Xf) {
+    CHFSI_TRY(s->Qf.ensure(sizeof(float) * N * N));
+    CHFSI_TRY(s->rot32.ensure(sizeof(float) * chunk * bs));
+    Lf = sptr<float>(s->Qf);
+    tmpf = sptr<float>(s->rot32);
+    CHFSI_HIP(launch_cast_blocks(Linv, N, N, bs, 0, Lf, st));
+  }
+  for (int64_t n0 = 0; n0 < M; n0 += chunk) {
+    const int64_t nc = std::min(chunk, M - n0);
+    for (int i0 = 0; i0 < N; i0 += bs) {
+      const int bw = std::min(bs, N - i0);
+      if (!Xf) {
+        CHFSI_RB(rocblas_dgemm(s->blas, rocblas_operation_none, rocblas_operation_none, bw,
+                               (rocblas_int)nc, i0 + bw, &one, Linv + i0, N, X + n0 * N, N, &zero,
+                               tmp + i0, N));
+      } else {
+        CHFSI_RB(rocblas_dgemm(s->blas, rocblas_operation_none, rocblas_operation_none, bw,
+                               (rocblas_int)nc, bw, &one, Linv + i0 + (int64_t)i0 * N, N,
+                               X + n0 * N + i0, N, &zero, tmp + i0, N));
+        if (i0 > 0) {
+          CHFSI_RB(rocblas_sgemm(s->blas, rocblas_operation_none, rocblas_operation_none, bw,
+                                 (rocblas_int)nc, i0, &onef, Lf + i0, N, Xf + n0 * N, N, &zerof,
+                                 tmpf, bw));
+          CHFSI_HIP(launch_add_f32_to_f64(tmpf, bw, tmp + i0, N, bw, nc, st));
+        }
+      }
+    }
+    CHFSI_HIP(hipMemcpyAsync(X + n0 * N, tmp, sizeof(double) * nc * N, hipMemcpyDeviceToDevice,
+                             st));
+  }
+  return CHFSI_OK;
+}
+
+// Lower block triangle of S = Xc Xc^T by column blocks of bs; mixed
+// (fillParallelOverlapMatMixedPrecScalapack, :3831-4231): the diagonal blocks
+// in FP64, the blocks below them from the FP32 copy.
+chfsi_status overlap_lower(chfsi_solver *s, const double *X, const float *Xf, int64_t M, int N,
+                           int bs, double *S, hipStream_t st) {
+  for (int j = 0; j < N; j += bs) {
+    const int bw = std::min(bs, N - j);
+    double *Sj = S + j + (int64_t)j * N;
+    if (!Xf) {
+      CHFSI_TRY(gemm_nt_splitk(s, X + j, N, X + j, N, N - j, bw, M, Sj, N, st));
+    } else {
+      CHFSI_TRY(gemm_nt_splitk(s, X + j, N, X + j, N, bw, bw, M, Sj, N, st));
+      if (N - j - bw > 0)
+        CHFSI_TRY(sgemm_nt_splitk(s, Xf + j + bw, N, Xf + j, N, N - j - bw, bw, M, Sj + bw, N,
+                                  st));
+    }
+  }
+  return CHFSI_OK;
+}
+
+// S = X^T X (lower), S = L L^T, X <- X L^{-T}.  Column-major view: Xc = X^T is
+// N x M with leading dimension N, so S = Xc Xc^T and Xc <- L^{-1} Xc, applied
+// as block-triangular GEMMs with the explicit triangular inverse.
+chfsi_status cholgs_impl(chfsi_solver *s, double *X, int64_t M, int N, int bs, bool mixed_o,
+                         bool mixed_sr, hipStream_t st) {
   CHFSI_TRY(ensure_blas(s, st));
   CHFSI_TRY(s->Hp.ensure(sizeof(double) * N * N));
   CHFSI_TRY(s->info.ensure(sizeof(rocblas_int)));
   double *S = sptr<double>(s->Hp);
-  CHFSI_TRY(gemm_nt_splitk(s, X, N, X, N, N, N, M, S, N, st));
+  float *Xf = nullptr;
+  if (mixed_o || mixed_sr) CHFSI_TRY(cast_x(s, X, M, N, &Xf, st));
+  CHFSI_TRY(overlap_lower(s, X, mixed_o ? Xf : nullptr, M, N, bs, S, st));
   rocblas_int *dinfo = sptr<rocblas_int>(s->info);
   CHFSI_RB(rocsolver_dpotrf(s->blas, rocblas_fill_lower, N, S, N, dinfo));
   rocblas_int hinfo = 0;
@@ -320,25 +448,41 @@ chfsi_status cholgs_impl(chfsi_solver *s, double *X, int64_t M, int N, hipStream
   CHFSI_RB(rocsolver_dtrtri(s->blas, rocblas_fill_lower, rocblas_diagonal_non_unit, N, S, N,
                             dinfo));
   CHFSI_HIP(launch_zero_upper(S, N, N, st));
-  return rotate(s, X, M, N, S, rocblas_operation_none, st);
+  return rotate_lower(s, X, mixed_sr ? Xf : nullptr, M, N, S, bs, st);
 }
 
-// Hp = X^T H~ X (lower triangle by column blocks), eig, X <- X Q.
+// Hp = X^T H~ X (lower triangle by column blocks), eig, X <- X Q.  Mixed
+// X^T H X (XtHXMixedPrec, kohnShamDFTOperatorDevice.cc:4161-): diagonal blocks
+// in FP64, blocks below from FP32 copies of X and of the HX block.
 chfsi_status rr_impl(chfsi_solver *s, chfsi_operator *op, double *X, int64_t M, int N, int B,
-                     double *eig_host, hipStream_t st) {
+                     double *eig_host, bool mixed_xthx, bool mixed_rot, hipStream_t st) {
   CHFSI_TRY(ensure_blas(s, st));
   CHFSI_TRY(s->Hp.ensure(sizeof(double) * N * N));
   CHFSI_TRY(s->HXb.ensure(sizeof(double) * M * B));
   CHFSI_TRY(s->evals.ensure(sizeof(double) * 2 * N));
   CHFSI_TRY(s->info.ensure(sizeof(rocblas_int)));
   double *Hp = sptr<double>(s->Hp), *HX = sptr<double>(s->HXb);
-  const double one = 1.0, zero = 0.0;
+  float *Xf = nullptr, *HXf = nullptr;
+  if (mixed_xthx || mixed_rot) CHFSI_TRY(cast_x(s, X, M, N, &Xf, st));
+  if (mixed_xthx) {
+    CHFSI_TRY(s->HXf.ensure(sizeof(float) * M * B));
+    HXf = sptr<float>(s->HXf);
+  }
   for (int j = 0; j < N; j += B) {
     const int bw = std::min(B, N - j);
     CHFSI_TRY(hx_fused<double>(op, X + j, N, HX, bw, bw, 1.0, 0.0, 0.0, 1, true, st));
     if (op->nc) CHFSI_HIP(launch_zero_rows(op->d_c_rows, op->nc, HX, bw, bw, st));
-    // Hp[j:N, j:j+bw] = Xc[j:N, :] * HXc^T
-    CHFSI_TRY(gemm_nt_splitk(s, X + j, N, HX, bw, N - j, bw, M, Hp + j + (int64_t)j * N, N, st));
+    double *Hj = Hp + j + (int64_t)j * N;
+    if (!mixed_xthx) {
+      // Hp[j:N, j:j+bw] = Xc[j:N, :] * HXc^T
+      CHFSI_TRY(gemm_nt_splitk(s, X + j, N, HX, bw, N - j, bw, M, Hj, N, st));
+    } else {
+      CHFSI_TRY(gemm_nt_splitk(s, X + j, N, HX, bw, bw, bw, M, Hj, N, st));
+      if (N - j - bw > 0) {
+        CHFSI_HIP(launch_copy_block<double, float>(HX, bw, HXf, bw, M, bw, st));
+        CHFSI_TRY(sgemm_nt_splitk(s, Xf + j + bw, N, HXf, bw, N - j - bw, bw, M, Hj + bw, N, st));
+      }
+    }
   }
   double *D = sptr<double>(s->evals), *E = D + N;
   rocblas_int *dinfo = sptr<rocblas_int>(s->info);
@@ -352,7 +496,7 @@ chfsi_status rr_impl(chfsi_solver *s, chfsi_operator *op, double *X, int64_t M, 
     set_error("projected eigensolve failed (syevd info %d)", (int)hinfo);
     return CHFSI_ERR_EIG;
   }
-  return rotate(s, X, M, N, Hp, rocblas_operation_transpose, st);
+  return rotate_dense(s, X, mixed_rot ? Xf : nullptr, M, N, Hp, B, st);
 }
 
 chfsi_status residual_impl(chfsi_solver *s, chfsi_operator *op, const double *X, int64_t M, int N,
@@ -393,6 +537,10 @@ extern "C" void chfsi_default_params(chfsi_params *p) {
   p->filter_precision = CHFSI_FP32;
   p->use_shep = 1;                      // utils/dftParameters.cc:934-938
   p->fp64_tail_degrees = 0;
+  p->mixed_prec_overlap = 0;            // utils/dftParameters.cc:1246-1249
+  p->mixed_prec_cgs_rotation = 0;
+  p->mixed_prec_xthx = 0;
+  p->mixed_prec_rr_rotation = 0;
 }
 
 extern "C" chfsi_status chfsi_solver_set_params(chfsi_solver *s, const chfsi_params *params) {
@@ -464,19 +612,18 @@ extern "C" chfsi_status chfsi_chebyshev_filter(chfsi_operator *op, double *X, in
 //   L_{k-1} += a1_k (H~ - c) L_k                  (one fused HX, beta = 1)
 //   L_{k-2} += a2_k L_k
 // and X0bar = L_0.  Forward iterates come from checkpoints (pairs X_{k-1}, X_k
-// every `every` degrees) re-run one segment at a time.
-extern "C" chfsi_status chfsi_chebyshev_filter_backward(
-    chfsi_operator *op, const double *X0, int64_t ldx, const double *Ybar, int64_t ldy, int32_t B,
-    int32_t m, double a, double b, double a0, int32_t every, double *Xbar, int64_t ldxb,
-    double *g, void *stream) {
-  CHFSI_ARG(op && X0 && Ybar && Xbar && g && B > 0 && m >= 1, "bad argument");
-  CHFSI_ARG(ldx >= B && ldy >= B && ldxb >= B, "bad leading dimension");
-  CHFSI_ARG(op->d_Nq && op->d_jxw, "operator was created without shape data / JxW");
-  hipStream_t st = as_stream(stream);
+// every `every` degrees) re-run one segment at a time.  T is the arithmetic of
+// the iterates and adjoints (FP64, or FP32 on the FP32 apply); g is always
+// accumulated in FP64 from FP64 interpolations of the FP32 vectors.
+namespace {
+template <typename T>
+chfsi_status filter_backward_impl(chfsi_operator *op, const double *X0, int64_t ldx,
+                                  const double *Ybar, int64_t ldy, int B, int m, double a,
+                                  double b, double a0, int every, double *Xbar, int64_t ldxb,
+                                  double *g, hipStream_t st) {
   const int64_t M = op->M;
   const size_t vec = size_t(M) * B;
   every = std::max(1, std::min<int>(every, m));
-  // coefficients of every degree
   ChebState cs = cheb_init(a, b, a0);
   std::vector<double> a1(m + 1, 0.0), a2(m + 1, 0.0);
   a1[1] = cs.sigma / cs.e;
@@ -489,75 +636,95 @@ extern "C" chfsi_status chfsi_chebyshev_filter_backward(
       t.sigma = sigma2;
     }
   }
+  const T c = T(cs.c);
   // storage: X_0 copy, checkpoint pairs, one segment of iterates, 3 adjoint buffers
   const int nseg = (m + every - 1) / every;
   Scratch ck, seg, adj;
-  CHFSI_TRY(ck.ensure(sizeof(double) * vec * (1 + 2 * nseg)));
-  CHFSI_TRY(seg.ensure(sizeof(double) * vec * (every + 2)));
-  CHFSI_TRY(adj.ensure(sizeof(double) * vec * 3));
-  double *ckp = sptr<double>(ck), *segp = sptr<double>(seg), *adjp = sptr<double>(adj);
+  CHFSI_TRY(ck.ensure(sizeof(T) * vec * (1 + 2 * nseg)));
+  CHFSI_TRY(seg.ensure(sizeof(T) * vec * (every + 2)));
+  CHFSI_TRY(adj.ensure(sizeof(T) * vec * 3));
+  T *ckp = sptr<T>(ck), *segp = sptr<T>(seg), *adjp = sptr<T>(adj);
   auto ckv = [&](int i) { return ckp + vec * i; };
-  CHFSI_HIP(launch_copy_block<double, double>(X0, ldx, ckv(0), B, M, B, st));
-  CHFSI_HIP(hipMemsetAsync(segp, 0, sizeof(double) * vec * (every + 2), st));
+  CHFSI_HIP(launch_copy_block<double, T>(X0, ldx, ckv(0), B, M, B, st));
+  CHFSI_HIP(hipMemsetAsync(segp, 0, sizeof(T) * vec * (every + 2), st));
   // forward sweep storing (X_{k0-1}, X_{k0}) for every segment start k0 = s*every >= 1
   {
-    double *prev = segp, *cur = segp + vec;
-    CHFSI_HIP(launch_copy_block<double, double>(ckv(0), B, prev, B, M, B, st));
+    T *prev = segp, *cur = segp + vec;
+    CHFSI_HIP(launch_copy_block<T, T>(ckv(0), B, prev, B, M, B, st));
     int64_t ldc = B, ldp = B;
-    CHFSI_TRY(hx_fused<double>(op, prev, B, cur, B, B, a1[1], 0.0, -cs.c * a1[1], 1, true, st));
+    CHFSI_TRY(hx_fused<T>(op, prev, B, cur, B, B, T(a1[1]), T(0), T(-c * a1[1]), 1, true, st));
     for (int k = 1; k <= m; ++k) {
       if (k % every == 0 && k < m) {
-        const int s = k / every;
-        CHFSI_HIP(launch_copy_block<double, double>(prev, ldp, ckv(2 * s - 1), B, M, B, st));
-        CHFSI_HIP(launch_copy_block<double, double>(cur, ldc, ckv(2 * s), B, M, B, st));
+        const int sg = k / every;
+        CHFSI_HIP(launch_copy_block<T, T>(prev, ldp, ckv(2 * sg - 1), B, M, B, st));
+        CHFSI_HIP(launch_copy_block<T, T>(cur, ldc, ckv(2 * sg), B, M, B, st));
       }
       if (k == m) break;
-      CHFSI_TRY(hx_fused<double>(op, cur, ldc, prev, ldp, B, a1[k + 1], a2[k + 1],
-                                 -cs.c * a1[k + 1], 0, true, st));
+      CHFSI_TRY(hx_fused<T>(op, cur, ldc, prev, ldp, B, T(a1[k + 1]), T(a2[k + 1]),
+                            T(-c * a1[k + 1]), 0, true, st));
       std::swap(cur, prev);
       std::swap(ldc, ldp);
     }
   }
   // adjoint sweep: Lk = L_k, Lm1 = L_{k-1}, Lm2 = L_{k-2}
-  double *Lk = adjp, *Lm1 = adjp + vec, *Lm2 = adjp + 2 * vec;
-  CHFSI_HIP(launch_copy_block<double, double>(Ybar, ldy, Lk, B, M, B, st));
-  CHFSI_HIP(hipMemsetAsync(Lm1, 0, sizeof(double) * vec * 2, st));
-  for (int s = nseg - 1; s >= 0; --s) {
-    const int k0 = s * every, k1 = std::min(m, k0 + every);
+  T *Lk = adjp, *Lm1 = adjp + vec, *Lm2 = adjp + 2 * vec;
+  CHFSI_HIP(launch_copy_block<double, T>(Ybar, ldy, Lk, B, M, B, st));
+  CHFSI_HIP(hipMemsetAsync(Lm1, 0, sizeof(T) * vec * 2, st));
+  for (int sg = nseg - 1; sg >= 0; --sg) {
+    const int k0 = sg * every, k1 = std::min(m, k0 + every);
     // iterates X_{k0-1 .. k1-1} into seg[0 .. ]: seg[i] = X_{k0 - 1 + i}
     auto X = [&](int k) { return segp + vec * (k - k0 + 1); };
     if (k0 == 0) {
-      CHFSI_HIP(launch_copy_block<double, double>(ckv(0), B, X(0), B, M, B, st));
+      CHFSI_HIP(launch_copy_block<T, T>(ckv(0), B, X(0), B, M, B, st));
     } else {
-      CHFSI_HIP(launch_copy_block<double, double>(ckv(2 * s - 1), B, X(k0 - 1), B, M, B, st));
-      CHFSI_HIP(launch_copy_block<double, double>(ckv(2 * s), B, X(k0), B, M, B, st));
+      CHFSI_HIP(launch_copy_block<T, T>(ckv(2 * sg - 1), B, X(k0 - 1), B, M, B, st));
+      CHFSI_HIP(launch_copy_block<T, T>(ckv(2 * sg), B, X(k0), B, M, B, st));
     }
     for (int k = k0 + 1; k <= k1 - 1; ++k) {
       if (k == 1)
-        CHFSI_TRY(hx_fused<double>(op, X(0), B, X(1), B, B, a1[1], 0.0, -cs.c * a1[1], 1, true,
-                                   st));
+        CHFSI_TRY(hx_fused<T>(op, X(0), B, X(1), B, B, T(a1[1]), T(0), T(-c * a1[1]), 1, true,
+                              st));
       else {
-        CHFSI_HIP(launch_copy_block<double, double>(X(k - 2), B, X(k), B, M, B, st));
-        CHFSI_TRY(hx_fused<double>(op, X(k - 1), B, X(k), B, B, a1[k], a2[k], -cs.c * a1[k], 0,
-                                   true, st));
+        CHFSI_HIP(launch_copy_block<T, T>(X(k - 2), B, X(k), B, M, B, st));
+        CHFSI_TRY(hx_fused<T>(op, X(k - 1), B, X(k), B, B, T(a1[k]), T(a2[k]), T(-c * a1[k]), 0,
+                              true, st));
       }
     }
     for (int k = k1; k >= k0 + 1; --k) {
       CHFSI_HIP(launch_quad_product(op->d_map, op->C, op->ne, op->nq, op->d_Nq, op->d_jxw, Lk, B,
                                     X(k - 1), B, B, op->d_inv_sqrt_mass, nullptr, a1[k], g, 1, st));
-      CHFSI_TRY(hx_fused<double>(op, Lk, B, Lm1, B, B, a1[k], 1.0, -cs.c * a1[k], 0, true, st));
-      if (k >= 2) CHFSI_HIP(launch_axpby<double>((int64_t)vec, a2[k], Lk, 1.0, Lm2, st));
+      CHFSI_TRY(hx_fused<T>(op, Lk, B, Lm1, B, B, T(a1[k]), T(1), T(-c * a1[k]), 0, true, st));
+      if (k >= 2) CHFSI_HIP(launch_axpby<T>((int64_t)vec, T(a2[k]), Lk, T(1), Lm2, st));
       // shift: (L_k, L_{k-1}, L_{k-2}) <- (L_{k-1}, L_{k-2}, 0)
-      double *t = Lk;
+      T *t = Lk;
       Lk = Lm1;
       Lm1 = Lm2;
       Lm2 = t;
-      CHFSI_HIP(hipMemsetAsync(Lm2, 0, sizeof(double) * vec, st));
+      CHFSI_HIP(hipMemsetAsync(Lm2, 0, sizeof(T) * vec, st));
     }
   }
-  CHFSI_HIP(launch_copy_block<double, double>(Lk, B, Xbar, ldxb, M, B, st));
+  CHFSI_HIP(launch_copy_block<T, double>(Lk, B, Xbar, ldxb, M, B, st));
   CHFSI_HIP(hipStreamSynchronize(st));
   return CHFSI_OK;
+}
+
+}  // namespace
+
+extern "C" chfsi_status chfsi_chebyshev_filter_backward(
+    chfsi_operator *op, const double *X0, int64_t ldx, const double *Ybar, int64_t ldy, int32_t B,
+    int32_t m, double a, double b, double a0, int32_t every, double *Xbar, int64_t ldxb,
+    double *g, int32_t precision, void *stream) {
+  CHFSI_ARG(op && X0 && Ybar && Xbar && g && B > 0 && m >= 1, "bad argument");
+  CHFSI_ARG(ldx >= B && ldy >= B && ldxb >= B, "bad leading dimension");
+  CHFSI_ARG(op->d_Nq && op->d_jxw, "operator was created without shape data / JxW");
+  hipStream_t st = as_stream(stream);
+  if (precision == CHFSI_FP64)
+    return filter_backward_impl<double>(op, X0, ldx, Ybar, ldy, B, m, a, b, a0, every, Xbar, ldxb,
+                                        g, st);
+  CHFSI_ARG(precision == CHFSI_FP32, "filter backward precision must be CHFSI_FP64 or CHFSI_FP32");
+  CHFSI_ARG(op->nc == 0, "FP32 filter backward supports unconstrained layouts only");
+  return filter_backward_impl<float>(op, X0, ldx, Ybar, ldy, B, m, a, b, a0, every, Xbar, ldxb, g,
+                                     st);
 }
 
 extern "C" chfsi_status chfsi_lanczos_bounds(chfsi_operator *op, const double *v0, int32_t steps,
@@ -572,14 +739,15 @@ extern "C" chfsi_status chfsi_cholgs(chfsi_operator *op, double *X, int64_t M, i
                                      void *stream) {
   CHFSI_ARG(op && X && M == op->M && N > 0, "bad argument");
   chfsi_solver tmp;
-  return cholgs_impl(&tmp, X, M, N, as_stream(stream));
+  return cholgs_impl(&tmp, X, M, N, std::min<int>(N, 256), false, false, as_stream(stream));
 }
 
 extern "C" chfsi_status chfsi_rayleigh_ritz(chfsi_operator *op, double *X, int64_t M, int32_t N,
                                             int32_t block, double *eig_host, void *stream) {
   CHFSI_ARG(op && X && eig_host && M == op->M && N > 0 && block > 0, "bad argument");
   chfsi_solver tmp;
-  return rr_impl(&tmp, op, X, M, N, std::min<int>(block, N), eig_host, as_stream(stream));
+  return rr_impl(&tmp, op, X, M, N, std::min<int>(block, N), eig_host, false, false,
+                 as_stream(stream));
 }
 
 extern "C" chfsi_status chfsi_residual_norms(chfsi_operator *op, const double *X, int64_t M,
@@ -652,9 +820,12 @@ extern "C" chfsi_status chfsi_solve(chfsi_solver *s, chfsi_operator *op, double 
   CHFSI_HIP(hipEventRecord(ev[2], st));
 
   // CholGS + Rayleigh-Ritz (SHEP route, …SolverDevice.cc:609-643)
-  CHFSI_TRY(cholgs_impl(s, X, M, N, st));
+  const bool mx = use_mixed_prec_overall != 0;
+  CHFSI_TRY(cholgs_impl(s, X, M, N, B, mx && P.mixed_prec_overlap, mx && P.mixed_prec_cgs_rotation,
+                        st));
   CHFSI_HIP(hipEventRecord(ev[3], st));
-  CHFSI_TRY(rr_impl(s, op, X, M, N, B, eig_host, st));
+  CHFSI_TRY(rr_impl(s, op, X, M, N, B, eig_host, mx && P.mixed_prec_xthx,
+                    mx && P.mixed_prec_rr_rotation, st));
   CHFSI_HIP(hipEventRecord(ev[4], st));
 
   if (compute_residual) CHFSI_TRY(residual_impl(s, op, X, M, N, B, eig_host, res_host, st));

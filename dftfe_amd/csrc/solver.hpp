@@ -23,11 +23,14 @@ struct chfsi_solver {
   double a0 = 0, bLow = 0, bUp = 0;
   rocblas_handle blas = nullptr;
   chfsi::Scratch Y64, X32, Y32, HXb, Hp, rot, evals, info, part, lam, v0, split;
+  // mixed-precision dense steps: FP32 copies of X, of an HX block and of the
+  // N x N factor, FP32 split-K partials and rotation panel
+  chfsi::Scratch Xf, HXf, Qf, split32, rot32;
   bool have_v0 = false;
   ~chfsi_solver() {
     if (blas) rocblas_destroy_handle(blas);
     for (chfsi::Scratch *s : {&Y64, &X32, &Y32, &HXb, &Hp, &rot, &evals, &info, &part, &lam, &v0,
-                              &split})
+                              &split, &Xf, &HXf, &Qf, &split32, &rot32})
       s->release();
   }
 };

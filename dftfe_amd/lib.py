@@ -63,6 +63,8 @@ class Params(ct.Structure):
         ("reuse_lanczos_upper", ct.c_int32), ("first_scf_degree_factor", ct.c_double),
         ("filter_precision", ct.c_int32), ("use_shep", ct.c_int32),
         ("fp64_tail_degrees", ct.c_int32),
+        ("mixed_prec_overlap", ct.c_int32), ("mixed_prec_cgs_rotation", ct.c_int32),
+        ("mixed_prec_xthx", ct.c_int32), ("mixed_prec_rr_rotation", ct.c_int32),
     ]
 
 
@@ -77,6 +79,7 @@ SIGNATURES = {
     "chfsi_operator_set_cell_matrices": (ct.c_int, [_VP, _VP, _VP]),
     "chfsi_operator_get_cell_matrices": (ct.c_int, [_VP, _VP, _VP]),
     "chfsi_operator_set_hx_variant": (ct.c_int, [_VP, ct.c_int32, ct.c_int32]),
+    "chfsi_operator_set_proj_variant": (ct.c_int, [_VP, ct.c_int32, ct.c_int32]),
     "chfsi_operator_autotune": (ct.c_int, [_VP, ct.c_int32, c_int32_p, c_int32_p,
                                            ct.POINTER(ct.c_float), _VP]),
     "chfsi_operator_sqrt_mass": (_VP, [_VP]),
@@ -95,7 +98,7 @@ SIGNATURES = {
     "chfsi_chebyshev_filter_backward": (ct.c_int, [_VP, _VP, ct.c_int64, _VP, ct.c_int64,
                                                    ct.c_int32, ct.c_int32, ct.c_double,
                                                    ct.c_double, ct.c_double, ct.c_int32, _VP,
-                                                   ct.c_int64, _VP, _VP]),
+                                                   ct.c_int64, _VP, ct.c_int32, _VP]),
     "chfsi_constraints_distribute": (ct.c_int, [_VP, _VP, ct.c_int64, ct.c_int32, _VP]),
     "chfsi_constraints_slave_to_master": (ct.c_int, [_VP, _VP, ct.c_int64, ct.c_int32, _VP]),
     "chfsi_constraints_set_zero": (ct.c_int, [_VP, _VP, ct.c_int64, ct.c_int32, _VP]),

@@ -133,6 +133,14 @@ typedef struct {
   int32_t use_shep;                /* SUBSPACE PROJ SHEP GPU: 1 = CholGS + RR   */
   int32_t fp64_tail_degrees;       /* last degrees of a mixed-precision filter run
                                       in FP64 (0: whole filter in filter_precision) */
+  /* Mixed-precision dense steps, applied when use_mixed_prec_overall is set:
+   * diagonal blocks (cheby_block_size wide) in FP64, off-diagonal blocks in
+   * FP32 (linearAlgebraOperationsDevice.cc:3831-4609, :2243-3077,
+   * kohnShamDFTOperatorDevice.cc:4161-; parameters utils/dftParameters.cc:942-966) */
+  int32_t mixed_prec_overlap;      /* USE MIXED PREC CGS O: S = X^T X            */
+  int32_t mixed_prec_cgs_rotation; /* USE MIXED PREC CGS SR: X <- X L^{-T}        */
+  int32_t mixed_prec_xthx;         /* USE MIXED PREC XTHX: Hp = X^T H~ X          */
+  int32_t mixed_prec_rr_rotation;  /* USE MIXED PREC RR_SR: X <- X Q              */
 } chfsi_params;
 
 const char *chfsi_last_error(void);
@@ -157,10 +165,14 @@ chfsi_status chfsi_operator_get_cell_matrices(chfsi_operator *op, double *H_dev,
 
 /* Tile-shape selection of the fused HX kernel (no reference counterpart: the
  * reference calls hipBLAS gemmStridedBatched).  chfsi_operator_autotune times
- * every tile variant of both precisions on a B-column block, keeps the fastest
- * and reports times_ms[0..7] (FP64 variants) and [8..15] (FP32 variants),
- * -1 where a variant does not exist. */
+ * every tile variant of both precisions on a B-column block, keeps the fastest,
+ * then times the projection-kernel variants with it and keeps the fastest.
+ * times_ms (32 entries) [0..11]: FP64 tile variants, [12..15]: FP64 projection
+ * variants, [16..27]: FP32 tile variants, [28..31]: FP32 projection variants
+ * (ms per HX; -1 where a variant does not exist). */
 chfsi_status chfsi_operator_set_hx_variant(chfsi_operator *op, int32_t precision, int32_t variant);
+chfsi_status chfsi_operator_set_proj_variant(chfsi_operator *op, int32_t precision,
+                                             int32_t variant);
 chfsi_status chfsi_operator_autotune(chfsi_operator *op, int32_t B, int32_t *best64,
                                      int32_t *best32, float *times_ms, void *stream);
 
@@ -217,15 +229,18 @@ chfsi_status chfsi_hx_adjoint_v(chfsi_operator *op, const double *Ybar, int64_t 
                                 const double *X, int64_t ldx, int32_t B, double scalar,
                                 double *g_dev, int32_t accumulate, void *stream);
 
-/* Reverse pass of chfsi_chebyshev_filter (FP64; no reference counterpart).
+/* Reverse pass of chfsi_chebyshev_filter (no reference counterpart).
  * Given the filter input X0 and Ybar = dL/dY of its output, writes
  * Xbar = dL/dX0 and accumulates dL/dV(c,q) into g.  Forward iterates are
- * recomputed from checkpoints stored every `checkpoint_every` degrees. */
+ * recomputed from checkpoints stored every `checkpoint_every` degrees.
+ * precision CHFSI_FP64, or CHFSI_FP32: iterates and adjoints in FP32 on the
+ * FP32 apply, the gradient accumulated in FP64. */
 chfsi_status chfsi_chebyshev_filter_backward(chfsi_operator *op, const double *X0, int64_t ldx,
                                              const double *Ybar, int64_t ldy, int32_t B,
                                              int32_t m, double a, double b, double a0,
                                              int32_t checkpoint_every, double *Xbar,
-                                             int64_t ldxb, double *g_dev, void *stream);
+                                             int64_t ldxb, double *g_dev, int32_t precision,
+                                             void *stream);
 
 /* ---------------- linear algebra (linearAlgebraOperationsDevice) ---------------- */
 

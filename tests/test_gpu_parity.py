@@ -66,7 +66,7 @@ def test_hx_fp64_matches_oracle(small, B):
     assert _rel(Y, ref.apply(X)) < 1e-13
 
 
-@pytest.mark.parametrize("precision,variant", [(0, 0), (0, 1), (0, 2), (1, 0), (1, 1), (1, 2), (1, 3), (1, 4)])
+@pytest.mark.parametrize("precision,variant", [(0, v) for v in range(5)] + [(1, v) for v in range(7)])
 def test_hx_tile_variants(small, precision, variant):
     """Every HX tile variant gives the oracle's H~X (FP64: 1e-13, FP32: 2e-6)."""
     prob, ref, op = small
@@ -79,6 +79,23 @@ def test_hx_tile_variants(small, precision, variant):
         op.HX(Xd, False, 1.0, Yd)
     finally:
         op.set_hx_variant(precision, 0)
+    want = ref.apply(X if precision == 0 else X.astype(np.float32).astype(np.float64))
+    assert _rel(Yd.double().cpu().numpy(), want) < (1e-13 if precision == 0 else 2e-6)
+
+
+@pytest.mark.parametrize("precision,variant", [(p, v) for p in (0, 1) for v in range(3)])
+def test_projection_variants(small, precision, variant):
+    """Every projection-kernel variant gives the oracle's H~X."""
+    prob, ref, op = small
+    X = prob.initial_guess(N=72, seed_offset=22)
+    dt = torch.float64 if precision == 0 else torch.float32
+    Xd = torch.tensor(X, device="cuda", dtype=dt)
+    Yd = torch.zeros_like(Xd)
+    op.set_proj_variant(precision, variant)
+    try:
+        op.HX(Xd, False, 1.0, Yd)
+    finally:
+        op.set_proj_variant(precision, 0)
     want = ref.apply(X if precision == 0 else X.astype(np.float32).astype(np.float64))
     assert _rel(Yd.double().cpu().numpy(), want) < (1e-13 if precision == 0 else 2e-6)
 
@@ -261,6 +278,29 @@ def test_pass_mixed_fp32(small):
     err = np.abs(lam - lam_o).max()
     print(f"mixed-precision single pass max|dlambda| = {err:.3e} Ha")
     assert err < 1e-5
+
+
+def test_pass_blocks_and_mixed_dense(small):
+    """N = 64 in blocks of 16: the block-triangular overlap / rotation match the
+    oracle (FP64); the mixed-precision dense steps (FP32 off-diagonal blocks,
+    FP64 filter) stay within FP32 rounding of it."""
+    from dftfe_amd import lib as L
+    from dftfe_amd.chfsi import ChebyshevSolverDevice
+
+    prob, ref, op = small
+    lo, up, _, _ = O.lanczos_bounds(ref, prob.lanczos_start, 10)
+    bounds = O.first_call_bounds(lo, up, 64, prob.M)
+    X = prob.initial_guess()
+    lam_o, res_o, _ = O.chfsi_pass(ref, X, *bounds, 20, 16)
+    for mixed in (False, True):
+        solver = ChebyshevSolverDevice(*bounds, cheby_block_size=16, cheby_degree=20,
+                                       reuse_lanczos_upper=True, filter_precision=L.FP64,
+                                       mixed_dense=mixed)
+        Xd = torch.tensor(X, device="cuda")
+        lam, res, _ = solver.solve(op, Xd, use_mixed_prec_overall=True)
+        err = float(np.abs(lam - lam_o).max())
+        print(f"blocks of 16, mixed dense={mixed}: max|dlambda| = {err:.3e} Ha")
+        assert err < (1e-6 if mixed else 1e-10)
 
 
 def test_dealii_layout_matches_unique(small, small_dealii):
