@@ -129,9 +129,9 @@ def main():
                           "calibration_copy_bytes": 4 << 28, "autotune": tune}), flush=True)
         return
 
-    def run_passes(mixed: bool, npass: int):
+    def run_passes(mixed: bool, npass: int, dense: bool = False):
         solver = ChebyshevSolverDevice(cheby_block_size=B, cheby_degree=m,
-                                       lanczos_steps=wl.lanczos_steps)
+                                       lanczos_steps=wl.lanczos_steps, mixed_dense=dense)
         solver.set_lanczos_start(v0)
         X = X0.clone()
         lams, ress, phases = [], [], []
@@ -161,6 +161,11 @@ def main():
     if not args.no_fp64:
         fp64_ms, lam_64, res_64, ph_64 = run_passes(False, args.steps)
         log(f"fp64: {fp64_ms:.1f} ms/pass; last max residual {res_64[-1].max():.3e}")
+    # the same mixed passes with the reference's opt-in mixed-precision dense
+    # steps (FP32 off-diagonal blocks of the overlap, CholGS rotation, X^T H X
+    # and Rayleigh-Ritz rotation)
+    md_ms, lam_md, _, ph_md = run_passes(True, args.steps, dense=True)
+    log(f"mixed + mixed dense: {md_ms:.1f} ms/pass")
     del X0
 
     hx32_ms = time_hx(torch.float32)
@@ -191,6 +196,8 @@ def main():
                    "fe_degree": wl.p, "wavefunctions": N, "block": B, "cheby_degree": m,
                    "atoms": wl.n_atoms, "passes": args.steps, "parallelism": "single GPU"},
         "mixed_pass_ms": round(mixed_ms, 2),
+        "mixed_dense_pass_ms": round(md_ms, 2),
+        "phase_ms_mixed_dense_last": ph_md[-1],
         "fp64_pass_ms": round(fp64_ms, 2) if fp64_ms else None,
         "mixed_over_fp64_speedup": round(fp64_ms / mixed_ms, 3) if fp64_ms else None,
         "hx_fp32_ms": round(hx32_ms, 4),
@@ -204,6 +211,8 @@ def main():
         "max_residual_mixed": [float(r.max()) for r in res_mx],
     }
     if lam_64 is not None:
+        out["max_eig_err_mixed_dense_vs_fp64_Ha"] = [float(np.abs(a - b).max())
+                                                    for a, b in zip(lam_md, lam_64)]
         out["max_eig_err_mixed_vs_fp64_Ha"] = [float(np.abs(a - b).max())
                                               for a, b in zip(lam_mx, lam_64)]
         out["max_residual_fp64"] = [float(r.max()) for r in res_64]

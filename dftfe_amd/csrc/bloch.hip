@@ -155,6 +155,7 @@ chfsi_status zapply_round(chfsi_operator *op, const ZHxItem<T> *dev_items,
     pl.nk = n;
     pl.cells = op->d_nl_cells;
     pl.ncells = op->n_nl_cells;
+    pl.max_srb = op->max_srb;
     pl.map = op->d_map;
     pl.ne = op->ne;
     pl.ne_pad = op->ne_pad;

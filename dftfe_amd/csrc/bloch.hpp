@@ -60,6 +60,7 @@ struct ZProjLaunch {
   const int *cells;          // cells that own slots
   int ncells;
   int per_xcd;
+  int max_srb;               // largest slot count of any cell / 16 (1..8)
   const uint32_t *map;
   int ne, ne_pad;
   const T *s_in;

@@ -66,7 +66,7 @@ __global__ __launch_bounds__(64 * NW) void hx_kernel(HxLaunch<T> a) {
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wave = tid >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform (scalar)
   const int wc = wave % WC;
   const int wr = wave / WC;
   constexpr int nrb = NRB;
@@ -311,7 +311,7 @@ __global__ __launch_bounds__(64 * NW) void hx_bf16_kernel(HxLaunch<float> a, con
   const int v0 = (item % a.ntiles) * TN;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wave = tid >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform (scalar)
   const int wc = wave % WC;
   const int wr = wave / WC;
   constexpr int nkt = NRB;
@@ -482,7 +482,7 @@ __global__ __launch_bounds__(64 * NW) void proj_kernel(ProjLaunch<T> a) {
   const int v0 = (item % a.ntiles) * TN;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wave = tid >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform (scalar)
   const int s0 = a.nl_off[cell];
   const int nslot = a.nl_off[cell + 1] - s0;
   const int nsrb = nslot >> 4;
@@ -1010,7 +1010,10 @@ static hipError_t hx_dispatch(const HxLaunch<T> &a, hipStream_t s) {
 //         2: 16 x 64 x 4 (four row groups, 16 waves per CU)
 //   prefetch depth two (two slices in flight): FP32 5: 8 x 64 x 4, 6: 16 x 128 x 8;
 //   FP64 3: 16 x 64 x 4, 4: 8 x 64 x 4
-int hx_variants(bool fp64) { return fp64 ? 5 : 7; }
+//   fewer column waves (each A operand feeds more column blocks):
+//   FP32 7: 8 x 64 x 2, 8: 8 x 64 x 1, 9: 16 x 128 x 4; FP64 5: 8 x 64 x 2, 6: 16 x 64 x 2
+//   sixteen waves on a 64-column tile: FP32 10: 16 x 64 x 4
+int hx_variants(bool fp64) { return fp64 ? 7 : 11; }
 
 template <>
 hipError_t launch_hx<float>(const HxLaunch<float> &a, hipStream_t s) {
@@ -1021,6 +1024,10 @@ hipError_t launch_hx<float>(const HxLaunch<float> &a, hipStream_t s) {
     case 4: return hx_dispatch<float, 8, 32, 2>(a, s);
     case 5: return hx_dispatch<float, 8, 64, 4, 2>(a, s);
     case 6: return hx_dispatch<float, 16, 128, 8, 2>(a, s);
+    case 7: return hx_dispatch<float, 8, 64, 2>(a, s);
+    case 8: return hx_dispatch<float, 8, 64, 1>(a, s);
+    case 9: return hx_dispatch<float, 16, 128, 4>(a, s);
+    case 10: return hx_dispatch<float, 16, 64, 4>(a, s);
     default: return hx_dispatch<float, 8, 128, 8>(a, s);
   }
 }
@@ -1032,6 +1039,8 @@ hipError_t launch_hx<double>(const HxLaunch<double> &a, hipStream_t s) {
     case 2: return hx_dispatch<double, 16, 64, 4>(a, s);
     case 3: return hx_dispatch<double, 16, 64, 4, 2>(a, s);
     case 4: return hx_dispatch<double, 8, 64, 4, 2>(a, s);
+    case 5: return hx_dispatch<double, 8, 64, 2>(a, s);
+    case 6: return hx_dispatch<double, 16, 64, 2>(a, s);
     default: return hx_dispatch<double, 8, 64, 4>(a, s);
   }
 }

@@ -67,7 +67,7 @@ __global__ __launch_bounds__(64 * NW) void zhx_kernel(ZHxLaunch<T> a) {
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wave = tid >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform (scalar)
   const int wc = wave % WC;
   const int wr = wave / WC;
   constexpr int nrb = NRB;
@@ -278,7 +278,7 @@ __global__ __launch_bounds__(64 * NW) void zproj_kernel(ZProjLaunch<T> a) {
   const int v0 = (rel % ntiles) * TN;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wave = tid >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform (scalar)
   const int s0 = a.nl_off[cell];
   const int nslot = a.nl_off[cell + 1] - s0;
   const int nsrb = nslot >> 4;
@@ -527,7 +527,7 @@ hipError_t launch_zhx<double>(ZHxLaunch<double> a, const ZHxItem<double> *h, int
   }
 }
 
-template <typename T, int NRB>
+template <typename T, int NRB, int MAXSRB>
 static hipError_t zproj_go(ZProjLaunch<T> a, const ZHxItem<T> *h, hipStream_t s) {
   constexpr int TN = sizeof(T) == 4 ? 128 : 64;
   int total = 0;
@@ -538,20 +538,29 @@ static hipError_t zproj_go(ZProjLaunch<T> a, const ZHxItem<T> *h, hipStream_t s)
   a.kbegin[a.nk] = total;
   if (total == 0) return hipSuccess;
   a.per_xcd = (total + 7) / 8;
-  hipLaunchKernelGGL((zproj_kernel<T, 4, TN, 8, NRB>), dim3(8 * a.per_xcd), dim3(256), 0, s, a);
+  hipLaunchKernelGGL((zproj_kernel<T, 4, TN, MAXSRB, NRB>), dim3(8 * a.per_xcd), dim3(256), 0, s,
+                     a);
   return hipGetLastError();
+}
+
+// accumulators sized to the largest slot count of any cell
+template <typename T, int NRB>
+static hipError_t zproj_fit(ZProjLaunch<T> a, const ZHxItem<T> *h, hipStream_t s) {
+  if (a.max_srb <= 2) return zproj_go<T, NRB, 2>(a, h, s);
+  if (a.max_srb <= 4) return zproj_go<T, NRB, 4>(a, h, s);
+  return zproj_go<T, NRB, 8>(a, h, s);
 }
 
 template <typename T>
 hipError_t launch_zproj(ZProjLaunch<T> a, const ZHxItem<T> *h, hipStream_t s) {
-  if (a.nk < 1 || a.nk > kMaxBatch) return hipErrorInvalidValue;
+  if (a.nk < 1 || a.nk > kMaxBatch || a.max_srb < 1 || a.max_srb > 8) return hipErrorInvalidValue;
   switch (a.ne_pad >> 4) {
-    case 1: return zproj_go<T, 1>(a, h, s);
-    case 2: return zproj_go<T, 2>(a, h, s);
-    case 4: return zproj_go<T, 4>(a, h, s);
-    case 8: return zproj_go<T, 8>(a, h, s);
-    case 14: return zproj_go<T, 14>(a, h, s);
-    case 22: return zproj_go<T, 22>(a, h, s);
+    case 1: return zproj_fit<T, 1>(a, h, s);
+    case 2: return zproj_fit<T, 2>(a, h, s);
+    case 4: return zproj_fit<T, 4>(a, h, s);
+    case 8: return zproj_fit<T, 8>(a, h, s);
+    case 14: return zproj_fit<T, 14>(a, h, s);
+    case 22: return zproj_fit<T, 22>(a, h, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -66,7 +66,7 @@ def test_hx_fp64_matches_oracle(small, B):
     assert _rel(Y, ref.apply(X)) < 1e-13
 
 
-@pytest.mark.parametrize("precision,variant", [(0, v) for v in range(5)] + [(1, v) for v in range(7)])
+@pytest.mark.parametrize("precision,variant", [(0, v) for v in range(7)] + [(1, v) for v in range(11)])
 def test_hx_tile_variants(small, precision, variant):
     """Every HX tile variant gives the oracle's H~X (FP64: 1e-13, FP32: 2e-6)."""
     prob, ref, op = small
