@@ -66,7 +66,7 @@ __global__ __launch_bounds__(64 * NW) void hx_kernel(HxLaunch<T> a) {
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform (scalar)
+  const int wave = tid >> 6;
   const int wc = wave % WC;
   const int wr = wave / WC;
   constexpr int nrb = NRB;
@@ -311,7 +311,7 @@ __global__ __launch_bounds__(64 * NW) void hx_bf16_kernel(HxLaunch<float> a, con
   const int v0 = (item % a.ntiles) * TN;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform (scalar)
+  const int wave = tid >> 6;
   const int wc = wave % WC;
   const int wr = wave / WC;
   constexpr int nkt = NRB;
@@ -482,7 +482,7 @@ __global__ __launch_bounds__(64 * NW) void proj_kernel(ProjLaunch<T> a) {
   const int v0 = (item % a.ntiles) * TN;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform (scalar)
+  const int wave = tid >> 6;
   const int s0 = a.nl_off[cell];
   const int nslot = a.nl_off[cell + 1] - s0;
   const int nsrb = nslot >> 4;

@@ -67,7 +67,7 @@ __global__ __launch_bounds__(64 * NW) void zhx_kernel(ZHxLaunch<T> a) {
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform (scalar)
+  const int wave = tid >> 6;
   const int wc = wave % WC;
   const int wr = wave / WC;
   constexpr int nrb = NRB;
@@ -278,7 +278,7 @@ __global__ __launch_bounds__(64 * NW) void zproj_kernel(ZProjLaunch<T> a) {
   const int v0 = (rel % ntiles) * TN;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform (scalar)
+  const int wave = tid >> 6;
   const int s0 = a.nl_off[cell];
   const int nslot = a.nl_off[cell + 1] - s0;
   const int nsrb = nslot >> 4;
