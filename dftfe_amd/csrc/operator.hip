@@ -580,7 +580,8 @@ extern "C" chfsi_status chfsi_operator_set_proj_variant(chfsi_operator *op, int3
 namespace {
 template <typename T>
 chfsi_status time_variants(chfsi_operator *op, int B, int *best, float *times, hipStream_t st) {
-  const int nv = hx_variants(sizeof(T) == 8);
+  // times[0..11]: tile variants, times[12..15]: projection variants
+  const int nv = hx_variants(sizeof(T) == 8) < 12 ? hx_variants(sizeof(T) == 8) : 12;
   Scratch x, y;
   CHFSI_TRY(x.ensure(sizeof(T) * op->M * B));
   CHFSI_TRY(y.ensure(sizeof(T) * op->M * B));
@@ -617,7 +618,7 @@ chfsi_status time_variants(chfsi_operator *op, int B, int *best, float *times, h
     int &pv = sizeof(T) == 8 ? op->proj_variant64 : op->proj_variant32;
     float best_p = 1e30f;
     int bp = 0;
-    for (int v = 0; v < proj_variants() && v < 3; ++v) {
+    for (int v = 0; v < proj_variants() && v < 4; ++v) {
       pv = v;
       const T *xs = static_cast<const T *>(x.ptr);
       T *ys = static_cast<T *>(y.ptr);
