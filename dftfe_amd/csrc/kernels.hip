@@ -21,6 +21,88 @@
 
 namespace chfsi {
 
+// Epilogue of the fused HX: scaled scatter of the wave's accumulators with the
+// first-touch combine (rows rb = wr + WR*j, column blocks wc*CBW + cb).
+// Branch-free: the destination (and, with gamma != 0, source) values of JB
+// row blocks are loaded from clamped, always-valid addresses before any is
+// used, so their latencies overlap; the combine is a select and only the
+// stores are predicated.  sout holds the output scale of each local row.
+template <typename T, int NRB, int WR, int RBW, int CBW>
+__device__ __forceinline__ void hx_scatter(const HxLaunch<T> &a,
+                                           const typename V4<T>::t (&acc)[RBW][CBW],
+                                           const uint32_t *smap, const T *sout, int wr, int wc,
+                                           int lane, int v0) {
+  // row blocks per batch: 16 (FP32) or 8 (FP64) elements per thread
+  constexpr int JBMAX = (sizeof(T) == 4 ? 4 : 2) / CBW > 0 ? (sizeof(T) == 4 ? 4 : 2) / CBW : 1;
+  constexpr int JB = RBW < JBMAX ? RBW : JBMAX;
+  const bool with_src = a.gamma != T(0);
+#pragma unroll
+  for (int j0 = 0; j0 < RBW; j0 += JB) {
+    T dv[JB][CBW][4], sv[JB][CBW][4];
+    uint32_t gg[JB][4];
+#pragma unroll
+    for (int jj = 0; jj < JB; ++jj) {
+      const int rb = wr + WR * (j0 + jj);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = rb * 16 + acc_row<T>(lane, r);
+        const int rowc = (j0 + jj < RBW && rb < NRB && row < a.ne) ? row : 0;
+        gg[jj][r] = smap[rowc] & 0x7fffffffu;
+      }
+    }
+#pragma unroll
+    for (int jj = 0; jj < JB; ++jj)
+#pragma unroll
+      for (int cb = 0; cb < CBW; ++cb) {
+        const int v = v0 + (wc * CBW + cb) * 16 + (lane & 15);
+        const int vc = v < a.B ? v : a.B - 1;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) dv[jj][cb][r] = a.dst[(int64_t)gg[jj][r] * a.ld_dst + vc];
+      }
+    if (with_src) {
+#pragma unroll
+      for (int jj = 0; jj < JB; ++jj)
+#pragma unroll
+        for (int cb = 0; cb < CBW; ++cb) {
+          const int v = v0 + (wc * CBW + cb) * 16 + (lane & 15);
+          const int vc = v < a.B ? v : a.B - 1;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) sv[jj][cb][r] = a.src[(int64_t)gg[jj][r] * a.ld_src + vc];
+        }
+    } else {
+#pragma unroll
+      for (int jj = 0; jj < JB; ++jj)
+#pragma unroll
+        for (int cb = 0; cb < CBW; ++cb)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) sv[jj][cb][r] = T(0);
+    }
+#pragma unroll
+    for (int jj = 0; jj < JB; ++jj) {
+      const int j = j0 + jj;
+      if (j >= RBW) continue;
+      const int rb = wr + WR * j;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = rb * 16 + acc_row<T>(lane, r);
+        const bool rok = rb < NRB && row < a.ne;
+        const int rowc = rok ? row : 0;
+        const bool first = (smap[rowc] >> 31) != 0;
+        const T so = sout[rowc];
+#pragma unroll
+        for (int cb = 0; cb < CBW; ++cb) {
+          const int v = v0 + (wc * CBW + cb) * 16 + (lane & 15);
+          const T d = dv[jj][cb][r];
+          const T keep = a.beta_mode == 0 ? a.beta * d : (a.beta_mode == 2 ? a.beta * so * d : T(0));
+          const T base = first ? keep + a.gamma * sv[jj][cb][r] : d;
+          if (rok && v < a.B)
+            a.dst[(int64_t)gg[jj][r] * a.ld_dst + v] = base + a.alpha * so * acc[j][cb][r];
+        }
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Fused HX over one colour class
 // ---------------------------------------------------------------------------
@@ -33,8 +115,13 @@ namespace chfsi {
 // row blocks rb = wr + WR*j.  MFMA operand mapping (16x16x4): lane l supplies
 // A[l%16][k] and B[k][l%16] with k = 4*(l/16) + r for the r-th of four MFMAs
 // over a 16-wide K slice; result lane l holds column l%16, rows acc_row().
-template <typename T, int NW, int TN, int WC, int NRB, int PD>
-__global__ __launch_bounds__(64 * NW) void hx_kernel(HxLaunch<T> a) {
+// Eight-wave FP32 tiles of up to 64 columns fit two workgroups per CU in LDS;
+// with one column block per wave their registers are held to 128 so that both
+// are resident.
+template <typename T, int NW, int TN, int WC, int NRB>
+__global__ __launch_bounds__(64 * NW) __attribute__((
+    amdgpu_waves_per_eu(NW == 8 && TN <= 64 && TN == 16 * WC && sizeof(T) == 4 ? 4 : 1))) void hx_kernel(
+    HxLaunch<T> a) {
   using v4 = typename V4<T>::t;
   constexpr int NT = 64 * NW;
   constexpr int WR = NW / WC;
@@ -46,15 +133,20 @@ __global__ __launch_bounds__(64 * NW) void hx_kernel(HxLaunch<T> a) {
   static_assert(KPT >= 1, "column tile too narrow");
   static_assert(CBW >= 1 && TN % (16 * WC) == 0, "bad column split");
   constexpr int HCH = (ROWS_MAX * 4 + NT - 1) / NT;
-  constexpr int NLMAX = 128;  // projector slots per cell
+  constexpr int NLMAX = 128;             // projector slots per cell
+  constexpr int KMAX = ROWS_MAX + NLMAX;  // K rows: cell nodes, then projector slots
 
   // double-buffered K slices of the cell matrix and of the gathered block
   __shared__ __align__(16) T Hs[2][ROWS_MAX * RS];
   __shared__ __align__(16) T Xs[2][TN * RS];
-  // per-cell gather tables: DOF and input scale of each local node, projector of each slot
+  // per-cell tables: DOF and first-touch bit of each local node (epilogue);
+  // element offset of the source row and input scale of every K row (a DOF
+  // row of src for the cell's nodes, a row of W for its projector slots;
+  // scale 0 pads)
   __shared__ uint32_t smap[ROWS_MAX];
-  __shared__ T sscale[ROWS_MAX];
-  __shared__ int sproj[NLMAX];
+  __shared__ T sout[ROWS_MAX];
+  __shared__ int64_t soff[KMAX];
+  __shared__ T sscl[KMAX];
 
   const int wid = blockIdx.x;
   const int local = wid >> 3;
@@ -65,6 +157,7 @@ __global__ __launch_bounds__(64 * NW) void hx_kernel(HxLaunch<T> a) {
   const int v0 = (item % a.ntiles) * TN;
 
   const int tid = threadIdx.x;
+  __builtin_assume(tid < NT);
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wc = wave % WC;
@@ -83,18 +176,33 @@ __global__ __launch_bounds__(64 * NW) void hx_kernel(HxLaunch<T> a) {
   const T *Hcell = a.Ht + (int64_t)cell * a.ne_pad * a.ne_pad;
   const T *NLcell = a.nl_C ? a.nl_C + (int64_t)nl_begin * a.ne_pad : Hcell;
 
-  // gather tables (padding rows point at DOF 0 with scale 0)
-  for (int i = tid; i < ROWS_MAX; i += NT) {
-    uint32_t w = 0;
+  // gather tables (padding rows and empty slots read row 0 with scale 0)
+  for (int i = tid; i < KMAX; i += NT) {
+    int64_t off = 0;
     T sc = T(0);
-    if (i < a.ne) {
-      w = cmap[i];
-      sc = a.s_in ? a.s_in[w & 0x7fffffffu] : T(1);
+    if (i < ROWS_MAX) {
+      uint32_t w = 0;
+      T so = T(1);
+      if (i < a.ne) {
+        w = cmap[i];
+        const uint32_t r = w & 0x7fffffffu;
+        off = (int64_t)r * a.ld_src;
+        sc = a.s_in ? a.s_in[r] : T(1);
+        if (a.s_out) so = a.s_out[r];
+      }
+      smap[i] = w;
+      sout[i] = so;
+    } else {
+      const int sl = i - ROWS_MAX;
+      const int pj = sl < nslot ? a.slot_proj[nl_begin + sl] : -1;
+      if (pj >= 0) {
+        off = (int64_t)pj * a.ldW;
+        sc = T(1);
+      }
     }
-    smap[i] = w;
-    sscale[i] = sc;
+    soff[i] = off;
+    sscl[i] = sc;
   }
-  for (int s = tid; s < NLMAX; s += NT) sproj[s] = (s < nslot) ? a.slot_proj[nl_begin + s] : -1;
 
   const int gv = tid % TN;
   const int gk = (tid / TN) * KPT;
@@ -108,40 +216,31 @@ __global__ __launch_bounds__(64 * NW) void hx_kernel(HxLaunch<T> a) {
 #pragma unroll
     for (int cb = 0; cb < CBW; ++cb) acc[j][cb] = v4{0, 0, 0, 0};
 
-  // register staging of the next slice(s): PD = 1 loads slice t+1 while slice
-  // t is multiplied; PD = 2 keeps two slices in flight (t+1 in one set, t+2
-  // issued into the other) so a gather has two slices of MFMA work to land
-  v4 hregA[HCH], hregB[HCH];
-  T xregA[KPT], xsclA[KPT], xregB[KPT], xsclB[KPT];
+  // register staging of the next slice: loaded while slice t is multiplied
+  v4 hregA[HCH];
+  T xregA[KPT], xsclA[KPT];
 
   // Branch-free global loads into registers: indices are clamped to valid
-  // addresses and out-of-range values are multiplied by zero, so every load
-  // of a slice is unconditional, all are in flight together and they overlap
-  // the MFMAs of the previous slice.
+  // addresses and out-of-range values are multiplied by zero; the cell-matrix
+  // or projector tile and the src or W rows are selected per slice without a
+  // branch, so all loads of a slice are in flight together and overlap the
+  // MFMAs of the previous slice.
   const T vmask = vok ? T(1) : T(0);
   auto load_tile = [&](int t, v4 (&hreg)[HCH], T (&xreg)[KPT], T (&xscl)[KPT]) {
-    const T *tile_src = (t < nkt) ? Hcell + (int64_t)t * ROWS_MAX * 16
-                                  : NLcell + (int64_t)(t - nkt) * ROWS_MAX * 16;
+    const bool own = t < nkt;
+    const T *tile_src = own ? Hcell + (int64_t)t * ROWS_MAX * 16
+                            : NLcell + (int64_t)(t - nkt) * ROWS_MAX * 16;
 #pragma unroll
     for (int j = 0; j < HCH; ++j) {
       const int chunk = (tid + NT * j < nchunk) ? tid + NT * j : nchunk - 1;
       hreg[j] = reinterpret_cast<const v4 *>(tile_src)[chunk];
     }
-    if (t < nkt) {
+    const T *xb = (own ? a.src : a.W) + vcl;
 #pragma unroll
-      for (int kk = 0; kk < KPT; ++kk) {
-        const int k = t * 16 + gk + kk;
-        const uint32_t g = smap[k] & 0x7fffffffu;
-        xreg[kk] = a.src[(int64_t)g * a.ld_src + vcl];
-        xscl[kk] = vmask * sscale[k];
-      }
-    } else {
-#pragma unroll
-      for (int kk = 0; kk < KPT; ++kk) {
-        const int pj = sproj[(t - nkt) * 16 + gk + kk];
-        xreg[kk] = a.W[(int64_t)(pj < 0 ? 0 : pj) * a.ldW + vcl];
-        xscl[kk] = pj >= 0 ? vmask : T(0);
-      }
+    for (int kk = 0; kk < KPT; ++kk) {
+      const int k = t * 16 + gk + kk;
+      xreg[kk] = xb[soff[k]];
+      xscl[kk] = vmask * sscl[k];
     }
   };
   auto store_tile = [&](int buf, const v4 (&hreg)[HCH], const T (&xreg)[KPT],
@@ -202,70 +301,258 @@ __global__ __launch_bounds__(64 * NW) void hx_kernel(HxLaunch<T> a) {
     }
   };
 
-  __syncthreads();  // gather tables ready
-  if (ntot > 0) {
-    load_tile(0, hregA, xregA, xsclA);
-    store_tile(0, hregA, xregA, xsclA);
-  }
-  if constexpr (PD == 1) {
+  __syncthreads();  // gather tables ready (ntot >= NRB >= 1)
+  load_tile(0, hregA, xregA, xsclA);
+  store_tile(0, hregA, xregA, xsclA);
+  __syncthreads();
+  for (int t = 0; t < ntot; ++t) {
+    const int buf = t & 1;
+    if (t + 1 < ntot) load_tile(t + 1, hregA, xregA, xsclA);
+    compute(buf);
+    if (t + 1 < ntot) store_tile(buf ^ 1, hregA, xregA, xsclA);
     __syncthreads();
-    for (int t = 0; t < ntot; ++t) {
-      const int buf = t & 1;
-      if (t + 1 < ntot) load_tile(t + 1, hregA, xregA, xsclA);
-      compute(buf);
-      if (t + 1 < ntot) store_tile(buf ^ 1, hregA, xregA, xsclA);
-      __syncthreads();
-    }
-  } else {
-    if (ntot > 1) load_tile(1, hregB, xregB, xsclB);
-    __syncthreads();
-    // step t: issue slice t+2 into the free set, multiply slice t, park slice
-    // t+1 (loaded one step earlier) in the other LDS buffer
-    auto step = [&](int t, v4 (&hl)[HCH], T (&xl)[KPT], T (&sl)[KPT], const v4 (&hs)[HCH],
-                    const T (&xs)[KPT], const T (&ss)[KPT]) {
-      if (t + 2 < ntot) load_tile(t + 2, hl, xl, sl);
-      compute(t & 1);
-      if (t + 1 < ntot) store_tile((t + 1) & 1, hs, xs, ss);
-      __syncthreads();
-    };
-    for (int t = 0; t < ntot; t += 2) {
-      step(t, hregA, xregA, xsclA, hregB, xregB, xsclB);
-      if (t + 1 < ntot) step(t + 1, hregB, xregB, xsclB, hregA, xregA, xsclA);
-    }
   }
 
-  // epilogue: scaled scatter with the first-touch combine
-#pragma unroll
-  for (int j = 0; j < RBW; ++j) {
-    const int rb = wr + WR * j;
-    if (rb >= nrb) continue;
-#pragma unroll
-    for (int cb = 0; cb < CBW; ++cb) {
-      const int v = v0 + (wc * CBW + cb) * 16 + (lane & 15);
-      if (v >= a.B) continue;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = rb * 16 + acc_row<T>(lane, r);
-        if (row >= a.ne) continue;
-        const uint32_t w = smap[row];
-        const int64_t g = w & 0x7fffffffu;
-        const T so = a.s_out ? a.s_out[g] : T(1);
-        const int64_t addr = g * a.ld_dst + v;
-        T base;
-        if (w >> 31) {
-          base = T(0);
-          if (a.beta_mode == 0)
-            base = a.beta * a.dst[addr];
-          else if (a.beta_mode == 2)
-            base = a.beta * so * a.dst[addr];
-          if (a.gamma != T(0)) base += a.gamma * a.src[g * a.ld_src + v];
-        } else {
-          base = a.dst[addr];
-        }
-        a.dst[addr] = base + a.alpha * so * acc[j][cb][r];
+  hx_scatter<T, NRB, WR, RBW, CBW>(a, acc, smap, sout, wr, wc, lane, v0);
+}
+
+// ---------------------------------------------------------------------------
+// Fused HX (FP32) fed by direct global->LDS loads through an S-stage ring
+// ---------------------------------------------------------------------------
+// Same work items, colouring, MFMA operand mapping and epilogue as hx_kernel;
+// only the way K slices reach LDS differs.  global_load_lds_dwordx4 copies 16
+// bytes per lane from global memory to LDS address M0 + 16*lane without
+// passing through registers, so S-1 slices can be in flight while slice t is
+// multiplied, at no register cost.  The copies are issued from inline
+// assembly: the compiler does not count them, so it neither waits for them
+// nor can its own waits fall short of them (waits are "at most N outstanding"
+// in issue order, and uncounted older copies only make them stricter).  The
+// kernel waits for its copies itself.  Every wave issues the same number D of
+// copies per slice (duplicates of valid copies fill the count), so "this
+// wave's copies of slice t have landed" is exactly vmcnt((S-2)*D).
+//
+// One barrier per slice: wait for slice t, barrier (every wave's copies of t
+// have landed and every wave is done with slice t-1), refill the stage that
+// held slice t-1 with slice t+S-1, multiply slice t.
+//
+// LDS layouts (a copy fills 1 KB contiguously):
+//   cell-matrix slice: ROWS_MAX rows of 16 floats, four 16-byte units per
+//     row; unit q of row r sits at unit position q ^ f(r) with
+//     f(r) = ((r >> 1) ^ (r >> 3)) & 3, so the A-operand reads (16 rows, one
+//     unit per lane quarter) of any 8 or 16 consecutive lanes hit distinct
+//     banks;
+//   gathered slice: [k][column], RPO = 256/TN rows per 1 KB copy, the copies
+//     spaced 16*RPO bytes apart so that the four k rows of a B-operand read
+//     fall in distinct 64-byte bank windows.
+// Requirements (checked by the launcher): B a multiple of 4, 16-byte aligned
+// rows of src and W (leading dimensions multiples of 4).
+__device__ __forceinline__ uint32_t lds_address(const void *p) {
+  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void *)p;
+}
+
+// 16 bytes per lane from g to LDS address m0 + 16*lane (M0 restored after)
+__device__ __forceinline__ void copy16_to_lds(const void *g, uint32_t m0) {
+  uint32_t saved;
+  asm volatile(
+      "s_mov_b32 %0, m0\n\t"
+      "s_mov_b32 m0, %2\n\t"
+      "s_nop 0\n\t"
+      "global_load_lds_dwordx4 %1, off\n\t"
+      "s_mov_b32 m0, %0"
+      : "=&s"(saved)
+      : "v"(g), "s"(m0)
+      : "memory");
+}
+
+// s_waitcnt vmcnt(N) (expcnt and lgkmcnt not waited for)
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt(0x0F70 | (N & 15) | ((N >> 4) << 14));
+}
+
+template <int NW, int TN, int WC, int NRB, int S>
+__global__ __launch_bounds__(64 * NW)
+__attribute__((amdgpu_waves_per_eu(NW == 8 && S == 2 ? 4 : 1))) void hxp_kernel(HxLaunch<float> a) {
+  using T = float;
+  using v4 = f32x4;
+  constexpr int NT = 64 * NW;
+  constexpr int WR = NW / WC;
+  constexpr int RBW = (NRB + WR - 1) / WR;
+  constexpr int ROWS_MAX = NRB * 16;
+  constexpr int CBW = TN / 16 / WC;
+  static_assert(CBW >= 1 && TN % (16 * WC) == 0, "bad column split");
+  static_assert(TN == 64 || TN == 128, "gathered-slice layout needs TN 64 or 128");
+  static_assert(S >= 2, "at least two stages");
+  constexpr int NLMAX = 128;
+  constexpr int KMAX = ROWS_MAX + NLMAX;
+  constexpr int HB = ROWS_MAX * 16 * 4;   // cell-matrix slice bytes (NRB copies)
+  constexpr int RPO = 256 / TN;           // gathered rows per copy
+  constexpr int LPR = TN / 4;             // lanes per gathered row
+  constexpr int XG = 1024 + 16 * RPO;     // bytes per gathered copy incl. spacing
+  constexpr int NX = 16 / RPO;            // gathered copies per slice
+  constexpr int SB = HB + NX * XG;        // bytes per stage
+  constexpr int NOP = NRB + NX;
+  constexpr int D = (NOP + NW - 1) / NW;  // copies per wave per slice
+  static_assert((S - 2) * D < 64, "vmcnt range");
+
+  __shared__ __align__(16) unsigned char ring[S * SB];
+  __shared__ uint32_t smap[ROWS_MAX];
+  __shared__ T sout[ROWS_MAX];
+  __shared__ __align__(16) int64_t soff[KMAX];
+  __shared__ __align__(16) T sscl[KMAX];
+
+  const int wid = blockIdx.x;
+  const int local = wid >> 3;
+  if (local >= a.per_xcd) return;
+  const int item = (wid & 7) * a.per_xcd + local;
+  if (item >= a.ncells * a.ntiles) return;
+  const int cell = a.cells[item / a.ntiles];
+  const int v0 = (item % a.ntiles) * TN;
+
+  const int tid = threadIdx.x;
+  __builtin_assume(tid < NT);
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wave_s = __builtin_amdgcn_readfirstlane(wave);
+  const int wc = wave % WC;
+  const int wr = wave / WC;
+  constexpr int nkt = NRB;
+
+  int nl_begin = 0, nslot = 0;
+  if (a.nl_off) {
+    nl_begin = a.nl_off[cell];
+    nslot = a.nl_off[cell + 1] - nl_begin;
+  }
+  const int ntot = nkt + (nslot >> 4);
+  const uint32_t *cmap = a.map + (int64_t)cell * a.ne;
+  const T *Hcell = a.Ht + (int64_t)cell * a.ne_pad * a.ne_pad;
+  const T *NLcell = a.nl_C ? a.nl_C + (int64_t)nl_begin * a.ne_pad : Hcell;
+
+  // gather tables (padding rows and empty slots read row 0 with scale 0)
+  for (int i = tid; i < KMAX; i += NT) {
+    int64_t off = 0;
+    T sc = T(0);
+    if (i < ROWS_MAX) {
+      uint32_t w = 0;
+      T so = T(1);
+      if (i < a.ne) {
+        w = cmap[i];
+        const uint32_t r = w & 0x7fffffffu;
+        off = (int64_t)r * a.ld_src;
+        sc = a.s_in ? a.s_in[r] : T(1);
+        if (a.s_out) so = a.s_out[r];
+      }
+      smap[i] = w;
+      sout[i] = so;
+    } else {
+      const int sl = i - ROWS_MAX;
+      const int pj = sl < nslot ? a.slot_proj[nl_begin + sl] : -1;
+      if (pj >= 0) {
+        off = (int64_t)pj * a.ldW;
+        sc = T(1);
       }
     }
+    soff[i] = off;
+    sscl[i] = sc;
   }
+  __syncthreads();
+
+  // per-lane parts of the copy addresses
+  const uint32_t ring0 = lds_address(ring);
+  const int hr = lane >> 2;  // row within a copy's 16 rows (bits 0..3 of the row)
+  const int hoff = hr * 16 + (((lane & 3) ^ (((hr >> 1) ^ (hr >> 3)) & 3)) * 4);
+  const int xsub = lane / LPR;
+  const int xc = v0 + 4 * (lane % LPR);
+  const int xcl = xc < a.B - 4 ? xc : a.B - 4;  // columns >= B are never stored
+
+  auto issue = [&](int t, int stage) {
+    const bool own = t < nkt;
+    const T *tile = own ? Hcell + (int64_t)t * ROWS_MAX * 16
+                        : NLcell + (int64_t)(t - nkt) * ROWS_MAX * 16;
+    const T *xb = (own ? a.src : a.W) + xcl;
+    const uint32_t sbase = ring0 + stage * SB;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const int o = (wave_s + NW * j) % NOP;  // wave-uniform
+      if (o < NRB) {
+        copy16_to_lds(tile + o * 256 + hoff, sbase + o * 1024);
+      } else {
+        const int x = o - NRB;
+        copy16_to_lds(xb + soff[t * 16 + RPO * x + xsub], sbase + HB + x * XG);
+      }
+    }
+  };
+
+  v4 acc[RBW][CBW];
+#pragma unroll
+  for (int j = 0; j < RBW; ++j)
+#pragma unroll
+    for (int cb = 0; cb < CBW; ++cb) acc[j][cb] = v4{0, 0, 0, 0};
+
+  const int arow = lane & 15;
+  const int aoff = arow * 16 + (((lane >> 4) ^ (((arow >> 1) ^ (arow >> 3)) & 3)) * 4);
+  const int kq = 4 * (lane >> 4);  // first of the lane's four K rows
+
+  auto compute = [&](int stage, int t) {
+    const T *Hs = reinterpret_cast<const T *>(ring + stage * SB);
+    const T *Xs = reinterpret_cast<const T *>(ring + stage * SB + HB);
+    const v4 sc = *reinterpret_cast<const v4 *>(&sscl[t * 16 + kq]);
+    v4 b4[CBW];
+#pragma unroll
+    for (int cb = 0; cb < CBW; ++cb) {
+      const int col = (wc * CBW + cb) * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k = kq + r;
+        b4[cb][r] = Xs[(k / RPO) * (XG / 4) + (k % RPO) * TN + col] * sc[r];
+      }
+    }
+    constexpr int G = RBW < 4 ? RBW : 4;
+    const T *hb = Hs + aoff;
+    v4 a4[G], a4n[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int rb = wr + WR * g;
+      a4[g] = *reinterpret_cast<const v4 *>(hb + (rb < NRB ? rb : 0) * 256);
+    }
+#pragma unroll
+    for (int j0 = 0; j0 < RBW; j0 += G) {
+      if (j0 + G < RBW) {
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          const int jn = j0 + G + g;
+          const int rb = wr + WR * jn;
+          if (jn < RBW) a4n[g] = *reinterpret_cast<const v4 *>(hb + (rb < NRB ? rb : 0) * 256);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          const int j = j0 + g;
+          if (j < RBW && wr + WR * j < NRB) {
+#pragma unroll
+            for (int cb = 0; cb < CBW; ++cb) acc[j][cb] = mfma16(a4[g][r], b4[cb][r], acc[j][cb]);
+          }
+        }
+#pragma unroll
+      for (int g = 0; g < G; ++g) a4[g] = a4n[g];
+    }
+  };
+
+#pragma unroll
+  for (int s = 0; s < S - 1; ++s) issue(s < ntot ? s : ntot - 1, s);
+  for (int t = 0; t < ntot; ++t) {
+    wait_vmcnt<(S - 2) * D>();
+    __syncthreads();
+    const int tn = t + S - 1;
+    issue(tn < ntot ? tn : ntot - 1, tn % S);
+    compute(t % S, t);
+  }
+  wait_vmcnt<0>();  // no copy may still target LDS when the workgroup ends
+
+  hx_scatter<T, NRB, WR, RBW, CBW>(a, acc, smap, sout, wr, wc, lane, v0);
 }
 
 // ---------------------------------------------------------------------------
@@ -973,28 +1260,62 @@ static inline unsigned grid_for(int64_t n, int block, unsigned cap = 1u << 20) {
   return (unsigned)g;
 }
 
-template <typename T, int NW, int TN, int WC, int NRB, int PD>
+template <typename T, int NW, int TN, int WC, int NRB>
 static hipError_t hx_go(HxLaunch<T> a, hipStream_t s) {
   a.ntiles = (a.B + TN - 1) / TN;
   const int items = a.ncells * a.ntiles;
   if (items == 0) return hipSuccess;
   a.per_xcd = (items + 7) / 8;
-  hipLaunchKernelGGL((hx_kernel<T, NW, TN, WC, NRB, PD>), dim3(8 * a.per_xcd), dim3(64 * NW), 0,
+  hipLaunchKernelGGL((hx_kernel<T, NW, TN, WC, NRB>), dim3(8 * a.per_xcd), dim3(64 * NW), 0,
                      s, a);
   return hipGetLastError();
 }
 
 // Row-block count of the padded cell matrix is a compile-time constant so the
 // MFMA loop has no guards: (p+1)^3 padded to 16 for p = 1..6.
-template <typename T, int NW, int TN, int WC, int PD = 1>
+template <typename T, int NW, int TN, int WC>
 static hipError_t hx_dispatch(const HxLaunch<T> &a, hipStream_t s) {
   switch (a.ne_pad >> 4) {
-    case 1: return hx_go<T, NW, TN, WC, 1, PD>(a, s);
-    case 2: return hx_go<T, NW, TN, WC, 2, PD>(a, s);
-    case 4: return hx_go<T, NW, TN, WC, 4, PD>(a, s);
-    case 8: return hx_go<T, NW, TN, WC, 8, PD>(a, s);
-    case 14: return hx_go<T, NW, TN, WC, 14, PD>(a, s);
-    case 22: return hx_go<T, NW, TN, WC, 22, PD>(a, s);
+    case 1: return hx_go<T, NW, TN, WC, 1>(a, s);
+    case 2: return hx_go<T, NW, TN, WC, 2>(a, s);
+    case 4: return hx_go<T, NW, TN, WC, 4>(a, s);
+    case 8: return hx_go<T, NW, TN, WC, 8>(a, s);
+    case 14: return hx_go<T, NW, TN, WC, 14>(a, s);
+    case 22: return hx_go<T, NW, TN, WC, 22>(a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int NW, int TN, int WC, int NRB, int S>
+static hipError_t hxp_go(HxLaunch<float> a, hipStream_t s) {
+  a.ntiles = (a.B + TN - 1) / TN;
+  const int items = a.ncells * a.ntiles;
+  if (items == 0) return hipSuccess;
+  a.per_xcd = (items + 7) / 8;
+  hipLaunchKernelGGL((hxp_kernel<NW, TN, WC, NRB, S>), dim3(8 * a.per_xcd), dim3(64 * NW), 0, s,
+                     a);
+  return hipGetLastError();
+}
+
+// The direct-to-LDS copies move 16-byte units: B a multiple of 4 and 16-byte
+// aligned rows of src and W.  Other shapes take the register-staged kernel.
+static bool hxp_fits(const HxLaunch<float> &a) {
+  auto al = [](const void *p) { return ((uintptr_t)p & 15) == 0; };
+  bool ok = a.B >= 4 && a.B % 4 == 0 && a.ld_src % 4 == 0 && al(a.src) && al(a.Ht);
+  if (a.nl_off) ok = ok && a.ldW % 4 == 0 && al(a.W) && al(a.nl_C);
+  return ok;
+}
+
+template <int NW, int TN, int WC, int S>
+static hipError_t hxp_dispatch(const HxLaunch<float> &a, hipStream_t s) {
+  if (!hxp_fits(a)) return hx_dispatch<float, 8, 64, 4>(a, s);
+  switch (a.ne_pad >> 4) {
+    case 1: return hxp_go<NW, TN, WC, 1, S>(a, s);
+    case 2: return hxp_go<NW, TN, WC, 2, S>(a, s);
+    case 4: return hxp_go<NW, TN, WC, 4, S>(a, s);
+    case 8: return hxp_go<NW, TN, WC, 8, S>(a, s);
+    case 14: return hxp_go<NW, TN, WC, 14, S>(a, s);
+    case 22: return hxp_go<NW, TN, WC, 22, S>(a, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -1005,15 +1326,15 @@ static hipError_t hx_dispatch(const HxLaunch<T> &a, hipStream_t s) {
 //         2: 8 x  64 x 4 (two row groups)
 //         3: 16 x 128 x 8 (two row groups, one cell matrix read per 128 columns)
 //         4: 8 x  32 x 2 (four row groups)
+//         7: 8 x 64 x 2, 8: 8 x 64 x 1, 9: 16 x 128 x 4 (fewer column waves: each
+//            A operand feeds more column blocks), 10: 16 x 64 x 4
+//   direct-to-LDS ring (stages): 5: 16 x 128 x 8 (4), 6: 8 x 64 x 4 (2, two
+//            workgroups per CU), 11: 16 x 64 x 4 (4)
 //   FP64  0: 8 x  64 x 4 (two row groups)
 //         1: 4 x  32 x 2
 //         2: 16 x 64 x 4 (four row groups, 16 waves per CU)
-//   prefetch depth two (two slices in flight): FP32 5: 8 x 64 x 4, 6: 16 x 128 x 8;
-//   FP64 3: 16 x 64 x 4, 4: 8 x 64 x 4
-//   fewer column waves (each A operand feeds more column blocks):
-//   FP32 7: 8 x 64 x 2, 8: 8 x 64 x 1, 9: 16 x 128 x 4; FP64 5: 8 x 64 x 2, 6: 16 x 64 x 2
-//   sixteen waves on a 64-column tile: FP32 10: 16 x 64 x 4
-int hx_variants(bool fp64) { return fp64 ? 7 : 11; }
+//         3: 8 x 64 x 2, 4: 16 x 64 x 2 (fewer column waves)
+int hx_variants(bool fp64) { return fp64 ? 5 : 12; }
 
 template <>
 hipError_t launch_hx<float>(const HxLaunch<float> &a, hipStream_t s) {
@@ -1022,12 +1343,13 @@ hipError_t launch_hx<float>(const HxLaunch<float> &a, hipStream_t s) {
     case 2: return hx_dispatch<float, 8, 64, 4>(a, s);
     case 3: return hx_dispatch<float, 16, 128, 8>(a, s);
     case 4: return hx_dispatch<float, 8, 32, 2>(a, s);
-    case 5: return hx_dispatch<float, 8, 64, 4, 2>(a, s);
-    case 6: return hx_dispatch<float, 16, 128, 8, 2>(a, s);
+    case 5: return hxp_dispatch<16, 128, 8, 4>(a, s);
+    case 6: return hxp_dispatch<8, 64, 4, 2>(a, s);
     case 7: return hx_dispatch<float, 8, 64, 2>(a, s);
     case 8: return hx_dispatch<float, 8, 64, 1>(a, s);
     case 9: return hx_dispatch<float, 16, 128, 4>(a, s);
     case 10: return hx_dispatch<float, 16, 64, 4>(a, s);
+    case 11: return hxp_dispatch<16, 64, 4, 4>(a, s);
     default: return hx_dispatch<float, 8, 128, 8>(a, s);
   }
 }
@@ -1037,10 +1359,8 @@ hipError_t launch_hx<double>(const HxLaunch<double> &a, hipStream_t s) {
   switch (a.variant) {
     case 1: return hx_dispatch<double, 4, 32, 2>(a, s);
     case 2: return hx_dispatch<double, 16, 64, 4>(a, s);
-    case 3: return hx_dispatch<double, 16, 64, 4, 2>(a, s);
-    case 4: return hx_dispatch<double, 8, 64, 4, 2>(a, s);
-    case 5: return hx_dispatch<double, 8, 64, 2>(a, s);
-    case 6: return hx_dispatch<double, 16, 64, 2>(a, s);
+    case 3: return hx_dispatch<double, 8, 64, 2>(a, s);
+    case 4: return hx_dispatch<double, 16, 64, 2>(a, s);
     default: return hx_dispatch<double, 8, 64, 4>(a, s);
   }
 }

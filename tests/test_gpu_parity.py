@@ -66,7 +66,7 @@ def test_hx_fp64_matches_oracle(small, B):
     assert _rel(Y, ref.apply(X)) < 1e-13
 
 
-@pytest.mark.parametrize("precision,variant", [(0, v) for v in range(7)] + [(1, v) for v in range(11)])
+@pytest.mark.parametrize("precision,variant", [(0, v) for v in range(5)] + [(1, v) for v in range(12)])
 def test_hx_tile_variants(small, precision, variant):
     """Every HX tile variant gives the oracle's H~X (FP64: 1e-13, FP32: 2e-6)."""
     prob, ref, op = small
@@ -81,6 +81,34 @@ def test_hx_tile_variants(small, precision, variant):
         op.set_hx_variant(precision, 0)
     want = ref.apply(X if precision == 0 else X.astype(np.float32).astype(np.float64))
     assert _rel(Yd.double().cpu().numpy(), want) < (1e-13 if precision == 0 else 2e-6)
+
+
+@pytest.mark.parametrize("B", [72, 256, 70])
+@pytest.mark.parametrize("variant", [5, 6, 11])
+def test_hx_ring_variants_bitwise(small, variant, B):
+    """The direct-to-LDS ring variants (FP32 5, 6, 11) add the same products in
+    the same order as the register-staged kernel (variant 2), so a degree-4 FP32
+    filter (scaled input, first-touch Chebyshev combine, nonlocal slices) gives
+    bitwise-equal blocks: B = 72 leaves a partial column tile, B = 256 two full
+    128-column tiles, and B = 70 (not a multiple of 4) takes the register-staged
+    kernel instead."""
+    from dftfe_amd import lib as L
+    from dftfe_amd.chfsi import chebyshev_filter
+
+    prob, ref, op = small
+    X = torch.tensor(prob.sqrt_mass[:, None] * prob.initial_guess(N=B, seed_offset=B + 3),
+                     device="cuda")
+    out = {}
+    for v in (2, variant):
+        Xd = X.clone()
+        op.set_hx_variant(L.FP32, v)
+        try:
+            chebyshev_filter(op, Xd, 4, 5.0, 41.0, 0.0, precision=L.FP32)
+        finally:
+            op.set_hx_variant(L.FP32, 0)
+        out[v] = Xd
+    assert torch.isfinite(out[2]).all()
+    assert torch.equal(out[2], out[variant])
 
 
 @pytest.mark.parametrize("precision,variant", [(p, v) for p in (0, 1) for v in range(3)])
