@@ -18,90 +18,9 @@
 
 #include "kernels.hpp"
 #include "mfma.hpp"
+#include "scatter.hpp"
 
 namespace chfsi {
-
-// Epilogue of the fused HX: scaled scatter of the wave's accumulators with the
-// first-touch combine (rows rb = wr + WR*j, column blocks wc*CBW + cb).
-// Branch-free: the destination (and, with gamma != 0, source) values of JB
-// row blocks are loaded from clamped, always-valid addresses before any is
-// used, so their latencies overlap; the combine is a select and only the
-// stores are predicated.  sout holds the output scale of each local row.
-template <typename T, int NRB, int WR, int RBW, int CBW>
-__device__ __forceinline__ void hx_scatter(const HxLaunch<T> &a,
-                                           const typename V4<T>::t (&acc)[RBW][CBW],
-                                           const uint32_t *smap, const T *sout, int wr, int wc,
-                                           int lane, int v0) {
-  // row blocks per batch: 16 (FP32) or 8 (FP64) elements per thread
-  constexpr int JBMAX = (sizeof(T) == 4 ? 4 : 2) / CBW > 0 ? (sizeof(T) == 4 ? 4 : 2) / CBW : 1;
-  constexpr int JB = RBW < JBMAX ? RBW : JBMAX;
-  const bool with_src = a.gamma != T(0);
-#pragma unroll
-  for (int j0 = 0; j0 < RBW; j0 += JB) {
-    T dv[JB][CBW][4], sv[JB][CBW][4];
-    uint32_t gg[JB][4];
-#pragma unroll
-    for (int jj = 0; jj < JB; ++jj) {
-      const int rb = wr + WR * (j0 + jj);
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = rb * 16 + acc_row<T>(lane, r);
-        const int rowc = (j0 + jj < RBW && rb < NRB && row < a.ne) ? row : 0;
-        gg[jj][r] = smap[rowc] & 0x7fffffffu;
-      }
-    }
-#pragma unroll
-    for (int jj = 0; jj < JB; ++jj)
-#pragma unroll
-      for (int cb = 0; cb < CBW; ++cb) {
-        const int v = v0 + (wc * CBW + cb) * 16 + (lane & 15);
-        const int vc = v < a.B ? v : a.B - 1;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) dv[jj][cb][r] = a.dst[(int64_t)gg[jj][r] * a.ld_dst + vc];
-      }
-    if (with_src) {
-#pragma unroll
-      for (int jj = 0; jj < JB; ++jj)
-#pragma unroll
-        for (int cb = 0; cb < CBW; ++cb) {
-          const int v = v0 + (wc * CBW + cb) * 16 + (lane & 15);
-          const int vc = v < a.B ? v : a.B - 1;
-#pragma unroll
-          for (int r = 0; r < 4; ++r) sv[jj][cb][r] = a.src[(int64_t)gg[jj][r] * a.ld_src + vc];
-        }
-    } else {
-#pragma unroll
-      for (int jj = 0; jj < JB; ++jj)
-#pragma unroll
-        for (int cb = 0; cb < CBW; ++cb)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) sv[jj][cb][r] = T(0);
-    }
-#pragma unroll
-    for (int jj = 0; jj < JB; ++jj) {
-      const int j = j0 + jj;
-      if (j >= RBW) continue;
-      const int rb = wr + WR * j;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = rb * 16 + acc_row<T>(lane, r);
-        const bool rok = rb < NRB && row < a.ne;
-        const int rowc = rok ? row : 0;
-        const bool first = (smap[rowc] >> 31) != 0;
-        const T so = sout[rowc];
-#pragma unroll
-        for (int cb = 0; cb < CBW; ++cb) {
-          const int v = v0 + (wc * CBW + cb) * 16 + (lane & 15);
-          const T d = dv[jj][cb][r];
-          const T keep = a.beta_mode == 0 ? a.beta * d : (a.beta_mode == 2 ? a.beta * so * d : T(0));
-          const T base = first ? keep + a.gamma * sv[jj][cb][r] : d;
-          if (rok && v < a.B)
-            a.dst[(int64_t)gg[jj][r] * a.ld_dst + v] = base + a.alpha * so * acc[j][cb][r];
-        }
-      }
-    }
-  }
-}
 
 // ---------------------------------------------------------------------------
 // Fused HX over one colour class
@@ -313,7 +232,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((
     __syncthreads();
   }
 
-  hx_scatter<T, NRB, WR, RBW, CBW>(a, acc, smap, sout, wr, wc, lane, v0);
+  hx_scatter<T, NRB, WR, RBW, CBW>(a, a.ne, acc, smap, sout, wr, wc, lane, v0);
 }
 
 // ---------------------------------------------------------------------------
@@ -552,7 +471,7 @@ __attribute__((amdgpu_waves_per_eu(NW == 8 && S == 2 ? 4 : 1))) void hxp_kernel(
   }
   wait_vmcnt<0>();  // no copy may still target LDS when the workgroup ends
 
-  hx_scatter<T, NRB, WR, RBW, CBW>(a, acc, smap, sout, wr, wc, lane, v0);
+  hx_scatter<T, NRB, WR, RBW, CBW, 8>(a, a.ne, acc, smap, sout, wr, wc, lane, v0);
 }
 
 // ---------------------------------------------------------------------------

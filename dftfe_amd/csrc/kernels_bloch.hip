@@ -13,6 +13,7 @@
 #include "bloch.hpp"
 #include "kernels.hpp"
 #include "mfma.hpp"
+#include "scatter.hpp"
 
 namespace chfsi {
 
@@ -51,6 +52,7 @@ __global__ __launch_bounds__(64 * NW) void zhx_kernel(ZHxLaunch<T> a) {
   __shared__ __align__(16) T Xs[2][TN * RS];
   __shared__ uint32_t smap[ROWS_MAX];
   __shared__ T sscale[ROWS_MAX];
+  __shared__ T sout[ROWS_MAX];
   __shared__ int sproj[NLMAX];
 
   const int wid = blockIdx.x;
@@ -86,13 +88,15 @@ __global__ __launch_bounds__(64 * NW) void zhx_kernel(ZHxLaunch<T> a) {
 
   for (int i = tid; i < ROWS_MAX; i += NT) {
     uint32_t w = 0;
-    T sc = T(0);
+    T sc = T(0), so = T(1);
     if (i < a.ne) {
       w = cmap[i];
       sc = a.s_in ? a.s_in[w & 0x7fffffffu] : T(1);
+      if (a.s_out) so = a.s_out[w & 0x7fffffffu];
     }
     smap[i] = w;
     sscale[i] = sc;
+    sout[i] = so;
   }
   for (int s = tid; s < NLMAX; s += NT) sproj[s] = (s < nslot) ? a.slot_proj[nl_begin + s] : -1;
 
@@ -212,37 +216,7 @@ __global__ __launch_bounds__(64 * NW) void zhx_kernel(ZHxLaunch<T> a) {
   }
 
   // epilogue: scaled scatter with the first-touch combine (real scalars)
-#pragma unroll
-  for (int j = 0; j < RBW; ++j) {
-    const int rb = wr + WR * j;
-    if (rb >= nrb) continue;
-#pragma unroll
-    for (int cb = 0; cb < CBW; ++cb) {
-      const int v = v0 + (wc * CBW + cb) * 16 + (lane & 15);
-      if (v >= it.B) continue;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = rb * 16 + acc_row<T>(lane, r);
-        if (row >= a.ne) continue;
-        const uint32_t w = smap[row];
-        const int64_t g = w & 0x7fffffffu;
-        const T so = a.s_out ? a.s_out[g] : T(1);
-        const int64_t addr = g * it.ld_dst + v;
-        T base;
-        if (w >> 31) {
-          base = T(0);
-          if (it.beta_mode == 0)
-            base = it.beta * it.dst[addr];
-          else if (it.beta_mode == 2)
-            base = it.beta * so * it.dst[addr];
-          if (it.gamma != T(0)) base += it.gamma * it.src[g * it.ld_src + v];
-        } else {
-          base = it.dst[addr];
-        }
-        it.dst[addr] = base + it.alpha * so * acc[j][cb][r];
-      }
-    }
-  }
+  hx_scatter<T, NRB, WR, RBW, CBW, 8>(it, a.ne, acc, smap, sout, wr, wc, lane, v0);
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,110 @@
+// Epilogue shared by the fused cell-GEMM kernels (real and complex forms).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mfma.hpp"
+
+namespace chfsi {
+
+// Loads and stores through the global address space: the multivector blocks
+// are device memory, and pointers read from a batch table would otherwise be
+// treated as generic (flat instructions, which also count against the LDS
+// wait counter).
+template <typename T>
+__device__ __forceinline__ T gload(const T *p, int64_t i) {
+  return ((const __attribute__((address_space(1))) T *)p)[i];
+}
+template <typename T>
+__device__ __forceinline__ void gstore(T *p, int64_t i, T v) {
+  ((__attribute__((address_space(1))) T *)p)[i] = v;
+}
+
+// Epilogue of the fused HX: scaled scatter of the wave's accumulators with the
+// first-touch combine (rows rb = wr + WR*j, column blocks wc*CBW + cb).
+// P is the launch (or batch-entry) description providing dst, src, ld_dst,
+// ld_src, alpha, beta, gamma, beta_mode and B (real columns); ne is the
+// number of local rows, smap the DOF and first-touch bit of each, sout its
+// output scale.  Branch-free: the destination (and, with gamma != 0, source)
+// values of a batch of row blocks (ELEMS elements per thread) are loaded from
+// clamped, always-valid addresses before any is used, so their latencies
+// overlap; the combine is a select and only the stores are predicated.
+template <typename T, int NRB, int WR, int RBW, int CBW, int ELEMS = (sizeof(T) == 4 ? 16 : 8),
+          typename P>
+__device__ __forceinline__ void hx_scatter(const P &a, int ne,
+                                           const typename V4<T>::t (&acc)[RBW][CBW],
+                                           const uint32_t *smap, const T *sout, int wr, int wc,
+                                           int lane, int v0) {
+  // row blocks per batch: ELEMS elements per thread
+  constexpr int JBMAX = ELEMS / (4 * CBW) > 0 ? ELEMS / (4 * CBW) : 1;
+  constexpr int JB = RBW < JBMAX ? RBW : JBMAX;
+  const bool with_src = a.gamma != T(0);
+#pragma unroll
+  for (int j0 = 0; j0 < RBW; j0 += JB) {
+    T dv[JB][CBW][4], sv[JB][CBW][4];
+    uint32_t gg[JB][4];
+#pragma unroll
+    for (int jj = 0; jj < JB; ++jj) {
+      const int rb = wr + WR * (j0 + jj);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = rb * 16 + acc_row<T>(lane, r);
+        const int rowc = (j0 + jj < RBW && rb < NRB && row < ne) ? row : 0;
+        gg[jj][r] = smap[rowc] & 0x7fffffffu;
+      }
+    }
+#pragma unroll
+    for (int jj = 0; jj < JB; ++jj)
+#pragma unroll
+      for (int cb = 0; cb < CBW; ++cb) {
+        const int v = v0 + (wc * CBW + cb) * 16 + (lane & 15);
+        const int vc = v < a.B ? v : a.B - 1;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) dv[jj][cb][r] = gload(a.dst, (int64_t)gg[jj][r] * a.ld_dst + vc);
+      }
+    if (with_src) {
+#pragma unroll
+      for (int jj = 0; jj < JB; ++jj)
+#pragma unroll
+        for (int cb = 0; cb < CBW; ++cb) {
+          const int v = v0 + (wc * CBW + cb) * 16 + (lane & 15);
+          const int vc = v < a.B ? v : a.B - 1;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) sv[jj][cb][r] = gload(a.src, (int64_t)gg[jj][r] * a.ld_src + vc);
+        }
+    } else {
+#pragma unroll
+      for (int jj = 0; jj < JB; ++jj)
+#pragma unroll
+        for (int cb = 0; cb < CBW; ++cb)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) sv[jj][cb][r] = T(0);
+    }
+#pragma unroll
+    for (int jj = 0; jj < JB; ++jj) {
+      const int j = j0 + jj;
+      if (j >= RBW) continue;
+      const int rb = wr + WR * j;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = rb * 16 + acc_row<T>(lane, r);
+        const bool rok = rb < NRB && row < ne;
+        const int rowc = rok ? row : 0;
+        const bool first = (smap[rowc] >> 31) != 0;
+        const T so = sout[rowc];
+#pragma unroll
+        for (int cb = 0; cb < CBW; ++cb) {
+          const int v = v0 + (wc * CBW + cb) * 16 + (lane & 15);
+          const T d = dv[jj][cb][r];
+          const T keep = a.beta_mode == 0 ? a.beta * d : (a.beta_mode == 2 ? a.beta * so * d : T(0));
+          const T base = first ? keep + a.gamma * sv[jj][cb][r] : d;
+          if (rok && v < a.B)
+            gstore(a.dst, (int64_t)gg[jj][r] * a.ld_dst + v, base + a.alpha * so * acc[j][cb][r]);
+        }
+      }
+    }
+  }
+}
+
+}  // namespace chfsi
