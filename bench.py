@@ -44,6 +44,24 @@ def hx_bytes(prob, B, s):
     return prob.M * B * s + prob.C * prob.ne ** 2 * s + 2 * prob.M * B * s + prob.C * prob.ne * 4
 
 
+def measured_hx_traffic(workload, B, variant, precision="fp32"):
+    """HBM bytes of one fused block HX (8 colour launches of the given tile
+    variant + projection + reduction) from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE table committed under profiles/ for this workload and block
+    width; None when no measurement covers the configuration."""
+    path = os.path.join(ROOT, "profiles", f"hx_traffic_{workload}.json")
+    if B != 256 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)[precision]
+    tile = t["tile_variants"].get(str(variant))
+    if tile is None:
+        return None
+    n = t["colour_launches_per_hx"]
+    return int(n * (tile["fetch_bytes"] + tile["write_bytes"])
+               + sum(t[k]["fetch_bytes"] + t[k]["write_bytes"] for k in ("projection", "reduction")))
+
+
 def pass_flops(prob, N, B, m, lanczos_steps):
     """Algorithmic flops of one pass: filter, Lanczos, CholGS (syrk + trsm), RR
     (N/B HX + lower X^T HX + rotation + eig), residual (N/B HX)."""
@@ -75,7 +93,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
-              "sharding is described in DESIGN.md and not built.", file=sys.stderr)
+              "sharding of the path is not built.", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -217,13 +235,15 @@ def main():
                                               for a, b in zip(lam_mx, lam_64)]
         out["max_residual_fp64"] = [float(r.max()) for r in res_64]
     out["roofline"] = {
-        "kernel": "fused HX (8 colour launches of hx_kernel<float> + projection)",
+        "kernel": f"fused FP32 HX (8 colour launches of tile variant {tune['fp32']} + projection)",
         "bound": "mfma",
         "achieved": round(F / hx32_ms / 1e9, 2),
         "peak": PEAK_TFLOPS["fp32"],
         "unit": "TFLOP/s",
         "frac": round(F / hx32_ms / 1e9 / PEAK_TFLOPS["fp32"], 4),
-        "traffic": None,
+        "traffic": measured_hx_traffic(args.workload, B, tune["fp32"]),
+        "traffic_source": f"profiles/hx_traffic_{args.workload}.json, FP32 tile variant "
+                          f"{tune['fp32']} (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
         "algorithmic_bytes": hx_bytes(prob, B, 4),
         "fp64": {"achieved": round(F / hx64_ms / 1e9, 2), "peak": PEAK_TFLOPS["fp64"],
                  "frac": round(F / hx64_ms / 1e9 / PEAK_TFLOPS["fp64"], 4)},
