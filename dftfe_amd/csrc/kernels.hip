@@ -232,7 +232,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((
     __syncthreads();
   }
 
-  hx_scatter<T, NRB, WR, RBW, CBW>(a, a.ne, acc, smap, sout, wr, wc, lane, v0);
+  hx_scatter<T, NRB, WR, RBW, CBW, 8>(a, a.ne, acc, smap, sout, wr, wc, lane, v0);
 }
 
 // ---------------------------------------------------------------------------

@@ -19,6 +19,11 @@ struct V4<double> {
   using t = f64x4;
 };
 
+// a * b + c with one rounding, whatever the compiler's contraction choices:
+// kernels that must agree bit for bit use it for every add that is not an MFMA
+__device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 __device__ __forceinline__ f32x4 mfma16(float a, float b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }

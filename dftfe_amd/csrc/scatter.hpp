@@ -26,10 +26,11 @@ __device__ __forceinline__ void gstore(T *p, int64_t i, T v) {
 // P is the launch (or batch-entry) description providing dst, src, ld_dst,
 // ld_src, alpha, beta, gamma, beta_mode and B (real columns); ne is the
 // number of local rows, smap the DOF and first-touch bit of each, sout its
-// output scale.  Branch-free: the destination (and, with gamma != 0, source)
-// values of a batch of row blocks (ELEMS elements per thread) are loaded from
-// clamped, always-valid addresses before any is used, so their latencies
-// overlap; the combine is a select and only the stores are predicated.
+// output scale.  The destination values of a batch of row blocks (ELEMS
+// elements per thread) are loaded from clamped, always-valid addresses before
+// any is used, so their latencies overlap; the gamma term reads the source
+// rows of first-touch rows only.  Loads are never skipped by a branch; the
+// combine is a select and the stores are predicated.
 template <typename T, int NRB, int WR, int RBW, int CBW, int ELEMS = (sizeof(T) == 4 ? 16 : 8),
           typename P>
 __device__ __forceinline__ void hx_scatter(const P &a, int ne,
@@ -39,11 +40,12 @@ __device__ __forceinline__ void hx_scatter(const P &a, int ne,
   // row blocks per batch: ELEMS elements per thread
   constexpr int JBMAX = ELEMS / (4 * CBW) > 0 ? ELEMS / (4 * CBW) : 1;
   constexpr int JB = RBW < JBMAX ? RBW : JBMAX;
-  const bool with_src = a.gamma != T(0);
+  const bool any_src = a.gamma != T(0);
 #pragma unroll
   for (int j0 = 0; j0 < RBW; j0 += JB) {
     T dv[JB][CBW][4], sv[JB][CBW][4];
     uint32_t gg[JB][4];
+    bool nsrc[JB][4];
 #pragma unroll
     for (int jj = 0; jj < JB; ++jj) {
       const int rb = wr + WR * (j0 + jj);
@@ -51,7 +53,9 @@ __device__ __forceinline__ void hx_scatter(const P &a, int ne,
       for (int r = 0; r < 4; ++r) {
         const int row = rb * 16 + acc_row<T>(lane, r);
         const int rowc = (j0 + jj < RBW && rb < NRB && row < ne) ? row : 0;
-        gg[jj][r] = smap[rowc] & 0x7fffffffu;
+        const uint32_t w = smap[rowc];
+        gg[jj][r] = w & 0x7fffffffu;
+        nsrc[jj][r] = any_src && (w >> 31) != 0;
       }
     }
 #pragma unroll
@@ -63,7 +67,7 @@ __device__ __forceinline__ void hx_scatter(const P &a, int ne,
 #pragma unroll
         for (int r = 0; r < 4; ++r) dv[jj][cb][r] = gload(a.dst, (int64_t)gg[jj][r] * a.ld_dst + vc);
       }
-    if (with_src) {
+    if (any_src) {
 #pragma unroll
       for (int jj = 0; jj < JB; ++jj)
 #pragma unroll
@@ -71,7 +75,12 @@ __device__ __forceinline__ void hx_scatter(const P &a, int ne,
           const int v = v0 + (wc * CBW + cb) * 16 + (lane & 15);
           const int vc = v < a.B ? v : a.B - 1;
 #pragma unroll
-          for (int r = 0; r < 4; ++r) sv[jj][cb][r] = gload(a.src, (int64_t)gg[jj][r] * a.ld_src + vc);
+          for (int r = 0; r < 4; ++r) {
+            // rows that need no source read element 0 of the block instead:
+            // one cache line for all of them, and no branch between the loads
+            const T x = gload(a.src, nsrc[jj][r] ? (int64_t)gg[jj][r] * a.ld_src + vc : 0);
+            sv[jj][cb][r] = nsrc[jj][r] ? x : T(0);
+          }
         }
     } else {
 #pragma unroll
@@ -97,10 +106,10 @@ __device__ __forceinline__ void hx_scatter(const P &a, int ne,
         for (int cb = 0; cb < CBW; ++cb) {
           const int v = v0 + (wc * CBW + cb) * 16 + (lane & 15);
           const T d = dv[jj][cb][r];
-          const T keep = a.beta_mode == 0 ? a.beta * d : (a.beta_mode == 2 ? a.beta * so * d : T(0));
-          const T base = first ? keep + a.gamma * sv[jj][cb][r] : d;
+          const T keep = a.beta_mode == 0 ? a.beta * d : (a.beta_mode == 2 ? (a.beta * so) * d : T(0));
+          const T base = first ? fmadd(a.gamma, sv[jj][cb][r], keep) : d;
           if (rok && v < a.B)
-            gstore(a.dst, (int64_t)gg[jj][r] * a.ld_dst + v, base + a.alpha * so * acc[j][cb][r]);
+            gstore(a.dst, (int64_t)gg[jj][r] * a.ld_dst + v, fmadd(a.alpha * so, acc[j][cb][r], base));
         }
       }
     }

@@ -87,11 +87,11 @@ def test_hx_tile_variants(small, precision, variant):
 @pytest.mark.parametrize("variant", [5, 6, 11])
 def test_hx_ring_variants_bitwise(small, variant, B):
     """The direct-to-LDS ring variants (FP32 5, 6, 11) add the same products in
-    the same order as the register-staged kernel (variant 2), so a degree-4 FP32
-    filter (scaled input, first-touch Chebyshev combine, nonlocal slices) gives
-    bitwise-equal blocks: B = 72 leaves a partial column tile, B = 256 full
-    tiles, and B = 70 (not a multiple of 4) takes the register-staged kernel
-    instead."""
+    the same order as the register-staged kernel (variant 2), and every other add
+    is an explicit FMA, so a degree-4 FP32 filter (scaled input, first-touch
+    Chebyshev combine, nonlocal slices) gives bitwise-equal blocks: B = 72
+    leaves a partial column tile, B = 256 full tiles, and B = 70 (not a multiple
+    of 4) takes the register-staged kernel instead."""
     from dftfe_amd import lib as L
     from dftfe_amd.chfsi import chebyshev_filter
 
