@@ -398,6 +398,24 @@ def rayleigh_ritz(op: KohnShamOperatorDevice, X: torch.Tensor, block: int) -> np
     return ev
 
 
+def rayleigh_ritz_spectrum_split(op: KohnShamOperatorDevice, X: torch.Tensor, n_core: int,
+                                 block: int, Xfrac: Optional[torch.Tensor] = None,
+                                 mixed_xthx: bool = False):
+    """rayleighRitzGEPSpectrumSplitDirect (rayleighRitzDevice.cc:822-1278):
+    orthonormalise X in place (Cholesky), then keep the top N - n_core Ritz
+    pairs of X^T H~ X.  Returns (eigenvalues, Xfrac) with Xfrac = X Q[:, n_core:]
+    (M x (N - n_core), written into the given tensor when one is passed)."""
+    M, N = X.shape
+    nfr = N - n_core
+    if Xfrac is None:
+        Xfrac = torch.empty(M, nfr, dtype=X.dtype, device=X.device)
+    ev = np.zeros(nfr)
+    L.check(op.lib.chfsi_rayleigh_ritz_spectrum_split(
+        op.handle, _ptr(X), M, N, n_core, _ptr(Xfrac), Xfrac.stride(0), block,
+        ev.ctypes.data_as(L.c_double_p), int(mixed_xthx), _stream()))
+    return ev, Xfrac
+
+
 def residual_norms(op: KohnShamOperatorDevice, X: torch.Tensor, lam: np.ndarray,
                    block: int) -> np.ndarray:
     N = X.shape[1]
@@ -433,7 +451,7 @@ class ChebyshevSolverDevice:
         p.reuse_lanczos_upper = int(reuse_lanczos_upper)
         p.fp64_tail_degrees = fp64_tail_degrees
         for f in ("mixed_prec_overlap", "mixed_prec_cgs_rotation", "mixed_prec_xthx",
-                  "mixed_prec_rr_rotation"):
+                  "mixed_prec_rr_rotation", "mixed_prec_xthx_spectrum_split"):
             setattr(p, f, int(mixed_dense))
         self.params = p
         h = ct.c_void_p()
@@ -486,6 +504,34 @@ class ChebyshevSolverDevice:
         self.last_timings_ms = dict(zip(["lanczos", "filter", "cholgs", "rayleigh_ritz",
                                          "residual", "total"], t.tolist()))
         return ev, res, bup.value
+
+    def solve_spectrum_split(self, op: KohnShamOperatorDevice, X: torch.Tensor, n_core: int,
+                             Xfrac: Optional[torch.Tensor] = None, *,
+                             is_first_filtering_call: bool = False,
+                             compute_residual: bool = True,
+                             use_mixed_prec_overall: bool = False, is_first_scf: bool = False):
+        """One pass with the spectrum split (the solve branch taken when the
+        caller asks for fewer eigenvalues than wavefunctions): X is filtered and
+        orthonormalised in place, the top N - n_core Ritz vectors go to Xfrac.
+        Returns (eigenvalues, residual norms, bUp, Xfrac), N - n_core values each.
+        The caller's bookkeeping for the next pass is bLow = eigenvalues[-1]
+        with a0 kept (kohnShamEigenSolve.cc:483-498)."""
+        M, N = X.shape
+        nfr = N - n_core
+        if Xfrac is None:
+            Xfrac = torch.empty(M, nfr, dtype=X.dtype, device=X.device)
+        ev = np.zeros(nfr)
+        res = np.zeros(nfr)
+        bup = ct.c_double()
+        t = np.zeros(6)
+        L.check(self.lib.chfsi_solve_spectrum_split(
+            self.handle, op.handle, _ptr(X), M, N, n_core, _ptr(Xfrac), Xfrac.stride(0),
+            ev.ctypes.data_as(L.c_double_p), res.ctypes.data_as(L.c_double_p),
+            int(is_first_filtering_call), int(compute_residual), int(use_mixed_prec_overall),
+            int(is_first_scf), ct.byref(bup), t.ctypes.data_as(L.c_double_p), _stream()))
+        self.last_timings_ms = dict(zip(["lanczos", "filter", "cholgs", "rayleigh_ritz",
+                                         "residual", "total"], t.tolist()))
+        return ev, res, bup.value, Xfrac
 
     def solve_complex(self, op: KohnShamOperatorDevice, X: torch.Tensor, *,
                       is_first_filtering_call: bool = False, compute_residual: bool = True,

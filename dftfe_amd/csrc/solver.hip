@@ -499,8 +499,9 @@ chfsi_status rr_impl(chfsi_solver *s, chfsi_operator *op, double *X, int64_t M, 
   return rotate_dense(s, X, mixed_rot ? Xf : nullptr, M, N, Hp, B, st);
 }
 
-chfsi_status residual_impl(chfsi_solver *s, chfsi_operator *op, const double *X, int64_t M, int N,
-                           int B, const double *eig_host, double *res_host, hipStream_t st) {
+chfsi_status residual_impl(chfsi_solver *s, chfsi_operator *op, const double *X, int64_t ldX,
+                           int64_t M, int N, int B, const double *eig_host, double *res_host,
+                           hipStream_t st) {
   CHFSI_TRY(s->HXb.ensure(sizeof(double) * M * B));
   CHFSI_TRY(s->lam.ensure(sizeof(double) * N));
   const int nch = 512;
@@ -510,9 +511,9 @@ chfsi_status residual_impl(chfsi_solver *s, chfsi_operator *op, const double *X,
   std::vector<double> h(size_t(nch) * B);
   for (int j = 0; j < N; j += B) {
     const int bw = std::min(B, N - j);
-    CHFSI_TRY(hx_fused<double>(op, X + j, N, HX, bw, bw, 1.0, 0.0, 0.0, 1, true, st));
+    CHFSI_TRY(hx_fused<double>(op, X + j, ldX, HX, bw, bw, 1.0, 0.0, 0.0, 1, true, st));
     if (op->nc) CHFSI_HIP(launch_zero_rows(op->d_c_rows, op->nc, HX, bw, bw, st));
-    CHFSI_HIP(launch_residual_partial(HX, bw, X + j, N, lam + j, M, bw, part, nch, st));
+    CHFSI_HIP(launch_residual_partial(HX, bw, X + j, ldX, lam + j, M, bw, part, nch, st));
     CHFSI_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * nch * bw, hipMemcpyDeviceToHost, st));
     CHFSI_HIP(hipStreamSynchronize(st));
     for (int v = 0; v < bw; ++v) {
@@ -521,6 +522,63 @@ chfsi_status residual_impl(chfsi_solver *s, chfsi_operator *op, const double *X,
       res_host[j + v] = std::sqrt(acc);
     }
   }
+  return CHFSI_OK;
+}
+
+// Spectrum-split Rayleigh-Ritz on an orthonormal X (rayleighRitzGEPSpectrumSplitDirect,
+// rayleighRitzDevice.cc:822-1278, after its Cholesky step): Hp = X^T H~ X (lower
+// triangle by column blocks), full eigendecomposition, and only the top
+// Nfr = N - Noc pairs kept: Xfrac = X Q[:, Noc:N] (M x Nfr, leading dimension
+// ldf), eigenvalues ascending into eig_host[0..Nfr).  X itself is not
+// rotated.  Mixed (useMixedPrecXTHXSpectrumSplit, XtHXMixedPrec,
+// kohnShamDFTOperatorDevice.cc:4552-5081): column blocks that lie entirely
+// within the first Noc states are formed from FP32 copies of X and of the HX
+// block; the blocks reaching the top Nfr states stay FP64.
+chfsi_status rr_split_impl(chfsi_solver *s, chfsi_operator *op, const double *X, int64_t M, int N,
+                           int Noc, int B, double *Xfrac, int64_t ldf, double *eig_host,
+                           bool mixed_xthx, hipStream_t st) {
+  const int Nfr = N - Noc;
+  CHFSI_TRY(ensure_blas(s, st));
+  CHFSI_TRY(s->Hp.ensure(sizeof(double) * N * N));
+  CHFSI_TRY(s->HXb.ensure(sizeof(double) * M * B));
+  CHFSI_TRY(s->evals.ensure(sizeof(double) * 2 * N));
+  CHFSI_TRY(s->info.ensure(sizeof(rocblas_int)));
+  double *Hp = sptr<double>(s->Hp), *HX = sptr<double>(s->HXb);
+  float *Xf = nullptr, *HXf = nullptr;
+  if (mixed_xthx && Noc >= B) {
+    CHFSI_TRY(cast_x(s, X, M, N, &Xf, st));
+    CHFSI_TRY(s->HXf.ensure(sizeof(float) * M * B));
+    HXf = sptr<float>(s->HXf);
+  }
+  for (int j = 0; j < N; j += B) {
+    const int bw = std::min(B, N - j);
+    CHFSI_TRY(hx_fused<double>(op, X + j, N, HX, bw, bw, 1.0, 0.0, 0.0, 1, true, st));
+    if (op->nc) CHFSI_HIP(launch_zero_rows(op->d_c_rows, op->nc, HX, bw, bw, st));
+    double *Hj = Hp + j + (int64_t)j * N;
+    if (Xf && j + bw <= Noc) {
+      CHFSI_HIP(launch_copy_block<double, float>(HX, bw, HXf, bw, M, bw, st));
+      CHFSI_TRY(sgemm_nt_splitk(s, Xf + j, N, HXf, bw, N - j, bw, M, Hj, N, st));
+    } else {
+      CHFSI_TRY(gemm_nt_splitk(s, X + j, N, HX, bw, N - j, bw, M, Hj, N, st));
+    }
+  }
+  double *D = sptr<double>(s->evals), *E = D + N;
+  rocblas_int *dinfo = sptr<rocblas_int>(s->info);
+  CHFSI_RB(rocsolver_dsyevd(s->blas, rocblas_evect_original, rocblas_fill_lower, N, Hp, N, D, E,
+                            dinfo));
+  rocblas_int hinfo = 0;
+  CHFSI_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(rocblas_int), hipMemcpyDeviceToHost, st));
+  CHFSI_HIP(hipMemcpyAsync(eig_host, D + Noc, sizeof(double) * Nfr, hipMemcpyDeviceToHost, st));
+  CHFSI_HIP(hipStreamSynchronize(st));
+  if (hinfo != 0) {
+    set_error("projected eigensolve failed (syevd info %d)", (int)hinfo);
+    return CHFSI_ERR_EIG;
+  }
+  // Xfrac^T (Nfr x M, ld ldf) = Q[:, Noc:N]^T (Nfr x N) * Xc (N x M, ld N)
+  const double one = 1.0, zero = 0.0;
+  CHFSI_RB(rocblas_dgemm(s->blas, rocblas_operation_transpose, rocblas_operation_none, Nfr,
+                         (rocblas_int)M, N, &one, Hp + (int64_t)Noc * N, N, X, N, &zero, Xfrac,
+                         (rocblas_int)ldf));
   return CHFSI_OK;
 }
 
@@ -541,6 +599,7 @@ extern "C" void chfsi_default_params(chfsi_params *p) {
   p->mixed_prec_cgs_rotation = 0;
   p->mixed_prec_xthx = 0;
   p->mixed_prec_rr_rotation = 0;
+  p->mixed_prec_xthx_spectrum_split = 0;
 }
 
 extern "C" chfsi_status chfsi_solver_set_params(chfsi_solver *s, const chfsi_params *params) {
@@ -755,31 +814,33 @@ extern "C" chfsi_status chfsi_residual_norms(chfsi_operator *op, const double *X
                                              double *res_host, void *stream) {
   CHFSI_ARG(op && X && eig_host && res_host && M == op->M && N > 0 && block > 0, "bad argument");
   chfsi_solver tmp;
-  return residual_impl(&tmp, op, X, M, N, std::min<int>(block, N), eig_host, res_host,
+  return residual_impl(&tmp, op, X, N, M, N, std::min<int>(block, N), eig_host, res_host,
                        as_stream(stream));
 }
 
-extern "C" chfsi_status chfsi_solve(chfsi_solver *s, chfsi_operator *op, double *X, int64_t M,
-                                    int32_t N, double *eig_host, double *res_host,
-                                    int32_t is_first_filtering_call, int32_t compute_residual,
-                                    int32_t use_mixed_prec_overall, int32_t is_first_scf,
-                                    double *bUp_out, double *timings_ms, void *stream) {
-  CHFSI_ARG(s && op && X && eig_host, "null argument");
-  CHFSI_ARG(M == op->M && N > 0, "X must be n_dofs x N");
-  CHFSI_ARG(!compute_residual || res_host, "res_host required when computing residuals");
+extern "C" chfsi_status chfsi_rayleigh_ritz_spectrum_split(chfsi_operator *op, double *X, int64_t M,
+                                                           int32_t N, int32_t n_core,
+                                                           double *Xfrac, int64_t ld_frac,
+                                                           int32_t block, double *eig_host,
+                                                           int32_t mixed_xthx, void *stream) {
+  CHFSI_ARG(op && X && Xfrac && eig_host && M == op->M && N > 0 && block > 0, "bad argument");
+  CHFSI_ARG(n_core >= 0 && n_core < N && ld_frac >= N - n_core, "need 0 <= n_core < N, ld_frac >= N - n_core");
   hipStream_t st = as_stream(stream);
-  const chfsi_params &P = s->params;
-  hipEvent_t ev[6];
-  for (auto &e : ev) CHFSI_HIP(hipEventCreate(&e));
-  struct EvGuard {
-    hipEvent_t *e;
-    ~EvGuard() {
-      for (int i = 0; i < 6; ++i) hipEventDestroy(e[i]);
-    }
-  } guard{ev};
-  CHFSI_HIP(hipEventRecord(ev[0], st));
+  chfsi_solver tmp;
+  const int B = std::min<int>(block, N);
+  CHFSI_TRY(cholgs_impl(&tmp, X, M, N, B, false, false, st));
+  return rr_split_impl(&tmp, op, X, M, N, n_core, B, Xfrac, ld_frac, eig_host, mixed_xthx != 0, st);
+}
 
-  // Lanczos bounds (…SolverDevice.cc:241-297)
+namespace {
+// Start of a pass shared by both solve forms (…SolverDevice.cc:241-450): Lanczos
+// bounds when needed, degree, M^{1/2} scaling into the Loewdin basis and the
+// blocked filter.  Records ev[1] after Lanczos and ev[2] after the filter.
+chfsi_status lanczos_scale_filter(chfsi_solver *s, chfsi_operator *op, double *X, int64_t M, int N,
+                                  int32_t is_first_filtering_call, int32_t use_mixed_prec_overall,
+                                  int32_t is_first_scf, int *B_out, hipEvent_t *ev,
+                                  hipStream_t st) {
+  const chfsi_params &P = s->params;
   if (is_first_filtering_call || !P.reuse_lanczos_upper) {
     double lo, up;
     const double *v0 = s->have_v0 ? sptr<double>(s->v0) : nullptr;
@@ -808,7 +869,6 @@ extern "C" chfsi_status chfsi_solve(chfsi_solver *s, chfsi_operator *op, double 
   unsigned m = P.cheby_degree > 0 ? (unsigned)P.cheby_degree : cheby_order_lookup(s->bUp);
   if (is_first_scf) m = (unsigned)(m * P.first_scf_degree_factor);
 
-  // M^{1/2} scaling into the Loewdin basis, then blocked filtering
   CHFSI_HIP(launch_scale_rows<double>(X, N, M, N, 1.0, op->d_sqrt_mass, st));
   const int B = std::min<int>(P.cheby_block_size > 0 ? P.cheby_block_size : N, N);
   const int prec = use_mixed_prec_overall ? P.filter_precision : CHFSI_FP64;
@@ -818,23 +878,24 @@ extern "C" chfsi_status chfsi_solve(chfsi_solver *s, chfsi_operator *op, double 
                            use_mixed_prec_overall ? P.fp64_tail_degrees : 0, st));
   }
   CHFSI_HIP(hipEventRecord(ev[2], st));
+  *B_out = B;
+  return CHFSI_OK;
+}
 
-  // CholGS + Rayleigh-Ritz (SHEP route, …SolverDevice.cc:609-643)
-  const bool mx = use_mixed_prec_overall != 0;
-  CHFSI_TRY(cholgs_impl(s, X, M, N, B, mx && P.mixed_prec_overlap, mx && P.mixed_prec_cgs_rotation,
-                        st));
-  CHFSI_HIP(hipEventRecord(ev[3], st));
-  CHFSI_TRY(rr_impl(s, op, X, M, N, B, eig_host, mx && P.mixed_prec_xthx,
-                    mx && P.mixed_prec_rr_rotation, st));
-  CHFSI_HIP(hipEventRecord(ev[4], st));
-
-  if (compute_residual) CHFSI_TRY(residual_impl(s, op, X, M, N, B, eig_host, res_host, st));
-
-  // back to the FE basis
-  CHFSI_HIP(launch_scale_rows<double>(X, N, M, N, 1.0, op->d_inv_sqrt_mass, st));
-  CHFSI_HIP(hipEventRecord(ev[5], st));
-  CHFSI_HIP(hipEventSynchronize(ev[5]));
-  if (timings_ms) {
+struct PassEvents {
+  hipEvent_t ev[6] = {};
+  ~PassEvents() {
+    for (hipEvent_t e : ev)
+      if (e) hipEventDestroy(e);
+  }
+  chfsi_status create() {
+    for (auto &e : ev) CHFSI_HIP(hipEventCreate(&e));
+    return CHFSI_OK;
+  }
+  // per-phase times {lanczos, filter, cholgs, rayleigh_ritz, residual, total}
+  chfsi_status report(double *timings_ms) {
+    CHFSI_HIP(hipEventSynchronize(ev[5]));
+    if (!timings_ms) return CHFSI_OK;
     float t;
     for (int k = 0; k < 4; ++k) {
       CHFSI_HIP(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
@@ -844,7 +905,88 @@ extern "C" chfsi_status chfsi_solve(chfsi_solver *s, chfsi_operator *op, double 
     timings_ms[4] = t;
     CHFSI_HIP(hipEventElapsedTime(&t, ev[0], ev[5]));
     timings_ms[5] = t;
+    return CHFSI_OK;
   }
+};
+}  // namespace
+
+extern "C" chfsi_status chfsi_solve(chfsi_solver *s, chfsi_operator *op, double *X, int64_t M,
+                                    int32_t N, double *eig_host, double *res_host,
+                                    int32_t is_first_filtering_call, int32_t compute_residual,
+                                    int32_t use_mixed_prec_overall, int32_t is_first_scf,
+                                    double *bUp_out, double *timings_ms, void *stream) {
+  CHFSI_ARG(s && op && X && eig_host, "null argument");
+  CHFSI_ARG(M == op->M && N > 0, "X must be n_dofs x N");
+  CHFSI_ARG(!compute_residual || res_host, "res_host required when computing residuals");
+  hipStream_t st = as_stream(stream);
+  const chfsi_params &P = s->params;
+  PassEvents pe;
+  CHFSI_TRY(pe.create());
+  CHFSI_HIP(hipEventRecord(pe.ev[0], st));
+  int B = 0;
+  CHFSI_TRY(lanczos_scale_filter(s, op, X, M, N, is_first_filtering_call, use_mixed_prec_overall,
+                                 is_first_scf, &B, pe.ev, st));
+
+  // CholGS + Rayleigh-Ritz (SHEP route, …SolverDevice.cc:609-643)
+  const bool mx = use_mixed_prec_overall != 0;
+  CHFSI_TRY(cholgs_impl(s, X, M, N, B, mx && P.mixed_prec_overlap, mx && P.mixed_prec_cgs_rotation,
+                        st));
+  CHFSI_HIP(hipEventRecord(pe.ev[3], st));
+  CHFSI_TRY(rr_impl(s, op, X, M, N, B, eig_host, mx && P.mixed_prec_xthx,
+                    mx && P.mixed_prec_rr_rotation, st));
+  CHFSI_HIP(hipEventRecord(pe.ev[4], st));
+
+  if (compute_residual) CHFSI_TRY(residual_impl(s, op, X, N, M, N, B, eig_host, res_host, st));
+
+  // back to the FE basis
+  CHFSI_HIP(launch_scale_rows<double>(X, N, M, N, 1.0, op->d_inv_sqrt_mass, st));
+  CHFSI_HIP(hipEventRecord(pe.ev[5], st));
+  CHFSI_TRY(pe.report(timings_ms));
+  if (bUp_out) *bUp_out = s->bUp;
+  return CHFSI_OK;
+}
+
+extern "C" chfsi_status chfsi_solve_spectrum_split(
+    chfsi_solver *s, chfsi_operator *op, double *X, int64_t M, int32_t N, int32_t n_core,
+    double *Xfrac, int64_t ld_frac, double *eig_host, double *res_host,
+    int32_t is_first_filtering_call, int32_t compute_residual, int32_t use_mixed_prec_overall,
+    int32_t is_first_scf, double *bUp_out, double *timings_ms, void *stream) {
+  CHFSI_ARG(s && op && X && Xfrac && eig_host, "null argument");
+  CHFSI_ARG(M == op->M && N > 0, "X must be n_dofs x N");
+  CHFSI_ARG(n_core >= 0 && n_core < N && ld_frac >= N - n_core,
+            "need 0 <= n_core < N, ld_frac >= N - n_core");
+  CHFSI_ARG(!compute_residual || res_host, "res_host required when computing residuals");
+  hipStream_t st = as_stream(stream);
+  const chfsi_params &P = s->params;
+  const int Nfr = N - n_core;
+  PassEvents pe;
+  CHFSI_TRY(pe.create());
+  CHFSI_HIP(hipEventRecord(pe.ev[0], st));
+  int B = 0;
+  CHFSI_TRY(lanczos_scale_filter(s, op, X, M, N, is_first_filtering_call, use_mixed_prec_overall,
+                                 is_first_scf, &B, pe.ev, st));
+
+  // Rayleigh-Ritz in the generalised form with the spectrum split
+  // (…SolverDevice.cc:584-607): Cholesky orthonormalisation, projected
+  // Hamiltonian, top Nfr pairs into Xfrac
+  const bool mx = use_mixed_prec_overall != 0;
+  CHFSI_TRY(cholgs_impl(s, X, M, N, B, mx && P.mixed_prec_overlap, mx && P.mixed_prec_cgs_rotation,
+                        st));
+  CHFSI_HIP(hipEventRecord(pe.ev[3], st));
+  CHFSI_TRY(rr_split_impl(s, op, X, M, N, n_core, B, Xfrac, ld_frac, eig_host,
+                          mx && P.mixed_prec_xthx_spectrum_split, st));
+  CHFSI_HIP(hipEventRecord(pe.ev[4], st));
+
+  // residuals of the Nfr Ritz pairs (…SolverDevice.cc:676-690)
+  if (compute_residual)
+    CHFSI_TRY(residual_impl(s, op, Xfrac, ld_frac, M, Nfr, std::min(B, Nfr), eig_host, res_host,
+                            st));
+
+  // both back to the FE basis (…SolverDevice.cc:718-732)
+  CHFSI_HIP(launch_scale_rows<double>(X, N, M, N, 1.0, op->d_inv_sqrt_mass, st));
+  CHFSI_HIP(launch_scale_rows<double>(Xfrac, ld_frac, M, Nfr, 1.0, op->d_inv_sqrt_mass, st));
+  CHFSI_HIP(hipEventRecord(pe.ev[5], st));
+  CHFSI_TRY(pe.report(timings_ms));
   if (bUp_out) *bUp_out = s->bUp;
   return CHFSI_OK;
 }

@@ -65,6 +65,7 @@ class Params(ct.Structure):
         ("fp64_tail_degrees", ct.c_int32),
         ("mixed_prec_overlap", ct.c_int32), ("mixed_prec_cgs_rotation", ct.c_int32),
         ("mixed_prec_xthx", ct.c_int32), ("mixed_prec_rr_rotation", ct.c_int32),
+        ("mixed_prec_xthx_spectrum_split", ct.c_int32),
     ]
 
 
@@ -121,6 +122,14 @@ SIGNATURES = {
     "chfsi_solve": (ct.c_int, [_VP, _VP, _VP, ct.c_int64, ct.c_int32, c_double_p, c_double_p,
                                ct.c_int32, ct.c_int32, ct.c_int32, ct.c_int32, c_double_p,
                                c_double_p, _VP]),
+    # spectrum split (top N - n_core Ritz pairs)
+    "chfsi_rayleigh_ritz_spectrum_split": (ct.c_int, [_VP, _VP, ct.c_int64, ct.c_int32, ct.c_int32,
+                                                      _VP, ct.c_int64, ct.c_int32, c_double_p,
+                                                      ct.c_int32, _VP]),
+    "chfsi_solve_spectrum_split": (ct.c_int, [_VP, _VP, _VP, ct.c_int64, ct.c_int32, ct.c_int32,
+                                              _VP, ct.c_int64, c_double_p, c_double_p,
+                                              ct.c_int32, ct.c_int32, ct.c_int32, ct.c_int32,
+                                              c_double_p, c_double_p, _VP]),
     # complex (Bloch k-point) form
     "chfsi_operator_set_kpoint": (ct.c_int, [_VP, ct.c_int32]),
     "chfsi_operator_n_kpoints": (ct.c_int32, [_VP]),

@@ -141,6 +141,9 @@ typedef struct {
   int32_t mixed_prec_cgs_rotation; /* USE MIXED PREC CGS SR: X <- X L^{-T}        */
   int32_t mixed_prec_xthx;         /* USE MIXED PREC XTHX: Hp = X^T H~ X          */
   int32_t mixed_prec_rr_rotation;  /* USE MIXED PREC RR_SR: X <- X Q              */
+  int32_t mixed_prec_xthx_spectrum_split; /* USE MIXED PREC XTHX SPECTRUM SPLIT:
+                                      column blocks within the first n_core states
+                                      of the spectrum-split X^T H~ X in FP32      */
 } chfsi_params;
 
 const char *chfsi_last_error(void);
@@ -296,6 +299,33 @@ chfsi_status chfsi_solve(chfsi_solver *solver, chfsi_operator *op, double *X, in
                          int32_t is_first_filtering_call, int32_t compute_residual,
                          int32_t use_mixed_prec_overall, int32_t is_first_scf, double *bUp_out,
                          double *timings_ms, void *stream);
+
+/* rayleighRitzGEPSpectrumSplitDirect (rayleighRitzDevice.cc:822-1278): on X
+ * (device, node-major M x N, Loewdin basis) S = X^T X = L L^T and
+ * X <- X L^{-T}, then Hp = X^T H~ X, its eigendecomposition, and only the top
+ * N - n_core pairs kept: Xfrac (device, M x (N - n_core), leading dimension
+ * ld_frac) = X Q[:, n_core:N] and the N - n_core eigenvalues ascending in
+ * eig_host.  X is left orthonormal and unrotated.  mixed_xthx: column blocks
+ * of Hp within the first n_core states from FP32 copies. */
+chfsi_status chfsi_rayleigh_ritz_spectrum_split(chfsi_operator *op, double *X, int64_t M,
+                                                int32_t N, int32_t n_core, double *Xfrac,
+                                                int64_t ld_frac, int32_t block, double *eig_host,
+                                                int32_t mixed_xthx, void *stream);
+
+/* solve with the spectrum split (…SolverDevice.cc:155-736, the branch taken
+ * when eigenValues.size() != N): Lanczos and filter as chfsi_solve, then the
+ * spectrum-split Rayleigh-Ritz above (no separate CholGS), residual norms of
+ * the N - n_core Ritz pairs in Xfrac, and X and Xfrac both returned in the FE
+ * basis.  eig_host and res_host hold N - n_core values.  As in the reference's
+ * caller (kohnShamEigenSolve.cc:483-498), the next pass sets bLow to the last
+ * eigenvalue and keeps a0. */
+chfsi_status chfsi_solve_spectrum_split(chfsi_solver *solver, chfsi_operator *op, double *X,
+                                        int64_t M, int32_t N, int32_t n_core, double *Xfrac,
+                                        int64_t ld_frac, double *eig_host, double *res_host,
+                                        int32_t is_first_filtering_call,
+                                        int32_t compute_residual,
+                                        int32_t use_mixed_prec_overall, int32_t is_first_scf,
+                                        double *bUp_out, double *timings_ms, void *stream);
 
 /* ---------------- complex (Bloch k-point) form ----------------
  * Complex multivectors are std::complex<double> (or <float>) node-major,

@@ -403,6 +403,45 @@ def chfsi_pass(op, X: np.ndarray, a0: float, bLow: float, bUp: float, m: int, bl
     return lam, res, op.s[:, None] * Xt
 
 
+def rayleigh_ritz_spectrum_split(op, X: np.ndarray, n_core: int, block: int):
+    """Generalised Rayleigh-Ritz keeping only the top N - n_core pairs:
+    X <- X L^{-T} (S = X^T X = L L^T), Hp = X^T H~ X, Hp = Q diag(l) Q^T, and
+    Xfrac = X Q[:, n_core:].  Returns (l[n_core:], X orthonormalised, Xfrac).
+    Reference: src/linAlg/rayleighRitzDevice.cc:822-1278
+    (rayleighRitzGEPSpectrumSplitDirect; eigenvalues by index n_core..N-1 at
+    :1401-1406, rotation of the top states only at :1430-1441)."""
+    X = cholgs(X)
+    N = X.shape[1]
+    XH = X.conj().T
+    Hp = np.zeros((N, N), dtype=X.dtype)
+    for j in range(0, N, block):
+        Hp[:, j:j + block] = XH @ op.apply(X[:, j:j + block])
+    Hp = 0.5 * (Hp + Hp.conj().T)
+    lam, Q = np.linalg.eigh(Hp)
+    return lam[n_core:], X, X @ Q[:, n_core:]
+
+
+def chfsi_pass_spectrum_split(op, X: np.ndarray, a0: float, bLow: float, bUp: float, m: int,
+                              block: int, n_core: int):
+    """One pass with the spectrum split, in the physical FE basis: filter as
+    chfsi_pass, then the split Rayleigh-Ritz (no separate CholGS), residuals of
+    the top pairs only, and both X and Xfrac unscaled.  Reference:
+    chebyshevOrthogonalizedSubspaceIterationSolverDevice.cc:584-607 (split
+    branch), :676-690 (residuals on Xfrac), :718-732 (unscaling).
+    Returns (eigenvalues, residual norms, X, Xfrac)."""
+    Xt = op.sq[:, None] * X
+    N = X.shape[1]
+
+    def hx(src, scale_flag, scalar, dst):
+        return op.HX(src, scale_flag, scalar, dst)
+
+    for j in range(0, N, block):
+        Xt[:, j:j + block] = chebyshev_filter(hx, Xt[:, j:j + block], m, bLow, bUp, a0)
+    lam, Xt, Xf = rayleigh_ritz_spectrum_split(op, Xt, n_core, block)
+    res = residual_norms(op, Xf, lam, block)
+    return lam, res, op.s[:, None] * Xt, op.s[:, None] * Xf
+
+
 def next_bounds(lam: np.ndarray):
     """After a pass: a0 = lambda_0, bLow = lambda_{N-1} (src/dft/kohnShamEigenSolve.cc:489-498)."""
     return float(lam[0]), float(lam[-1])

@@ -212,6 +212,49 @@ def test_rayleigh_ritz_routes(brute):
     assert np.allclose(r, np.linalg.norm(H @ Xr - Xr * lam, axis=0), atol=1e-12)
 
 
+def test_rayleigh_ritz_spectrum_split(brute):
+    """Spectrum-split Rayleigh-Ritz on a non-orthonormal block: the kept
+    eigenvalues are the top N - n_core eigenvalues of the generalised problem
+    (X^T H X, X^T X) solved by LAPACK's sygvd; Xfrac is orthonormal and
+    diagonalises H with those values; it spans the top columns of the full
+    Rayleigh-Ritz; and X comes back orthonormalised but not rotated."""
+    prob, ref, H = brute
+    X = prob.sqrt_mass[:, None] * prob.initial_guess(N=12, seed_offset=7)
+    n_core = 7
+    lam, Xo, Xf = O.rayleigh_ritz_spectrum_split(ref, X, n_core, 5)
+    gen = sla.eigh(X.T @ H @ X, X.T @ X, eigvals_only=True)
+    assert lam.shape == (5,)
+    assert np.abs(lam - gen[n_core:]).max() < 1e-10 * np.abs(gen).max()
+    assert np.abs(Xf.T @ Xf - np.eye(5)).max() < 1e-12
+    assert np.abs(Xf.T @ H @ Xf - np.diag(lam)).max() < 1e-10
+    lam_full, Xr = O.rayleigh_ritz(ref, O.cholgs(X), 5)
+    P = Xr[:, n_core:] @ Xr[:, n_core:].T
+    assert np.abs(P @ Xf - Xf).max() < 1e-10
+    assert np.abs(Xo - O.cholgs(X)).max() < 1e-12
+    # a wrong split index (keeping the bottom instead of the top) fails
+    assert np.abs(lam_full[:5] - lam).max() > 1e-3
+
+
+def test_chfsi_pass_spectrum_split_matches_full_pass(brute):
+    """The split pass filters exactly as the full pass; its eigenvalues and
+    residual norms are the top N - n_core of the full pass's, its X is the full
+    pass's filtered block after CholGS only, and Xfrac spans the top Ritz vectors."""
+    prob, ref, H = brute
+    N, n_core = 12, 8
+    lo, up, _, _ = O.lanczos_bounds(ref, prob.lanczos_start, 10)
+    a0, bLow, bUp = O.first_call_bounds(lo, up, N, prob.M, reproducible=True)
+    X = prob.initial_guess(N=N, seed_offset=3)
+    lam, res, Xf_full = O.chfsi_pass(ref, X, a0, bLow, bUp, 6, 4)
+    lam_s, res_s, Xs, Xfrac = O.chfsi_pass_spectrum_split(ref, X, a0, bLow, bUp, 6, 4, n_core)
+    assert np.abs(lam_s - lam[n_core:]).max() < 1e-10
+    assert np.abs(res_s - res[n_core:]).max() < 1e-9
+    # physical basis: M-orthonormal columns, same span as the full pass's top vectors
+    Mm = prob.mass
+    assert np.abs(Xs.T @ (Mm[:, None] * Xs) - np.eye(N)).max() < 1e-10
+    P = Xf_full[:, n_core:] @ (Xf_full[:, n_core:].T * Mm[None, :])
+    assert np.abs(P @ Xfrac - Xfrac).max() < 1e-8 * np.abs(Xfrac).max()
+
+
 def test_lanczos_bounds_bracket(brute):
     prob, ref, H = brute
     ev = np.linalg.eigvalsh(H)
