@@ -235,6 +235,19 @@ class KohnShamOperatorDevice:
                                        _ptr(rho), 0, _stream()))
         return rho
 
+    def density_complex(self, X: torch.Tensor, w: torch.Tensor,
+                        rho: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """rho(c, q) (+)= sum_v w_v |psi_v(q)|^2 for complex FE-basis wavefunctions
+        X (M x N, complex128); with rho given, the k-point's contribution is added."""
+        p = self.prob
+        acc = rho is not None
+        if rho is None:
+            rho = torch.empty((p.C, p.nqc), dtype=torch.float64, device=self.device)
+        w = w.to(device=self.device, dtype=torch.float64).contiguous()
+        L.check(self.lib.chfsi_density_complex(self.handle, _ptr(X), X.stride(0), X.shape[1],
+                                               _ptr(w), _ptr(rho), int(acc), _stream()))
+        return rho
+
     def hx_adjoint_v(self, Ybar: torch.Tensor, X: torch.Tensor, scalar: float = 1.0,
                      g: Optional[torch.Tensor] = None) -> torch.Tensor:
         """dL/dV(c, q) for Y = scalar H~(V) X given Ybar = dL/dY (accumulates into g)."""

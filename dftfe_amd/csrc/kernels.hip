@@ -1064,7 +1064,7 @@ template <int QPT, typename TIn>
 __global__ __launch_bounds__(256) void quad_product_kernel(
     const uint32_t *map, int ne, int nqc, const double *Nq, const double *JxW, const TIn *a,
     int64_t lda, const TIn *b, int64_t ldb, int B, const double *s_in, const double *w,
-    double wscale, double *g, int accumulate) {
+    int wshift, double wscale, double *g, int accumulate) {
   constexpr int VC = 16;  // columns per chunk
   constexpr int KT = 16;  // nodes per slice
   __shared__ double Ns[256 * QPT][KT + 1];
@@ -1126,7 +1126,7 @@ __global__ __launch_bounds__(256) void quad_product_kernel(
     for (int qi = 0; qi < QPT; ++qi)
 #pragma unroll
       for (int vv = 0; vv < VC; ++vv)
-        if (v0 + vv < B) gacc[qi] += (w ? w[v0 + vv] : 1.0) * acc_a[qi][vv] * acc_b[qi][vv];
+        if (v0 + vv < B) gacc[qi] += (w ? w[(v0 + vv) >> wshift] : 1.0) * acc_a[qi][vv] * acc_b[qi][vv];
   }
 #pragma unroll
   for (int qi = 0; qi < QPT; ++qi) {
@@ -1141,31 +1141,33 @@ __global__ __launch_bounds__(256) void quad_product_kernel(
 template <typename TIn>
 static hipError_t quad_go(const uint32_t *map, int C, int ne, int nqc, const double *Nq,
                           const double *JxW, const TIn *a, int64_t lda, const TIn *b, int64_t ldb,
-                          int B, const double *s_in, const double *w, double wscale, double *g,
-                          int accumulate, hipStream_t s) {
+                          int B, const double *s_in, const double *w, int wshift, double wscale,
+                          double *g, int accumulate, hipStream_t s) {
   if (ne > 384 || nqc > 512) return hipErrorInvalidValue;
   if (nqc <= 256)
     hipLaunchKernelGGL((quad_product_kernel<1, TIn>), dim3(C), dim3(256), 0, s, map, ne, nqc, Nq,
-                       JxW, a, lda, b, ldb, B, s_in, w, wscale, g, accumulate);
+                       JxW, a, lda, b, ldb, B, s_in, w, wshift, wscale, g, accumulate);
   else
     hipLaunchKernelGGL((quad_product_kernel<2, TIn>), dim3(C), dim3(256), 0, s, map, ne, nqc, Nq,
-                       JxW, a, lda, b, ldb, B, s_in, w, wscale, g, accumulate);
+                       JxW, a, lda, b, ldb, B, s_in, w, wshift, wscale, g, accumulate);
   return hipGetLastError();
 }
 
 hipError_t launch_quad_product(const uint32_t *map, int C, int ne, int nqc, const double *Nq,
                                const double *JxW, const double *a, int64_t lda, const double *b,
                                int64_t ldb, int B, const double *s_in, const double *w,
-                               double wscale, double *g, int accumulate, hipStream_t s) {
-  return quad_go<double>(map, C, ne, nqc, Nq, JxW, a, lda, b, ldb, B, s_in, w, wscale, g,
+                               double wscale, double *g, int accumulate, hipStream_t s,
+                               int wshift) {
+  return quad_go<double>(map, C, ne, nqc, Nq, JxW, a, lda, b, ldb, B, s_in, w, wshift, wscale, g,
                          accumulate, s);
 }
 
 hipError_t launch_quad_product(const uint32_t *map, int C, int ne, int nqc, const double *Nq,
                                const double *JxW, const float *a, int64_t lda, const float *b,
                                int64_t ldb, int B, const double *s_in, const double *w,
-                               double wscale, double *g, int accumulate, hipStream_t s) {
-  return quad_go<float>(map, C, ne, nqc, Nq, JxW, a, lda, b, ldb, B, s_in, w, wscale, g,
+                               double wscale, double *g, int accumulate, hipStream_t s,
+                               int wshift) {
+  return quad_go<float>(map, C, ne, nqc, Nq, JxW, a, lda, b, ldb, B, s_in, w, wshift, wscale, g,
                         accumulate, s);
 }
 

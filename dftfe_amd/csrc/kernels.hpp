@@ -110,17 +110,20 @@ hipError_t launch_residual_partial(const double *HX, int64_t ldH, const double *
 hipError_t launch_dot_partial(const double *x, const double *y, int64_t n, double *partial,
                               int nchunks, hipStream_t st);
 
-// g[c*nqc + q] (+)= wscale JxW[c*nqc + q] sum_v w_v (N(q).(s a_v)|_c)(N(q).(s b_v)|_c)
-// s, w and JxW nullable (== 1)
+// g[c*nqc + q] (+)= wscale JxW[c*nqc + q] sum_v w_{v >> wshift} (N(q).(s a_v)|_c)(N(q).(s b_v)|_c)
+// s, w and JxW nullable (== 1); wshift = 1 shares each weight between the two
+// real columns of an interleaved complex column
 hipError_t launch_quad_product(const uint32_t *map, int C, int ne, int nqc, const double *Nq,
                                const double *JxW, const double *a, int64_t lda, const double *b,
                                int64_t ldb, int B, const double *s_in, const double *w,
-                               double wscale, double *g, int accumulate, hipStream_t s);
+                               double wscale, double *g, int accumulate, hipStream_t s,
+                               int wshift = 0);
 // the same with FP32 inputs (products and sums in FP64)
 hipError_t launch_quad_product(const uint32_t *map, int C, int ne, int nqc, const double *Nq,
                                const double *JxW, const float *a, int64_t lda, const float *b,
                                int64_t ldb, int B, const double *s_in, const double *w,
-                               double wscale, double *g, int accumulate, hipStream_t s);
+                               double wscale, double *g, int accumulate, hipStream_t s,
+                               int wshift = 0);
 
 // C[i + j*ldc] = sum_k P[k*m*n + i + j*m]  (split-K reduction, column-major m x n)
 hipError_t launch_sum_partials(const double *P, int nk, int m, int n, double *C, int64_t ldc,

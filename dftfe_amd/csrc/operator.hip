@@ -674,6 +674,20 @@ extern "C" chfsi_status chfsi_density(chfsi_operator *op, const double *X, int64
   return CHFSI_OK;
 }
 
+extern "C" chfsi_status chfsi_density_complex(chfsi_operator *op, const void *X, int64_t ld,
+                                              int32_t N, const double *w_dev, double *rho_dev,
+                                              int32_t accumulate, void *stream) {
+  CHFSI_ARG(op && X && rho_dev && N > 0 && ld >= N, "bad argument");
+  CHFSI_ARG(op->d_Nq, "operator was created without shape data");
+  // |psi|^2 = Re^2 + Im^2: the real contraction over the 2N interleaved real
+  // columns, each weight shared by the pair
+  const double *Xr = static_cast<const double *>(X);
+  CHFSI_HIP(launch_quad_product(op->d_map, op->C, op->ne, op->nq, op->d_Nq, nullptr, Xr, 2 * ld,
+                                Xr, 2 * ld, 2 * N, nullptr, w_dev, 1.0, rho_dev, accumulate,
+                                as_stream(stream), 1));
+  return CHFSI_OK;
+}
+
 extern "C" chfsi_status chfsi_hx_adjoint_v(chfsi_operator *op, const double *Ybar, int64_t ldy,
                                            const double *X, int64_t ldx, int32_t B,
                                            double scalar, double *g_dev, int32_t accumulate,

@@ -94,6 +94,8 @@ SIGNATURES = {
                                  ct.c_float, _VP]),
     "chfsi_hx_cheby": (ct.c_int, [_VP, _VP, ct.c_int64, _VP, ct.c_int64, ct.c_int32, _VP]),
     "chfsi_density": (ct.c_int, [_VP, _VP, ct.c_int64, ct.c_int32, _VP, _VP, ct.c_int32, _VP]),
+    "chfsi_density_complex": (ct.c_int, [_VP, _VP, ct.c_int64, ct.c_int32, _VP, _VP, ct.c_int32,
+                                         _VP]),
     "chfsi_hx_adjoint_v": (ct.c_int, [_VP, _VP, ct.c_int64, _VP, ct.c_int64, ct.c_int32,
                                       ct.c_double, _VP, ct.c_int32, _VP]),
     "chfsi_chebyshev_filter_backward": (ct.c_int, [_VP, _VP, ct.c_int64, _VP, ct.c_int64,

@@ -223,6 +223,15 @@ chfsi_status chfsi_density(chfsi_operator *op, const double *X, int64_t ld, int3
                            const double *w_dev, double *rho_dev, int32_t accumulate,
                            void *stream);
 
+/* The same contraction for complex (Bloch) wavefunctions, |psi|^2 = Re^2 + Im^2
+ * (densityCalculator.cc:212-254 with dataTypes::number complex): X is complex
+ * double interleaved (re, im), ld and N in complex elements, w holds N real
+ * weights (spin factor * k weight * occupation); contributions of several
+ * k-points add with accumulate = 1. */
+chfsi_status chfsi_density_complex(chfsi_operator *op, const void *X, int64_t ld, int32_t N,
+                                   const double *w_dev, double *rho_dev, int32_t accumulate,
+                                   void *stream);
+
 /* Adjoint of HX with respect to the local potential at the quadrature points
  * (no reference counterpart; differentiation harness).  For Y = scalar * H~ X,
  *   g[c*n_q + q] (+)= scalar * JxW(c,q) * sum_v (N(q).(M^{-1/2} Ybar_v)) (N(q).(M^{-1/2} X_v))

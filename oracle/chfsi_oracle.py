@@ -528,5 +528,5 @@ def density_at_quadrature(prob, X: np.ndarray, occ: np.ndarray, spin_factor: flo
         c1 = min(prob.C, c0 + 256)
         Xc = X[prob.cell_map[c0:c1]]                      # (k, ne, N)
         psi = np.einsum("qi,kiv->kqv", prob.Nq, Xc)
-        rho[c0:c1] = np.einsum("kqv,v->kq", psi * psi, w)
+        rho[c0:c1] = np.einsum("kqv,v->kq", (psi * psi.conj()).real, w)
     return rho

@@ -206,3 +206,20 @@ def test_complex_pass_matches_oracle(kw, mixed):
     assert err < (1e-6 if mixed else 1e-10)
     if not mixed:
         assert np.abs(res - res_o).max() < 1e-8
+
+
+def test_complex_density_matches_oracle(kw):
+    """rho(q) = sum_k sum_v w_kv |psi_kv(q)|^2 accumulated over two k-points on the
+    device (complex wavefunctions, FE basis) against the oracle, 1e-12 relative."""
+    prob, kpts, op, ref = kw
+    rng = np.random.default_rng(21)
+    rho_o = np.zeros((prob.C, prob.nqc))
+    rho = None
+    for k, nv in ((0, 40), (3, 17)):
+        X = (rng.standard_normal((prob.M, nv)) + 1j * rng.standard_normal((prob.M, nv))) * 0.1
+        f = rng.uniform(0.0, 1.0, nv)
+        wk = 0.5
+        rho_o += O.density_at_quadrature(prob, X, f, spin_factor=2.0, kweight=wk)
+        w = torch.tensor(2.0 * wk * f, device="cuda")
+        rho = op.density_complex(_dev(X), w, rho)
+    assert _rel(rho.cpu().numpy(), rho_o) < 1e-12

@@ -160,3 +160,20 @@ def test_complex_chfsi_converges_to_exact(kprob):
         lam, res, X = O.chfsi_pass(ref, X, a0, bLow, bUp, 12, 8)
         a0, bLow = O.next_bounds(lam)
     assert np.abs(lam[:N] - exact).max() < 1e-8
+
+
+def test_complex_density_is_modulus_squared(kprob):
+    """The oracle's density for complex wavefunctions is sum_v w_v |psi_v(q)|^2:
+    invariant under a global phase (in particular psi -> i psi, which flips the
+    sign of psi^2), and equal to the densities of the real and imaginary parts
+    added; for real input it is the real density unchanged."""
+    prob, _, _ = kprob
+    rng = np.random.default_rng(9)
+    X = rng.standard_normal((prob.M, 5)) + 1j * rng.standard_normal((prob.M, 5))
+    f = rng.uniform(0.2, 1.0, 5)
+    rho = O.density_at_quadrature(prob, X, f, spin_factor=2.0)
+    assert np.abs(O.density_at_quadrature(prob, 1j * X, f) - rho).max() < 1e-12 * rho.max()
+    assert np.abs(O.density_at_quadrature(prob, np.exp(0.7j) * X, f) - rho).max() < 1e-12 * rho.max()
+    parts = O.density_at_quadrature(prob, X.real, f) + O.density_at_quadrature(prob, X.imag, f)
+    assert np.abs(parts - rho).max() < 1e-12 * rho.max()
+    assert (rho >= 0).all()
