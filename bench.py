@@ -44,13 +44,13 @@ def hx_bytes(prob, B, s):
     return prob.M * B * s + prob.C * prob.ne ** 2 * s + 2 * prob.M * B * s + prob.C * prob.ne * 4
 
 
-def measured_hx_traffic(workload, B, variant, precision="fp32"):
-    """HBM bytes of one fused block HX (8 colour launches of the given tile
-    variant + projection + reduction) from the rocprofv3 FETCH_SIZE /
-    WRITE_SIZE table committed under profiles/ for this workload and block
-    width; None when no measurement covers the configuration."""
-    path = os.path.join(ROOT, "profiles", f"hx_traffic_{workload}.json")
-    if B != 256 or not os.path.exists(path):
+def measured_hx_traffic(workload, width, variant, precision="fp32"):
+    """HBM bytes of one fused HX on `width` columns (8 colour launches of the
+    given tile variant + projection + reduction) from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE table committed under profiles/ for this workload and width;
+    None when no measurement covers the configuration."""
+    path = os.path.join(ROOT, "profiles", f"hx_traffic_{workload}_w{width}.json")
+    if not os.path.exists(path):
         return None
     with open(path) as f:
         t = json.load(f)[precision]
@@ -113,14 +113,18 @@ def main():
         f"pairs={len(prob.pair_cell)} N={wl.N} B={wl.B} m={wl.m}")
 
     N, B, m = wl.N, wl.B, wl.m
-    tune = op.autotune(B)
-    log(f"autotune (B={B}): {tune}")
+    # columns the solver filters in one recurrence (and applies H~ to at once in
+    # Rayleigh-Ritz and the residuals): its default rule, whole blocks within a
+    # 16 GiB scratch of 8 bytes per DOF and column
+    W = min(N, max(B, ((16 << 30) // (8 * prob.M)) // B * B))
+    tune = op.autotune(W)
+    log(f"autotune (width {W}): {tune}")
     X0 = torch.tensor(prob.initial_guess(), device=dev)
     v0 = torch.tensor(prob.lanczos_start, device=dev)
 
     # ---- fused HX timing with HIP events on the stream the library launches on
-    def time_hx(dtype):
-        Xb = torch.randn(prob.M, B, device=dev, dtype=dtype)
+    def time_hx(dtype, width):
+        Xb = torch.randn(prob.M, width, device=dev, dtype=dtype)
         Yb = torch.zeros_like(Xb)
         for _ in range(3):
             op.HX(Xb, False, 1.0, Yb)
@@ -134,14 +138,15 @@ def main():
         return e0.elapsed_time(e1) / args.hx_reps
 
     if args.hx_only:
-        hx32_ms = time_hx(torch.float32)
-        hx64_ms = time_hx(torch.float64)
+        hx32_ms = time_hx(torch.float32, W)
+        hx64_ms = time_hx(torch.float64, W)
         src = torch.empty(1 << 28, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
-        F = hx_flops(prob, B)
-        print(json.dumps({"hx_fp32_ms": round(hx32_ms, 4), "hx_fp64_ms": round(hx64_ms, 4),
+        F = hx_flops(prob, W)
+        print(json.dumps({"width": W, "hx_fp32_ms": round(hx32_ms, 4),
+                          "hx_fp64_ms": round(hx64_ms, 4),
                           "hx_fp32_tflops": round(F / hx32_ms / 1e9, 2),
                           "hx_fp64_tflops": round(F / hx64_ms / 1e9, 2),
                           "calibration_copy_bytes": 4 << 28, "autotune": tune}), flush=True)
@@ -219,13 +224,16 @@ def main():
         f"max|dlambda| vs full mixed {max(sp_err):.2e}")
     del X0
 
-    hx32_ms = time_hx(torch.float32)
-    hx64_ms = time_hx(torch.float64)
-    F = hx_flops(prob, B)
+    hx32_ms = time_hx(torch.float32, W)
+    hx64_ms = time_hx(torch.float64, W)
+    F = hx_flops(prob, W)
     hx32_gf = F / hx32_ms / 1e6
     hx64_gf = F / hx64_ms / 1e6
-    log(f"HX B={B}: fp32 {hx32_ms:.3f} ms = {hx32_gf:.0f} GFLOP/s; "
+    log(f"HX {W} columns: fp32 {hx32_ms:.3f} ms = {hx32_gf:.0f} GFLOP/s; "
         f"fp64 {hx64_ms:.3f} ms = {hx64_gf:.0f} GFLOP/s")
+    hx32_blk = time_hx(torch.float32, B)
+    hx64_blk = time_hx(torch.float64, B)
+    log(f"HX {B} columns: fp32 {hx32_blk:.3f} ms, fp64 {hx64_blk:.3f} ms")
 
     PF = pass_flops(prob, N, B, m, wl.lanczos_steps)
     value = PF / mixed_ms / 1e6
@@ -244,7 +252,8 @@ def main():
         "dtype": "fp32 filter / fp64 CholGS,RR,residual",
         "data": "synthetic (seeded FE problem and start vectors)",
         "config": {"workload": args.workload, "dofs": prob.M, "cells": prob.C,
-                   "fe_degree": wl.p, "wavefunctions": N, "block": B, "cheby_degree": m,
+                   "fe_degree": wl.p, "wavefunctions": N, "block": B, "filter_columns": W,
+                   "cheby_degree": m,
                    "atoms": wl.n_atoms, "passes": args.steps, "parallelism": "single GPU"},
         "mixed_pass_ms": round(mixed_ms, 2),
         "mixed_dense_pass_ms": round(md_ms, 2),
@@ -255,10 +264,14 @@ def main():
         "max_eig_err_spectrum_split_vs_mixed_top_Ha": sp_err,
         "fp64_pass_ms": round(fp64_ms, 2) if fp64_ms else None,
         "mixed_over_fp64_speedup": round(fp64_ms / mixed_ms, 3) if fp64_ms else None,
+        "hx_columns": W,
         "hx_fp32_ms": round(hx32_ms, 4),
         "hx_fp32_gflops": round(hx32_gf, 1),
         "hx_fp64_ms": round(hx64_ms, 4),
         "hx_fp64_gflops": round(hx64_gf, 1),
+        "hx_block_columns": B,
+        "hx_fp32_ms_block": round(hx32_blk, 4),
+        "hx_fp64_ms_block": round(hx64_blk, 4),
         "pass_gflop": round(PF / 1e9, 1),
         "hx_tile_autotune": tune,
         "phase_ms_mixed_last": ph_mx[-1],
@@ -272,16 +285,17 @@ def main():
                                               for a, b in zip(lam_mx, lam_64)]
         out["max_residual_fp64"] = [float(r.max()) for r in res_64]
     out["roofline"] = {
-        "kernel": f"fused FP32 HX (8 colour launches of tile variant {tune['fp32']} + projection)",
+        "kernel": f"fused FP32 HX on {W} columns (8 colour launches of tile variant "
+                  f"{tune['fp32']} + projection)",
         "bound": "mfma",
         "achieved": round(F / hx32_ms / 1e9, 2),
         "peak": PEAK_TFLOPS["fp32"],
         "unit": "TFLOP/s",
         "frac": round(F / hx32_ms / 1e9 / PEAK_TFLOPS["fp32"], 4),
-        "traffic": measured_hx_traffic(args.workload, B, tune["fp32"]),
+        "traffic": measured_hx_traffic(args.workload, W, tune["fp32"]),
         "traffic_source": f"profiles/hx_traffic_{args.workload}.json, FP32 tile variant "
                           f"{tune['fp32']} (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
-        "algorithmic_bytes": hx_bytes(prob, B, 4),
+        "algorithmic_bytes": hx_bytes(prob, W, 4),
         "fp64": {"achieved": round(F / hx64_ms / 1e9, 2), "peak": PEAK_TFLOPS["fp64"],
                  "frac": round(F / hx64_ms / 1e9 / PEAK_TFLOPS["fp64"], 4)},
     }

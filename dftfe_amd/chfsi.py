@@ -132,14 +132,14 @@ class KohnShamOperatorDevice:
     def autotune(self, B: int) -> dict:
         """Time every HX tile variant on a B-column block and keep the fastest."""
         b64, b32 = ct.c_int32(), ct.c_int32()
-        t = (ct.c_float * 32)()
+        t = (ct.c_float * 40)()
         L.check(self.lib.chfsi_operator_autotune(self.handle, B, ct.byref(b64), ct.byref(b32), t,
                                                  _stream()))
         return {"fp64": b64.value, "fp32": b32.value,
-                "fp64_ms": [x for x in t[0:12] if x >= 0],
-                "fp32_ms": [x for x in t[16:28] if x >= 0],
-                "fp64_proj_ms": [x for x in t[12:16] if x >= 0],
-                "fp32_proj_ms": [x for x in t[28:32] if x >= 0]}
+                "fp64_ms": [x for x in t[0:16] if x >= 0],
+                "fp32_ms": [x for x in t[20:36] if x >= 0],
+                "fp64_proj_ms": [x for x in t[16:20] if x >= 0],
+                "fp32_proj_ms": [x for x in t[36:40] if x >= 0]}
 
     def getSqrtMassVec(self) -> int:
         return self.lib.chfsi_operator_sqrt_mass(self.handle)
@@ -447,11 +447,14 @@ class ChebyshevSolverDevice:
                  cheby_block_size: int = 400, cheby_degree: int = 0, lanczos_steps: int = 20,
                  reproducible: bool = False, filter_precision: int = L.FP32,
                  first_scf_degree_factor: float = 1.34, reuse_lanczos_upper: bool = False,
-                 fp64_tail_degrees: int = 0, mixed_dense: bool = False):
+                 fp64_tail_degrees: int = 0, mixed_dense: bool = False,
+                 filter_columns: int = 0):
         """mixed_dense: FP32 off-diagonal blocks in the overlap, CholGS rotation,
         X^T H X and Rayleigh-Ritz rotation when a pass runs with
         use_mixed_prec_overall (the reference's USE MIXED PREC CGS O / CGS SR /
-        XTHX / RR_SR together)."""
+        XTHX / RR_SR together).  filter_columns: columns one filter recurrence
+        runs on, whole blocks of cheby_block_size (0: all, within a 16 GiB
+        scratch); every column's result is the same whatever the grouping."""
         self.lib = L.load()
         p = L.Params()
         self.lib.chfsi_default_params(ct.byref(p))
@@ -463,6 +466,7 @@ class ChebyshevSolverDevice:
         p.first_scf_degree_factor = first_scf_degree_factor
         p.reuse_lanczos_upper = int(reuse_lanczos_upper)
         p.fp64_tail_degrees = fp64_tail_degrees
+        p.filter_columns = filter_columns
         for f in ("mixed_prec_overlap", "mixed_prec_cgs_rotation", "mixed_prec_xthx",
                   "mixed_prec_rr_rotation", "mixed_prec_xthx_spectrum_split"):
             setattr(p, f, int(mixed_dense))

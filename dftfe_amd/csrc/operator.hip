@@ -580,8 +580,8 @@ extern "C" chfsi_status chfsi_operator_set_proj_variant(chfsi_operator *op, int3
 namespace {
 template <typename T>
 chfsi_status time_variants(chfsi_operator *op, int B, int *best, float *times, hipStream_t st) {
-  // times[0..11]: tile variants, times[12..15]: projection variants
-  const int nv = hx_variants(sizeof(T) == 8) < 12 ? hx_variants(sizeof(T) == 8) : 12;
+  // times[0..15]: tile variants, times[16..19]: projection variants
+  const int nv = hx_variants(sizeof(T) == 8) < 16 ? hx_variants(sizeof(T) == 8) : 16;
   Scratch x, y;
   CHFSI_TRY(x.ensure(sizeof(T) * op->M * B));
   CHFSI_TRY(y.ensure(sizeof(T) * op->M * B));
@@ -632,7 +632,7 @@ chfsi_status time_variants(chfsi_operator *op, int B, int *best, float *times, h
         CHFSI_HIP(hipEventSynchronize(e1));
         if (stt == CHFSI_OK) CHFSI_HIP(hipEventElapsedTime(&ms, e0, e1));
       }
-      times[12 + v] = stt == CHFSI_OK ? ms / 3 : -1.0f;
+      times[16 + v] = stt == CHFSI_OK ? ms / 3 : -1.0f;
       if (stt == CHFSI_OK && ms < best_p) {
         best_p = ms;
         bp = v;
@@ -652,15 +652,15 @@ extern "C" chfsi_status chfsi_operator_autotune(chfsi_operator *op, int32_t B, i
                                                 int32_t *best32, float *times_ms, void *stream) {
   CHFSI_ARG(op && B > 0, "bad argument");
   hipStream_t st = as_stream(stream);
-  float t[32];
+  float t[40];
   for (float &v : t) v = -1.0f;
   int b64 = 0, b32 = 0;
   CHFSI_TRY(time_variants<double>(op, B, &b64, t, st));
-  CHFSI_TRY(time_variants<float>(op, B, &b32, t + 16, st));
+  CHFSI_TRY(time_variants<float>(op, B, &b32, t + 20, st));
   if (best64) *best64 = b64;
   if (best32) *best32 = b32;
   if (times_ms)
-    for (int i = 0; i < 32; ++i) times_ms[i] = t[i];
+    for (int i = 0; i < 40; ++i) times_ms[i] = t[i];
   return CHFSI_OK;
 }
 

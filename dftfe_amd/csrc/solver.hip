@@ -156,6 +156,18 @@ chfsi_status cheb_steps(chfsi_operator *op, T **cur, int64_t *ldc, T **prev, int
   return CHFSI_OK;
 }
 
+// Columns one filter recurrence runs on: whole blocks of B, all N unless the
+// filter scratch (8 bytes per DOF and column in every precision: the FP64
+// iterate, or the two FP32 iterates) would exceed 16 GiB.  A wider recurrence
+// fills more of each colour launch (the per-launch ramp and drain are shared by
+// more items) and gives every column the same arithmetic as block-by-block.
+int filter_group_columns(const chfsi_params &P, int64_t M, int N, int B) {
+  if (P.filter_columns > 0) return std::min(N, std::max(B, P.filter_columns / B * B));
+  const int64_t budget = int64_t(16) << 30;
+  const int64_t fit = std::max<int64_t>(1, budget / (8 * std::max<int64_t>(M, 1)) / B);
+  return (int)std::min<int64_t>(N, fit * B);
+}
+
 // In-place filter of the B-column block X (leading dimension ld).  FP32 and
 // BF16 forms cast the block to FP32 once; with tail > 0 the last `tail`
 // degrees run in FP64 from the cast-back pair of iterates, so that the
@@ -458,28 +470,35 @@ chfsi_status rr_impl(chfsi_solver *s, chfsi_operator *op, double *X, int64_t M, 
                      double *eig_host, bool mixed_xthx, bool mixed_rot, hipStream_t st) {
   CHFSI_TRY(ensure_blas(s, st));
   CHFSI_TRY(s->Hp.ensure(sizeof(double) * N * N));
-  CHFSI_TRY(s->HXb.ensure(sizeof(double) * M * B));
+  const int G = filter_group_columns(s->params, M, N, B);
+  CHFSI_TRY(s->HXb.ensure(sizeof(double) * M * G));
   CHFSI_TRY(s->evals.ensure(sizeof(double) * 2 * N));
   CHFSI_TRY(s->info.ensure(sizeof(rocblas_int)));
-  double *Hp = sptr<double>(s->Hp), *HX = sptr<double>(s->HXb);
+  double *Hp = sptr<double>(s->Hp), *HXg = sptr<double>(s->HXb);
   float *Xf = nullptr, *HXf = nullptr;
   if (mixed_xthx || mixed_rot) CHFSI_TRY(cast_x(s, X, M, N, &Xf, st));
   if (mixed_xthx) {
     CHFSI_TRY(s->HXf.ensure(sizeof(float) * M * B));
     HXf = sptr<float>(s->HXf);
   }
+  int g0 = 0, gw = 0;  // columns [g0, g0 + gw) of H~X are in HXg (ld gw)
   for (int j = 0; j < N; j += B) {
     const int bw = std::min(B, N - j);
-    CHFSI_TRY(hx_fused<double>(op, X + j, N, HX, bw, bw, 1.0, 0.0, 0.0, 1, true, st));
-    if (op->nc) CHFSI_HIP(launch_zero_rows(op->d_c_rows, op->nc, HX, bw, bw, st));
+    if (j >= g0 + gw) {
+      g0 = j;
+      gw = std::min(G, N - j);
+      CHFSI_TRY(hx_fused<double>(op, X + j, N, HXg, gw, gw, 1.0, 0.0, 0.0, 1, true, st));
+      if (op->nc) CHFSI_HIP(launch_zero_rows(op->d_c_rows, op->nc, HXg, gw, gw, st));
+    }
+    const double *HX = HXg + (j - g0);
     double *Hj = Hp + j + (int64_t)j * N;
     if (!mixed_xthx) {
       // Hp[j:N, j:j+bw] = Xc[j:N, :] * HXc^T
-      CHFSI_TRY(gemm_nt_splitk(s, X + j, N, HX, bw, N - j, bw, M, Hj, N, st));
+      CHFSI_TRY(gemm_nt_splitk(s, X + j, N, HX, gw, N - j, bw, M, Hj, N, st));
     } else {
-      CHFSI_TRY(gemm_nt_splitk(s, X + j, N, HX, bw, bw, bw, M, Hj, N, st));
+      CHFSI_TRY(gemm_nt_splitk(s, X + j, N, HX, gw, bw, bw, M, Hj, N, st));
       if (N - j - bw > 0) {
-        CHFSI_HIP(launch_copy_block<double, float>(HX, bw, HXf, bw, M, bw, st));
+        CHFSI_HIP(launch_copy_block<double, float>(HX, gw, HXf, bw, M, bw, st));
         CHFSI_TRY(sgemm_nt_splitk(s, Xf + j + bw, N, HXf, bw, N - j - bw, bw, M, Hj + bw, N, st));
       }
     }
@@ -502,18 +521,25 @@ chfsi_status rr_impl(chfsi_solver *s, chfsi_operator *op, double *X, int64_t M, 
 chfsi_status residual_impl(chfsi_solver *s, chfsi_operator *op, const double *X, int64_t ldX,
                            int64_t M, int N, int B, const double *eig_host, double *res_host,
                            hipStream_t st) {
-  CHFSI_TRY(s->HXb.ensure(sizeof(double) * M * B));
+  const int G = filter_group_columns(s->params, M, N, B);
+  CHFSI_TRY(s->HXb.ensure(sizeof(double) * M * G));
   CHFSI_TRY(s->lam.ensure(sizeof(double) * N));
   const int nch = 512;
   CHFSI_TRY(s->part.ensure(sizeof(double) * nch * B));
   double *HX = sptr<double>(s->HXb), *lam = sptr<double>(s->lam), *part = sptr<double>(s->part);
   CHFSI_HIP(hipMemcpyAsync(lam, eig_host, sizeof(double) * N, hipMemcpyHostToDevice, st));
   std::vector<double> h(size_t(nch) * B);
+  int g0 = 0, gw = 0;  // columns [g0, g0 + gw) of H~X are in HX (ld gw)
   for (int j = 0; j < N; j += B) {
     const int bw = std::min(B, N - j);
-    CHFSI_TRY(hx_fused<double>(op, X + j, ldX, HX, bw, bw, 1.0, 0.0, 0.0, 1, true, st));
-    if (op->nc) CHFSI_HIP(launch_zero_rows(op->d_c_rows, op->nc, HX, bw, bw, st));
-    CHFSI_HIP(launch_residual_partial(HX, bw, X + j, ldX, lam + j, M, bw, part, nch, st));
+    if (j >= g0 + gw) {
+      g0 = j;
+      gw = std::min(G, N - j);
+      CHFSI_TRY(hx_fused<double>(op, X + j, ldX, HX, gw, gw, 1.0, 0.0, 0.0, 1, true, st));
+      if (op->nc) CHFSI_HIP(launch_zero_rows(op->d_c_rows, op->nc, HX, gw, gw, st));
+    }
+    CHFSI_HIP(launch_residual_partial(HX + (j - g0), gw, X + j, ldX, lam + j, M, bw, part, nch,
+                                      st));
     CHFSI_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * nch * bw, hipMemcpyDeviceToHost, st));
     CHFSI_HIP(hipStreamSynchronize(st));
     for (int v = 0; v < bw; ++v) {
@@ -540,26 +566,33 @@ chfsi_status rr_split_impl(chfsi_solver *s, chfsi_operator *op, const double *X,
   const int Nfr = N - Noc;
   CHFSI_TRY(ensure_blas(s, st));
   CHFSI_TRY(s->Hp.ensure(sizeof(double) * N * N));
-  CHFSI_TRY(s->HXb.ensure(sizeof(double) * M * B));
+  const int G = filter_group_columns(s->params, M, N, B);
+  CHFSI_TRY(s->HXb.ensure(sizeof(double) * M * G));
   CHFSI_TRY(s->evals.ensure(sizeof(double) * 2 * N));
   CHFSI_TRY(s->info.ensure(sizeof(rocblas_int)));
-  double *Hp = sptr<double>(s->Hp), *HX = sptr<double>(s->HXb);
+  double *Hp = sptr<double>(s->Hp), *HXg = sptr<double>(s->HXb);
   float *Xf = nullptr, *HXf = nullptr;
   if (mixed_xthx && Noc >= B) {
     CHFSI_TRY(cast_x(s, X, M, N, &Xf, st));
     CHFSI_TRY(s->HXf.ensure(sizeof(float) * M * B));
     HXf = sptr<float>(s->HXf);
   }
+  int g0 = 0, gw = 0;  // columns [g0, g0 + gw) of H~X are in HXg (ld gw)
   for (int j = 0; j < N; j += B) {
     const int bw = std::min(B, N - j);
-    CHFSI_TRY(hx_fused<double>(op, X + j, N, HX, bw, bw, 1.0, 0.0, 0.0, 1, true, st));
-    if (op->nc) CHFSI_HIP(launch_zero_rows(op->d_c_rows, op->nc, HX, bw, bw, st));
+    if (j >= g0 + gw) {
+      g0 = j;
+      gw = std::min(G, N - j);
+      CHFSI_TRY(hx_fused<double>(op, X + j, N, HXg, gw, gw, 1.0, 0.0, 0.0, 1, true, st));
+      if (op->nc) CHFSI_HIP(launch_zero_rows(op->d_c_rows, op->nc, HXg, gw, gw, st));
+    }
+    const double *HX = HXg + (j - g0);
     double *Hj = Hp + j + (int64_t)j * N;
     if (Xf && j + bw <= Noc) {
-      CHFSI_HIP(launch_copy_block<double, float>(HX, bw, HXf, bw, M, bw, st));
+      CHFSI_HIP(launch_copy_block<double, float>(HX, gw, HXf, bw, M, bw, st));
       CHFSI_TRY(sgemm_nt_splitk(s, Xf + j, N, HXf, bw, N - j, bw, M, Hj, N, st));
     } else {
-      CHFSI_TRY(gemm_nt_splitk(s, X + j, N, HX, bw, N - j, bw, M, Hj, N, st));
+      CHFSI_TRY(gemm_nt_splitk(s, X + j, N, HX, gw, N - j, bw, M, Hj, N, st));
     }
   }
   double *D = sptr<double>(s->evals), *E = D + N;
@@ -600,6 +633,7 @@ extern "C" void chfsi_default_params(chfsi_params *p) {
   p->mixed_prec_xthx = 0;
   p->mixed_prec_rr_rotation = 0;
   p->mixed_prec_xthx_spectrum_split = 0;
+  p->filter_columns = 0;
 }
 
 extern "C" chfsi_status chfsi_solver_set_params(chfsi_solver *s, const chfsi_params *params) {
@@ -872,8 +906,9 @@ chfsi_status lanczos_scale_filter(chfsi_solver *s, chfsi_operator *op, double *X
   CHFSI_HIP(launch_scale_rows<double>(X, N, M, N, 1.0, op->d_sqrt_mass, st));
   const int B = std::min<int>(P.cheby_block_size > 0 ? P.cheby_block_size : N, N);
   const int prec = use_mixed_prec_overall ? P.filter_precision : CHFSI_FP64;
-  for (int j = 0; j < N; j += B) {
-    const int bw = std::min(B, N - j);
+  const int G = filter_group_columns(P, M, N, B);
+  for (int j = 0; j < N; j += G) {
+    const int bw = std::min(G, N - j);
     CHFSI_TRY(filter_block(s, op, X + j, N, bw, (int)m, s->bLow, s->bUp, s->a0, prec,
                            use_mixed_prec_overall ? P.fp64_tail_degrees : 0, st));
   }

@@ -19,7 +19,7 @@
   } while (0)
 
 struct chfsi_solver {
-  chfsi_params params;
+  chfsi_params params{};  // zero unless set (entry points that build a temporary solver)
   double a0 = 0, bLow = 0, bUp = 0;
   rocblas_handle blas = nullptr;
   chfsi::Scratch Y64, X32, Y32, HXb, Hp, rot, evals, info, part, lam, v0, split;

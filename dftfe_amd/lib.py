@@ -66,6 +66,7 @@ class Params(ct.Structure):
         ("mixed_prec_overlap", ct.c_int32), ("mixed_prec_cgs_rotation", ct.c_int32),
         ("mixed_prec_xthx", ct.c_int32), ("mixed_prec_rr_rotation", ct.c_int32),
         ("mixed_prec_xthx_spectrum_split", ct.c_int32),
+        ("filter_columns", ct.c_int32),
     ]
 
 

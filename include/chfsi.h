@@ -144,6 +144,11 @@ typedef struct {
   int32_t mixed_prec_xthx_spectrum_split; /* USE MIXED PREC XTHX SPECTRUM SPLIT:
                                       column blocks within the first n_core states
                                       of the spectrum-split X^T H~ X in FP32      */
+  int32_t filter_columns;          /* columns filtered by one recurrence: a multiple
+                                      of cheby_block_size (0: all N, within a 16 GiB
+                                      filter scratch).  No reference counterpart: the
+                                      reference filters one block at a time to bound
+                                      memory; every column's arithmetic is the same */
 } chfsi_params;
 
 const char *chfsi_last_error(void);
@@ -170,8 +175,8 @@ chfsi_status chfsi_operator_get_cell_matrices(chfsi_operator *op, double *H_dev,
  * reference calls hipBLAS gemmStridedBatched).  chfsi_operator_autotune times
  * every tile variant of both precisions on a B-column block, keeps the fastest,
  * then times the projection-kernel variants with it and keeps the fastest.
- * times_ms (32 entries) [0..11]: FP64 tile variants, [12..15]: FP64 projection
- * variants, [16..27]: FP32 tile variants, [28..31]: FP32 projection variants
+ * times_ms (40 entries) [0..15]: FP64 tile variants, [16..19]: FP64 projection
+ * variants, [20..35]: FP32 tile variants, [36..39]: FP32 projection variants
  * (ms per HX; -1 where a variant does not exist). */
 chfsi_status chfsi_operator_set_hx_variant(chfsi_operator *op, int32_t precision, int32_t variant);
 chfsi_status chfsi_operator_set_proj_variant(chfsi_operator *op, int32_t precision,
