@@ -1324,6 +1324,168 @@ hipError_t launch_f64_to_bf16(const double *x, int64_t n, uint16_t *y, hipStream
   return hipGetLastError();
 }
 
+// FP32 projection P_slot = C_e^T (s X)_e with the K slices copied global ->
+// LDS directly (the ring of hxp_kernel): per slice, the cell's slot rows of
+// nl_Ct (16 slots = one 1 KB copy, 16-byte units XOR-swizzled as for the cell
+// matrices) and the 16 gathered source rows ([k][column] groups).  All waves
+// split the columns and hold every slot block of the cell.  The products and
+// their order are proj_kernel's (B operand X*s, four 16x16x4 MFMAs per slice),
+// so the partials agree bit for bit.  Requirements as hxp_kernel's.
+template <int NW, int TN, int MAXSRB, int NRB, int S>
+__global__ __launch_bounds__(64 * NW) void projp_kernel(ProjLaunch<float> a) {
+  using T = float;
+  using v4 = f32x4;
+  constexpr int NT = 64 * NW;
+  constexpr int CBW = TN / 16 / NW;
+  constexpr int ROWS = NRB * 16;
+  constexpr int RPO = 256 / TN;
+  constexpr int LPR = TN / 4;
+  constexpr int XG = 1024 + 16 * RPO;
+  constexpr int NX = 16 / RPO;
+  constexpr int AB = MAXSRB * 1024;  // slot rows of one slice
+  constexpr int SB = AB + NX * XG;
+  constexpr int D = (MAXSRB + NX + NW - 1) / NW;
+  static_assert(CBW >= 1 && TN % (16 * NW) == 0, "bad column split");
+  static_assert(TN == 64 || TN == 128, "gathered-slice layout needs TN 64 or 128");
+  static_assert(S >= 2 && (S - 2) * D < 64, "ring depth");
+
+  __shared__ __align__(16) unsigned char ring[S * SB];
+  __shared__ __align__(16) int64_t soff[ROWS];
+  __shared__ __align__(16) T sscl[ROWS];
+
+  // XCD-aware: consecutive cells run on one XCD, the column tiles of a cell
+  // next to each other, so the cell's slot rows of nl_Ct are read from HBM
+  // once for all its tiles (ordering the tiles outermost instead, to share the
+  // face rows of neighbouring cells, measured slower)
+  const int wid = blockIdx.x;
+  const int local = wid >> 3;
+  if (local >= a.per_xcd) return;
+  const int item = (wid & 7) * a.per_xcd + local;
+  if (item >= a.ncells * a.ntiles) return;
+  const int cell = a.cells[item / a.ntiles];
+  const int v0 = (item % a.ntiles) * TN;
+  const int tid = threadIdx.x;
+  __builtin_assume(tid < NT);
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wave_s = __builtin_amdgcn_readfirstlane(wave);
+  const int s0 = a.nl_off[cell];
+  const int nslot = a.nl_off[cell + 1] - s0;
+  const int nsrb = nslot >> 4;
+  const int nops = nsrb + NX;
+  const T *Acell = a.nl_Ct + (int64_t)s0 * ROWS;
+  const uint32_t *cmap = a.map + (int64_t)cell * a.ne;
+
+  for (int i = tid; i < ROWS; i += NT) {
+    int64_t off = 0;
+    T sc = T(0);
+    if (i < a.ne) {
+      const uint32_t r = cmap[i] & 0x7fffffffu;
+      off = (int64_t)r * a.ld_src;
+      sc = a.s_in ? a.s_in[r] : T(1);
+    }
+    soff[i] = off;
+    sscl[i] = sc;
+  }
+  __syncthreads();
+
+  const uint32_t ring0 = lds_address(ring);
+  const int hr = lane >> 2;
+  const int hoff = hr * 16 + (((lane & 3) ^ (((hr >> 1) ^ (hr >> 3)) & 3)) * 4);
+  const int xsub = lane / LPR;
+  const int xc = v0 + 4 * (lane % LPR);
+  const int xcl = xc < a.B - 4 ? xc : a.B - 4;  // columns >= B are never stored
+  const T *xb = a.src + xcl;
+
+  auto issue = [&](int t, int stage) {
+    const T *tile = Acell + (int64_t)t * nslot * 16;
+    const uint32_t sbase = ring0 + stage * SB;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const int o = (wave_s + NW * j) % nops;  // wave-uniform
+      if (o < nsrb) {
+        copy16_to_lds(tile + o * 256 + hoff, sbase + o * 1024);
+      } else {
+        const int x = o - nsrb;
+        copy16_to_lds(xb + soff[t * 16 + RPO * x + xsub], sbase + AB + x * XG);
+      }
+    }
+  };
+
+  v4 acc[MAXSRB][CBW];
+#pragma unroll
+  for (int j = 0; j < MAXSRB; ++j)
+#pragma unroll
+    for (int cb = 0; cb < CBW; ++cb) acc[j][cb] = v4{0, 0, 0, 0};
+
+  const int arow = lane & 15;
+  const int aoff = arow * 16 + (((lane >> 4) ^ (((arow >> 1) ^ (arow >> 3)) & 3)) * 4);
+  const int kq = 4 * (lane >> 4);
+
+  auto compute = [&](int stage, int t) {
+    const T *As = reinterpret_cast<const T *>(ring + stage * SB);
+    const T *Xs = reinterpret_cast<const T *>(ring + stage * SB + AB);
+    const v4 sc = *reinterpret_cast<const v4 *>(&sscl[t * 16 + kq]);
+    v4 b4[CBW];
+#pragma unroll
+    for (int cb = 0; cb < CBW; ++cb) {
+      const int col = (wave * CBW + cb) * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k = kq + r;
+        b4[cb][r] = sc[r] * Xs[(k / RPO) * (XG / 4) + (k % RPO) * TN + col];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < MAXSRB; ++j) {
+      if (j < nsrb) {
+        const v4 a4 = *reinterpret_cast<const v4 *>(As + j * 256 + aoff);
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int cb = 0; cb < CBW; ++cb) acc[j][cb] = mfma16(a4[r], b4[cb][r], acc[j][cb]);
+      }
+    }
+  };
+
+#pragma unroll
+  for (int st = 0; st < S - 1; ++st) issue(st < NRB ? st : NRB - 1, st);
+  for (int t = 0; t < NRB; ++t) {
+    wait_vmcnt<(S - 2) * D>();
+    __syncthreads();
+    const int tn = t + S - 1;
+    issue(tn < NRB ? tn : NRB - 1, tn % S);
+    compute(t % S, t);
+  }
+  wait_vmcnt<0>();  // no copy may still target LDS when the workgroup ends
+
+#pragma unroll
+  for (int j = 0; j < MAXSRB; ++j) {
+    if (j >= nsrb) continue;
+#pragma unroll
+    for (int cb = 0; cb < CBW; ++cb) {
+      const int v = v0 + (wave * CBW + cb) * 16 + (lane & 15);
+      if (v >= a.B) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int srow = j * 16 + acc_row<T>(lane, r);
+        a.partial[(int64_t)(s0 + srow) * a.ldP + v] = acc[j][cb][r];
+      }
+    }
+  }
+}
+
+template <int NW, int TN, int MAXSRB, int NRB, int S>
+static hipError_t projp_go(ProjLaunch<float> a, hipStream_t s) {
+  a.ntiles = (a.B + TN - 1) / TN;
+  const int items = a.ncells * a.ntiles;
+  if (items == 0) return hipSuccess;
+  a.per_xcd = (items + 7) / 8;
+  hipLaunchKernelGGL((projp_kernel<NW, TN, MAXSRB, NRB, S>), dim3(8 * a.per_xcd), dim3(64 * NW),
+                     0, s, a);
+  return hipGetLastError();
+}
+
 template <typename T, int NW, int TN, int MAXSRB, int NRB, int ST>
 static hipError_t proj_go(ProjLaunch<T> a, hipStream_t s) {
   a.ntiles = (a.B + TN - 1) / TN;
@@ -1338,11 +1500,18 @@ static hipError_t proj_go(ProjLaunch<T> a, hipStream_t s) {
 // Projection variants: 0: 4 waves x wide column tile, one node tile per stage,
 // accumulators for 128 slots; 1: same tile, two node tiles per stage,
 // accumulators sized to the largest cell; 2: 2 waves x half-width tile, two
-// node tiles per stage, sized accumulators.
+// node tiles per stage, sized accumulators; 3 (FP32; FP64 and unaligned
+// blocks take 0): 8 waves x 128 columns, direct-to-LDS ring of 3 slices.
 template <typename T, int NRB, int MAXSRB>
 static hipError_t proj_variant(const ProjLaunch<T> &a, hipStream_t s) {
   constexpr int TNW = sizeof(T) == 4 ? 128 : 64;
   constexpr int ST = NRB % 2 == 0 ? 2 : 1;
+  if constexpr (sizeof(T) == 4) {
+    auto al = [](const void *p) { return ((uintptr_t)p & 15) == 0; };
+    if (a.variant == 3 && a.B >= 4 && a.B % 4 == 0 && a.ld_src % 4 == 0 && al(a.src) &&
+        al(a.nl_Ct))
+      return projp_go<8, 128, MAXSRB, NRB, 3>(a, s);
+  }
   switch (a.variant) {
     case 1: return proj_go<T, 4, TNW, MAXSRB, NRB, ST>(a, s);
     case 2: return proj_go<T, 2, TNW / 2, MAXSRB, NRB, ST>(a, s);
@@ -1357,7 +1526,7 @@ static hipError_t proj_fit(const ProjLaunch<T> &a, hipStream_t s) {
   return proj_variant<T, NRB, 8>(a, s);
 }
 
-int proj_variants() { return 3; }
+int proj_variants() { return 4; }
 
 template <typename T>
 hipError_t launch_proj(const ProjLaunch<T> &a, hipStream_t s) {

@@ -111,7 +111,31 @@ def test_hx_ring_variants_bitwise(small, variant, B):
     assert torch.equal(out[2], out[variant])
 
 
-@pytest.mark.parametrize("precision,variant", [(p, v) for p in (0, 1) for v in range(3)])
+@pytest.mark.parametrize("B", [72, 256, 70])
+def test_projection_ring_bitwise(small, B):
+    """The direct-to-LDS FP32 projection (variant 3) forms the same products in
+    the same order as variant 0, so a degree-4 FP32 filter gives bitwise-equal
+    blocks; B = 70 (not a multiple of 4) takes variant 0 itself."""
+    from dftfe_amd import lib as L
+    from dftfe_amd.chfsi import chebyshev_filter
+
+    prob, ref, op = small
+    X = torch.tensor(prob.sqrt_mass[:, None] * prob.initial_guess(N=B, seed_offset=B + 5),
+                     device="cuda")
+    out = {}
+    for v in (0, 3):
+        Xd = X.clone()
+        op.set_proj_variant(L.FP32, v)
+        try:
+            chebyshev_filter(op, Xd, 4, 5.0, 41.0, 0.0, precision=L.FP32)
+        finally:
+            op.set_proj_variant(L.FP32, 0)
+        out[v] = Xd
+    assert torch.isfinite(out[0]).all()
+    assert torch.equal(out[0], out[3])
+
+
+@pytest.mark.parametrize("precision,variant", [(p, v) for p in (0, 1) for v in range(4)])
 def test_projection_variants(small, precision, variant):
     """Every projection-kernel variant gives the oracle's H~X."""
     prob, ref, op = small
