@@ -87,6 +87,11 @@ def main():
     ap.add_argument("--no-kpoints", action="store_true",
                     help="skip the batched k-point filter measurement")
     ap.add_argument("--kpoint-reps", type=int, default=3)
+    ap.add_argument("--converge-passes", type=int, default=0,
+                    help="also run FP64, mixed and mixed defect-form passes from the same start "
+                         "this many times and report their eigenvalue differences (untimed)")
+    ap.add_argument("--monotone-blow", action="store_true",
+                    help="convergence leg: bLow never rises above its previous value")
     ap.add_argument("--hx-only", action="store_true",
                     help="setup, tile autotune and the HX timing only (for counter runs), "
                          "plus a 1 GiB device copy that calibrates the traffic counters")
@@ -152,9 +157,14 @@ def main():
                           "calibration_copy_bytes": 4 << 28, "autotune": tune}), flush=True)
         return
 
-    def run_passes(mixed: bool, npass: int, dense: bool = False):
+    def run_passes(mixed: bool, npass: int, dense: bool = False, defect: bool = False,
+                   monotone: bool = False):
+        """npass passes from the same start with the reference's bookkeeping
+        (a0 = lowest, bLow = top Ritz value, kohnShamEigenSolve.cc:483-498), or
+        with bLow never rising above its previous value (monotone)."""
         solver = ChebyshevSolverDevice(cheby_block_size=B, cheby_degree=m,
-                                       lanczos_steps=wl.lanczos_steps, mixed_dense=dense)
+                                       lanczos_steps=wl.lanczos_steps, mixed_dense=dense,
+                                       mixed_prec_defect_filter=defect)
         solver.set_lanczos_start(v0)
         X = X0.clone()
         lams, ress, phases = [], [], []
@@ -164,11 +174,14 @@ def main():
             lam, res, _ = solver.solve(op, X, is_first_filtering_call=(k == 0),
                                        use_mixed_prec_overall=mixed)
             a0, bLow = float(lam[0]), float(lam[-1])
+            if monotone and k > 0:
+                bLow = min(bLow, solver.bounds()[1])
             _, _, bUp = solver.bounds()
             solver.reinitSpectrumBounds(a0, bLow, bUp)
             lams.append(lam)
             ress.append(res)
-            phases.append(solver.last_timings_ms)
+            phases.append(dict(solver.last_timings_ms,
+                               filter_precision=solver.last_filter_precision()))
         torch.cuda.synchronize()
         return (time.perf_counter() - t) / npass * 1e3, lams, ress, phases
 
@@ -218,6 +231,35 @@ def main():
         torch.cuda.synchronize()
         return (time.perf_counter() - t) / npass * 1e3, lams, phases, n_core
 
+    # the mixed passes with the FP32 filter in defect form (X p(theta) + E)
+    df_ms, lam_df, res_df, ph_df = run_passes(True, args.steps, defect=True)
+    log(f"mixed, defect-form filter: {df_ms:.1f} ms/pass")
+
+    conv = None
+    if args.converge_passes > 0:
+        # the same start, P passes of FP64, of the plain FP32 filter and of its
+        # defect form: how far the mixed eigenvalues sit from the FP64 ones as
+        # the passes converge (all states, and the lowest 90%, the states an SCF
+        # occupies; the top ones sit next to the filter's cut)
+        P = args.converge_passes
+        mono = args.monotone_blow
+        _, lam_c64, res_c64, _ = run_passes(False, P, monotone=mono)
+        nlow = int(0.9 * N)
+        conv = {"passes": P, "lowest": nlow, "monotone_bLow": mono,
+                "max_residual_fp64_all": [float(r.max()) for r in res_c64],
+                "max_residual_fp64_lowest": [float(r[:nlow].max()) for r in res_c64]}
+        for name, defect in (("plain", False), ("defect", True)):
+            _, lam_c, res_c, _ = run_passes(True, P, defect=defect, monotone=mono)
+            conv[name] = {
+                "max_eig_diff_all_Ha": [float(np.abs(a - b).max()) for a, b in zip(lam_c, lam_c64)],
+                "max_eig_diff_lowest_Ha": [float(np.abs(a[:nlow] - b[:nlow]).max())
+                                           for a, b in zip(lam_c, lam_c64)],
+                "max_residual_lowest": [float(r[:nlow].max()) for r in res_c],
+                "max_residual_top": [float(r[nlow:].max()) for r in res_c]}
+            log(f"converge ({name}): {P} passes; last max|dlambda| all "
+                f"{conv[name]['max_eig_diff_all_Ha'][-1]:.2e}, lowest {nlow} "
+                f"{conv[name]['max_eig_diff_lowest_Ha'][-1]:.2e}")
+
     sp_ms, lam_sp, ph_sp, n_core = run_split_passes(args.steps)
     sp_err = [float(np.abs(a - b[n_core:]).max()) for a, b in zip(lam_sp, lam_mx)]
     log(f"mixed, spectrum split (top {N - n_core} kept): {sp_ms:.1f} ms/pass; "
@@ -257,6 +299,8 @@ def main():
                    "atoms": wl.n_atoms, "passes": args.steps, "parallelism": "single GPU"},
         "mixed_pass_ms": round(mixed_ms, 2),
         "mixed_dense_pass_ms": round(md_ms, 2),
+        "mixed_defect_pass_ms": round(df_ms, 2),
+        "phase_ms_mixed_defect_last": ph_df[-1],
         "phase_ms_mixed_dense_last": ph_md[-1],
         "spectrum_split_pass_ms": round(sp_ms, 2),
         "spectrum_split_kept": N - n_core,
@@ -283,6 +327,8 @@ def main():
                                                     for a, b in zip(lam_md, lam_64)]
         out["max_eig_err_mixed_vs_fp64_Ha"] = [float(np.abs(a - b).max())
                                               for a, b in zip(lam_mx, lam_64)]
+        out["max_eig_err_mixed_defect_vs_fp64_Ha"] = [float(np.abs(a - b).max())
+                                                     for a, b in zip(lam_df, lam_64)]
         out["max_residual_fp64"] = [float(r.max()) for r in res_64]
     out["roofline"] = {
         "kernel": f"fused FP32 HX on {W} columns (8 colour launches of tile variant "
@@ -293,13 +339,15 @@ def main():
         "unit": "TFLOP/s",
         "frac": round(F / hx32_ms / 1e9 / PEAK_TFLOPS["fp32"], 4),
         "traffic": measured_hx_traffic(args.workload, W, tune["fp32"]),
-        "traffic_source": f"profiles/hx_traffic_{args.workload}.json, FP32 tile variant "
+        "traffic_source": f"profiles/hx_traffic_{args.workload}_w{W}.json, FP32 tile variant "
                           f"{tune['fp32']} (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
         "algorithmic_bytes": hx_bytes(prob, W, 4),
         "fp64": {"achieved": round(F / hx64_ms / 1e9, 2), "peak": PEAK_TFLOPS["fp64"],
                  "frac": round(F / hx64_ms / 1e9 / PEAK_TFLOPS["fp64"], 4)},
     }
 
+    if conv is not None:
+        out["convergence"] = conv
     if not args.no_kpoints:
         out["kpoint_batch"] = kpoint_batch(args.kpoint_reps, log)
     if not args.no_cpu:

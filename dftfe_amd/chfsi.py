@@ -448,13 +448,19 @@ class ChebyshevSolverDevice:
                  reproducible: bool = False, filter_precision: int = L.FP32,
                  first_scf_degree_factor: float = 1.34, reuse_lanczos_upper: bool = False,
                  fp64_tail_degrees: int = 0, mixed_dense: bool = False,
-                 filter_columns: int = 0):
+                 filter_columns: int = 0, mixed_prec_switch_residual: float = 0.0,
+                 mixed_prec_defect_filter: bool = False):
         """mixed_dense: FP32 off-diagonal blocks in the overlap, CholGS rotation,
         X^T H X and Rayleigh-Ritz rotation when a pass runs with
         use_mixed_prec_overall (the reference's USE MIXED PREC CGS O / CGS SR /
         XTHX / RR_SR together).  filter_columns: columns one filter recurrence
         runs on, whole blocks of cheby_block_size (0: all, within a 16 GiB
-        scratch); every column's result is the same whatever the grouping."""
+        scratch); every column's result is the same whatever the grouping.
+        mixed_prec_switch_residual > 0: once a pass's residual norms over its
+        lowest 90% of states are all below it, later passes filter in FP64.
+        mixed_prec_defect_filter:
+        run the FP32 filter in defect form (X p(theta) + E, E in FP32), whose
+        rounding scales with the residual."""
         self.lib = L.load()
         p = L.Params()
         self.lib.chfsi_default_params(ct.byref(p))
@@ -467,6 +473,8 @@ class ChebyshevSolverDevice:
         p.reuse_lanczos_upper = int(reuse_lanczos_upper)
         p.fp64_tail_degrees = fp64_tail_degrees
         p.filter_columns = filter_columns
+        p.mixed_prec_switch_residual = mixed_prec_switch_residual
+        p.mixed_prec_defect_filter = int(mixed_prec_defect_filter)
         for f in ("mixed_prec_overlap", "mixed_prec_cgs_rotation", "mixed_prec_xthx",
                   "mixed_prec_rr_rotation", "mixed_prec_xthx_spectrum_split"):
             setattr(p, f, int(mixed_dense))
@@ -496,6 +504,12 @@ class ChebyshevSolverDevice:
         L.check(self.lib.chfsi_solver_get_bounds(self.handle, ct.byref(a), ct.byref(b),
                                                  ct.byref(c)))
         return a.value, b.value, c.value
+
+    def last_filter_precision(self) -> int:
+        """Filter precision of the last pass (L.FP64 once the residual switch fired)."""
+        p = ct.c_int32()
+        L.check(self.lib.chfsi_solver_last_filter_precision(self.handle, ct.byref(p)))
+        return p.value
 
     def set_lanczos_start(self, v0: Optional[torch.Tensor]) -> None:
         if v0 is None:

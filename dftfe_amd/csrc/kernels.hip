@@ -951,6 +951,60 @@ __global__ __launch_bounds__(256) void residual_partial_kernel(const double *HX,
     partial[(int64_t)blockIdx.x * B + v] = red[0][tv] + red[1][tv] + red[2][tv] + red[3][tv];
 }
 
+__global__ __launch_bounds__(256) void colstats_partial_kernel(const double *HX, int64_t ldH,
+                                                               const double *X, int64_t ldX,
+                                                               int64_t M, int B,
+                                                               double *partial, int nchunks) {
+  __shared__ double red[2][4][64];
+  const int tv = threadIdx.x & 63, tg = threadIdx.x >> 6;
+  const int v = blockIdx.y * 64 + tv;
+  const int64_t rows = (M + nchunks - 1) / nchunks;
+  const int64_t g0 = (int64_t)blockIdx.x * rows;
+  const int64_t g1 = g0 + rows < M ? g0 + rows : M;
+  double xh = 0.0, xx = 0.0;
+  if (v < B) {
+    for (int64_t g = g0 + tg; g < g1; g += 4) {
+      const double x = X[g * ldX + v];
+      xh += x * HX[g * ldH + v];
+      xx += x * x;
+    }
+  }
+  red[0][tg][tv] = xh;
+  red[1][tg][tv] = xx;
+  __syncthreads();
+  if (tg < 2 && v < B)
+    partial[((int64_t)blockIdx.x * 2 + tg) * B + v] =
+        red[tg][0][tv] + red[tg][1][tv] + red[tg][2][tv] + red[tg][3][tv];
+}
+
+__global__ __launch_bounds__(256) void defect_source_kernel(const double *HX, int64_t ldH,
+                                                            const double *X, int64_t ldX,
+                                                            const double *theta, double s,
+                                                            int64_t M, int B, float *R,
+                                                            float *E) {
+  const int64_t n = M * B;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t g = i / B;
+    const int v = (int)(i - g * B);
+    const double r = HX[g * ldH + v] - X[g * ldX + v] * theta[v];
+    R[i] = (float)r;
+    E[i] = (float)(s * r);
+  }
+}
+
+__global__ __launch_bounds__(256) void defect_finish_kernel(double *X, int64_t ldX,
+                                                            const double *D, const float *E,
+                                                            int64_t ldE, int64_t M, int B) {
+  const int64_t n = M * B;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t g = i / B;
+    const int v = (int)(i - g * B);
+    X[g * ldX + v] = fma(X[g * ldX + v], D[v], (double)E[g * ldE + v]);
+  }
+}
+
 __global__ __launch_bounds__(256) void dot_partial_kernel(const double *x, const double *y,
                                                           int64_t n, double *partial,
                                                           int nchunks) {
@@ -1628,6 +1682,31 @@ hipError_t launch_residual_partial(const double *HX, int64_t ldH, const double *
                                    int nchunks, hipStream_t st) {
   hipLaunchKernelGGL(residual_partial_kernel, dim3(nchunks, (B + 63) / 64), dim3(256), 0, st, HX,
                      ldH, X, ldX, lam, M, B, partial, nchunks);
+  return hipGetLastError();
+}
+
+hipError_t launch_colstats_partial(const double *HX, int64_t ldH, const double *X, int64_t ldX,
+                                   int64_t M, int B, double *partial, int nchunks,
+                                   hipStream_t st) {
+  hipLaunchKernelGGL(colstats_partial_kernel, dim3(nchunks, (B + 63) / 64), dim3(256), 0, st, HX,
+                     ldH, X, ldX, M, B, partial, nchunks);
+  return hipGetLastError();
+}
+
+hipError_t launch_defect_source(const double *HX, int64_t ldH, const double *X, int64_t ldX,
+                                const double *theta, double s, int64_t M, int B, float *R,
+                                float *E, hipStream_t st) {
+  if (M * B == 0) return hipSuccess;
+  hipLaunchKernelGGL(defect_source_kernel, dim3(grid_for(M * B, 256, 65536)), dim3(256), 0, st,
+                     HX, ldH, X, ldX, theta, s, M, B, R, E);
+  return hipGetLastError();
+}
+
+hipError_t launch_defect_finish(double *X, int64_t ldX, const double *D, const float *E,
+                                int64_t ldE, int64_t M, int B, hipStream_t st) {
+  if (M * B == 0) return hipSuccess;
+  hipLaunchKernelGGL(defect_finish_kernel, dim3(grid_for(M * B, 256, 65536)), dim3(256), 0, st, X,
+                     ldX, D, E, ldE, M, B);
   return hipGetLastError();
 }
 

@@ -45,6 +45,11 @@ struct HxLaunch {
   const T *W;              // n_proj x ldW
   int64_t ldW;
   int variant;             // tile variant (see hx_variants)
+  // defect source of the filter's defect form (nullable): first-touch rows
+  // add rcoef[v] * R[g, v] after the Chebyshev combine
+  const T *R;
+  int64_t ldR;
+  const T *rcoef;
 };
 
 // Arguments of the projection C_e^T (s_in * src)_e for all cells with slots.
@@ -106,6 +111,17 @@ hipError_t launch_axpby(int64_t n, T a, const T *x, T b, T *y, hipStream_t st);
 hipError_t launch_residual_partial(const double *HX, int64_t ldH, const double *X, int64_t ldX,
                                    const double *lam, int64_t M, int B, double *partial,
                                    int nchunks, hipStream_t st);
+// column sums of X*HX and X*X into partial[chunk][2][B] (Rayleigh quotients)
+hipError_t launch_colstats_partial(const double *HX, int64_t ldH, const double *X, int64_t ldX,
+                                   int64_t M, int B, double *partial, int nchunks, hipStream_t st);
+// defect source of the filter: r = HX - X*diag(theta) in FP64, R = (float) r,
+// E = (float)(s * r)
+hipError_t launch_defect_source(const double *HX, int64_t ldH, const double *X, int64_t ldX,
+                                const double *theta, double s, int64_t M, int B, float *R,
+                                float *E, hipStream_t st);
+// X[g, v] = X[g, v] * D[v] + (double) E[g, v]
+hipError_t launch_defect_finish(double *X, int64_t ldX, const double *D, const float *E,
+                                int64_t ldE, int64_t M, int B, hipStream_t st);
 // dot products x.y into partial[chunk]; then the host sums
 hipError_t launch_dot_partial(const double *x, const double *y, int64_t n, double *partial,
                               int nchunks, hipStream_t st);

@@ -414,8 +414,9 @@ struct OpData<float> {
 template <typename T>
 chfsi_status hx_fused(chfsi_operator *op, const T *src, int64_t ld_src, T *dst, int64_t ld_dst,
                       int B, T alpha, T beta, T gamma, int beta_mode, bool scaled,
-                      hipStream_t s, bool bf16) {
+                      hipStream_t s, bool bf16, const T *R, int64_t ldR, const T *rcoef) {
   CHFSI_ARG(!bf16 || sizeof(T) == 4, "the BF16 apply works on FP32 multivectors");
+  CHFSI_ARG(!R || (!bf16 && sizeof(T) == 4 && rcoef && ldR >= B), "bad defect source");
   if (!op->has_H) {
     set_error("cell matrices not set (call chfsi_operator_set_potential)");
     return CHFSI_ERR_STATE;
@@ -467,6 +468,9 @@ chfsi_status hx_fused(chfsi_operator *op, const T *src, int64_t ld_src, T *dst, 
   a.beta_mode = beta_mode;
   a.B = B;
   a.variant = sizeof(T) == 8 ? op->hx_variant64 : op->hx_variant32;
+  a.R = R;
+  a.ldR = ldR;
+  a.rcoef = rcoef;
   if (op->n_slots > 0) {
     a.nl_off = op->d_nl_off;
     a.nl_C = OpData<T>::nlC(op);
@@ -490,9 +494,11 @@ chfsi_status hx_fused(chfsi_operator *op, const T *src, int64_t ld_src, T *dst, 
 
 template chfsi_status hx_fused<double>(chfsi_operator *, const double *, int64_t, double *,
                                        int64_t, int, double, double, double, int, bool,
-                                       hipStream_t, bool);
+                                       hipStream_t, bool, const double *, int64_t,
+                                       const double *);
 template chfsi_status hx_fused<float>(chfsi_operator *, const float *, int64_t, float *, int64_t,
-                                      int, float, float, float, int, bool, hipStream_t, bool);
+                                      int, float, float, float, int, bool, hipStream_t, bool,
+                                      const float *, int64_t, const float *);
 
 }  // namespace chfsi
 

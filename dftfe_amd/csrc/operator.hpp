@@ -21,11 +21,13 @@ struct Scratch {
 
 // One fused apply  dst <- base + alpha * S H S src  over all colour classes
 // (see HxLaunch for base / beta_mode).  bf16 selects the BF16-input kernel
-// (T = float only).
+// (T = float only).  R (nullable, ld ldR) adds rcoef[v] * R[:, v] to every row
+// once (the defect form of the filter; FP32, not with bf16).
 template <typename T>
 chfsi_status hx_fused(chfsi_operator *op, const T *src, int64_t ld_src, T *dst, int64_t ld_dst,
                       int B, T alpha, T beta, T gamma, int beta_mode, bool scaled, hipStream_t s,
-                      bool bf16 = false);
+                      bool bf16 = false, const T *R = nullptr, int64_t ldR = 0,
+                      const T *rcoef = nullptr);
 
 }  // namespace chfsi
 

@@ -6,7 +6,16 @@
 
 #include "mfma.hpp"
 
+#include <type_traits>
+#include <utility>
+
 namespace chfsi {
+
+// Launch descriptions with a defect source (R, ldR, rcoef): the real HX.
+template <typename P, typename = void>
+struct HasDefect : std::false_type {};
+template <typename P>
+struct HasDefect<P, std::void_t<decltype(std::declval<P>().R)>> : std::true_type {};
 
 // Loads and stores through the global address space: the multivector blocks
 // are device memory, and pointers read from a batch table would otherwise be
@@ -41,11 +50,25 @@ __device__ __forceinline__ void hx_scatter(const P &a, int ne,
   constexpr int JBMAX = ELEMS / (4 * CBW) > 0 ? ELEMS / (4 * CBW) : 1;
   constexpr int JB = RBW < JBMAX ? RBW : JBMAX;
   const bool any_src = a.gamma != T(0);
+  bool any_r = false;
+  const T *Rp = nullptr;
+  int64_t ldR = 0;
+  T rc[CBW];
+  if constexpr (HasDefect<P>::value && sizeof(T) == 4) {
+    any_r = a.R != nullptr;
+    Rp = any_r ? a.R : a.dst;
+    ldR = a.ldR;
+#pragma unroll
+    for (int cb = 0; cb < CBW; ++cb) {
+      const int v = v0 + (wc * CBW + cb) * 16 + (lane & 15);
+      rc[cb] = any_r ? a.rcoef[v < a.B ? v : a.B - 1] : T(0);
+    }
+  }
 #pragma unroll
   for (int j0 = 0; j0 < RBW; j0 += JB) {
     T dv[JB][CBW][4], sv[JB][CBW][4];
     uint32_t gg[JB][4];
-    bool nsrc[JB][4];
+    bool nsrc[JB][4], fst[JB][4];
 #pragma unroll
     for (int jj = 0; jj < JB; ++jj) {
       const int rb = wr + WR * (j0 + jj);
@@ -55,7 +78,8 @@ __device__ __forceinline__ void hx_scatter(const P &a, int ne,
         const int rowc = (j0 + jj < RBW && rb < NRB && row < ne) ? row : 0;
         const uint32_t w = smap[rowc];
         gg[jj][r] = w & 0x7fffffffu;
-        nsrc[jj][r] = any_src && (w >> 31) != 0;
+        fst[jj][r] = (w >> 31) != 0;
+        nsrc[jj][r] = any_src && fst[jj][r];
       }
     }
 #pragma unroll
@@ -90,6 +114,25 @@ __device__ __forceinline__ void hx_scatter(const P &a, int ne,
 #pragma unroll
           for (int r = 0; r < 4; ++r) sv[jj][cb][r] = T(0);
     }
+    if constexpr (HasDefect<P>::value && sizeof(T) == 4) {
+      if (any_r) {
+        // defect form: first-touch rows also take rcoef * R, folded into the
+        // source term here (sv becomes gamma*src + rcoef*R); the other rows
+        // read element 0 of the block, as for the source
+#pragma unroll
+        for (int jj = 0; jj < JB; ++jj)
+#pragma unroll
+          for (int cb = 0; cb < CBW; ++cb) {
+            const int v = v0 + (wc * CBW + cb) * 16 + (lane & 15);
+            const int vc = v < a.B ? v : a.B - 1;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const T x = gload(Rp, fst[jj][r] ? (int64_t)gg[jj][r] * ldR + vc : 0);
+              sv[jj][cb][r] = fmadd(rc[cb], fst[jj][r] ? x : T(0), a.gamma * sv[jj][cb][r]);
+            }
+          }
+      }
+    }
 #pragma unroll
     for (int jj = 0; jj < JB; ++jj) {
       const int j = j0 + jj;
@@ -107,7 +150,10 @@ __device__ __forceinline__ void hx_scatter(const P &a, int ne,
           const int v = v0 + (wc * CBW + cb) * 16 + (lane & 15);
           const T d = dv[jj][cb][r];
           const T keep = a.beta_mode == 0 ? a.beta * d : (a.beta_mode == 2 ? (a.beta * so) * d : T(0));
-          const T base = first ? fmadd(a.gamma, sv[jj][cb][r], keep) : d;
+          T base = first ? fmadd(a.gamma, sv[jj][cb][r], keep) : d;
+          if constexpr (HasDefect<P>::value && sizeof(T) == 4) {
+            if (any_r) base = first ? sv[jj][cb][r] + keep : d;
+          }
           if (rok && v < a.B)
             gstore(a.dst, (int64_t)gg[jj][r] * a.ld_dst + v, fmadd(a.alpha * so, acc[j][cb][r], base));
         }

@@ -168,6 +168,86 @@ int filter_group_columns(const chfsi_params &P, int64_t M, int N, int B) {
   return (int)std::min<int64_t>(N, fit * B);
 }
 
+// Defect form of the FP32 filter on the B-column block X (FP64, ld):
+//   theta_v = <x_v, H~x_v> / <x_v, x_v>,  R = H~X - X diag(theta)  (one FP64 apply),
+//   Y_k = X diag(D_k) + E_k  with  D_1 = (sigma/e)(theta - c),  E_0 = 0,  E_1 = (sigma/e) R,
+//   D_k+1 = alpha1 (theta - c) D_k + alpha2 D_k-1,
+//   E_k+1 = alpha1 (H~ - c) E_k + alpha2 E_k-1 + alpha1 D_k R,
+// the scalars of the plain recurrence (cheb_steps).  E runs in FP32 through the
+// fused HX (the R term enters its first-touch combine); the rounding then
+// scales with |R| instead of |X|, so near convergence the filtered block keeps
+// the FP64 eigenvectors' accuracy.  Equal to the plain filter in exact
+// arithmetic.
+chfsi_status filter_block_defect(chfsi_solver *s, chfsi_operator *op, double *X, int64_t ld,
+                                 int B, int m, double a, double b, double a0, hipStream_t st) {
+  const int64_t M = op->M;
+  const ChebState cs = cheb_init(a, b, a0);
+  CHFSI_TRY(s->HXb.ensure(sizeof(double) * M * B));
+  double *HX = sptr<double>(s->HXb);
+  CHFSI_TRY(hx_fused<double>(op, X, ld, HX, B, B, 1.0, 0.0, 0.0, 1, true, st));
+  if (op->nc) CHFSI_HIP(launch_zero_rows(op->d_c_rows, op->nc, HX, B, B, st));
+  const int nch = 256;
+  CHFSI_TRY(s->part.ensure(sizeof(double) * nch * 2 * B));
+  double *part = sptr<double>(s->part);
+  CHFSI_HIP(launch_colstats_partial(HX, B, X, ld, M, B, part, nch, st));
+  std::vector<double> h(size_t(nch) * 2 * B);
+  CHFSI_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+  CHFSI_HIP(hipStreamSynchronize(st));
+  // host tables: theta (B), D_m (B), then the FP32 coefficient rows
+  // alpha1 * D_(d-1) of degrees d = 2..m
+  std::vector<double> th(2 * size_t(B));
+  for (int v = 0; v < B; ++v) {
+    double xh = 0, xx = 0;
+    for (int k = 0; k < nch; ++k) {
+      xh += h[(size_t(k) * 2) * B + v];
+      xx += h[(size_t(k) * 2 + 1) * B + v];
+    }
+    th[v] = xx > 0 ? xh / xx : 0.0;
+  }
+  std::vector<float> rc(size_t(m + 1) * B, 0.0f);
+  std::vector<double> Dp(B, 1.0), D(B), Dn(B);
+  for (int v = 0; v < B; ++v) D[v] = cs.sigma / cs.e * (th[v] - cs.c);
+  double sigma = cs.sigma;
+  for (int degree = 2; degree <= m; ++degree) {
+    const double sigma2 = 1.0 / (cs.gamma - sigma);
+    const double alpha1 = 2.0 * sigma2 / cs.e, alpha2 = -(sigma * sigma2);
+    for (int v = 0; v < B; ++v) {
+      rc[size_t(degree) * B + v] = (float)(alpha1 * D[v]);
+      Dn[v] = alpha1 * (th[v] - cs.c) * D[v] + alpha2 * Dp[v];
+    }
+    std::swap(Dp, D);
+    std::swap(D, Dn);
+    sigma = sigma2;
+  }
+  for (int v = 0; v < B; ++v) th[B + v] = D[v];
+  CHFSI_TRY(s->dcoef.ensure(sizeof(double) * 2 * B + sizeof(float) * rc.size()));
+  double *dth = sptr<double>(s->dcoef), *dD = dth + B;
+  float *drc = reinterpret_cast<float *>(dth + 2 * B);
+  CHFSI_HIP(hipMemcpyAsync(dth, th.data(), sizeof(double) * th.size(), hipMemcpyHostToDevice, st));
+  CHFSI_HIP(hipMemcpyAsync(drc, rc.data(), sizeof(float) * rc.size(), hipMemcpyHostToDevice, st));
+
+  CHFSI_TRY(s->X32.ensure(sizeof(float) * M * B));
+  CHFSI_TRY(s->Y32.ensure(sizeof(float) * M * B));
+  CHFSI_TRY(s->R32.ensure(sizeof(float) * M * B));
+  float *Ep = sptr<float>(s->X32), *Ec = sptr<float>(s->Y32), *R = sptr<float>(s->R32);
+  CHFSI_HIP(hipMemsetAsync(Ep, 0, sizeof(float) * M * B, st));
+  CHFSI_HIP(launch_defect_source(HX, B, X, ld, dth, cs.sigma / cs.e, M, B, R, Ec, st));
+  sigma = cs.sigma;
+  for (int degree = 2; degree <= m; ++degree) {
+    const double sigma2 = 1.0 / (cs.gamma - sigma);
+    const double alpha1 = 2.0 * sigma2 / cs.e, alpha2 = -(sigma * sigma2);
+    CHFSI_TRY(hx_fused<float>(op, Ec, B, Ep, B, B, float(alpha1), float(alpha2),
+                              float(-cs.c * alpha1), 0, true, st, false, R, B,
+                              drc + size_t(degree) * B));
+    std::swap(Ec, Ep);
+    sigma = sigma2;
+  }
+  CHFSI_HIP(launch_defect_finish(X, ld, dD, Ec, B, M, B, st));
+  // the host tables are read by the copies above: wait before they go
+  CHFSI_HIP(hipStreamSynchronize(st));
+  return CHFSI_OK;
+}
+
 // In-place filter of the B-column block X (leading dimension ld).  FP32 and
 // BF16 forms cast the block to FP32 once; with tail > 0 the last `tail`
 // degrees run in FP64 from the cast-back pair of iterates, so that the
@@ -193,6 +273,8 @@ chfsi_status filter_block(chfsi_solver *s, chfsi_operator *op, double *X, int64_
     return CHFSI_ERR_ARG;
   }
   const bool bf16 = precision == CHFSI_BF16;
+  if (!bf16 && s->params.mixed_prec_defect_filter)
+    return filter_block_defect(s, op, X, ld, B, m, a, b, a0, st);
   tail = std::max(0, std::min(tail, m - 1));
   const int mlow = m - tail;
   CHFSI_TRY(s->X32.ensure(sizeof(float) * M * B));
@@ -634,6 +716,8 @@ extern "C" void chfsi_default_params(chfsi_params *p) {
   p->mixed_prec_rr_rotation = 0;
   p->mixed_prec_xthx_spectrum_split = 0;
   p->filter_columns = 0;
+  p->mixed_prec_switch_residual = 0.0;
+  p->mixed_prec_defect_filter = 0;
 }
 
 extern "C" chfsi_status chfsi_solver_set_params(chfsi_solver *s, const chfsi_params *params) {
@@ -674,6 +758,13 @@ extern "C" chfsi_status chfsi_solver_get_bounds(const chfsi_solver *s, double *a
   *a0 = s->a0;
   *bLow = s->bLow;
   *bUp = s->bUp;
+  return CHFSI_OK;
+}
+
+extern "C" chfsi_status chfsi_solver_last_filter_precision(const chfsi_solver *s,
+                                                           int32_t *precision) {
+  CHFSI_ARG(s && precision, "null argument");
+  *precision = s->last_precision;
   return CHFSI_OK;
 }
 
@@ -905,7 +996,11 @@ chfsi_status lanczos_scale_filter(chfsi_solver *s, chfsi_operator *op, double *X
 
   CHFSI_HIP(launch_scale_rows<double>(X, N, M, N, 1.0, op->d_sqrt_mass, st));
   const int B = std::min<int>(P.cheby_block_size > 0 ? P.cheby_block_size : N, N);
-  const int prec = use_mixed_prec_overall ? P.filter_precision : CHFSI_FP64;
+  if (is_first_filtering_call) s->last_res_lowest = 1e300;
+  const bool switched =
+      P.mixed_prec_switch_residual > 0 && s->last_res_lowest < P.mixed_prec_switch_residual;
+  const int prec = use_mixed_prec_overall && !switched ? P.filter_precision : CHFSI_FP64;
+  s->last_precision = prec;
   const int G = filter_group_columns(P, M, N, B);
   for (int j = 0; j < N; j += G) {
     const int bw = std::min(G, N - j);
@@ -971,7 +1066,11 @@ extern "C" chfsi_status chfsi_solve(chfsi_solver *s, chfsi_operator *op, double 
                     mx && P.mixed_prec_rr_rotation, st));
   CHFSI_HIP(hipEventRecord(pe.ev[4], st));
 
-  if (compute_residual) CHFSI_TRY(residual_impl(s, op, X, N, M, N, B, eig_host, res_host, st));
+  if (compute_residual) {
+    CHFSI_TRY(residual_impl(s, op, X, N, M, N, B, eig_host, res_host, st));
+    const int nlow = std::max(1, (9 * N) / 10);
+    s->last_res_lowest = *std::max_element(res_host, res_host + nlow);
+  }
 
   // back to the FE basis
   CHFSI_HIP(launch_scale_rows<double>(X, N, M, N, 1.0, op->d_inv_sqrt_mass, st));

@@ -26,11 +26,17 @@ struct chfsi_solver {
   // mixed-precision dense steps: FP32 copies of X, of an HX block and of the
   // N x N factor, FP32 split-K partials and rotation panel
   chfsi::Scratch Xf, HXf, Qf, split32, rot32;
+  // defect form of the FP32 filter: FP32 defect source, coefficient tables
+  chfsi::Scratch R32, dcoef;
   bool have_v0 = false;
+  // largest residual norm over the lowest 90% of states of the last pass that
+  // computed residuals, and the filter precision the last pass used
+  double last_res_lowest = 1e300;
+  int last_precision = CHFSI_FP64;
   ~chfsi_solver() {
     if (blas) rocblas_destroy_handle(blas);
     for (chfsi::Scratch *s : {&Y64, &X32, &Y32, &HXb, &Hp, &rot, &evals, &info, &part, &lam, &v0,
-                              &split, &Xf, &HXf, &Qf, &split32, &rot32})
+                              &split, &Xf, &HXf, &Qf, &split32, &rot32, &R32, &dcoef})
       s->release();
   }
 };

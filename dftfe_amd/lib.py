@@ -67,6 +67,8 @@ class Params(ct.Structure):
         ("mixed_prec_xthx", ct.c_int32), ("mixed_prec_rr_rotation", ct.c_int32),
         ("mixed_prec_xthx_spectrum_split", ct.c_int32),
         ("filter_columns", ct.c_int32),
+        ("mixed_prec_switch_residual", ct.c_double),
+        ("mixed_prec_defect_filter", ct.c_int32),
     ]
 
 
@@ -121,6 +123,7 @@ SIGNATURES = {
     "chfsi_solver_set_params": (ct.c_int, [_VP, ct.POINTER(Params)]),
     "chfsi_solver_reinit_bounds": (ct.c_int, [_VP, ct.c_double, ct.c_double, ct.c_double]),
     "chfsi_solver_get_bounds": (ct.c_int, [_VP, c_double_p, c_double_p, c_double_p]),
+    "chfsi_solver_last_filter_precision": (ct.c_int, [_VP, c_int32_p]),
     "chfsi_solver_set_lanczos_start": (ct.c_int, [_VP, _VP, ct.c_int64]),
     "chfsi_solve": (ct.c_int, [_VP, _VP, _VP, ct.c_int64, ct.c_int32, c_double_p, c_double_p,
                                ct.c_int32, ct.c_int32, ct.c_int32, ct.c_int32, c_double_p,

@@ -149,6 +149,20 @@ typedef struct {
                                       filter scratch).  No reference counterpart: the
                                       reference filters one block at a time to bound
                                       memory; every column's arithmetic is the same */
+  double mixed_prec_switch_residual; /* > 0: once a pass's residual norms over its
+                                      lowest 90% of states are all below this, later
+                                      passes filter in FP64 even with
+                                      use_mixed_prec_overall (a per-pass form of the
+                                      reference's useMixedPrecOverall = !scfConverged,
+                                      dft.cc); 0: never.  Reset by a first filtering
+                                      call */
+  int32_t mixed_prec_defect_filter; /* with the FP32 filter: run it in defect form,
+                                      Y = X diag(p(theta)) + E with theta the
+                                      columns' Rayleigh quotients and R = H~X - X
+                                      theta from one FP64 apply, E filtered in FP32
+                                      (same recurrence plus the source p_k(theta) R),
+                                      so the FP32 rounding scales with the residual.
+                                      No reference counterpart; 0: plain FP32 */
 } chfsi_params;
 
 const char *chfsi_last_error(void);
@@ -297,6 +311,9 @@ chfsi_status chfsi_solver_set_params(chfsi_solver *solver, const chfsi_params *p
 /* reinitSpectrumBounds (…SolverDevice.h:121-124) */
 chfsi_status chfsi_solver_reinit_bounds(chfsi_solver *solver, double a0, double bLow, double bUp);
 chfsi_status chfsi_solver_get_bounds(const chfsi_solver *solver, double *a0, double *bLow, double *bUp);
+/* Filter precision (chfsi_precision) of the solver's last pass.  No reference
+ * counterpart: reports the mixed_prec_switch_residual decision. */
+chfsi_status chfsi_solver_last_filter_precision(const chfsi_solver *solver, int32_t *precision);
 /* Supplies the Lanczos start vector used on a first filtering call (device,
  * length M); without it a seeded uniform [0,1) vector is generated. */
 chfsi_status chfsi_solver_set_lanczos_start(chfsi_solver *solver, const double *v0_dev,

@@ -299,6 +299,46 @@ def chebyshev_filter(apply_hx, X: np.ndarray, m: int, a: float, b: float, a0: fl
     return Y
 
 
+def chebyshev_filter_defect(apply_hx, X: np.ndarray, m: int, a: float, b: float, a0: float,
+                            dtype=np.float64) -> np.ndarray:
+    """The filter of chebyshev_filter written as Y_k = X diag(D_k) + E_k (no
+    reference counterpart: the device's defect form of the low-precision
+    filter).  theta are the columns' Rayleigh quotients and R = H~X - X theta
+    (both FP64); D_k = p_k(theta) follows the scalar recurrence and E_k the same
+    three-term recurrence in `dtype` with the source term alpha1 D_k R:
+        D_1 = (sigma/e)(theta - c),  E_1 = (sigma/e) R,
+        D_k+1 = alpha1 (theta - c) D_k + alpha2 D_k-1,
+        E_k+1 = alpha1 (H~ - c) E_k + alpha2 E_k-1 + alpha1 D_k R.
+    Equal to chebyshev_filter in exact arithmetic; with dtype float32 the
+    rounding of E scales with |R|, not with |X|."""
+    e = (b - a) / 2.0
+    c = (b + a) / 2.0
+    sigma = e / (a0 - c)
+    gamma = 2.0 / sigma
+    HX = apply_hx(X, False, 1.0, np.zeros_like(X))
+    theta = (X * HX).sum(axis=0) / (X * X).sum(axis=0)
+    R = HX - X * theta[None, :]
+    Rd = R.astype(dtype)
+    D_prev, D = np.ones(X.shape[1]), (sigma / e) * (theta - c)
+    E_prev = np.zeros(X.shape, dtype=dtype)
+    E = (dtype(sigma / e) * Rd).astype(dtype)
+
+    def hx_low(src, scalar, dst):
+        # H~ applied in FP64 to the stored values, result rounded to dtype
+        return apply_hx(src.astype(np.float64), True, scalar, dst.astype(np.float64)).astype(dtype)
+
+    for _degree in range(2, m + 1):
+        sigma2 = 1.0 / (gamma - sigma)
+        alpha1, alpha2 = 2.0 * sigma2 / e, -(sigma * sigma2)
+        En = (dtype(alpha2) * E_prev + dtype(-c * alpha1) * E
+              + (alpha1 * D).astype(dtype)[None, :] * Rd).astype(dtype)
+        En = hx_low(E, alpha1, En)
+        D_prev, D = D, alpha1 * (theta - c) * D + alpha2 * D_prev
+        E_prev, E = E, En
+        sigma = sigma2
+    return X * D[None, :] + E.astype(np.float64)
+
+
 def chebyshev_scalar(lam: np.ndarray, m: int, a: float, b: float, a0: float) -> np.ndarray:
     """Closed form of the filter on an eigen-component: T_m((l-c)/e) / T_m((a0-c)/e)."""
     e = (b - a) / 2.0

@@ -186,6 +186,67 @@ def test_chebyshev_filter_closed_form():
                / np.polynomial.chebyshev.chebval((a0 - (a + b) / 2) / ((b - a) / 2), [0] * 20 + [1])) < 1e-12
 
 
+def _dense_operator(n=60, lo=-0.07, hi=315.0, seed=5):
+    rng = np.random.default_rng(seed)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    lam = np.concatenate([np.linspace(lo, 0.93, 20), np.geomspace(1.2, hi, n - 20)])
+    A = (Q * lam) @ Q.T
+    A = 0.5 * (A + A.T)
+
+    def hx(src, flag, scalar, dst):
+        return dst + scalar * (A @ src)
+    return A, lam, Q, hx
+
+
+def test_chebyshev_filter_defect_equals_plain():
+    """The defect form X D_m + E_m is the same polynomial filter: in FP64 it
+    agrees with chebyshev_filter to rounding, for a random block and for the
+    eigenbasis (where E = 0 and D_m is the closed form)."""
+    A, lam, Q, hx = _dense_operator()
+    m, a, b, a0 = 30, 0.93, 315.0, -0.07
+    X = np.random.default_rng(1).standard_normal((60, 8))
+    Yp = O.chebyshev_filter(hx, X, m, a, b, a0)
+    Yd = O.chebyshev_filter_defect(hx, X, m, a, b, a0)
+    assert np.abs(Yd - Yp).max() / np.abs(Yp).max() < 1e-11
+    Yd = O.chebyshev_filter_defect(hx, Q[:, :8], m, a, b, a0)
+    want = Q[:, :8] * O.chebyshev_scalar(lam[:8], m, a, b, a0)[None, :]
+    assert np.abs(Yd - want).max() < 1e-10
+
+
+def test_chebyshev_filter_defect_fp32_rounding_follows_residual():
+    """Near converged vectors the FP32 defect form is far more accurate than
+    the FP32 plain recurrence (same operator, same rounding of every stored
+    iterate): its rounding scales with |R|, the plain one with |X|."""
+    A, lam, Q, hx = _dense_operator()
+    m, a, b, a0 = 30, 0.93, 315.0, -0.07
+    rng = np.random.default_rng(2)
+    X = Q[:, :16] + 1e-6 * rng.standard_normal((60, 16))
+    Y64 = O.chebyshev_filter(hx, X, m, a, b, a0)
+
+    def plain32(X):
+        f = np.float32
+        e, c = (b - a) / 2, (b + a) / 2
+        sigma = e / (a0 - c)
+        gamma = 2 / sigma
+        Xa = X.astype(f)
+        Y = (f(sigma / e) * (hx(Xa.astype(float), False, 1.0, np.zeros(X.shape)).astype(f)
+                             + f(-c) * Xa)).astype(f)
+        for _ in range(2, m + 1):
+            s2 = 1 / (gamma - sigma)
+            a1, a2 = 2 * s2 / e, -(sigma * s2)
+            Xa = (f(a2) * Xa + f(-c * a1) * Y).astype(f)
+            Xa = hx(Y.astype(float), True, a1, Xa.astype(float)).astype(f)
+            Xa, Y = Y, Xa
+            sigma = s2
+        return Y.astype(float)
+
+    err_plain = np.abs(plain32(X) - Y64).max() / np.abs(Y64).max()
+    err_def = np.abs(O.chebyshev_filter_defect(hx, X, m, a, b, a0, dtype=np.float32) - Y64).max() \
+        / np.abs(Y64).max()
+    assert err_plain > 1e-8
+    assert err_def < err_plain / 100, (err_def, err_plain)
+
+
 # ---------------------------------------------------------------- dense steps
 def test_cholgs_orthonormal_and_span(brute):
     prob, _, _ = brute
