@@ -44,22 +44,24 @@ def hx_bytes(prob, B, s):
     return prob.M * B * s + prob.C * prob.ne ** 2 * s + 2 * prob.M * B * s + prob.C * prob.ne * 4
 
 
-def measured_hx_traffic(workload, width, variant, precision="fp32"):
+def measured_hx_traffic(workload, width, variant, proj_variant, precision="fp32"):
     """HBM bytes of one fused HX on `width` columns (8 colour launches of the
-    given tile variant + projection + reduction) from the rocprofv3 FETCH_SIZE /
-    WRITE_SIZE table committed under profiles/ for this workload and width;
-    None when no measurement covers the configuration."""
+    given tile variant + the given projection variant + reduction) from the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE table committed under profiles/ for this
+    workload and width; None when no measurement covers the configuration."""
     path = os.path.join(ROOT, "profiles", f"hx_traffic_{workload}_w{width}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         t = json.load(f)[precision]
     tile = t["tile_variants"].get(str(variant))
-    if tile is None:
+    proj = t.get("projection_variants", {}).get(str(proj_variant))
+    if tile is None or proj is None:
         return None
     n = t["colour_launches_per_hx"]
-    return int(n * (tile["fetch_bytes"] + tile["write_bytes"])
-               + sum(t[k]["fetch_bytes"] + t[k]["write_bytes"] for k in ("projection", "reduction")))
+    return int(n * (tile["fetch_bytes"] + tile["write_bytes"]) + proj["fetch_bytes"]
+               + proj["write_bytes"] + t["reduction"]["fetch_bytes"]
+               + t["reduction"]["write_bytes"])
 
 
 def pass_flops(prob, N, B, m, lanczos_steps):
@@ -330,17 +332,20 @@ def main():
         out["max_eig_err_mixed_defect_vs_fp64_Ha"] = [float(np.abs(a - b).max())
                                                      for a, b in zip(lam_df, lam_64)]
         out["max_residual_fp64"] = [float(r.max()) for r in res_64]
+    pm = tune.get("fp32_proj_ms") or [0.0]
+    proj32 = int(np.argmin(pm))  # the projection variant the autotuner kept
     out["roofline"] = {
         "kernel": f"fused FP32 HX on {W} columns (8 colour launches of tile variant "
-                  f"{tune['fp32']} + projection)",
+                  f"{tune['fp32']} + projection variant {proj32} + reduction)",
         "bound": "mfma",
         "achieved": round(F / hx32_ms / 1e9, 2),
         "peak": PEAK_TFLOPS["fp32"],
         "unit": "TFLOP/s",
         "frac": round(F / hx32_ms / 1e9 / PEAK_TFLOPS["fp32"], 4),
-        "traffic": measured_hx_traffic(args.workload, W, tune["fp32"]),
+        "traffic": measured_hx_traffic(args.workload, W, tune["fp32"], proj32),
         "traffic_source": f"profiles/hx_traffic_{args.workload}_w{W}.json, FP32 tile variant "
-                          f"{tune['fp32']} (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
+                          f"{tune['fp32']}, projection variant {proj32} "
+                          f"(rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
         "algorithmic_bytes": hx_bytes(prob, W, 4),
         "fp64": {"achieved": round(F / hx64_ms / 1e9, 2), "peak": PEAK_TFLOPS["fp64"],
                  "frac": round(F / hx64_ms / 1e9 / PEAK_TFLOPS["fp64"], 4)},

@@ -807,7 +807,18 @@ __global__ void proj_reduce_kernel(const int *proj_ptr, const int *proj_slots,
   const int p = (int)(idx / B);
   const int v = (int)(idx % B);
   T s = T(0);
-  for (int j = proj_ptr[p]; j < proj_ptr[p + 1]; ++j) s += partial[(int64_t)proj_slots[j] * ldP + v];
+  int j = proj_ptr[p];
+  const int j1 = proj_ptr[p + 1];
+  // eight loads in flight, then the adds in slot order (the sum is the same
+  // sequence of roundings as one add per load)
+  for (; j + 8 <= j1; j += 8) {
+    T x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) x[u] = partial[(int64_t)proj_slots[j + u] * ldP + v];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += x[u];
+  }
+  for (; j < j1; ++j) s += partial[(int64_t)proj_slots[j] * ldP + v];
   W[(int64_t)p * ldW + v] = T(proj_V[p]) * s;
 }
 
@@ -1400,7 +1411,7 @@ __global__ __launch_bounds__(64 * NW) void projp_kernel(ProjLaunch<float> a) {
   constexpr int SB = AB + NX * XG;
   constexpr int D = (MAXSRB + NX + NW - 1) / NW;
   static_assert(CBW >= 1 && TN % (16 * NW) == 0, "bad column split");
-  static_assert(TN == 64 || TN == 128, "gathered-slice layout needs TN 64 or 128");
+  static_assert(TN == 64 || TN == 128 || TN == 256, "gathered-slice layout needs TN 64..256");
   static_assert(S >= 2 && (S - 2) * D < 64, "ring depth");
 
   __shared__ __align__(16) unsigned char ring[S * SB];
@@ -1555,7 +1566,8 @@ static hipError_t proj_go(ProjLaunch<T> a, hipStream_t s) {
 // accumulators for 128 slots; 1: same tile, two node tiles per stage,
 // accumulators sized to the largest cell; 2: 2 waves x half-width tile, two
 // node tiles per stage, sized accumulators; 3 (FP32; FP64 and unaligned
-// blocks take 0): 8 waves x 128 columns, direct-to-LDS ring of 3 slices.
+// blocks take 0): 8 waves x 256 columns (128 for blocks of at most 256),
+// direct-to-LDS ring of 3 slices.
 template <typename T, int NRB, int MAXSRB>
 static hipError_t proj_variant(const ProjLaunch<T> &a, hipStream_t s) {
   constexpr int TNW = sizeof(T) == 4 ? 128 : 64;
@@ -1564,7 +1576,8 @@ static hipError_t proj_variant(const ProjLaunch<T> &a, hipStream_t s) {
     auto al = [](const void *p) { return ((uintptr_t)p & 15) == 0; };
     if (a.variant == 3 && a.B >= 4 && a.B % 4 == 0 && a.ld_src % 4 == 0 && al(a.src) &&
         al(a.nl_Ct))
-      return projp_go<8, 128, MAXSRB, NRB, 3>(a, s);
+      return a.B > 256 ? projp_go<8, 256, MAXSRB, NRB, 3>(a, s)
+                       : projp_go<8, 128, MAXSRB, NRB, 3>(a, s);
   }
   switch (a.variant) {
     case 1: return proj_go<T, 4, TNW, MAXSRB, NRB, ST>(a, s);

@@ -111,11 +111,12 @@ def test_hx_ring_variants_bitwise(small, variant, B):
     assert torch.equal(out[2], out[variant])
 
 
-@pytest.mark.parametrize("B", [72, 256, 70])
+@pytest.mark.parametrize("B", [72, 256, 70, 520])
 def test_projection_ring_bitwise(small, B):
     """The direct-to-LDS FP32 projection (variant 3) forms the same products in
     the same order as variant 0, so a degree-4 FP32 filter gives bitwise-equal
-    blocks; B = 70 (not a multiple of 4) takes variant 0 itself."""
+    blocks; B = 70 (not a multiple of 4) takes variant 0 itself, B = 520 the
+    256-column tiles with a partial last tile."""
     from dftfe_amd import lib as L
     from dftfe_amd.chfsi import chebyshev_filter
 
