@@ -1283,6 +1283,142 @@ static hipError_t hxp_go(HxLaunch<float> a, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Single-column HX (the Lanczos matrix-vector products).  With one column
+// the cell matrices stream once and bound the apply; the tile kernels would
+// use one column of each 64-wide MFMA tile.  One 4-wave workgroup per cell:
+// per K slice, the 16 values of a row are LPR lanes' 16-byte loads, so a wave
+// load covers RPW rows (1 KB contiguous) and the workgroup RPI rows; the
+// partial sums of a row are reduced over its LPR lanes at the end.  Gather
+// tables, nonlocal slices (rows of W) and the first-touch combine are those
+// of hx_kernel.
+template <typename T, int NRB>
+__global__ __launch_bounds__(256) void hx1_kernel(HxLaunch<T> a) {
+  constexpr int ROWS = NRB * 16;
+  constexpr int KMAX = ROWS + 128;
+  constexpr int EPL = 16 / sizeof(T);  // elements per 16-byte load
+  constexpr int LPR = 16 / EPL;        // lanes per row
+  constexpr int RPW = 64 / LPR;        // rows per wave load
+  constexpr int RPI = 4 * RPW;         // rows per workgroup load
+  constexpr int NG = (ROWS + RPI - 1) / RPI;
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  using L16 = typename std::conditional<sizeof(T) == 8, d2, f32x4>::type;
+  __shared__ uint32_t smap[ROWS];
+  __shared__ T sout[ROWS];
+  __shared__ T xs[KMAX];
+
+  const int wid = blockIdx.x;
+  const int local = wid >> 3;
+  if (local >= a.per_xcd) return;
+  const int item = (wid & 7) * a.per_xcd + local;
+  if (item >= a.ncells) return;
+  const int cell = a.cells[item];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  int nl_begin = 0, nslot = 0;
+  if (a.nl_off) {
+    nl_begin = a.nl_off[cell];
+    nslot = a.nl_off[cell + 1] - nl_begin;
+  }
+  const int ntot = NRB + (nslot >> 4);
+  const uint32_t *cmap = a.map + (int64_t)cell * a.ne;
+  const T *Hcell = a.Ht + (int64_t)cell * a.ne_pad * a.ne_pad;
+  const T *NLcell = a.nl_C ? a.nl_C + (int64_t)nl_begin * a.ne_pad : Hcell;
+
+  for (int i = tid; i < KMAX; i += 256) {
+    T x = T(0);
+    if (i < ROWS) {
+      uint32_t w = 0;
+      T so = T(1);
+      if (i < a.ne) {
+        w = cmap[i];
+        const uint32_t r = w & 0x7fffffffu;
+        x = (a.s_in ? a.s_in[r] : T(1)) * a.src[(int64_t)r * a.ld_src];
+        if (a.s_out) so = a.s_out[r];
+      }
+      smap[i] = w;
+      sout[i] = so;
+    } else {
+      const int sl = i - ROWS;
+      const int pj = sl < nslot ? a.slot_proj[nl_begin + sl] : -1;
+      if (pj >= 0) x = a.W[(int64_t)pj * a.ldW];
+    }
+    xs[i] = x;
+  }
+  __syncthreads();
+
+  const int lr = lane / LPR, lk = lane % LPR;
+  T acc[NG];
+#pragma unroll
+  for (int g = 0; g < NG; ++g) acc[g] = T(0);
+  for (int t = 0; t < ntot; ++t) {
+    const T *tile = t < NRB ? Hcell + (int64_t)t * ROWS * 16 : NLcell + (int64_t)(t - NRB) * ROWS * 16;
+    T xk[EPL];
+#pragma unroll
+    for (int e = 0; e < EPL; ++e) xk[e] = xs[t * 16 + lk * EPL + e];
+    L16 v[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+      const int row = g * RPI + wave * RPW + lr;
+      const int rc = row < ROWS ? row : 0;
+      v[g] = *reinterpret_cast<const L16 *>(tile + (int64_t)rc * 16 + lk * EPL);
+    }
+    // one common first use of every load of the slice, so that all of them
+    // are issued before the first wait (left alone, the compiler sinks each
+    // load to its own use and keeps one or two in flight)
+    static_assert(NG <= 11, "row groups per thread");
+    asm volatile("" ::"v"(v[0]), "v"(v[NG > 1 ? 1 : 0]), "v"(v[NG > 2 ? 2 : 0]),
+                 "v"(v[NG > 3 ? 3 : 0]), "v"(v[NG > 4 ? 4 : 0]), "v"(v[NG > 5 ? 5 : 0]),
+                 "v"(v[NG > 6 ? 6 : 0]), "v"(v[NG > 7 ? 7 : 0]), "v"(v[NG > 8 ? 8 : 0]),
+                 "v"(v[NG > 9 ? 9 : 0]), "v"(v[NG > 10 ? 10 : 0]));
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+      const T *e = reinterpret_cast<const T *>(&v[g]);
+#pragma unroll
+      for (int q = 0; q < EPL; ++q) acc[g] = fmadd(e[q], xk[q], acc[g]);
+    }
+  }
+#pragma unroll
+  for (int g = 0; g < NG; ++g)
+#pragma unroll
+    for (int o = 1; o < LPR; o <<= 1) acc[g] += __shfl_xor(acc[g], o);
+  if (lk != 0) return;
+#pragma unroll
+  for (int g = 0; g < NG; ++g) {
+    const int row = g * RPI + wave * RPW + lr;
+    if (row >= a.ne) continue;
+    const uint32_t w = smap[row];
+    const int64_t gd = w & 0x7fffffffu;
+    const bool first = (w >> 31) != 0;
+    const T so = sout[row];
+    const T d = a.dst[gd * a.ld_dst];
+    const T keep = a.beta_mode == 0 ? a.beta * d : (a.beta_mode == 2 ? (a.beta * so) * d : T(0));
+    const T base = first ? fmadd(a.gamma, a.src[gd * a.ld_src], keep) : d;
+    a.dst[gd * a.ld_dst] = fmadd(a.alpha * so, acc[g], base);
+  }
+}
+
+template <typename T, int NRB>
+static hipError_t hx1_go(HxLaunch<T> a, hipStream_t s) {
+  a.ntiles = 1;
+  if (a.ncells == 0) return hipSuccess;
+  a.per_xcd = (a.ncells + 7) / 8;
+  hipLaunchKernelGGL((hx1_kernel<T, NRB>), dim3(8 * a.per_xcd), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t hx1_dispatch(const HxLaunch<T> &a, hipStream_t s) {
+  switch (a.ne_pad >> 4) {
+    case 1: return hx1_go<T, 1>(a, s);
+    case 2: return hx1_go<T, 2>(a, s);
+    case 4: return hx1_go<T, 4>(a, s);
+    case 8: return hx1_go<T, 8>(a, s);
+    case 14: return hx1_go<T, 14>(a, s);
+    case 22: return hx1_go<T, 22>(a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 // The direct-to-LDS copies move 16-byte units: B a multiple of 4 and 16-byte
 // aligned rows of src and W.  Other shapes take the register-staged kernel.
 static bool hxp_fits(const HxLaunch<float> &a) {
@@ -1324,6 +1460,7 @@ int hx_variants(bool fp64) { return fp64 ? 5 : 12; }
 
 template <>
 hipError_t launch_hx<float>(const HxLaunch<float> &a, hipStream_t s) {
+  if (a.B == 1 && !a.R) return hx1_dispatch<float>(a, s);
   switch (a.variant) {
     case 1: return hx_dispatch<float, 4, 64, 4>(a, s);
     case 2: return hx_dispatch<float, 8, 64, 4>(a, s);
@@ -1342,6 +1479,7 @@ hipError_t launch_hx<float>(const HxLaunch<float> &a, hipStream_t s) {
 
 template <>
 hipError_t launch_hx<double>(const HxLaunch<double> &a, hipStream_t s) {
+  if (a.B == 1) return hx1_dispatch<double>(a, s);
   switch (a.variant) {
     case 1: return hx_dispatch<double, 4, 32, 2>(a, s);
     case 2: return hx_dispatch<double, 16, 64, 4>(a, s);

@@ -257,11 +257,26 @@ def test_lanczos_bounds_match_oracle(small):
     assert (lo, up) == (olo, oup)
 
 
-def test_filter_fp64_matches_oracle(small):
+def test_hx_fp32_single_column_matches_oracle(small):
+    """One column takes the single-column kernel (cell matrices streamed as a
+    matrix-vector product) in FP32 too."""
+    prob, ref, op = small
+    X = prob.initial_guess(N=1, seed_offset=3)
+    Xd = torch.tensor(X, device="cuda", dtype=torch.float32)
+    Yd = torch.zeros_like(Xd)
+    op.HX(Xd, False, 1.0, Yd)
+    want = ref.apply(X.astype(np.float32).astype(np.float64))
+    assert _rel(Yd.double().cpu().numpy(), want) < 2e-6
+
+
+@pytest.mark.parametrize("N", [64, 1])
+def test_filter_fp64_matches_oracle(small, N):
+    """N = 1 runs the whole recurrence (scaled apply, first-touch Chebyshev
+    combine with the source term) through the single-column kernel."""
     from dftfe_amd.chfsi import chebyshev_filter
 
     prob, ref, op = small
-    X = prob.sqrt_mass[:, None] * prob.initial_guess()
+    X = prob.sqrt_mass[:, None] * prob.initial_guess(N=N)
     a, b, a0 = 5.0, 41.0, 0.0
     Xd = torch.tensor(X, device="cuda")
     chebyshev_filter(op, Xd, 20, a, b, a0)
