@@ -351,6 +351,7 @@ def main():
                  "frac": round(F / hx64_ms / 1e9 / PEAK_TFLOPS["fp64"], 4)},
     }
 
+    out["cell_rebuild"] = cell_rebuild(prob, op, log)
     if conv is not None:
         out["convergence"] = conv
     if not args.no_kpoints:
@@ -358,6 +359,46 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(prob, B, log)
     print(json.dumps(out), flush=True)
+
+
+def cell_rebuild(prob, op, log, reps=5):
+    """Per-SCF rebuild of every cell Hamiltonian on the device from V_eff JxW
+    (LDA) and from V_eff JxW plus the GGA vector (C x n_q x 3), timed with HIP
+    events on the stream the library launches on; the LDA matrices of the
+    workload are restored afterwards.  Flops: 2 n_e^2 n_q per cell (LDA),
+    4 n_e^2 n_q (GGA); FP64 on the vector units."""
+    import torch
+    from dftfe_amd.chfsi import _ptr, _stream
+
+    dev = torch.device("cuda:0")
+    v = torch.tensor(prob.vJxW, device=dev)
+    g = 0.01 * torch.randn(prob.C, prob.nqc, 3, dtype=torch.float64, device=dev) \
+        * torch.tensor(prob.JxW, device=dev)[None, :, None]
+    lib = op.lib
+    calls = {
+        "lda": lambda: lib.chfsi_operator_set_potential(op.handle, _ptr(v), _stream()),
+        "gga": lambda: lib.chfsi_operator_set_potential_gga(op.handle, _ptr(v), _ptr(g),
+                                                            _stream()),
+    }
+    res = {}
+    for name, call in calls.items():
+        for _ in range(2):
+            if call() != 0:
+                raise RuntimeError(f"cell rebuild ({name}) failed")
+        st = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            call()
+        e1.record(st)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        fl = (2 if name == "lda" else 4) * prob.C * prob.ne ** 2 * prob.nqc
+        res[f"{name}_ms"] = round(ms, 3)
+        res[f"{name}_fp64_tflops"] = round(fl / ms / 1e9, 2)
+    op.set_potential(prob.vJxW)
+    log(f"cell rebuild: {res}")
+    return res
 
 
 def kpoint_batch(reps, log):

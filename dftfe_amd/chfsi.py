@@ -61,6 +61,8 @@ class KohnShamOperatorDevice:
         d.inv_sqrt_mass = arr("isq", prob.inv_sqrt_mass, np.float64, ct.c_double)
         d.shape_values = arr("N", prob.Nq, np.float64, ct.c_double)
         d.stiffness = arr("K", prob.K_e, np.float64, ct.c_double)
+        if prob.Gq is not None:
+            d.shape_gradients = arr("Gq", prob.Gq, np.float64, ct.c_double)
         d.n_proj = prob.n_proj
         d.n_pairs = len(prob.pair_cell)
         d.proj_per_pair = 4
@@ -112,6 +114,19 @@ class KohnShamOperatorDevice:
         """computeHamiltonianMatricesAllkpt: cell matrices from V_eff JxW on device."""
         v = torch.as_tensor(_as(vJxW, np.float64), device=self.device)
         L.check(self.lib.chfsi_operator_set_potential(self.handle, _ptr(v), _stream()))
+        torch.cuda.current_stream().synchronize()
+
+    def set_potential_gga(self, vJxW: np.ndarray, gJxW: np.ndarray) -> None:
+        """hamMatrixKernelGGAMemOpt: cell matrices from V_eff JxW and the
+        (C, n_q, 3) GGA vector d(E_xc)/d(sigma) grad rho JxW, on device."""
+        p = self.prob
+        g = _as(gJxW, np.float64)
+        if g.shape != (p.C, p.nqc, 3):
+            raise ValueError(f"gJxW must have shape {(p.C, p.nqc, 3)}, got {g.shape}")
+        v = torch.as_tensor(_as(vJxW, np.float64), device=self.device)
+        gt = torch.as_tensor(g, device=self.device)
+        L.check(self.lib.chfsi_operator_set_potential_gga(self.handle, _ptr(v), _ptr(gt),
+                                                          _stream()))
         torch.cuda.current_stream().synchronize()
 
     def set_cell_matrices(self, H: torch.Tensor) -> None:

@@ -827,8 +827,12 @@ __global__ void proj_reduce_kernel(const int *proj_ptr, const int *proj_slots,
 // (hamMatrixKernelLDA, hamiltonianMatrixCalculatorFlattenedDevice.cc:61-116)
 // Output written in the padded K-tiled layout [kt][row][16] of the HX kernel.
 // ---------------------------------------------------------------------------
+// GGA (G and gJxW non-null) sums over 2 n_q terms: for q < n_q the pair
+// (v N_i + 2 U_i, N_j), then (N_i, 2 U_j) with U_i(q) = sum_d gJxW(q, d) G(q, i, d),
+// which adds 2 sum_q g.(grad N_i N_j + N_i grad N_j) to the LDA matrix.
 __global__ __launch_bounds__(256) void build_h_kernel(const double *K, const double *Nq,
-                                                      const double *vJxW, int ne, int ne_pad,
+                                                      const double *vJxW, const double *G,
+                                                      const double *gJxW, int ne, int ne_pad,
                                                       int nq, double *Ht64, float *Ht32) {
   constexpr int TB = 32, TQ = 32;
   __shared__ double Ni[TQ][TB + 1];
@@ -839,16 +843,32 @@ __global__ __launch_bounds__(256) void build_h_kernel(const double *K, const dou
   const int tid = threadIdx.x;
   const int tx = tid % 16, ty = tid / 16;
   const double *v = vJxW + (int64_t)cell * nq;
+  const double *g = gJxW ? gJxW + (int64_t)cell * nq * 3 : nullptr;
+  const int nqe = g ? 2 * nq : nq;
   double acc[2][2] = {{0, 0}, {0, 0}};
-  for (int q0 = 0; q0 < nq; q0 += TQ) {
+  for (int q0 = 0; q0 < nqe; q0 += TQ) {
     for (int e = tid; e < TQ * TB; e += 256) {
       const int qq = e / TB, ii = e % TB;
       const int q = q0 + qq;
       const int i = ti * TB + ii, j = tj * TB + ii;
       double vi = 0, vj = 0;
       if (q < nq) {
-        if (i < ne) vi = Nq[(int64_t)q * ne + i] * v[q];
+        if (i < ne) {
+          vi = Nq[(int64_t)q * ne + i] * v[q];
+          if (g) {
+            const double *gi = G + ((int64_t)q * ne + i) * 3;
+            const double u = g[3 * q] * gi[0] + g[3 * q + 1] * gi[1] + g[3 * q + 2] * gi[2];
+            vi += 2.0 * u;
+          }
+        }
         if (j < ne) vj = Nq[(int64_t)q * ne + j];
+      } else if (q < nqe) {
+        const int qr = q - nq;
+        if (i < ne) vi = Nq[(int64_t)qr * ne + i];
+        if (j < ne) {
+          const double *gj = G + ((int64_t)qr * ne + j) * 3;
+          vj = 2.0 * (g[3 * qr] * gj[0] + g[3 * qr + 1] * gj[1] + g[3 * qr + 2] * gj[2]);
+        }
       }
       Ni[qq][ii] = vi;
       Nj[qq][ii] = vj;
@@ -1766,11 +1786,13 @@ template hipError_t launch_proj_reduce<double>(const int *, const int *, const d
                                                const double *, int64_t, double *, int64_t, int,
                                                hipStream_t);
 
-hipError_t launch_build_h(const double *K, const double *Nq, const double *vJxW, int C, int ne,
-                          int ne_pad, int nq, double *Ht64, float *Ht32, hipStream_t s) {
+hipError_t launch_build_h(const double *K, const double *Nq, const double *vJxW, const double *G,
+                          const double *gJxW, int C, int ne, int ne_pad, int nq, double *Ht64,
+                          float *Ht32, hipStream_t s) {
+  if (gJxW && !G) return hipErrorInvalidValue;
   const int ntile = (ne_pad + 31) / 32;
-  hipLaunchKernelGGL(build_h_kernel, dim3(ntile * ntile, C), dim3(256), 0, s, K, Nq, vJxW, ne,
-                     ne_pad, nq, Ht64, Ht32);
+  hipLaunchKernelGGL(build_h_kernel, dim3(ntile * ntile, C), dim3(256), 0, s, K, Nq, vJxW, G,
+                     gJxW, ne, ne_pad, nq, Ht64, Ht32);
   return hipGetLastError();
 }
 

@@ -72,7 +72,7 @@ using namespace chfsi;
 
 chfsi_operator::~chfsi_operator() {
   void *ptrs[] = {d_map, d_cells_by_color, d_sqrt_mass, d_inv_sqrt_mass, d_sqrt_mass32,
-                  d_inv_sqrt_mass32, d_K, d_Nq, d_jxw, d_Ht64, d_Ht32, d_Ht16, d_nlC16, d_nl_off,
+                  d_inv_sqrt_mass32, d_K, d_Nq, d_Gq, d_jxw, d_Ht64, d_Ht32, d_Ht16, d_nlC16, d_nl_off,
                   d_nlC64, d_nlC32,
                   d_nlCt64, d_nlCt32, d_slot_proj, d_proj_ptr, d_proj_slots, d_proj_V,
                   d_nl_cells, d_c_rows, d_c_ptr, d_c_cols, d_c_vals, d_m_rows, d_m_ptr,
@@ -217,6 +217,8 @@ extern "C" chfsi_status chfsi_operator_create(const chfsi_operator_desc *d, chfs
   if (d->shape_values && d->stiffness) {
     CHFSI_TRY(dev_upload(&op->d_Nq, d->shape_values, size_t(op->nq) * ne));
     CHFSI_TRY(dev_upload(&op->d_K, d->stiffness, size_t(ne) * ne));
+    if (d->shape_gradients)
+      CHFSI_TRY(dev_upload(&op->d_Gq, d->shape_gradients, size_t(op->nq) * ne * 3));
   }
   if (d->jxw) CHFSI_TRY(dev_upload(&op->d_jxw, d->jxw, size_t(C) * op->nq));
   if (d->n_kpoints > 0) {
@@ -350,8 +352,21 @@ extern "C" chfsi_status chfsi_operator_set_potential(chfsi_operator *op, const d
                                                      void *stream) {
   CHFSI_ARG(op && vJxW, "null argument");
   CHFSI_ARG(op->d_Nq && op->d_K, "operator was created without shape data");
-  CHFSI_HIP(launch_build_h(op->d_K, op->d_Nq, vJxW, op->C, op->ne, op->ne_pad, op->nq,
-                           op->d_Ht64, op->d_Ht32, as_stream(stream)));
+  CHFSI_HIP(launch_build_h(op->d_K, op->d_Nq, vJxW, nullptr, nullptr, op->C, op->ne, op->ne_pad,
+                           op->nq, op->d_Ht64, op->d_Ht32, as_stream(stream)));
+  CHFSI_HIP(launch_f64_to_bf16(op->d_Ht64, int64_t(op->C) * op->ne_pad * op->ne_pad, op->d_Ht16,
+                               as_stream(stream)));
+  op->has_H = true;
+  return rebuild_kpoint_matrices(op, as_stream(stream));
+}
+
+extern "C" chfsi_status chfsi_operator_set_potential_gga(chfsi_operator *op, const double *vJxW,
+                                                         const double *gJxW, void *stream) {
+  CHFSI_ARG(op && vJxW && gJxW, "null argument");
+  CHFSI_ARG(op->d_Nq && op->d_K, "operator was created without shape data");
+  CHFSI_ARG(op->d_Gq, "operator was created without shape gradients");
+  CHFSI_HIP(launch_build_h(op->d_K, op->d_Nq, vJxW, op->d_Gq, gJxW, op->C, op->ne, op->ne_pad,
+                           op->nq, op->d_Ht64, op->d_Ht32, as_stream(stream)));
   CHFSI_HIP(launch_f64_to_bf16(op->d_Ht64, int64_t(op->C) * op->ne_pad * op->ne_pad, op->d_Ht16,
                                as_stream(stream)));
   op->has_H = true;

@@ -43,7 +43,7 @@ struct chfsi_operator {
   double *d_sqrt_mass = nullptr, *d_inv_sqrt_mass = nullptr;
   float *d_sqrt_mass32 = nullptr, *d_inv_sqrt_mass32 = nullptr;
   // shape data and cell Hamiltonians (padded K-tiled layout)
-  double *d_K = nullptr, *d_Nq = nullptr, *d_jxw = nullptr;
+  double *d_K = nullptr, *d_Nq = nullptr, *d_Gq = nullptr, *d_jxw = nullptr;
   double *d_Ht64 = nullptr;
   float *d_Ht32 = nullptr;
   uint16_t *d_Ht16 = nullptr, *d_nlC16 = nullptr;  // BF16 copies

@@ -162,6 +162,7 @@ class FEProblem:
     kpoint: Optional[np.ndarray] = None   # Bloch wave vector (bohr^-1); None = real problem
     M_c: Optional[np.ndarray] = None      # (ne, ne) consistent cell mass  int N_i N_j
     G_c: Optional[np.ndarray] = None      # (3, ne, ne) int N_i dN_j/dx_d
+    Gq: Optional[np.ndarray] = None       # (nqc, ne, 3) physical shape gradients
 
     @property
     def vJxW(self) -> np.ndarray:
@@ -264,6 +265,12 @@ def build_problem(wl: Workload, mode: str = "unique", kpoint=None) -> FEProblem:
                 Nq[q, :] = phi[qx, la] * phi[qy, lb] * phi[qz, lc]
                 wq3[q] = gw[qx] * gw[qy] * gw[qz]
     JxW = wq3 * h ** 3
+    # physical shape gradients at the quadrature points (GGA cell term)
+    qi = np.arange(nqc)
+    qx, qy, qz = qi % nq, (qi // nq) % nq, qi // (nq * nq)
+    Gq = np.stack([dphi[qx][:, la] * phi[qy][:, lb] * phi[qz][:, lc],
+                   phi[qx][:, la] * dphi[qy][:, lb] * phi[qz][:, lc],
+                   phi[qx][:, la] * phi[qy][:, lb] * dphi[qz][:, lc]], axis=-1) / h
 
     # ---- cell stiffness via 1-D tensor factors (exact Gauss integration)
     M1 = (phi * gw[:, None]).T @ phi          # int phi_a phi_b on [0,1]
@@ -297,7 +304,7 @@ def build_problem(wl: Workload, mode: str = "unique", kpoint=None) -> FEProblem:
         pair_atom=np.zeros(0, np.int64), pair_cell=np.zeros(0, np.int64),
         pair_C=np.zeros((0, ne, 4)),
         kpoint=None if kpoint is None else np.asarray(kpoint, dtype=np.float64),
-        M_c=M_c, G_c=G_c,
+        M_c=M_c, G_c=G_c, Gq=Gq,
     )
 
     # ---- local potential V(x) = -sum_a D_a exp(-|x-R_a|^2 / w_a^2)

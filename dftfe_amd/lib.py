@@ -45,6 +45,7 @@ class OperatorDesc(ct.Structure):
         ("jxw", c_double_p),
         ("n_kpoints", ct.c_int32), ("kpoints", c_double_p), ("cell_mass", c_double_p),
         ("cell_grad", c_double_p), ("pair_C_kpoints", c_double_p),
+        ("shape_gradients", c_double_p),
     ]
 
 
@@ -80,6 +81,7 @@ SIGNATURES = {
     "chfsi_operator_create": (ct.c_int, [ct.POINTER(OperatorDesc), ct.POINTER(_VP)]),
     "chfsi_operator_destroy": (None, [_VP]),
     "chfsi_operator_set_potential": (ct.c_int, [_VP, _VP, _VP]),
+    "chfsi_operator_set_potential_gga": (ct.c_int, [_VP, _VP, _VP, _VP]),
     "chfsi_operator_set_cell_matrices": (ct.c_int, [_VP, _VP, _VP]),
     "chfsi_operator_get_cell_matrices": (ct.c_int, [_VP, _VP, _VP]),
     "chfsi_operator_set_hx_variant": (ct.c_int, [_VP, ct.c_int32, ct.c_int32]),

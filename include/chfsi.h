@@ -99,6 +99,11 @@ typedef struct {
   const double *cell_grad;        /* 3*n_e*n_e: [d][i][j] = int N_i dN_j/dx_d         */
   const double *pair_C_kpoints;   /* n_kpoints*n_pairs*n_e*proj_per_pair complex,
                                      interleaved (re, im): C_e at each k-point       */
+  /* physical shape gradients at the quadrature points, n_q*n_e*3, [q][i][d]
+   * (the reference-cell gradients times the inverse Jacobian of the Cartesian
+   * cells, hamiltonianMatrixCalculatorFlattenedDevice.cc:415-427); nullable,
+   * needed only by chfsi_operator_set_potential_gga */
+  const double *shape_gradients;
 } chfsi_operator_desc;
 
 /* Batch of k-point blocks filtered together (complex form). */
@@ -178,6 +183,15 @@ void chfsi_operator_destroy(chfsi_operator *op);
  * H_e = 1/2 K_e + N^T diag(vJxW_c) N.  Replaces computeHamiltonianMatricesAllkpt
  * / hamMatrixKernelLDA (hamiltonianMatrixCalculatorFlattenedDevice.cc:61-116, :1073-1375). */
 chfsi_status chfsi_operator_set_potential(chfsi_operator *op, const double *vJxW_dev, void *stream);
+
+/* GGA form of the same rebuild: gJxW_dev (device, C*n_q*3, [c][q][d]) is
+ * d(E_xc)/d(sigma) * grad rho * JxW at each quadrature point and
+ *   H_e = 1/2 K_e + N^T diag(vJxW_c) N
+ *         + 2 sum_q gJxW(q) . (grad N_i(q) N_j(q) + N_i(q) grad N_j(q)).
+ * Needs shape_gradients in the description.  Replaces hamMatrixKernelGGAMemOpt
+ * (hamiltonianMatrixCalculatorFlattenedDevice.cc:283-456, real case). */
+chfsi_status chfsi_operator_set_potential_gga(chfsi_operator *op, const double *vJxW_dev,
+                                              const double *gJxW_dev, void *stream);
 
 /* Direct upload of FP64 cell matrices (device, C*n_e*n_e, row i contiguous). */
 chfsi_status chfsi_operator_set_cell_matrices(chfsi_operator *op, const double *H_dev, void *stream);

@@ -46,11 +46,17 @@ def shape_gradients(prob) -> np.ndarray:
     return G
 
 
-def cell_hamiltonians(prob, V_q: Optional[np.ndarray] = None, cells=None) -> np.ndarray:
+def cell_hamiltonians(prob, V_q: Optional[np.ndarray] = None, cells=None,
+                      gJxW: Optional[np.ndarray] = None) -> np.ndarray:
     """H_e[i,j] = 1/2 int grad N_i . grad N_j + sum_q vJxW_q N_i(q) N_j(q).
 
     Reference: src/dftOperator/hamiltonianMatrixCalculatorFlattenedDevice.cc:61-116
     (hamMatrixKernelLDA, real case); CPU hamiltonianMatrixCalculator.cc:21-985.
+
+    gJxW (C, nqc, 3), the GGA vector d(E_xc)/d(sigma) grad rho JxW, adds
+    2 sum_q sum_d gJxW[q, d] (dN_i/dx_d N_j + N_i dN_j/dx_d) at each point,
+    term by term as hamMatrixKernelGGAMemOpt
+    (hamiltonianMatrixCalculatorFlattenedDevice.cc:283-456, val at :429-441).
     """
     V_q = prob.V_q if V_q is None else V_q
     cells = np.arange(prob.C) if cells is None else np.asarray(cells)
@@ -64,6 +70,9 @@ def cell_hamiltonians(prob, V_q: Optional[np.ndarray] = None, cells=None) -> np.
     H = np.empty((len(cells), prob.ne, prob.ne), dtype=np.result_type(np.float64, shift))
     for k in range(len(cells)):
         H[k] = 0.5 * K + (Nt * vJxW[k][None, :]) @ prob.Nq + shift
+        if gJxW is not None:
+            U = np.einsum("qd,qid->qi", gJxW[cells[k]], G)      # g . grad N_i at q
+            H[k] += 2.0 * (U.T @ prob.Nq + prob.Nq.T @ U)
     return H
 
 

@@ -89,6 +89,60 @@ def test_cell_hamiltonian_constant_and_linear_potential(brute):
     assert np.abs(H.sum(1) - want).max() < 1e-12
 
 
+def test_product_shape_gradients_match_oracle(brute):
+    """The product's vectorised gradient table == the oracle's point-by-point one."""
+    prob, _, _ = brute
+    assert np.abs(prob.Gq - O.shape_gradients(prob)).max() < 1e-13
+
+
+def _assemble(prob, He):
+    A = np.zeros((prob.M, prob.M))
+    for c in range(prob.C):
+        m = prob.cell_map[c]
+        np.add.at(A, (m[:, None], m[None, :]), He[c])
+    return A
+
+
+def test_cell_hamiltonian_gga_term(brute):
+    """GGA cell term 2 int g.(grad N_i N_j + N_i grad N_j), pinned without its formula:
+    - it is symmetric and linear in g;
+    - for constant g it is 2 int g.grad(N_i N_j), whose assembled sum over the
+      periodic mesh vanishes (a wrong relative sign of the two halves does not);
+    - its row sums are 2 g_d int dN_i/dx_d (sum_j N_j = 1), a boundary integral:
+      the 1-D factor phi_a(1) - phi_a(0) = delta_{a,p} - delta_{a,0} times the
+      integrals of the other two 1-D factors (pins the factor 2, the sign and
+      the direction index d)."""
+    prob, _, _ = brute
+    C, nqc, pp = prob.C, prob.nqc, prob.wl.p + 1
+    zero = np.zeros((C, nqc))
+    H0 = O.cell_hamiltonians(prob, V_q=zero)
+    rng = np.random.default_rng(3)
+    g1 = rng.standard_normal((C, nqc, 3)) * prob.JxW[None, :, None]
+    g2 = rng.standard_normal((C, nqc, 3)) * prob.JxW[None, :, None]
+    T1 = O.cell_hamiltonians(prob, V_q=zero, gJxW=g1) - H0
+    T2 = O.cell_hamiltonians(prob, V_q=zero, gJxW=g2) - H0
+    T12 = O.cell_hamiltonians(prob, V_q=zero, gJxW=0.5 * g1 - 2.0 * g2) - H0
+    scale = np.abs(T1).max()
+    assert np.abs(T1 - T1.transpose(0, 2, 1)).max() < 1e-14 * scale
+    assert np.abs(T12 - (0.5 * T1 - 2.0 * T2)).max() < 1e-13 * scale
+
+    gvec = np.array([0.3, -0.7, 1.1])
+    gc = np.broadcast_to(gvec[None, None, :] * prob.JxW[None, :, None], (C, nqc, 3))
+    Tc = O.cell_hamiltonians(prob, V_q=zero, gJxW=gc) - H0
+    assert np.abs(Tc).max() > 0.1
+    assert np.abs(_assemble(prob, Tc)).max() < 1e-12 * np.abs(Tc).max()
+
+    I0 = prob.gll_w * prob.h                  # int phi_b over [0, h] (GLL weights, exact)
+    edge = np.zeros(pp)
+    edge[-1], edge[0] = 1.0, -1.0              # phi_a(h) - phi_a(0)
+    i = np.arange(prob.ne)
+    a, b, c = i % pp, (i // pp) % pp, i // (pp * pp)
+    want = 2.0 * (gvec[0] * edge[a] * I0[b] * I0[c] + gvec[1] * I0[a] * edge[b] * I0[c]
+                  + gvec[2] * I0[a] * I0[b] * edge[c])
+    for k in (0, C - 1):
+        assert np.abs(Tc[k].sum(1) - want).max() < 1e-12
+
+
 def test_mass_vector():
     for mode in ("unique", "dealii"):
         prob = fe.build_workload("brute", mode)
