@@ -823,81 +823,141 @@ __global__ void proj_reduce_kernel(const int *proj_ptr, const int *proj_slots,
 }
 
 // ---------------------------------------------------------------------------
-// Cell Hamiltonian build  H_e = 1/2 K + N^T diag(vJxW_c) N
-// (hamMatrixKernelLDA, hamiltonianMatrixCalculatorFlattenedDevice.cc:61-116)
+// Cell Hamiltonian build  H_e = 1/2 K + N^T diag(vJxW_c) N  (+ GGA term)
+// (hamMatrixKernelLDA / hamMatrixKernelGGAMemOpt,
+//  hamiltonianMatrixCalculatorFlattenedDevice.cc:61-116, :283-456)
 // Output written in the padded K-tiled layout [kt][row][16] of the HX kernel.
+//
+// All cells share the shape functions, so the build is one GEMM of node pairs
+// against per-cell quadrature weights:
+//   H_e[i][j] = 1/2 K[i][j] + sum_k P_k(i, j) w_k(e)
+// LDA: k = q,       P = N_i(q) N_j(q),                          w = vJxW_e(q)
+// GGA: k = 4 q + s, s = 0 as LDA; s = 1..3 (d = s - 1):
+//                   P = 2 (dN_i/dx_d N_j + N_i dN_j/dx_d),      w = gJxW_e(q, d)
+// A workgroup takes a 16 x 16 block (ib <= jb) of node pairs and 64 cells:
+// wave w owns nodes i = 16 ib + 4 w + t (t < 4), each a 16 (j) x 64 (cells)
+// strip of FP64 16x16x4 MFMA results.  Off-diagonal blocks also write the
+// transposed entries, so each pair's quadrature sum is formed once.
 // ---------------------------------------------------------------------------
-// GGA (G and gJxW non-null) sums over 2 n_q terms: for q < n_q the pair
-// (v N_i + 2 U_i, N_j), then (N_i, 2 U_j) with U_i(q) = sum_d gJxW(q, d) G(q, i, d),
-// which adds 2 sum_q g.(grad N_i N_j + N_i grad N_j) to the LDA matrix.
-__global__ __launch_bounds__(256) void build_h_kernel(const double *K, const double *Nq,
-                                                      const double *vJxW, const double *G,
-                                                      const double *gJxW, int ne, int ne_pad,
-                                                      int nq, double *Ht64, float *Ht32) {
-  constexpr int TB = 32, TQ = 32;
-  __shared__ double Ni[TQ][TB + 1];
-  __shared__ double Nj[TQ][TB + 1];
-  const int ntile = (ne_pad + TB - 1) / TB;
-  const int ti = blockIdx.x / ntile, tj = blockIdx.x % ntile;
-  const int cell = blockIdx.y;
-  const int tid = threadIdx.x;
-  const int tx = tid % 16, ty = tid / 16;
-  const double *v = vJxW + (int64_t)cell * nq;
-  const double *g = gJxW ? gJxW + (int64_t)cell * nq * 3 : nullptr;
-  const int nqe = g ? 2 * nq : nq;
-  double acc[2][2] = {{0, 0}, {0, 0}};
-  for (int q0 = 0; q0 < nqe; q0 += TQ) {
-    for (int e = tid; e < TQ * TB; e += 256) {
-      const int qq = e / TB, ii = e % TB;
-      const int q = q0 + qq;
-      const int i = ti * TB + ii, j = tj * TB + ii;
-      double vi = 0, vj = 0;
-      if (q < nq) {
-        if (i < ne) {
-          vi = Nq[(int64_t)q * ne + i] * v[q];
-          if (g) {
-            const double *gi = G + ((int64_t)q * ne + i) * 3;
-            const double u = g[3 * q] * gi[0] + g[3 * q + 1] * gi[1] + g[3 * q + 2] * gi[2];
-            vi += 2.0 * u;
-          }
-        }
-        if (j < ne) vj = Nq[(int64_t)q * ne + j];
-      } else if (q < nqe) {
-        const int qr = q - nq;
-        if (i < ne) vi = Nq[(int64_t)qr * ne + i];
-        if (j < ne) {
-          const double *gj = G + ((int64_t)qr * ne + j) * 3;
-          vj = 2.0 * (g[3 * qr] * gj[0] + g[3 * qr + 1] * gj[1] + g[3 * qr + 2] * gj[2]);
+template <bool GGA>
+__global__ __launch_bounds__(256, 2) void build_h_kernel(const double *K, const double *Nq,
+                                                      const double *G, const double *vJxW,
+                                                      const double *gJxW, int C, int ne,
+                                                      int ne_pad, int nq, double *Ht64,
+                                                      float *Ht32) {
+  constexpr int QC = GGA ? 16 : 32;            // quadrature points per LDS chunk
+  constexpr int CT = 64;                       // cells per workgroup
+  constexpr int NS = 18;                       // padded row of 16 nodes
+  constexpr int WS = GGA ? CT * 4 + 4 : CT + 2;  // padded row of the cell weights
+  constexpr int GQ = GGA ? QC : 1;
+  __shared__ double sNi[QC][NS], sNj[QC][NS];
+  __shared__ double sGi[GQ][3][NS], sGj[GQ][3][NS];
+  __shared__ double sW[QC][WS];
+  const int nb = ne_pad / 16;
+  int p = blockIdx.x, ib = 0;
+  while (p >= nb - ib) {
+    p -= nb - ib;
+    ++ib;
+  }
+  const int jb = ib + p;
+  const int c0 = blockIdx.y * CT;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int n = lane & 15, g = lane >> 4;
+  f64x4 acc[4][4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) acc[t][cb] = f64x4{0, 0, 0, 0};
+
+  for (int q0 = 0; q0 < nq; q0 += QC) {
+    for (int e = tid; e < QC * 16; e += 256) {
+      const int qq = e >> 4, m = e & 15, q = q0 + qq;
+      const int i = ib * 16 + m, j = jb * 16 + m;
+      const bool iok = q < nq && i < ne, jok = q < nq && j < ne;
+      sNi[qq][m] = iok ? Nq[(int64_t)q * ne + i] : 0.0;
+      sNj[qq][m] = jok ? Nq[(int64_t)q * ne + j] : 0.0;
+      if constexpr (GGA) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          sGi[qq][d][m] = iok ? G[((int64_t)q * ne + i) * 3 + d] : 0.0;
+          sGj[qq][d][m] = jok ? G[((int64_t)q * ne + j) * 3 + d] : 0.0;
         }
       }
-      Ni[qq][ii] = vi;
-      Nj[qq][ii] = vj;
+    }
+    // cell weights, read along q (contiguous per cell)
+    for (int e = tid; e < CT * QC; e += 256) {
+      const int c = e / QC, qq = e % QC, q = q0 + qq, cell = c0 + c;
+      const double v = (q < nq && cell < C) ? vJxW[(int64_t)cell * nq + q] : 0.0;
+      if constexpr (GGA)
+        sW[qq][c * 4] = v;
+      else
+        sW[qq][c] = v;
+    }
+    if constexpr (GGA) {
+      for (int e = tid; e < CT * QC * 3; e += 256) {
+        const int c = e / (QC * 3), r = e % (QC * 3), qq = r / 3, d = r % 3;
+        const int q = q0 + qq, cell = c0 + c;
+        sW[qq][c * 4 + 1 + d] = (q < nq && cell < C) ? gJxW[((int64_t)cell * nq + q) * 3 + d] : 0.0;
+      }
     }
     __syncthreads();
-#pragma unroll 8
-    for (int qq = 0; qq < TQ; ++qq) {
-      const double a0 = Ni[qq][2 * ty], a1 = Ni[qq][2 * ty + 1];
-      const double b0 = Nj[qq][2 * tx], b1 = Nj[qq][2 * tx + 1];
-      acc[0][0] += a0 * b0;
-      acc[0][1] += a0 * b1;
-      acc[1][0] += a1 * b0;
-      acc[1][1] += a1 * b1;
+    constexpr int KS = GGA ? QC : QC / 4;
+#pragma unroll 4
+    for (int ks = 0; ks < KS; ++ks) {
+      double av[4], bv[4];
+      if constexpr (GGA) {
+        const int q = ks, d = g > 0 ? g - 1 : 0;
+        const double nj = sNj[q][n], gj = sGj[q][d][n];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const double ni = sNi[q][4 * w + t], gi = sGi[q][d][4 * w + t];
+          av[t] = g == 0 ? ni * nj : 2.0 * fmadd(gi, nj, ni * gj);
+        }
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) bv[cb] = sW[q][(cb * 16 + n) * 4 + g];
+      } else {
+        const int q = 4 * ks + g;
+        const double nj = sNj[q][n];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) av[t] = sNi[q][4 * w + t] * nj;
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) bv[cb] = sW[q][cb * 16 + n];
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) acc[t][cb] = mfma16(av[t], bv[cb], acc[t][cb]);
     }
     __syncthreads();
   }
-  const int64_t base = (int64_t)cell * ne_pad * ne_pad;
+
+  const int64_t per = (int64_t)ne_pad * ne_pad;
 #pragma unroll
-  for (int di = 0; di < 2; ++di)
+  for (int t = 0; t < 4; ++t) {
+    const int i = ib * 16 + 4 * w + t;
 #pragma unroll
-    for (int dj = 0; dj < 2; ++dj) {
-      const int i = ti * TB + 2 * ty + di, j = tj * TB + 2 * tx + dj;
-      if (i >= ne_pad || j >= ne_pad) continue;
-      double h = 0.0;
-      if (i < ne && j < ne) h = 0.5 * K[(int64_t)i * ne + j] + acc[di][dj];
-      const int64_t o = base + (int64_t)(j >> 4) * ne_pad * 16 + (int64_t)i * 16 + (j & 15);
-      Ht64[o] = h;
-      if (Ht32) Ht32[o] = (float)h;
+    for (int cb = 0; cb < 4; ++cb) {
+      const int cell = c0 + cb * 16 + n;
+      if (cell >= C) continue;
+      const int64_t base = (int64_t)cell * per;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int j = jb * 16 + acc_row<double>(lane, r);
+        const bool ok = i < ne && j < ne;
+        const double q = acc[t][cb][r];
+        const double h = ok ? 0.5 * K[(int64_t)i * ne + j] + q : 0.0;
+        const int64_t o = base + (int64_t)(j >> 4) * ne_pad * 16 + (int64_t)i * 16 + (j & 15);
+        Ht64[o] = h;
+        if (Ht32) Ht32[o] = (float)h;
+        if (ib != jb) {
+          const double hT = ok ? 0.5 * K[(int64_t)j * ne + i] + q : 0.0;
+          const int64_t oT = base + (int64_t)(i >> 4) * ne_pad * 16 + (int64_t)j * 16 + (i & 15);
+          Ht64[oT] = hT;
+          if (Ht32) Ht32[oT] = (float)hT;
+        }
+      }
     }
+  }
 }
 
 __global__ void untile_h_kernel(const double *Ht, int C, int ne, int ne_pad, double *H) {
@@ -1789,10 +1849,16 @@ template hipError_t launch_proj_reduce<double>(const int *, const int *, const d
 hipError_t launch_build_h(const double *K, const double *Nq, const double *vJxW, const double *G,
                           const double *gJxW, int C, int ne, int ne_pad, int nq, double *Ht64,
                           float *Ht32, hipStream_t s) {
-  if (gJxW && !G) return hipErrorInvalidValue;
-  const int ntile = (ne_pad + 31) / 32;
-  hipLaunchKernelGGL(build_h_kernel, dim3(ntile * ntile, C), dim3(256), 0, s, K, Nq, vJxW, G,
-                     gJxW, ne, ne_pad, nq, Ht64, Ht32);
+  if ((gJxW && !G) || ne_pad % 16 != 0 || ne_pad < ne || C <= 0) return hipErrorInvalidValue;
+  const int nb = ne_pad / 16;
+  const dim3 grid(nb * (nb + 1) / 2, (C + 63) / 64);
+  if (grid.y > 65535u) return hipErrorInvalidValue;
+  if (gJxW)
+    hipLaunchKernelGGL(build_h_kernel<true>, grid, dim3(256), 0, s, K, Nq, G, vJxW, gJxW, C, ne,
+                       ne_pad, nq, Ht64, Ht32);
+  else
+    hipLaunchKernelGGL(build_h_kernel<false>, grid, dim3(256), 0, s, K, Nq, G, vJxW, gJxW, C, ne,
+                       ne_pad, nq, Ht64, Ht32);
   return hipGetLastError();
 }
 

@@ -55,16 +55,17 @@ def test_gga_hx_matches_oracle(gga):
     assert _rel(Yd.cpu().numpy(), ref.apply(X)) < 1e-13
 
 
-def test_gga_zero_vector_equals_lda_bitwise(gga):
-    """gJxW = 0 adds exact zeros: the matrices equal the LDA build bit for bit,
-    and an LDA rebuild afterwards restores the LDA matrices."""
+def test_gga_zero_vector_equals_lda(gga):
+    """gJxW = 0 adds only zero terms: the matrices equal the LDA build to
+    rounding (the GGA build sums its quadrature terms in another grouping), and
+    an LDA rebuild afterwards restores the LDA matrices bit for bit."""
     prob, op, V, g = gga
     vJxW = V * prob.JxW[None, :]
     op.set_potential(vJxW)
     H_lda = op.cell_matrices().cpu().numpy()
     op.set_potential_gga(vJxW, np.zeros_like(g))
     H0 = op.cell_matrices().cpu().numpy()
-    assert np.array_equal(H0, H_lda)
+    assert _rel(H0, H_lda) < 1e-15
     op.set_potential_gga(vJxW, g)
     op.set_potential(vJxW)
     assert np.array_equal(op.cell_matrices().cpu().numpy(), H_lda)
