@@ -844,15 +844,21 @@ __global__ __launch_bounds__(256, 2) void build_h_kernel(const double *K, const 
                                                       const double *G, const double *vJxW,
                                                       const double *gJxW, int C, int ne,
                                                       int ne_pad, int nq, double *Ht64,
-                                                      float *Ht32) {
+                                                      float *Ht32, uint16_t *Ht16) {
   constexpr int QC = GGA ? 16 : 32;            // quadrature points per LDS chunk
   constexpr int CT = 64;                       // cells per workgroup
   constexpr int NS = 18;                       // padded row of 16 nodes
   constexpr int WS = GGA ? CT * 4 + 4 : CT + 2;  // padded row of the cell weights
   constexpr int GQ = GGA ? QC : 1;
-  __shared__ double sNi[QC][NS], sNj[QC][NS];
-  __shared__ double sGi[GQ][3][NS], sGj[GQ][3][NS];
-  __shared__ double sW[QC][WS];
+  constexpr int OS = 17;                       // padded row of the output staging
+  constexpr int SIN = 2 * QC * NS + 2 * GQ * 3 * NS + QC * WS;
+  constexpr int SOUT = 16 * 16 * OS;           // 16 cells x 16 x 16 results
+  __shared__ double smem[SIN > SOUT ? SIN : SOUT];
+  double(*sNi)[NS] = reinterpret_cast<double(*)[NS]>(smem);
+  double(*sNj)[NS] = reinterpret_cast<double(*)[NS]>(smem + QC * NS);
+  double(*sGi)[3][NS] = reinterpret_cast<double(*)[3][NS]>(smem + 2 * QC * NS);
+  double(*sGj)[3][NS] = reinterpret_cast<double(*)[3][NS]>(smem + 2 * QC * NS + GQ * 3 * NS);
+  double(*sW)[WS] = reinterpret_cast<double(*)[WS]>(smem + 2 * QC * NS + 2 * GQ * 3 * NS);
   const int nb = ne_pad / 16;
   int p = blockIdx.x, ib = 0;
   while (p >= nb - ib) {
@@ -931,32 +937,45 @@ __global__ __launch_bounds__(256, 2) void build_h_kernel(const double *K, const 
     __syncthreads();
   }
 
+  // Epilogue through LDS, 16 cells at a time: each cell's 16 x 16 block is one
+  // contiguous 2 KB run of the [kt][row][16] layout (kt = jb, rows of ib), and
+  // so is its transpose (kt = ib, rows of jb); threads write consecutive elements.
+  // The K loop ended with a barrier, so the staging may overwrite its operands.
   const int64_t per = (int64_t)ne_pad * ne_pad;
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const int i = ib * 16 + 4 * w + t;
+  for (int cb = 0; cb < 4; ++cb) {
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb) {
-      const int cell = c0 + cb * 16 + n;
-      if (cell >= C) continue;
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        smem[(n * 16 + 4 * w + t) * OS + acc_row<double>(lane, r)] = acc[t][cb][r];
+    __syncthreads();
+#pragma unroll 2
+    for (int e = tid; e < 16 * 256; e += 256) {
+      const int c = e >> 8, wi = e & 255;
+      const int cell = c0 + cb * 16 + c;
+      if (cell >= C) break;  // e grows with c: no later element has a cell
       const int64_t base = (int64_t)cell * per;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int j = jb * 16 + acc_row<double>(lane, r);
-        const bool ok = i < ne && j < ne;
-        const double q = acc[t][cb][r];
-        const double h = ok ? 0.5 * K[(int64_t)i * ne + j] + q : 0.0;
-        const int64_t o = base + (int64_t)(j >> 4) * ne_pad * 16 + (int64_t)i * 16 + (j & 15);
+      {
+        const int il = wi >> 4, jl = wi & 15, i = ib * 16 + il, j = jb * 16 + jl;
+        const double q = smem[(c * 16 + il) * OS + jl];
+        const double h = (i < ne && j < ne) ? 0.5 * K[(int64_t)i * ne + j] + q : 0.0;
+        const int64_t o = base + (int64_t)jb * ne_pad * 16 + (int64_t)i * 16 + jl;
         Ht64[o] = h;
         if (Ht32) Ht32[o] = (float)h;
-        if (ib != jb) {
-          const double hT = ok ? 0.5 * K[(int64_t)j * ne + i] + q : 0.0;
-          const int64_t oT = base + (int64_t)(i >> 4) * ne_pad * 16 + (int64_t)j * 16 + (i & 15);
-          Ht64[oT] = hT;
-          if (Ht32) Ht32[oT] = (float)hT;
-        }
+        if (Ht16) Ht16[o] = to_bf16((float)h);
+      }
+      if (ib != jb) {
+        const int jl = wi >> 4, il = wi & 15, i = ib * 16 + il, j = jb * 16 + jl;
+        const double q = smem[(c * 16 + il) * OS + jl];
+        const double h = (i < ne && j < ne) ? 0.5 * K[(int64_t)j * ne + i] + q : 0.0;
+        const int64_t o = base + (int64_t)ib * ne_pad * 16 + (int64_t)j * 16 + il;
+        Ht64[o] = h;
+        if (Ht32) Ht32[o] = (float)h;
+        if (Ht16) Ht16[o] = to_bf16((float)h);
       }
     }
+    __syncthreads();
   }
 }
 
@@ -1848,17 +1867,17 @@ template hipError_t launch_proj_reduce<double>(const int *, const int *, const d
 
 hipError_t launch_build_h(const double *K, const double *Nq, const double *vJxW, const double *G,
                           const double *gJxW, int C, int ne, int ne_pad, int nq, double *Ht64,
-                          float *Ht32, hipStream_t s) {
+                          float *Ht32, uint16_t *Ht16, hipStream_t s) {
   if ((gJxW && !G) || ne_pad % 16 != 0 || ne_pad < ne || C <= 0) return hipErrorInvalidValue;
   const int nb = ne_pad / 16;
   const dim3 grid(nb * (nb + 1) / 2, (C + 63) / 64);
   if (grid.y > 65535u) return hipErrorInvalidValue;
   if (gJxW)
     hipLaunchKernelGGL(build_h_kernel<true>, grid, dim3(256), 0, s, K, Nq, G, vJxW, gJxW, C, ne,
-                       ne_pad, nq, Ht64, Ht32);
+                       ne_pad, nq, Ht64, Ht32, Ht16);
   else
     hipLaunchKernelGGL(build_h_kernel<false>, grid, dim3(256), 0, s, K, Nq, G, vJxW, gJxW, C, ne,
-                       ne_pad, nq, Ht64, Ht32);
+                       ne_pad, nq, Ht64, Ht32, Ht16);
   return hipGetLastError();
 }
 

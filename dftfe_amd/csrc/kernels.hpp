@@ -88,12 +88,13 @@ hipError_t launch_proj_reduce(const int *proj_ptr, const int *proj_slots, const 
                               int n_proj, const T *partial, int64_t ldP, T *W, int64_t ldW,
                               int B, hipStream_t s);
 
-// H_e = 1/2 K + N^T diag(vJxW_c) N, written tiled/padded in FP64 and FP32.
+// H_e = 1/2 K + N^T diag(vJxW_c) N, written tiled/padded in FP64, FP32 and BF16
+// (Ht32, Ht16 nullable).
 // With gJxW (C*n_q*3, nullable) and the physical shape gradients G (n_q*n_e*3)
 // the GGA term 2 sum_q gJxW(q).(grad N_i(q) N_j(q) + N_i(q) grad N_j(q)) is added.
 hipError_t launch_build_h(const double *K, const double *Nq, const double *vJxW, const double *G,
                           const double *gJxW, int C, int ne, int ne_pad, int nq, double *Ht64,
-                          float *Ht32, hipStream_t s);
+                          float *Ht32, uint16_t *Ht16, hipStream_t s);
 // tiled padded -> dense C x ne x ne (row-major) and back
 hipError_t launch_untile_h(const double *Ht, int C, int ne, int ne_pad, double *H, hipStream_t s);
 hipError_t launch_tile_h(const double *H, int C, int ne, int ne_pad, double *Ht64, float *Ht32,

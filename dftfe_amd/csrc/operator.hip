@@ -353,9 +353,7 @@ extern "C" chfsi_status chfsi_operator_set_potential(chfsi_operator *op, const d
   CHFSI_ARG(op && vJxW, "null argument");
   CHFSI_ARG(op->d_Nq && op->d_K, "operator was created without shape data");
   CHFSI_HIP(launch_build_h(op->d_K, op->d_Nq, vJxW, nullptr, nullptr, op->C, op->ne, op->ne_pad,
-                           op->nq, op->d_Ht64, op->d_Ht32, as_stream(stream)));
-  CHFSI_HIP(launch_f64_to_bf16(op->d_Ht64, int64_t(op->C) * op->ne_pad * op->ne_pad, op->d_Ht16,
-                               as_stream(stream)));
+                           op->nq, op->d_Ht64, op->d_Ht32, op->d_Ht16, as_stream(stream)));
   op->has_H = true;
   return rebuild_kpoint_matrices(op, as_stream(stream));
 }
@@ -366,9 +364,7 @@ extern "C" chfsi_status chfsi_operator_set_potential_gga(chfsi_operator *op, con
   CHFSI_ARG(op->d_Nq && op->d_K, "operator was created without shape data");
   CHFSI_ARG(op->d_Gq, "operator was created without shape gradients");
   CHFSI_HIP(launch_build_h(op->d_K, op->d_Nq, vJxW, op->d_Gq, gJxW, op->C, op->ne, op->ne_pad,
-                           op->nq, op->d_Ht64, op->d_Ht32, as_stream(stream)));
-  CHFSI_HIP(launch_f64_to_bf16(op->d_Ht64, int64_t(op->C) * op->ne_pad * op->ne_pad, op->d_Ht16,
-                               as_stream(stream)));
+                           op->nq, op->d_Ht64, op->d_Ht32, op->d_Ht16, as_stream(stream)));
   op->has_H = true;
   return rebuild_kpoint_matrices(op, as_stream(stream));
 }

@@ -75,3 +75,23 @@ def test_gga_rejects_bad_shape(gga):
     prob, op, V, g = gga
     with pytest.raises(ValueError):
         op.set_potential_gga(V * prob.JxW[None, :], g[:, :, :2])
+
+
+def test_gga_bf16_copy_matches_separate_conversion(gga):
+    """The build writes the BF16 cell matrices itself; they equal the separate
+    FP64 -> BF16 conversion of an upload of the same FP64 matrices bit for bit
+    (seen through the BF16 apply), and the FP32 copy likewise."""
+    prob, op, V, g = gga
+    op.set_potential_gga(V * prob.JxW[None, :], g)
+    X = torch.tensor(prob.initial_guess(N=32, seed_offset=5), device="cuda", dtype=torch.float32)
+    Y16, Y32 = torch.zeros_like(X), torch.zeros_like(X)
+    op.HX(X, False, 1.0, Y16, bf16=True)
+    op.HX(X, False, 1.0, Y32)
+    H = op.cell_matrices()
+    op.set_cell_matrices(H)
+    Z16, Z32 = torch.zeros_like(X), torch.zeros_like(X)
+    op.HX(X, False, 1.0, Z16, bf16=True)
+    op.HX(X, False, 1.0, Z32)
+    torch.cuda.synchronize()
+    assert torch.equal(Y16, Z16)
+    assert torch.equal(Y32, Z32)
