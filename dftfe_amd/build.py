@@ -1,4 +1,6 @@
-"""Builds libchfsi.so (gfx950) in-tree with hipcc.
+"""Builds libchfsi.so (gfx950) in-tree with hipcc, and the program that drives
+the C++ drop-in classes of include/dftfe_chfsi.hpp over it (bin/dropin_solve,
+used by the GPU tests).
 
     python -m dftfe_amd.build
 """
@@ -14,6 +16,9 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libchfsi.so")
 SOURCES = ["kernels.hip", "kernels_bloch.hip", "operator.hip", "solver.hip", "bloch.hip"]
+ROOT = os.path.dirname(HERE)
+DROPIN_SRC = os.path.join(ROOT, "tests", "cpp", "dropin_solve.cpp")
+DROPIN = os.path.join(HERE, "bin", "dropin_solve")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
@@ -46,7 +51,26 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    if os.path.exists(DROPIN_SRC):
+        build_dropin(verbose)
     return LIB
+
+
+def build_dropin(verbose: bool = False) -> str:
+    """The C++ drop-in driver, linked against the in-tree libchfsi.so."""
+    os.makedirs(os.path.dirname(DROPIN), exist_ok=True)
+    cmd = [HIPCC, "-O2", "-std=c++17", "--offload-arch=gfx950",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           "-I", os.path.join(ROOT, "include"), DROPIN_SRC, "-o", DROPIN + ".tmp",
+           "-L", HERE, "-lchfsi", "-Wl,-rpath,$ORIGIN/..",
+           "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"drop-in driver build failed:\n{res.stderr}")
+    os.replace(DROPIN + ".tmp", DROPIN)
+    if verbose:
+        print("built", DROPIN)
+    return DROPIN
 
 
 if __name__ == "__main__":

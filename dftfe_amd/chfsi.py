@@ -551,6 +551,16 @@ class ChebyshevSolverDevice:
                                          "residual", "total"], t.tolist()))
         return ev, res, bup.value
 
+    def solve_no_rr(self, op: KohnShamOperatorDevice, X: torch.Tensor, n_passes: int = 1, *,
+                    scale_degree: bool = False, use_mixed_prec_overall: bool = False) -> None:
+        """solveNoRR: n_passes of filter + CholGS on X (FE basis, overwritten)
+        without Rayleigh-Ritz; a0 and bLow as last set, bUp refreshed by Lanczos
+        unless reuse_lanczos_upper."""
+        M, N = X.shape
+        L.check(self.lib.chfsi_solve_no_rr(self.handle, op.handle, _ptr(X), M, N, n_passes,
+                                           int(scale_degree), int(use_mixed_prec_overall),
+                                           _stream()))
+
     def solve_spectrum_split(self, op: KohnShamOperatorDevice, X: torch.Tensor, n_core: int,
                              Xfrac: Optional[torch.Tensor] = None, *,
                              is_first_filtering_call: bool = False,

@@ -961,6 +961,44 @@ namespace {
 // Start of a pass shared by both solve forms (…SolverDevice.cc:241-450): Lanczos
 // bounds when needed, degree, M^{1/2} scaling into the Loewdin basis and the
 // blocked filter.  Records ev[1] after Lanczos and ev[2] after the filter.
+// Lanczos bounds from the supplied start vector, or the seeded default one.
+chfsi_status run_lanczos(chfsi_solver *s, chfsi_operator *op, int64_t M, double *lo, double *up,
+                         hipStream_t st) {
+  const chfsi_params &P = s->params;
+  const double *v0 = s->have_v0 ? sptr<double>(s->v0) : nullptr;
+  std::vector<double> hv;
+  Scratch gen;
+  if (!v0) {
+    hv = default_lanczos_start(M);
+    CHFSI_TRY(gen.ensure(sizeof(double) * M));
+    CHFSI_HIP(hipMemcpy(gen.ptr, hv.data(), sizeof(double) * M, hipMemcpyHostToDevice));
+    v0 = sptr<double>(gen);
+  }
+  chfsi_status stl = lanczos_impl(s, op, v0, P.lanczos_steps, P.reproducible, lo, up, st);
+  gen.release();
+  return stl;
+}
+
+// Filter all N columns of the Loewdin-scaled X in groups with degree m; the
+// precision is the filter precision of a mixed pass unless the residual-driven
+// switch has handed the passes to FP64.
+chfsi_status filter_all(chfsi_solver *s, chfsi_operator *op, double *X, int64_t M, int N, int m,
+                        int32_t use_mixed_prec_overall, hipStream_t st) {
+  const chfsi_params &P = s->params;
+  const int B = std::min<int>(P.cheby_block_size > 0 ? P.cheby_block_size : N, N);
+  const bool switched =
+      P.mixed_prec_switch_residual > 0 && s->last_res_lowest < P.mixed_prec_switch_residual;
+  const int prec = use_mixed_prec_overall && !switched ? P.filter_precision : CHFSI_FP64;
+  s->last_precision = prec;
+  const int G = filter_group_columns(P, M, N, B);
+  for (int j = 0; j < N; j += G) {
+    const int bw = std::min(G, N - j);
+    CHFSI_TRY(filter_block(s, op, X + j, N, bw, m, s->bLow, s->bUp, s->a0, prec,
+                           use_mixed_prec_overall ? P.fp64_tail_degrees : 0, st));
+  }
+  return CHFSI_OK;
+}
+
 chfsi_status lanczos_scale_filter(chfsi_solver *s, chfsi_operator *op, double *X, int64_t M, int N,
                                   int32_t is_first_filtering_call, int32_t use_mixed_prec_overall,
                                   int32_t is_first_scf, int *B_out, hipEvent_t *ev,
@@ -968,18 +1006,7 @@ chfsi_status lanczos_scale_filter(chfsi_solver *s, chfsi_operator *op, double *X
   const chfsi_params &P = s->params;
   if (is_first_filtering_call || !P.reuse_lanczos_upper) {
     double lo, up;
-    const double *v0 = s->have_v0 ? sptr<double>(s->v0) : nullptr;
-    std::vector<double> hv;
-    Scratch gen;
-    if (!v0) {
-      hv = default_lanczos_start(M);
-      CHFSI_TRY(gen.ensure(sizeof(double) * M));
-      CHFSI_HIP(hipMemcpy(gen.ptr, hv.data(), sizeof(double) * M, hipMemcpyHostToDevice));
-      v0 = sptr<double>(gen);
-    }
-    chfsi_status stl = lanczos_impl(s, op, v0, P.lanczos_steps, P.reproducible, &lo, &up, st);
-    gen.release();
-    CHFSI_TRY(stl);
+    CHFSI_TRY(run_lanczos(s, op, M, &lo, &up, st));
     if (is_first_filtering_call) {
       s->a0 = lo;
       s->bUp = up;
@@ -997,16 +1024,7 @@ chfsi_status lanczos_scale_filter(chfsi_solver *s, chfsi_operator *op, double *X
   CHFSI_HIP(launch_scale_rows<double>(X, N, M, N, 1.0, op->d_sqrt_mass, st));
   const int B = std::min<int>(P.cheby_block_size > 0 ? P.cheby_block_size : N, N);
   if (is_first_filtering_call) s->last_res_lowest = 1e300;
-  const bool switched =
-      P.mixed_prec_switch_residual > 0 && s->last_res_lowest < P.mixed_prec_switch_residual;
-  const int prec = use_mixed_prec_overall && !switched ? P.filter_precision : CHFSI_FP64;
-  s->last_precision = prec;
-  const int G = filter_group_columns(P, M, N, B);
-  for (int j = 0; j < N; j += G) {
-    const int bw = std::min(G, N - j);
-    CHFSI_TRY(filter_block(s, op, X + j, N, bw, (int)m, s->bLow, s->bUp, s->a0, prec,
-                           use_mixed_prec_overall ? P.fp64_tail_degrees : 0, st));
-  }
+  CHFSI_TRY(filter_all(s, op, X, M, N, (int)m, use_mixed_prec_overall, st));
   CHFSI_HIP(hipEventRecord(ev[2], st));
   *B_out = B;
   return CHFSI_OK;
@@ -1077,6 +1095,37 @@ extern "C" chfsi_status chfsi_solve(chfsi_solver *s, chfsi_operator *op, double 
   CHFSI_HIP(hipEventRecord(pe.ev[5], st));
   CHFSI_TRY(pe.report(timings_ms));
   if (bUp_out) *bUp_out = s->bUp;
+  return CHFSI_OK;
+}
+
+extern "C" chfsi_status chfsi_solve_no_rr(chfsi_solver *s, chfsi_operator *op, double *X,
+                                          int64_t M, int32_t N, int32_t n_passes,
+                                          int32_t scale_degree, int32_t use_mixed_prec_overall,
+                                          void *stream) {
+  CHFSI_ARG(s && op && X, "null argument");
+  CHFSI_ARG(M == op->M && N > 0, "X must be n_dofs x N");
+  CHFSI_ARG(n_passes >= 1, "n_passes must be at least 1");
+  hipStream_t st = as_stream(stream);
+  const chfsi_params &P = s->params;
+  // upper bound only (…SolverDevice.cc:812-824); a0 and bLow as last set
+  if (!P.reuse_lanczos_upper) {
+    double lo, up;
+    CHFSI_TRY(run_lanczos(s, op, M, &lo, &up, st));
+    s->bUp = up;
+  }
+  // degree (…SolverDevice.cc:826-839): the scaling factor applies when scale_degree
+  unsigned m = P.cheby_degree > 0 ? (unsigned)P.cheby_degree : cheby_order_lookup(s->bUp);
+  if (scale_degree) m = (unsigned)(m * P.first_scf_degree_factor);
+  const int B = std::min<int>(P.cheby_block_size > 0 ? P.cheby_block_size : N, N);
+  const bool mx = use_mixed_prec_overall != 0;
+  CHFSI_HIP(launch_scale_rows<double>(X, N, M, N, 1.0, op->d_sqrt_mass, st));
+  for (int pass = 0; pass < n_passes; ++pass) {
+    CHFSI_TRY(filter_all(s, op, X, M, N, (int)m, use_mixed_prec_overall, st));
+    CHFSI_TRY(cholgs_impl(s, X, M, N, B, mx && P.mixed_prec_overlap,
+                          mx && P.mixed_prec_cgs_rotation, st));
+  }
+  CHFSI_HIP(launch_scale_rows<double>(X, N, M, N, 1.0, op->d_inv_sqrt_mass, st));
+  CHFSI_HIP(hipStreamSynchronize(st));
   return CHFSI_OK;
 }
 

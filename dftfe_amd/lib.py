@@ -134,6 +134,9 @@ SIGNATURES = {
     "chfsi_rayleigh_ritz_spectrum_split": (ct.c_int, [_VP, _VP, ct.c_int64, ct.c_int32, ct.c_int32,
                                                       _VP, ct.c_int64, ct.c_int32, c_double_p,
                                                       ct.c_int32, _VP]),
+    # filter + CholGS passes without Rayleigh-Ritz (solveNoRR)
+    "chfsi_solve_no_rr": (ct.c_int, [_VP, _VP, _VP, ct.c_int64, ct.c_int32, ct.c_int32,
+                                     ct.c_int32, ct.c_int32, _VP]),
     "chfsi_solve_spectrum_split": (ct.c_int, [_VP, _VP, _VP, ct.c_int64, ct.c_int32, ct.c_int32,
                                               _VP, ct.c_int64, c_double_p, c_double_p,
                                               ct.c_int32, ct.c_int32, ct.c_int32, ct.c_int32,

@@ -345,6 +345,15 @@ chfsi_status chfsi_solve(chfsi_solver *solver, chfsi_operator *op, double *X, in
                          int32_t use_mixed_prec_overall, int32_t is_first_scf, double *bUp_out,
                          double *timings_ms, void *stream);
 
+/* solveNoRR (…SolverDevice.h:83-95; .cc:742-1080): n_passes of filter + CholGS
+ * on X (device, node-major M x N, FE basis) without Rayleigh-Ritz, X returned
+ * in the FE basis.  Lanczos refreshes only bUp (unless reuse_lanczos_upper);
+ * a0 and bLow stay as last set.  scale_degree multiplies the degree by
+ * first_scf_degree_factor (the reference does so for pseudopotential runs). */
+chfsi_status chfsi_solve_no_rr(chfsi_solver *solver, chfsi_operator *op, double *X, int64_t M,
+                               int32_t N, int32_t n_passes, int32_t scale_degree,
+                               int32_t use_mixed_prec_overall, void *stream);
+
 /* rayleighRitzGEPSpectrumSplitDirect (rayleighRitzDevice.cc:822-1278): on X
  * (device, node-major M x N, Loewdin basis) S = X^T X = L L^T and
  * X <- X L^{-T}, then Hp = X^T H~ X, its eigendecomposition, and only the top
