@@ -89,6 +89,8 @@ def main():
     ap.add_argument("--no-kpoints", action="store_true",
                     help="skip the batched k-point filter measurement")
     ap.add_argument("--kpoint-reps", type=int, default=3)
+    ap.add_argument("--no-diff", action="store_true",
+                    help="skip the differentiation leg (24 bohr, 12^3 cells, p=5, 256 bands)")
     ap.add_argument("--converge-passes", type=int, default=0,
                     help="also run FP64, mixed and mixed defect-form passes from the same start "
                          "this many times and report their eigenvalue differences (untimed)")
@@ -352,6 +354,8 @@ def main():
     }
 
     out["cell_rebuild"] = cell_rebuild(prob, op, log)
+    if not args.no_diff:
+        out["differentiation"] = differentiation(log)
     if conv is not None:
         out["convergence"] = conv
     if not args.no_kpoints:
@@ -399,6 +403,59 @@ def cell_rebuild(prob, op, log, reps=5):
     op.set_potential(prob.vJxW)
     log(f"cell rebuild: {res}")
     return res
+
+
+def differentiation(log, reps=2):
+    """Differentiation form on its own workload: periodic 24 bohr cube, 12^3
+    cells, p=5 (216,000 DOFs), 32 wells, 256 bands in blocks of 256, degree from
+    the workload, Fermi-Dirac occupations at kT = 0.01 Ha.  After two ChFSI
+    passes, times the band energy's forward (filter with checkpoints every 5
+    degrees, CholGS, projected Hamiltonian, eigenvalues) and its backward
+    (checkpointed filter adjoint and the HX adjoint in V), FP64 filter and FP32
+    filter with FP64 gradient accumulation, with HIP events on torch's stream
+    (the library launches on it).  Reports the relative max-norm difference of
+    the two gradients."""
+    import torch
+    from dftfe_amd import fe
+    from dftfe_amd.autograd import band_energy, fermi_occupations
+    from dftfe_amd.chfsi import ChebyshevSolverDevice, KohnShamOperatorDevice
+
+    prob = fe.build_workload("cube24_p5_n256")
+    wl = prob.wl
+    op = KohnShamOperatorDevice(prob)
+    solver = ChebyshevSolverDevice(cheby_block_size=wl.B, cheby_degree=wl.m, lanczos_steps=10)
+    solver.set_lanczos_start(torch.tensor(prob.lanczos_start, device="cuda"))
+    X = torch.tensor(prob.initial_guess(), device="cuda")
+    for k in range(2):
+        lam, _, _ = solver.solve(op, X, is_first_filtering_call=(k == 0))
+        _, _, bUp = solver.bounds()
+        solver.reinitSpectrumBounds(float(lam[0]), float(lam[-1]), bUp)
+    occ, _ = fermi_occupations(torch.tensor(lam, device="cuda"), 0.8 * wl.N, 0.01)
+    st = torch.cuda.current_stream()
+    out = {"workload": "cube24_p5_n256", "dofs": prob.M, "bands": wl.N, "degree": wl.m,
+           "checkpoint_every": 5}
+    grads = {}
+    for prec, name in ((0, "fp64"), (1, "fp32")):
+        fwd, bwd = [], []
+        for _ in range(reps + 1):
+            V = torch.tensor(prob.V_q, device="cuda", requires_grad=True)
+            e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+            e0.record(st)
+            E, _, _ = band_energy(op, V, X.clone(), solver.bounds(), wl.m, occ, wl.B,
+                                  precision=prec)
+            e1.record(st)
+            E.backward()
+            e2.record(st)
+            e2.synchronize()
+            fwd.append(e0.elapsed_time(e1))
+            bwd.append(e1.elapsed_time(e2))
+            grads[name] = V.grad.detach()
+        out[f"{name}_forward_ms"] = round(float(np.median(fwd[1:])), 2)
+        out[f"{name}_backward_ms"] = round(float(np.median(bwd[1:])), 2)
+    out["grad_fp32_vs_fp64_rel"] = float((grads["fp32"] - grads["fp64"]).abs().max()
+                                         / grads["fp64"].abs().max())
+    log(f"differentiation: {out}")
+    return out
 
 
 def kpoint_batch(reps, log):
