@@ -1217,88 +1217,115 @@ __global__ void zero_rows_kernel(const int64_t *rows, int64_t nrows, double *X, 
 // ---------------------------------------------------------------------------
 // Products of two multivectors at the quadrature points
 //   g[c, q] (+)= JxW[q] * sum_v w_v (N(q) . s a_v|_c) (N(q) . s b_v|_c)
-// One workgroup per cell; the cell's nodal rows of a and b are interpolated to
-// its quadrature points in 16-column chunks (FP64 FMA, LDS-staged shape-value
-// slices), multiplied pointwise and summed over the columns in registers, so
-// each g[c, q] is written once (deterministic).
 //   a = b = X, w = spin * occupation        -> density rho(q) JxW (densityCalculator.cc:622-689)
 //   a = Ybar, b = X, w = scalar              -> dL/dV(q) of Y = scalar * S H(V) S X
+// One workgroup per cell.  The interpolation N (n_q x n_e) times the cell's
+// gathered rows (n_e x 64 columns) is a GEMM on FP64 16x16x4 MFMA: each wave
+// owns 2 blocks of 16 quadrature points of a 128-point group and the 4 column
+// blocks; per slice of 16 nodes the shape values and the scaled gathered rows
+// are staged in LDS.  The pointwise products are weighted, summed over the
+// wave's columns and over the 16 lanes of a row, and added per point into an
+// LDS sum in a fixed order, so each g[c, q] is written once (deterministic).
+// SAME (a and b the same array, the density) interpolates once.
 // ---------------------------------------------------------------------------
-template <int QPT, typename TIn>
-__global__ __launch_bounds__(256) void quad_product_kernel(
+template <bool SAME, typename TIn>
+__global__ __launch_bounds__(256, 2) void quad_product_kernel(
     const uint32_t *map, int ne, int nqc, const double *Nq, const double *JxW, const TIn *a,
     int64_t lda, const TIn *b, int64_t ldb, int B, const double *s_in, const double *w,
     int wshift, double wscale, double *g, int accumulate) {
-  constexpr int VC = 16;  // columns per chunk
-  constexpr int KT = 16;  // nodes per slice
-  __shared__ double Ns[256 * QPT][KT + 1];
-  __shared__ double As[KT][VC];
-  __shared__ double Bs[KT][VC];
+  constexpr int QG = 128;  // quadrature points per group (4 waves x 2 blocks of 16)
+  constexpr int VC = 64;   // columns per chunk
+  constexpr int KT = 16;   // nodes per slice
+  constexpr int NS = KT + 1, XS = VC + 2;
+  __shared__ double sN[QG][NS];
+  __shared__ double sXa[KT][XS];
+  __shared__ double sXb[SAME ? 1 : KT][XS];
   __shared__ uint32_t smap[384];
   __shared__ double sscale[384];
+  __shared__ double gsum[512];
   const int cell = blockIdx.x;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int n = lane & 15, gl = lane >> 4;
   const uint32_t *cmap = map + (int64_t)cell * ne;
   for (int i = tid; i < ne; i += 256) {
     const uint32_t gg = cmap[i] & 0x7fffffffu;
     smap[i] = gg;
     sscale[i] = s_in ? s_in[gg] : 1.0;
   }
+  for (int q = tid; q < nqc; q += 256) gsum[q] = 0.0;
   __syncthreads();
-  double gacc[QPT];
-#pragma unroll
-  for (int qi = 0; qi < QPT; ++qi) gacc[qi] = 0.0;
   for (int v0 = 0; v0 < B; v0 += VC) {
-    double acc_a[QPT][VC], acc_b[QPT][VC];
+    for (int q0 = 0; q0 < nqc; q0 += QG) {
+      f64x4 acca[2][4], accb[2][4];
 #pragma unroll
-    for (int qi = 0; qi < QPT; ++qi)
+      for (int qb = 0; qb < 2; ++qb)
 #pragma unroll
-      for (int vv = 0; vv < VC; ++vv) acc_a[qi][vv] = acc_b[qi][vv] = 0.0;
-    for (int j0 = 0; j0 < ne; j0 += KT) {
-      for (int e = tid; e < 256 * QPT * KT; e += 256) {
-        const int q = e / KT, jj = e % KT;
-        Ns[q][jj] = (q < nqc && j0 + jj < ne) ? Nq[(int64_t)q * ne + j0 + jj] : 0.0;
-      }
-      {
-        const int jj = tid / VC, vv = tid % VC;  // 256 threads = 16 x 16
-        const int j = j0 + jj, v = v0 + vv;
-        double xa = 0.0, xb = 0.0;
-        if (j < ne && v < B) {
-          const int64_t gg = smap[j];
-          xa = sscale[j] * (double)a[gg * lda + v];
-          xb = sscale[j] * (double)b[gg * ldb + v];
+        for (int cb = 0; cb < 4; ++cb) acca[qb][cb] = accb[qb][cb] = f64x4{0, 0, 0, 0};
+      for (int k0 = 0; k0 < ne; k0 += KT) {
+        for (int e = tid; e < QG * KT; e += 256) {
+          const int qq = e >> 4, kk = e & 15;
+          const int q = q0 + qq, k = k0 + kk;
+          sN[qq][kk] = (q < nqc && k < ne) ? Nq[(int64_t)q * ne + k] : 0.0;
         }
-        As[jj][vv] = xa;
-        Bs[jj][vv] = xb;
-      }
-      __syncthreads();
-#pragma unroll 4
-      for (int jj = 0; jj < KT; ++jj) {
-#pragma unroll
-        for (int qi = 0; qi < QPT; ++qi) {
-          const double n = Ns[tid + 256 * qi][jj];
-#pragma unroll
-          for (int vv = 0; vv < VC; ++vv) {
-            acc_a[qi][vv] += n * As[jj][vv];
-            acc_b[qi][vv] += n * Bs[jj][vv];
+        for (int e = tid; e < KT * VC; e += 256) {
+          const int kk = e / VC, vv = e % VC;
+          const int k = k0 + kk, v = v0 + vv;
+          double xa = 0.0, xb = 0.0;
+          if (k < ne && v < B) {
+            const int64_t gg = smap[k];
+            const double sc = sscale[k];
+            xa = sc * (double)a[gg * lda + v];
+            if (!SAME) xb = sc * (double)b[gg * ldb + v];
           }
+          sXa[kk][vv] = xa;
+          if (!SAME) sXb[kk][vv] = xb;
         }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int kk = 4 * t + gl;
+          double av[2], ba[4], bb[4];
+#pragma unroll
+          for (int qb = 0; qb < 2; ++qb) av[qb] = sN[(wv * 2 + qb) * 16 + n][kk];
+#pragma unroll
+          for (int cb = 0; cb < 4; ++cb) {
+            ba[cb] = sXa[kk][cb * 16 + n];
+            if (!SAME) bb[cb] = sXb[kk][cb * 16 + n];
+          }
+#pragma unroll
+          for (int qb = 0; qb < 2; ++qb)
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) {
+              acca[qb][cb] = mfma16(av[qb], ba[cb], acca[qb][cb]);
+              if (!SAME) accb[qb][cb] = mfma16(av[qb], bb[cb], accb[qb][cb]);
+            }
+        }
+        __syncthreads();
       }
-      __syncthreads();
+      // row q = q0 + 16 (2 wv + qb) + acc_row, column v = v0 + 16 cb + n
+#pragma unroll
+      for (int qb = 0; qb < 2; ++qb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          double sacc = 0.0;
+#pragma unroll
+          for (int cb = 0; cb < 4; ++cb) {
+            const int v = v0 + cb * 16 + n;
+            const double x = acca[qb][cb][r];
+            const double y = SAME ? x : accb[qb][cb][r];
+            if (v < B) sacc += (w ? w[v >> wshift] : 1.0) * x * y;
+          }
+#pragma unroll
+          for (int off = 8; off >= 1; off >>= 1) sacc += __shfl_xor(sacc, off);
+          const int q = q0 + (wv * 2 + qb) * 16 + acc_row<double>(lane, r);
+          if (n == 0 && q < nqc) gsum[q] += sacc;
+        }
     }
-#pragma unroll
-    for (int qi = 0; qi < QPT; ++qi)
-#pragma unroll
-      for (int vv = 0; vv < VC; ++vv)
-        if (v0 + vv < B) gacc[qi] += (w ? w[(v0 + vv) >> wshift] : 1.0) * acc_a[qi][vv] * acc_b[qi][vv];
   }
-#pragma unroll
-  for (int qi = 0; qi < QPT; ++qi) {
-    const int q = tid + 256 * qi;
-    if (q < nqc) {
-      const int64_t o = (int64_t)cell * nqc + q;
-      g[o] = (accumulate ? g[o] : 0.0) + (JxW ? JxW[o] : 1.0) * wscale * gacc[qi];
-    }
+  __syncthreads();
+  for (int q = tid; q < nqc; q += 256) {
+    const int64_t o = (int64_t)cell * nqc + q;
+    g[o] = (accumulate ? g[o] : 0.0) + (JxW ? JxW[o] : 1.0) * wscale * gsum[q];
   }
 }
 
@@ -1308,12 +1335,12 @@ static hipError_t quad_go(const uint32_t *map, int C, int ne, int nqc, const dou
                           int B, const double *s_in, const double *w, int wshift, double wscale,
                           double *g, int accumulate, hipStream_t s) {
   if (ne > 384 || nqc > 512) return hipErrorInvalidValue;
-  if (nqc <= 256)
-    hipLaunchKernelGGL((quad_product_kernel<1, TIn>), dim3(C), dim3(256), 0, s, map, ne, nqc, Nq,
-                       JxW, a, lda, b, ldb, B, s_in, w, wshift, wscale, g, accumulate);
+  if (a == b && lda == ldb)
+    hipLaunchKernelGGL((quad_product_kernel<true, TIn>), dim3(C), dim3(256), 0, s, map, ne, nqc,
+                       Nq, JxW, a, lda, b, ldb, B, s_in, w, wshift, wscale, g, accumulate);
   else
-    hipLaunchKernelGGL((quad_product_kernel<2, TIn>), dim3(C), dim3(256), 0, s, map, ne, nqc, Nq,
-                       JxW, a, lda, b, ldb, B, s_in, w, wshift, wscale, g, accumulate);
+    hipLaunchKernelGGL((quad_product_kernel<false, TIn>), dim3(C), dim3(256), 0, s, map, ne, nqc,
+                       Nq, JxW, a, lda, b, ldb, B, s_in, w, wshift, wscale, g, accumulate);
   return hipGetLastError();
 }
 
