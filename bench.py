@@ -354,6 +354,7 @@ def main():
     }
 
     out["cell_rebuild"] = cell_rebuild(prob, op, log)
+    out["density"] = density_leg(prob, op, X0, log)
     if not args.no_diff:
         out["differentiation"] = differentiation(log)
     if conv is not None:
@@ -402,6 +403,31 @@ def cell_rebuild(prob, op, log, reps=5):
         res[f"{name}_fp64_tflops"] = round(fl / ms / 1e9, 2)
     op.set_potential(prob.vJxW)
     log(f"cell rebuild: {res}")
+    return res
+
+
+def density_leg(prob, op, X, log, reps=5):
+    """Density at the quadrature points from all N wavefunctions of the
+    workload (computeRhoFromPSI's contraction, one interpolation per column:
+    2 n_q n_e flops per cell and column, FP64 MFMA), timed with HIP events on
+    the stream the library launches on."""
+    import torch
+
+    N = X.shape[1]
+    w = torch.full((N,), 2.0, dtype=torch.float64)
+    for _ in range(2):
+        op.density(X, w)
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        op.density(X, w)
+    e1.record(st)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    fl = 2 * prob.C * prob.nqc * prob.ne * N
+    res = {"columns": N, "ms": round(ms, 3), "fp64_tflops": round(fl / ms / 1e9, 2)}
+    log(f"density: {res}")
     return res
 
 
