@@ -354,7 +354,7 @@ def main():
     }
 
     out["cell_rebuild"] = cell_rebuild(prob, op, log)
-    out["density"] = density_leg(prob, op, X0, log)
+    out["density"] = density_leg(prob, op, torch.tensor(prob.initial_guess(), device=dev), log)
     if not args.no_diff:
         out["differentiation"] = differentiation(log)
     if conv is not None:
