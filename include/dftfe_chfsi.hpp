@@ -318,7 +318,9 @@ class kohnShamDFTOperatorDeviceClass : public operatorDFTDeviceClass {
 };
 
 /* chebyshevOrthogonalizedSubspaceIterationSolverDevice
- * (chebyshevOrthogonalizedSubspaceIterationSolverDevice.h:37-166; .cc:155-1080). */
+ * (chebyshevOrthogonalizedSubspaceIterationSolverDevice.h:37-166; .cc:155-1080).
+ * As in the reference, the solver keeps a reference to dftParams, which must
+ * outlive it; the library parameters are taken from it at construction. */
 class chebyshevOrthogonalizedSubspaceIterationSolverDevice {
  public:
   chebyshevOrthogonalizedSubspaceIterationSolverDevice(const MPI_Comm &, const MPI_Comm &,
