@@ -161,11 +161,14 @@ def main():
                           "calibration_copy_bytes": 4 << 28, "autotune": tune}), flush=True)
         return
 
+    finals = {}
+
     def run_passes(mixed: bool, npass: int, dense: bool = False, defect: bool = False,
-                   monotone: bool = False):
+                   monotone: bool = False, label=None):
         """npass passes from the same start with the reference's bookkeeping
         (a0 = lowest, bLow = top Ritz value, kohnShamEigenSolve.cc:483-498), or
-        with bLow never rising above its previous value (monotone)."""
+        with bLow never rising above its previous value (monotone).  With a
+        label, the last Ritz vectors and values are kept for the error checks."""
         solver = ChebyshevSolverDevice(cheby_block_size=B, cheby_degree=m,
                                        lanczos_steps=wl.lanczos_steps, mixed_dense=dense,
                                        mixed_prec_defect_filter=defect)
@@ -187,7 +190,10 @@ def main():
             phases.append(dict(solver.last_timings_ms,
                                filter_precision=solver.last_filter_precision()))
         torch.cuda.synchronize()
-        return (time.perf_counter() - t) / npass * 1e3, lams, ress, phases
+        ms = (time.perf_counter() - t) / npass * 1e3
+        if label:
+            finals[label] = (X, lams[-1])
+        return ms, lams, ress, phases
 
     for _ in range(args.warmup):
         run_passes(True, 1)
@@ -195,11 +201,11 @@ def main():
             run_passes(False, 1)
     log("warmup done")
 
-    mixed_ms, lam_mx, res_mx, ph_mx = run_passes(True, args.steps)
+    mixed_ms, lam_mx, res_mx, ph_mx = run_passes(True, args.steps, label="mixed")
     log(f"mixed: {mixed_ms:.1f} ms/pass; last max residual {res_mx[-1].max():.3e}")
     fp64_ms, lam_64, res_64, ph_64 = (None, None, None, None)
     if not args.no_fp64:
-        fp64_ms, lam_64, res_64, ph_64 = run_passes(False, args.steps)
+        fp64_ms, lam_64, res_64, ph_64 = run_passes(False, args.steps, label="fp64")
         log(f"fp64: {fp64_ms:.1f} ms/pass; last max residual {res_64[-1].max():.3e}")
     # the same mixed passes with the reference's opt-in mixed-precision dense
     # steps (FP32 off-diagonal blocks of the overlap, CholGS rotation, X^T H X
@@ -236,7 +242,9 @@ def main():
         return (time.perf_counter() - t) / npass * 1e3, lams, phases, n_core
 
     # the mixed passes with the FP32 filter in defect form (X p(theta) + E)
-    df_ms, lam_df, res_df, ph_df = run_passes(True, args.steps, defect=True)
+    df_ms, lam_df, res_df, ph_df = run_passes(True, args.steps, defect=True, label="defect")
+    errors = density_band_errors(prob, op, finals, log) if "fp64" in finals else None
+    finals.clear()
     log(f"mixed, defect-form filter: {df_ms:.1f} ms/pass")
 
     conv = None
@@ -353,6 +361,8 @@ def main():
                  "frac": round(F / hx64_ms / 1e9 / PEAK_TFLOPS["fp64"], 4)},
     }
 
+    if errors:
+        out["density_band_energy_vs_fp64"] = errors
     out["cell_rebuild"] = cell_rebuild(prob, op, log)
     out["density"] = density_leg(prob, op, torch.tensor(prob.initial_guess(), device=dev), log)
     if not args.no_diff:
@@ -403,6 +413,36 @@ def cell_rebuild(prob, op, log, reps=5):
         res[f"{name}_fp64_tflops"] = round(fl / ms / 1e9, 2)
     op.set_potential(prob.vJxW)
     log(f"cell rebuild: {res}")
+    return res
+
+
+def density_band_errors(prob, op, finals, log):
+    """After the timed passes: density and band energy of each mixed-precision
+    form against the FP64 passes from the same start, with the same
+    occupations (Fermi-Dirac, kT = 0.01 Ha, 0.8 N electron pairs, from the
+    FP64 eigenvalues; spin factor 2).  Density error: quadrature-weighted L2
+    norm of the difference, relative to the L2 norm of the FP64 density."""
+    import torch
+    from dftfe_amd.autograd import fermi_occupations
+
+    X64, lam64 = finals["fp64"]
+    N = X64.shape[1]
+    occ, _ = fermi_occupations(torch.tensor(lam64, device="cuda"), 0.8 * N, 0.01)
+    w = (2.0 * occ).cpu()
+    jxw = torch.tensor(prob.JxW, device="cuda")[None, :]
+    rho64 = op.density(X64, w)
+    E64 = float((w.numpy() * lam64).sum())
+    n64 = float(torch.sqrt((rho64 * rho64 * jxw).sum()))
+    res = {}
+    for label in ("mixed", "defect"):
+        if label not in finals:
+            continue
+        X, lam = finals[label]
+        d = op.density(X, w) - rho64
+        res[label] = {"density_l2": float(torch.sqrt((d * d * jxw).sum())),
+                      "density_l2_rel": float(torch.sqrt((d * d * jxw).sum())) / n64,
+                      "band_energy_Ha": abs(float((w.numpy() * lam).sum()) - E64)}
+    log(f"errors vs fp64 after the timed passes: {res}")
     return res
 
 
