@@ -840,7 +840,7 @@ __global__ void proj_reduce_kernel(const int *proj_ptr, const int *proj_slots,
 // transposed entries, so each pair's quadrature sum is formed once.
 // ---------------------------------------------------------------------------
 template <bool GGA>
-__global__ __launch_bounds__(256, GGA ? 2 : 3) void build_h_kernel(const double *K, const double *Nq,
+__global__ __launch_bounds__(256, 3) void build_h_kernel(const double *K, const double *Nq,
                                                       const double *G, const double *vJxW,
                                                       const double *gJxW, int C, int ne,
                                                       int ne_pad, int nq, double *Ht64,
