@@ -1231,7 +1231,7 @@ __global__ void zero_rows_kernel(const int64_t *rows, int64_t nrows, double *X, 
 // SAME (a and b the same array, the density) interpolates once.
 // ---------------------------------------------------------------------------
 template <bool SAME, typename TIn, int VC>
-__global__ __launch_bounds__(256, 2) void quad_product_kernel(
+__global__ __launch_bounds__(256, SAME ? 2 : 3) void quad_product_kernel(
     const uint32_t *map, int ne, int nqc, const double *Nq, const double *JxW, const TIn *a,
     int64_t lda, const TIn *b, int64_t ldb, int B, const double *s_in, const double *w,
     int wshift, double wscale, double *g, int accumulate) {
