@@ -464,7 +464,9 @@ class ChebyshevSolverDevice:
                  first_scf_degree_factor: float = 1.34, reuse_lanczos_upper: bool = False,
                  fp64_tail_degrees: int = 0, mixed_dense: bool = False,
                  filter_columns: int = 0, mixed_prec_switch_residual: float = 0.0,
-                 mixed_prec_defect_filter: bool = False):
+                 mixed_prec_defect_filter: bool = False, use_shep: bool = True,
+                 lanczos_full_residual_margin: bool = False,
+                 is_pseudopotential: bool = False):
         """mixed_dense: FP32 off-diagonal blocks in the overlap, CholGS rotation,
         X^T H X and Rayleigh-Ritz rotation when a pass runs with
         use_mixed_prec_overall (the reference's USE MIXED PREC CGS O / CGS SR /
@@ -475,7 +477,12 @@ class ChebyshevSolverDevice:
         lowest 90% of states are all below it, later passes filter in FP64.
         mixed_prec_defect_filter:
         run the FP32 filter in defect form (X p(theta) + E, E in FP32), whose
-        rounding scales with the residual."""
+        rounding scales with the residual.  use_shep=False: Rayleigh-Ritz in
+        the generalised form without CholGS (rayleighRitzGEP).
+        lanczos_full_residual_margin: bUp = ceil(theta_max + |f|) (the
+        reference's reproducible-output margin) instead of |f| / 10.
+        is_pseudopotential: gates the first-SCF degree factor and the halving
+        of a looked-up degree for all-electron runs."""
         self.lib = L.load()
         p = L.Params()
         self.lib.chfsi_default_params(ct.byref(p))
@@ -490,6 +497,9 @@ class ChebyshevSolverDevice:
         p.filter_columns = filter_columns
         p.mixed_prec_switch_residual = mixed_prec_switch_residual
         p.mixed_prec_defect_filter = int(mixed_prec_defect_filter)
+        p.use_shep = int(use_shep)
+        p.lanczos_full_residual_margin = int(lanczos_full_residual_margin)
+        p.is_pseudopotential = int(is_pseudopotential)
         for f in ("mixed_prec_overlap", "mixed_prec_cgs_rotation", "mixed_prec_xthx",
                   "mixed_prec_rr_rotation", "mixed_prec_xthx_spectrum_split"):
             setattr(p, f, int(mixed_dense))

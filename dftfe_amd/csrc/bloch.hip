@@ -680,25 +680,28 @@ chfsi_status zapply(chfsi_operator *op, const double *x, int64_t ldx, double *y,
   return CHFSI_OK;
 }
 
-// Hp = X^H H~ X by column blocks; G = Xc HXc^H = conj(Hp) = Q' D Q'^H with
-// Q' = conj(Q); X Q in the Xc view is Q'^H Xc.
-chfsi_status zrr_impl(chfsi_solver *s, chfsi_operator *op, zc *X, int64_t M, int N, int B,
-                      double *eig_host, hipStream_t st) {
-  CHFSI_TRY(ensure_blas(s, st));
-  CHFSI_TRY(s->Hp.ensure(sizeof(zc) * N * N));
+// Lower triangle of G = Xc HXc^H = conj(X^H H~ X) by column blocks of B.
+chfsi_status zxthx_lower(chfsi_solver *s, chfsi_operator *op, const zc *X, int64_t M, int N,
+                         int B, zc *Hp, hipStream_t st) {
   CHFSI_TRY(s->HXb.ensure(sizeof(zc) * M * B));
-  CHFSI_TRY(s->evals.ensure(sizeof(double) * 2 * N));
-  CHFSI_TRY(s->info.ensure(sizeof(rocblas_int)));
-  zc *Hp = sp<zc>(s->Hp), *HX = sp<zc>(s->HXb);
+  zc *HX = sp<zc>(s->HXb);
   for (int j = 0; j < N; j += B) {
     const int bw = std::min(B, N - j);
     CHFSI_TRY(zapply(op, reinterpret_cast<const double *>(X + j), N,
                      reinterpret_cast<double *>(HX), bw, bw, st));
     CHFSI_TRY(zgemm_nc_splitk(s, X + j, N, HX, bw, N - j, bw, M, Hp + j + (int64_t)j * N, N, st));
   }
+  return CHFSI_OK;
+}
+
+// Hermitian eigendecomposition in place (lower triangle used): A <- Q,
+// ascending eigenvalues to the host.
+chfsi_status zeig(chfsi_solver *s, zc *A, int N, double *eig_host, hipStream_t st) {
+  CHFSI_TRY(s->evals.ensure(sizeof(double) * 2 * N));
+  CHFSI_TRY(s->info.ensure(sizeof(rocblas_int)));
   double *D = sp<double>(s->evals), *E = D + N;
   rocblas_int *dinfo = sp<rocblas_int>(s->info);
-  CHFSI_RB(rocsolver_zheevd(s->blas, rocblas_evect_original, rocblas_fill_lower, N, Hp, N, D, E,
+  CHFSI_RB(rocsolver_zheevd(s->blas, rocblas_evect_original, rocblas_fill_lower, N, A, N, D, E,
                             dinfo));
   rocblas_int hinfo = 0;
   CHFSI_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(rocblas_int), hipMemcpyDeviceToHost, st));
@@ -708,7 +711,58 @@ chfsi_status zrr_impl(chfsi_solver *s, chfsi_operator *op, zc *X, int64_t M, int
     set_error("projected eigensolve failed (zheevd info %d)", (int)hinfo);
     return CHFSI_ERR_EIG;
   }
+  return CHFSI_OK;
+}
+
+// Hp = X^H H~ X by column blocks; G = Xc HXc^H = conj(Hp) = Q' D Q'^H with
+// Q' = conj(Q); X Q in the Xc view is Q'^H Xc.
+chfsi_status zrr_impl(chfsi_solver *s, chfsi_operator *op, zc *X, int64_t M, int N, int B,
+                      double *eig_host, hipStream_t st) {
+  CHFSI_TRY(ensure_blas(s, st));
+  CHFSI_TRY(s->Hp.ensure(sizeof(zc) * N * N));
+  zc *Hp = sp<zc>(s->Hp);
+  CHFSI_TRY(zxthx_lower(s, op, X, M, N, B, Hp, st));
+  CHFSI_TRY(zeig(s, Hp, N, eig_host, st));
   return zrotate(s, X, M, N, Hp, rocblas_operation_conjugate_transpose, st);
+}
+
+// Generalised Rayleigh-Ritz without CholGS (rayleighRitzGEP,
+// rayleighRitzDevice.cc:355-819, complex).  In the Xc view the overlap is
+// Xc Xc^H = conj(X^H X) = L' L'^H and the projected Hamiltonian G = conj(Hp);
+// Ghat = L'^{-1} G L'^{-H} = Q' D Q'^H, and X <- X L^{-H} Q is
+// Xc <- (L'^{-H} Q')^H Xc.
+chfsi_status zgep_impl(chfsi_solver *s, chfsi_operator *op, zc *X, int64_t M, int N, int B,
+                       double *eig_host, hipStream_t st) {
+  CHFSI_TRY(ensure_blas(s, st));
+  CHFSI_TRY(s->Hp.ensure(sizeof(zc) * N * N));
+  CHFSI_TRY(s->gep.ensure(sizeof(zc) * 2 * N * N));
+  CHFSI_TRY(s->info.ensure(sizeof(rocblas_int)));
+  zc *G = sp<zc>(s->Hp), *Li = sp<zc>(s->gep), *W = Li + (int64_t)N * N;
+  CHFSI_TRY(zgemm_nc_splitk(s, X, N, X, N, N, N, M, Li, N, st));
+  rocblas_int *dinfo = sp<rocblas_int>(s->info);
+  CHFSI_RB(rocsolver_zpotrf(s->blas, rocblas_fill_lower, N, Li, N, dinfo));
+  rocblas_int hinfo = 0;
+  CHFSI_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(rocblas_int), hipMemcpyDeviceToHost, st));
+  CHFSI_HIP(hipStreamSynchronize(st));
+  if (hinfo != 0) {
+    set_error("overlap matrix is not positive definite (zpotrf info %d)", (int)hinfo);
+    return CHFSI_ERR_NOT_SPD;
+  }
+  CHFSI_RB(rocsolver_ztrtri(s->blas, rocblas_fill_lower, rocblas_diagonal_non_unit, N, Li, N,
+                            dinfo));
+  CHFSI_HIP(launch_zero_upper_complex(reinterpret_cast<double *>(Li), N, N, st));
+  CHFSI_TRY(zxthx_lower(s, op, X, M, N, B, G, st));
+  const zc one{1.0, 0.0}, zero{0.0, 0.0};
+  CHFSI_RB(rocblas_zhemm(s->blas, rocblas_side_right, rocblas_fill_lower, N, N, &one, G, N, Li,
+                         N, &zero, W, N));
+  CHFSI_RB(rocblas_ztrmm(s->blas, rocblas_side_right, rocblas_fill_lower,
+                         rocblas_operation_conjugate_transpose, rocblas_diagonal_non_unit, N, N,
+                         &one, Li, N, W, N, G, N));
+  CHFSI_TRY(zeig(s, G, N, eig_host, st));
+  CHFSI_RB(rocblas_ztrmm(s->blas, rocblas_side_left, rocblas_fill_lower,
+                         rocblas_operation_conjugate_transpose, rocblas_diagonal_non_unit, N, N,
+                         &one, Li, N, G, N, W, N));
+  return zrotate(s, X, M, N, W, rocblas_operation_conjugate_transpose, st);
 }
 
 // |H~ x_j - lambda_j x_j| over 2N real columns with each eigenvalue repeated
@@ -865,8 +919,9 @@ extern "C" chfsi_status chfsi_solve_complex(chfsi_solver *s, chfsi_operator *op,
       CHFSI_HIP(hipMemcpy(gen.ptr, h2.data(), sizeof(double) * 2 * M, hipMemcpyHostToDevice));
     }
     double lo, up;
-    chfsi_status stl = zlanczos_impl(s, op, sp<double>(gen), P.lanczos_steps, P.reproducible, &lo,
-                                     &up, st);
+    chfsi_status stl = zlanczos_impl(s, op, sp<double>(gen), P.lanczos_steps,
+                                     P.reproducible || P.lanczos_full_residual_margin, &lo, &up,
+                                     st);
     CHFSI_HIP(hipStreamSynchronize(st));
     gen.release();
     CHFSI_TRY(stl);
@@ -880,8 +935,7 @@ extern "C" chfsi_status chfsi_solve_complex(chfsi_solver *s, chfsi_operator *op,
   }
   CHFSI_HIP(hipEventRecord(ev[1], st));
 
-  unsigned m = P.cheby_degree > 0 ? (unsigned)P.cheby_degree : cheby_order_lookup(s->bUp);
-  if (is_first_scf) m = (unsigned)(m * P.first_scf_degree_factor);
+  const unsigned m = solve_degree(P, s->bUp, is_first_scf != 0);
 
   CHFSI_HIP(launch_scale_rows<double>(Xr, 2 * (int64_t)N, M, 2 * N, 1.0, op->d_sqrt_mass, st));
   const int B = std::min<int>(P.cheby_block_size > 0 ? P.cheby_block_size : N, N);
@@ -904,9 +958,14 @@ extern "C" chfsi_status chfsi_solve_complex(chfsi_solver *s, chfsi_operator *op,
     CHFSI_TRY(chfsi_batch_filter(op, &kb, stream));
   }
   CHFSI_HIP(hipEventRecord(ev[2], st));
-  CHFSI_TRY(zcholgs_impl(s, X, M, N, st));
-  CHFSI_HIP(hipEventRecord(ev[3], st));
-  CHFSI_TRY(zrr_impl(s, op, X, M, N, B, eig_host, st));
+  if (P.use_shep) {
+    CHFSI_TRY(zcholgs_impl(s, X, M, N, st));
+    CHFSI_HIP(hipEventRecord(ev[3], st));
+    CHFSI_TRY(zrr_impl(s, op, X, M, N, B, eig_host, st));
+  } else {
+    CHFSI_HIP(hipEventRecord(ev[3], st));
+    CHFSI_TRY(zgep_impl(s, op, X, M, N, B, eig_host, st));
+  }
   CHFSI_HIP(hipEventRecord(ev[4], st));
   if (compute_residual) CHFSI_TRY(zresidual_impl(s, op, X, M, N, B, eig_host, res_host, st));
   CHFSI_HIP(launch_scale_rows<double>(Xr, 2 * (int64_t)N, M, 2 * N, 1.0, op->d_inv_sqrt_mass,

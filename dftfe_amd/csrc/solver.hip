@@ -36,6 +36,21 @@ unsigned cheby_order_lookup(double upper) {
   return 1250;
 }
 
+// Degree of a solve pass (…SolverDevice.cc:307-326): the configured degree,
+// or the table entry for bUp halved for all-electron runs (orthogType CGS, the
+// device's only orthogonalisation); then the first-SCF factor, applied only to
+// pseudopotential runs.  Both products truncate, as the reference's unsigned
+// arithmetic does.
+unsigned solve_degree(const chfsi_params &P, double bUp, bool is_first_scf) {
+  unsigned m = (unsigned)std::max(0, P.cheby_degree);
+  if (m == 0) {
+    m = cheby_order_lookup(bUp);
+    if (!P.is_pseudopotential) m = (unsigned)(m * 0.5);
+  }
+  if (is_first_scf && P.is_pseudopotential) m = (unsigned)(m * P.first_scf_degree_factor);
+  return m;
+}
+
 chfsi_status ensure_blas(chfsi_solver *s, hipStream_t st) {
   if (!s->blas) CHFSI_RB(rocblas_create_handle(&s->blas));
   CHFSI_RB(rocblas_set_stream(s->blas, st));
@@ -545,21 +560,17 @@ chfsi_status cholgs_impl(chfsi_solver *s, double *X, int64_t M, int N, int bs, b
   return rotate_lower(s, X, mixed_sr ? Xf : nullptr, M, N, S, bs, st);
 }
 
-// Hp = X^T H~ X (lower triangle by column blocks), eig, X <- X Q.  Mixed
-// X^T H X (XtHXMixedPrec, kohnShamDFTOperatorDevice.cc:4161-): diagonal blocks
-// in FP64, blocks below from FP32 copies of X and of the HX block.
-chfsi_status rr_impl(chfsi_solver *s, chfsi_operator *op, double *X, int64_t M, int N, int B,
-                     double *eig_host, bool mixed_xthx, bool mixed_rot, hipStream_t st) {
-  CHFSI_TRY(ensure_blas(s, st));
-  CHFSI_TRY(s->Hp.ensure(sizeof(double) * N * N));
+// Lower triangle of Hp = X^T H~ X by column blocks of B (XtHX,
+// kohnShamDFTOperatorDevice.cc:4000-4157).  Mixed (XtHXMixedPrec, :4161-):
+// the diagonal blocks in FP64, the blocks below them from the FP32 copy Xf of
+// X and an FP32 copy of the HX block.
+chfsi_status xthx_lower(chfsi_solver *s, chfsi_operator *op, const double *X, int64_t M, int N,
+                        int B, double *Hp, const float *Xf, hipStream_t st) {
   const int G = filter_group_columns(s->params, M, N, B);
   CHFSI_TRY(s->HXb.ensure(sizeof(double) * M * G));
-  CHFSI_TRY(s->evals.ensure(sizeof(double) * 2 * N));
-  CHFSI_TRY(s->info.ensure(sizeof(rocblas_int)));
-  double *Hp = sptr<double>(s->Hp), *HXg = sptr<double>(s->HXb);
-  float *Xf = nullptr, *HXf = nullptr;
-  if (mixed_xthx || mixed_rot) CHFSI_TRY(cast_x(s, X, M, N, &Xf, st));
-  if (mixed_xthx) {
+  double *HXg = sptr<double>(s->HXb);
+  float *HXf = nullptr;
+  if (Xf) {
     CHFSI_TRY(s->HXf.ensure(sizeof(float) * M * B));
     HXf = sptr<float>(s->HXf);
   }
@@ -574,7 +585,7 @@ chfsi_status rr_impl(chfsi_solver *s, chfsi_operator *op, double *X, int64_t M, 
     }
     const double *HX = HXg + (j - g0);
     double *Hj = Hp + j + (int64_t)j * N;
-    if (!mixed_xthx) {
+    if (!Xf) {
       // Hp[j:N, j:j+bw] = Xc[j:N, :] * HXc^T
       CHFSI_TRY(gemm_nt_splitk(s, X + j, N, HX, gw, N - j, bw, M, Hj, N, st));
     } else {
@@ -585,9 +596,17 @@ chfsi_status rr_impl(chfsi_solver *s, chfsi_operator *op, double *X, int64_t M, 
       }
     }
   }
+  return CHFSI_OK;
+}
+
+// Eigendecomposition of the symmetric N x N matrix A (lower triangle used) in
+// place: A <- Q, ascending eigenvalues to the host.
+chfsi_status sym_eig(chfsi_solver *s, double *A, int N, double *eig_host, hipStream_t st) {
+  CHFSI_TRY(s->evals.ensure(sizeof(double) * 2 * N));
+  CHFSI_TRY(s->info.ensure(sizeof(rocblas_int)));
   double *D = sptr<double>(s->evals), *E = D + N;
   rocblas_int *dinfo = sptr<rocblas_int>(s->info);
-  CHFSI_RB(rocsolver_dsyevd(s->blas, rocblas_evect_original, rocblas_fill_lower, N, Hp, N, D, E,
+  CHFSI_RB(rocsolver_dsyevd(s->blas, rocblas_evect_original, rocblas_fill_lower, N, A, N, D, E,
                             dinfo));
   rocblas_int hinfo = 0;
   CHFSI_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(rocblas_int), hipMemcpyDeviceToHost, st));
@@ -597,7 +616,66 @@ chfsi_status rr_impl(chfsi_solver *s, chfsi_operator *op, double *X, int64_t M, 
     set_error("projected eigensolve failed (syevd info %d)", (int)hinfo);
     return CHFSI_ERR_EIG;
   }
+  return CHFSI_OK;
+}
+
+// Hp = X^T H~ X (lower triangle by column blocks), eig, X <- X Q
+// (rayleighRitz, rayleighRitzDevice.cc:80-353).  mixed_xthx / mixed_rot: the
+// reference's USE MIXED PREC XTHX / RR_SR forms.
+chfsi_status rr_impl(chfsi_solver *s, chfsi_operator *op, double *X, int64_t M, int N, int B,
+                     double *eig_host, bool mixed_xthx, bool mixed_rot, hipStream_t st) {
+  CHFSI_TRY(ensure_blas(s, st));
+  CHFSI_TRY(s->Hp.ensure(sizeof(double) * N * N));
+  double *Hp = sptr<double>(s->Hp);
+  float *Xf = nullptr;
+  if (mixed_xthx || mixed_rot) CHFSI_TRY(cast_x(s, X, M, N, &Xf, st));
+  CHFSI_TRY(xthx_lower(s, op, X, M, N, B, Hp, mixed_xthx ? Xf : nullptr, st));
+  CHFSI_TRY(sym_eig(s, Hp, N, eig_host, st));
   return rotate_dense(s, X, mixed_rot ? Xf : nullptr, M, N, Hp, B, st);
+}
+
+// Rayleigh-Ritz in the generalised form on a filtered, not orthonormalised X
+// (rayleighRitzGEP, rayleighRitzDevice.cc:355-819): S = X^T X = L L^T,
+// Hp = X^T H~ X, Hhat = L^{-1} Hp L^{-T} = Q diag(lambda) Q^T, and
+// X <- X (L^{-T} Q).  Mixed forms as rr_impl, with USE MIXED PREC CGS O for S.
+chfsi_status gep_impl(chfsi_solver *s, chfsi_operator *op, double *X, int64_t M, int N, int B,
+                      double *eig_host, bool mixed_o, bool mixed_xthx, bool mixed_rot,
+                      hipStream_t st) {
+  CHFSI_TRY(ensure_blas(s, st));
+  CHFSI_TRY(s->Hp.ensure(sizeof(double) * N * N));
+  CHFSI_TRY(s->gep.ensure(sizeof(double) * 2 * N * N));
+  CHFSI_TRY(s->info.ensure(sizeof(rocblas_int)));
+  double *Hp = sptr<double>(s->Hp), *Linv = sptr<double>(s->gep), *W = Linv + (int64_t)N * N;
+  float *Xf = nullptr;
+  if (mixed_o || mixed_xthx || mixed_rot) CHFSI_TRY(cast_x(s, X, M, N, &Xf, st));
+  // overlap and its Cholesky factor (pseudoGSDevice.cc:121-301 steps)
+  CHFSI_TRY(overlap_lower(s, X, mixed_o ? Xf : nullptr, M, N, B, Linv, st));
+  rocblas_int *dinfo = sptr<rocblas_int>(s->info);
+  CHFSI_RB(rocsolver_dpotrf(s->blas, rocblas_fill_lower, N, Linv, N, dinfo));
+  rocblas_int hinfo = 0;
+  CHFSI_HIP(hipMemcpyAsync(&hinfo, dinfo, sizeof(rocblas_int), hipMemcpyDeviceToHost, st));
+  CHFSI_HIP(hipStreamSynchronize(st));
+  if (hinfo != 0) {
+    set_error("overlap matrix is not positive definite (potrf info %d)", (int)hinfo);
+    return CHFSI_ERR_NOT_SPD;
+  }
+  CHFSI_RB(rocsolver_dtrtri(s->blas, rocblas_fill_lower, rocblas_diagonal_non_unit, N, Linv, N,
+                            dinfo));
+  CHFSI_HIP(launch_zero_upper(Linv, N, N, st));
+  // projected Hamiltonian of the unrotated X, then Hhat = Linv Hp Linv^T
+  CHFSI_TRY(xthx_lower(s, op, X, M, N, B, Hp, mixed_xthx ? Xf : nullptr, st));
+  const double one = 1.0, zero = 0.0;
+  CHFSI_RB(rocblas_dsymm(s->blas, rocblas_side_right, rocblas_fill_lower, N, N, &one, Hp, N,
+                         Linv, N, &zero, W, N));
+  CHFSI_RB(rocblas_dtrmm(s->blas, rocblas_side_right, rocblas_fill_lower,
+                         rocblas_operation_transpose, rocblas_diagonal_non_unit, N, N, &one, Linv,
+                         N, W, N, Hp, N));
+  CHFSI_TRY(sym_eig(s, Hp, N, eig_host, st));
+  // rotation matrix L^{-T} Q
+  CHFSI_RB(rocblas_dtrmm(s->blas, rocblas_side_left, rocblas_fill_lower,
+                         rocblas_operation_transpose, rocblas_diagonal_non_unit, N, N, &one, Linv,
+                         N, Hp, N, W, N));
+  return rotate_dense(s, X, mixed_rot ? Xf : nullptr, M, N, W, B, st);
 }
 
 chfsi_status residual_impl(chfsi_solver *s, chfsi_operator *op, const double *X, int64_t ldX,
@@ -707,7 +785,7 @@ extern "C" void chfsi_default_params(chfsi_params *p) {
   p->reproducible = 0;
   p->reuse_lanczos_upper = 0;
   p->first_scf_degree_factor = 1.34;    // utils/dftParameters.cc:872-876
-  p->filter_precision = CHFSI_FP32;
+  p->filter_precision = CHFSI_FP64;    // USE MIXED PREC CHEBY false, utils/dftParameters.cc:1262
   p->use_shep = 1;                      // utils/dftParameters.cc:934-938
   p->fp64_tail_degrees = 0;
   p->mixed_prec_overlap = 0;            // utils/dftParameters.cc:1246-1249
@@ -718,6 +796,15 @@ extern "C" void chfsi_default_params(chfsi_params *p) {
   p->filter_columns = 0;
   p->mixed_prec_switch_residual = 0.0;
   p->mixed_prec_defect_filter = 0;
+  p->is_pseudopotential = 0;            // utils/dftParameters.cc:1173
+  p->lanczos_full_residual_margin = 0;
+}
+
+extern "C" chfsi_status chfsi_chebyshev_degree(const chfsi_params *params, double bUp,
+                                               int32_t is_first_scf, int32_t *degree) {
+  CHFSI_ARG(params && degree, "null argument");
+  *degree = (int32_t)solve_degree(*params, bUp, is_first_scf != 0);
+  return CHFSI_OK;
 }
 
 extern "C" chfsi_status chfsi_solver_set_params(chfsi_solver *s, const chfsi_params *params) {
@@ -974,7 +1061,8 @@ chfsi_status run_lanczos(chfsi_solver *s, chfsi_operator *op, int64_t M, double 
     CHFSI_HIP(hipMemcpy(gen.ptr, hv.data(), sizeof(double) * M, hipMemcpyHostToDevice));
     v0 = sptr<double>(gen);
   }
-  chfsi_status stl = lanczos_impl(s, op, v0, P.lanczos_steps, P.reproducible, lo, up, st);
+  chfsi_status stl = lanczos_impl(s, op, v0, P.lanczos_steps,
+                                  P.reproducible || P.lanczos_full_residual_margin, lo, up, st);
   gen.release();
   return stl;
 }
@@ -1017,9 +1105,7 @@ chfsi_status lanczos_scale_filter(chfsi_solver *s, chfsi_operator *op, double *X
   }
   CHFSI_HIP(hipEventRecord(ev[1], st));
 
-  // degree (…SolverDevice.cc:307-326)
-  unsigned m = P.cheby_degree > 0 ? (unsigned)P.cheby_degree : cheby_order_lookup(s->bUp);
-  if (is_first_scf) m = (unsigned)(m * P.first_scf_degree_factor);
+  const unsigned m = solve_degree(P, s->bUp, is_first_scf != 0);
 
   CHFSI_HIP(launch_scale_rows<double>(X, N, M, N, 1.0, op->d_sqrt_mass, st));
   const int B = std::min<int>(P.cheby_block_size > 0 ? P.cheby_block_size : N, N);
@@ -1075,13 +1161,20 @@ extern "C" chfsi_status chfsi_solve(chfsi_solver *s, chfsi_operator *op, double 
   CHFSI_TRY(lanczos_scale_filter(s, op, X, M, N, is_first_filtering_call, use_mixed_prec_overall,
                                  is_first_scf, &B, pe.ev, st));
 
-  // CholGS + Rayleigh-Ritz (SHEP route, …SolverDevice.cc:609-643)
   const bool mx = use_mixed_prec_overall != 0;
-  CHFSI_TRY(cholgs_impl(s, X, M, N, B, mx && P.mixed_prec_overlap, mx && P.mixed_prec_cgs_rotation,
-                        st));
-  CHFSI_HIP(hipEventRecord(pe.ev[3], st));
-  CHFSI_TRY(rr_impl(s, op, X, M, N, B, eig_host, mx && P.mixed_prec_xthx,
-                    mx && P.mixed_prec_rr_rotation, st));
+  if (P.use_shep) {
+    // CholGS + Rayleigh-Ritz (SHEP route, …SolverDevice.cc:609-643)
+    CHFSI_TRY(cholgs_impl(s, X, M, N, B, mx && P.mixed_prec_overlap,
+                          mx && P.mixed_prec_cgs_rotation, st));
+    CHFSI_HIP(hipEventRecord(pe.ev[3], st));
+    CHFSI_TRY(rr_impl(s, op, X, M, N, B, eig_host, mx && P.mixed_prec_xthx,
+                      mx && P.mixed_prec_rr_rotation, st));
+  } else {
+    // generalised Rayleigh-Ritz without CholGS (…SolverDevice.cc:645-664)
+    CHFSI_HIP(hipEventRecord(pe.ev[3], st));
+    CHFSI_TRY(gep_impl(s, op, X, M, N, B, eig_host, mx && P.mixed_prec_overlap,
+                       mx && P.mixed_prec_xthx, mx && P.mixed_prec_rr_rotation, st));
+  }
   CHFSI_HIP(hipEventRecord(pe.ev[4], st));
 
   if (compute_residual) {

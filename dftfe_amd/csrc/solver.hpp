@@ -28,6 +28,8 @@ struct chfsi_solver {
   chfsi::Scratch Xf, HXf, Qf, split32, rot32;
   // defect form of the FP32 filter: FP32 defect source, coefficient tables
   chfsi::Scratch R32, dcoef;
+  // generalised Rayleigh-Ritz: L^{-1} and a product buffer (2 N x N)
+  chfsi::Scratch gep;
   bool have_v0 = false;
   // largest residual norm over the lowest 90% of states of the last pass that
   // computed residuals, and the filter precision the last pass used
@@ -36,7 +38,7 @@ struct chfsi_solver {
   ~chfsi_solver() {
     if (blas) rocblas_destroy_handle(blas);
     for (chfsi::Scratch *s : {&Y64, &X32, &Y32, &HXb, &Hp, &rot, &evals, &info, &part, &lam, &v0,
-                              &split, &Xf, &HXf, &Qf, &split32, &rot32, &R32, &dcoef})
+                              &split, &Xf, &HXf, &Qf, &split32, &rot32, &R32, &dcoef, &gep})
       s->release();
   }
 };
@@ -45,6 +47,10 @@ namespace chfsi {
 
 // Degree lookup on the upper bound (…SolverDevice.cc:29-46, :96-105).
 unsigned cheby_order_lookup(double upper);
+// Degree of a solve pass: configured, or looked up (halved for all-electron
+// runs), times the first-SCF factor for pseudopotential runs
+// (…SolverDevice.cc:307-326).
+unsigned solve_degree(const chfsi_params &P, double bUp, bool is_first_scf);
 chfsi_status ensure_blas(chfsi_solver *s, hipStream_t st);
 // x . y over n doubles (synchronises the stream)
 chfsi_status dot(chfsi_solver *s, const double *x, const double *y, int64_t n, double *out,

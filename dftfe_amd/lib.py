@@ -70,6 +70,7 @@ class Params(ct.Structure):
         ("filter_columns", ct.c_int32),
         ("mixed_prec_switch_residual", ct.c_double),
         ("mixed_prec_defect_filter", ct.c_int32),
+        ("is_pseudopotential", ct.c_int32), ("lanczos_full_residual_margin", ct.c_int32),
     ]
 
 
@@ -78,6 +79,8 @@ _VP = ct.c_void_p
 SIGNATURES = {
     "chfsi_last_error": (ct.c_char_p, []),
     "chfsi_default_params": (None, [ct.POINTER(Params)]),
+    "chfsi_chebyshev_degree": (ct.c_int, [ct.POINTER(Params), ct.c_double, ct.c_int32,
+                                          c_int32_p]),
     "chfsi_operator_create": (ct.c_int, [ct.POINTER(OperatorDesc), ct.POINTER(_VP)]),
     "chfsi_operator_destroy": (None, [_VP]),
     "chfsi_operator_set_potential": (ct.c_int, [_VP, _VP, _VP]),

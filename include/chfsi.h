@@ -134,8 +134,13 @@ typedef struct {
   double first_scf_degree_factor;  /* CHEBYSHEV FILTER POLYNOMIAL DEGREE SCALING
                                       FACTOR FIRST SCF (1.34)                   */
   int32_t filter_precision;        /* chfsi_precision for the filter bulk when
-                                      use_mixed_prec_overall is set             */
-  int32_t use_shep;                /* SUBSPACE PROJ SHEP GPU: 1 = CholGS + RR   */
+                                      use_mixed_prec_overall is set (default
+                                      CHFSI_FP64, as USE MIXED PREC CHEBY is false
+                                      by default, utils/dftParameters.cc:1262)  */
+  int32_t use_shep;                /* SUBSPACE PROJ SHEP GPU: 1 = CholGS + RR
+                                      (rayleighRitz), 0 = Rayleigh-Ritz in the
+                                      generalised form without CholGS
+                                      (rayleighRitzGEP, rayleighRitzDevice.cc:355-819) */
   int32_t fp64_tail_degrees;       /* last degrees of a mixed-precision filter run
                                       in FP64 (0: whole filter in filter_precision) */
   /* Mixed-precision dense steps, applied when use_mixed_prec_overall is set:
@@ -168,10 +173,34 @@ typedef struct {
                                       (same recurrence plus the source p_k(theta) R),
                                       so the FP32 rounding scales with the residual.
                                       No reference counterpart; 0: plain FP32 */
+  int32_t is_pseudopotential;      /* PSEUDOPOTENTIAL CALCULATION (default false,
+                                      utils/dftParameters.cc:1173): the first-SCF
+                                      degree factor applies only when set, and a
+                                      degree taken from the table is halved when not
+                                      set (…SolverDevice.cc:307-326; CGS is the only
+                                      orthogonalisation on the device)             */
+  int32_t lanczos_full_residual_margin; /* 1: bUp = ceil(theta_max + |f|), the margin
+                                      the reference takes under REPRODUCIBLE OUTPUT,
+                                      without that flag's other effects (Lanczos
+                                      steps, first-call bLow factor); 0: |f| / 10
+                                      unless reproducible
+                                      (linearAlgebraOperationsDevice.cc:510-525).
+                                      With 10 Lanczos steps the |f| / 10 margin can
+                                      leave bUp below the top of the spectrum, and
+                                      the filter then amplifies the states above
+                                      bUp every pass                               */
 } chfsi_params;
 
 const char *chfsi_last_error(void);
 void chfsi_default_params(chfsi_params *params);
+
+/* Chebyshev degree of a solve pass (…SolverDevice.cc:29-46 order_lookup,
+ * :96-105 setChebyshevOrder, :307-326): params->cheby_degree, or when it is 0
+ * the table entry for bUp, halved (truncated) unless is_pseudopotential; then
+ * times first_scf_degree_factor (truncated) when is_first_scf and
+ * is_pseudopotential.  Host only; no device needed. */
+chfsi_status chfsi_chebyshev_degree(const chfsi_params *params, double bUp, int32_t is_first_scf,
+                                    int32_t *degree);
 
 /* ---------------- operator (operatorDFTDeviceClass) ---------------- */
 

@@ -81,7 +81,7 @@ struct dftParameters {
   bool reproducible_output = false;
   bool reuseLanczosUpperBoundFromFirstCall = false;
   double chebyshevFilterPolyDegreeFirstScfScalingFactor = 1.34;
-  bool isPseudopotential = true;
+  bool isPseudopotential = false;
   bool useSubspaceProjectedSHEPGPU = true;
   bool useMixedPrecCheby = false;
   bool useMixedPrecCGS_O = false;
@@ -103,6 +103,14 @@ inline chfsi_params toChfsiParams(const dftParameters &p) {
   c.reuse_lanczos_upper = p.reuseLanczosUpperBoundFromFirstCall ? 1 : 0;
   c.first_scf_degree_factor = p.chebyshevFilterPolyDegreeFirstScfScalingFactor;
   c.use_shep = p.useSubspaceProjectedSHEPGPU ? 1 : 0;
+  c.is_pseudopotential = p.isPseudopotential ? 1 : 0;
+  /* the filter bulk leaves FP64 only under USE MIXED PREC CHEBY, and then only
+   * in passes the caller runs with useMixedPrecOverall
+   * (linearAlgebraOperationsDevice.cc:627).  The library's FP32 recurrence is
+   * a stronger form than the reference's, which keeps the cell products in
+   * FP64 and sends only the ghost exchange in FP32
+   * (kohnShamDFTOperatorDevice.cc:3895-3926). */
+  c.filter_precision = p.useMixedPrecCheby ? CHFSI_FP32 : CHFSI_FP64;
   c.mixed_prec_overlap = p.useMixedPrecCGS_O ? 1 : 0;
   c.mixed_prec_cgs_rotation = p.useMixedPrecCGS_SR ? 1 : 0;
   c.mixed_prec_rr_rotation = p.useMixedPrecSubspaceRotRR ? 1 : 0;

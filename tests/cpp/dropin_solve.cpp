@@ -10,6 +10,8 @@
 //   eig1.f64 res1.f64 X1.f64   first solve (full Rayleigh-Ritz)
 //   eig2.f64 res2.f64          second solve after the caller's bound update
 //   Xnr.f64                    solveNoRR, one pass, from X0
+//   eig_mixed_default.f64      solve from X0 with useMixedPrecOverall and default parameters
+//   eig_gep.f64 res_gep.f64    solve from X0 on the generalised Rayleigh-Ritz route
 //   eig_split.f64 Xfrac.f64    spectrum-split solve from X0 (top n_split states)
 //   hx_gga.f64                 H~ X0 after a GGA rebuild
 // Exit status 0 on success; any library error ends the program with a message.
@@ -167,6 +169,27 @@ int main(int argc, char **argv) {
     std::vector<double> unused;
     solver.solveNoRR(op, blas, elpa, X.begin(), unsigned(M * N), N, unused, ccl, comm, 1, false);
     save("Xnr.f64", toHost(X.begin(), size_t(M) * N).data(), size_t(M) * N);
+
+    // useMixedPrecOverall with the default parameters (USE MIXED PREC CHEBY
+    // false): the filter stays FP64, as in the reference
+    solver.reinitSpectrumBounds(meta.at("a0"), meta.at("bLow"), meta.at("bUp"));
+    toDevice(X.begin(), X0);
+    solver.solve(op, blas, elpa, X.begin(), nullptr, unsigned(M * N), N, eig, res, ccl, comm,
+                 false, true, /*useMixedPrecOverall=*/true);
+    save("eig_mixed_default.f64", eig.data(), eig.size());
+
+    // SUBSPACE PROJ SHEP GPU false: generalised Rayleigh-Ritz without CholGS
+    {
+      dftfe::dftParameters prm_gep = prm;
+      prm_gep.useSubspaceProjectedSHEPGPU = false;
+      dftfe::chebyshevOrthogonalizedSubspaceIterationSolverDevice solver_gep(
+          comm, comm, meta.at("a0"), meta.at("bLow"), meta.at("bUp"), prm_gep);
+      toDevice(X.begin(), X0);
+      solver_gep.solve(op, blas, elpa, X.begin(), nullptr, unsigned(M * N), N, eig, res, ccl,
+                       comm, false, true);
+      save("eig_gep.f64", eig.data(), eig.size());
+      save("res_gep.f64", res.data(), res.size());
+    }
 
     // spectrum split from X0: eigenValues sized to the kept states
     toDevice(X.begin(), X0);

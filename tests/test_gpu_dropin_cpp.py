@@ -98,6 +98,23 @@ def test_dropin_solve_two_passes(run):
     assert np.abs(_read(d, "res2.f64", (N,)) - res2_o).max() <= 1e-8
 
 
+def test_dropin_mixed_overall_default_params_stays_fp64(run):
+    """useMixedPrecOverall = true with the reference's default parameters
+    (USE MIXED PREC CHEBY false) must run the FP64 filter: FP64 parity."""
+    prob, ref, bounds, X0, _, d = run
+    lam_o, _, _ = O.chfsi_pass(ref, X0, *bounds, DEGREE, BLOCK)
+    assert np.abs(_read(d, "eig_mixed_default.f64", (N,)) - lam_o).max() <= 1e-10
+
+
+def test_dropin_gep_route(run):
+    """useSubspaceProjectedSHEPGPU = false: Rayleigh-Ritz in the generalised form
+    (rayleighRitzGEP) on the filtered, unorthonormalised block."""
+    prob, ref, bounds, X0, _, d = run
+    lam_o, res_o, _ = O.chfsi_pass(ref, X0, *bounds, DEGREE, BLOCK, route="gep")
+    assert np.abs(_read(d, "eig_gep.f64", (N,)) - lam_o).max() <= 1e-10
+    assert np.abs(_read(d, "res_gep.f64", (N,)) - res_o).max() <= 1e-8
+
+
 def test_dropin_solve_no_rr(run):
     """solveNoRR, one pass: the filtered block after CholGS, in the FE basis
     (CholGS is unique: X L^{-T} with the Cholesky factor of X^T X)."""

@@ -447,3 +447,54 @@ def test_density_integrates_to_occupation(brute):
     one = np.ones((prob.M, 1))
     rho = O.density_at_quadrature(prob, one, np.array([1.0]), spin_factor=1.0)
     assert abs((rho * prob.JxW[None, :]).sum() - prob.wl.L ** 3) < 1e-10
+
+
+# ----------------------------------------------------------------------------
+# Fermi-Dirac occupations (utils/dftUtils.cc:56-64)
+# ----------------------------------------------------------------------------
+
+
+def test_fermi_dirac_closed_forms():
+    kT, mu = 0.01, -0.3
+    x = np.linspace(-0.5, 0.5, 2001)
+    f_plus, f_minus = O.fermi_dirac(mu + x, mu, kT), O.fermi_dirac(mu - x, mu, kT)
+    # f(mu) = 1/2 exactly, particle-hole symmetry f(mu + x) + f(mu - x) = 1
+    assert O.fermi_dirac(np.array([mu]), mu, kT)[0] == 0.5
+    assert np.abs(f_plus + f_minus - 1.0).max() <= 2.3e-16
+    # limits without overflow; monotone decreasing
+    far = O.fermi_dirac(np.array([mu - 1e3, mu + 1e3, mu - 1e6 * kT, mu + 1e6 * kT]), mu, kT)
+    assert np.all(np.isfinite(far))
+    assert far[0] == 1.0 and far[2] == 1.0 and far[1] < 1e-300 and far[3] == 0.0
+    assert np.all(np.diff(f_plus) <= 0)
+    # the textbook form 1 / (1 + e^t) where it does not overflow (|t| <= 30)
+    t = np.linspace(-30, 30, 601)
+    ref = 1.0 / (1.0 + np.exp(t))
+    got = O.fermi_dirac(mu + t * kT, mu, kT)
+    assert np.abs(got - ref).max() < 1e-15
+    assert np.abs(got / ref - 1.0).max() < 1e-13
+    # slope at the chemical potential: -1 / (4 kT)
+    h = 1e-6
+    slope = (O.fermi_dirac(np.array([mu + h]), mu, kT) - O.fermi_dirac(np.array([mu - h]), mu, kT))
+    assert slope[0] / (2 * h) == pytest.approx(-1.0 / (4 * kT), rel=1e-8)
+    # the reference's branch form at its switch point and either side
+    for fac in (-3.0, -1e-9, 0.0, 1e-9, 3.0):
+        want = np.exp(-fac) / (1 + np.exp(-fac)) if fac >= 0 else 1.0 / (1 + np.exp(fac))
+        assert O.fermi_dirac(np.array([mu + fac * kT]), mu, kT)[0] == pytest.approx(want,
+                                                                                   rel=1e-15)
+
+
+def test_fermi_occupations_matches_oracle():
+    """The product's bisection on the chemical potential (dftfe_amd.autograd,
+    CPU torch): occupations equal the oracle's Fermi-Dirac at the returned mu,
+    and they sum to the requested count."""
+    torch = pytest.importorskip("torch")
+    from dftfe_amd.autograd import fermi_occupations
+
+    rng = np.random.default_rng(5)
+    eps = np.sort(rng.uniform(-1.0, 2.0, 256))
+    for n_occ, kT in ((204.8, 0.01), (100.0, 0.001), (10.5, 0.05)):
+        occ, mu = fermi_occupations(torch.tensor(eps), n_occ, kT)
+        occ = occ.numpy()
+        assert np.abs(occ - O.fermi_dirac(eps, mu, kT)).max() < 1e-15
+        assert occ.sum() == pytest.approx(n_occ, abs=1e-9)
+        assert np.all((occ >= 0) & (occ <= 1))
