@@ -112,6 +112,13 @@ WORKLOADS = {
     # periodic 24 bohr cube, 12^3 cells, p=5, 32 wells, 256 wavefunctions
     "cube24_p5_n256": Workload("cube24_p5_n256", 24.0, 12, 5, 6, 32, 256, 256,
                                20, seed=1004),
+    # small meshes at the production degrees: the p=6 (343 nodes, 8^3 Gauss
+    # points) and p=5 (216 nodes, 6^3 points) kernel instantiations at sizes
+    # the oracle applies in well under a second
+    "cube18_p6_n64": Workload("cube18_p6_n64", 18.0, 6, 6, 8, 8, 64, 32, 30,
+                              seed=1006),
+    "cube18_p5_n64": Workload("cube18_p5_n64", 18.0, 6, 5, 6, 8, 64, 32, 20,
+                              seed=1005),
     # tiny cases for brute-force checks (dense eigendecomposition)
     "tiny": Workload("tiny", 8.0, 2, 2, 3, 2, 8, 8, 8, seed=7),
     "brute": Workload("brute", 10.0, 4, 3, 4, 3, 16, 16, 12, seed=11),

@@ -32,6 +32,19 @@ def test_problem_checksums(golden):
         assert got[k] == pytest.approx(v, rel=1e-12, abs=1e-12), k
 
 
+@pytest.mark.parametrize("name", ["cube30_p6_n1024", "cube24_p5_n256"])
+def test_problem_checksums_full_size(name):
+    """The full-size workloads regenerate the inputs their fixtures were made from."""
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    from make_golden import fixture_path, problem_checksums
+
+    with open(fixture_path(name)) as f:
+        gold = json.load(f)
+    got = problem_checksums(fe.build_workload(name), N=gold["states"])
+    for k, v in gold["checksums"].items():
+        assert got[k] == pytest.approx(v, rel=1e-12, abs=1e-12), k
+
+
 def test_oracle_pass_matches_golden(golden):
     prob = fe.build_workload("cube20_p4_n64")
     ref = O.OracleOperator(prob)
