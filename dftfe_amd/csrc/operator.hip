@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <vector>
 
 #include "bloch.hpp"
@@ -401,6 +402,41 @@ extern "C" int64_t chfsi_operator_n_dofs(const chfsi_operator *op) { return op ?
 
 namespace chfsi {
 
+// ---------------------------------------------------------------------------
+// Fixed device scratch of the no-allocation HX entry point
+// ---------------------------------------------------------------------------
+// One array per GPU, resident for the life of the process.  In host code the
+// symbol is only a stand-in: its device address is looked up once per device
+// (with that device current) and every use goes through that address.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void *g_scratch_addr[kMaxDevices] = {};
+}  // namespace
+
+// Device address of `scratch` on the current device (looked up on first use).
+static chfsi_status noalloc_scratch(void **out) {
+  int dev = 0;
+  CHFSI_HIP(hipGetDevice(&dev));
+  CHFSI_ARG(dev >= 0 && dev < kMaxDevices, "device index %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_addr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                hipGetErrorString(e));
+      return CHFSI_ERR_HIP;
+    }
+    g_scratch_addr[dev] = p;
+  }
+  *out = g_scratch_addr[dev];
+  return CHFSI_OK;
+}
+
 template <typename T>
 struct OpData;
 template <>
@@ -425,7 +461,8 @@ struct OpData<float> {
 template <typename T>
 chfsi_status hx_fused(chfsi_operator *op, const T *src, int64_t ld_src, T *dst, int64_t ld_dst,
                       int B, T alpha, T beta, T gamma, int beta_mode, bool scaled,
-                      hipStream_t s, bool bf16, const T *R, int64_t ldR, const T *rcoef) {
+                      hipStream_t s, bool bf16, const T *R, int64_t ldR, const T *rcoef,
+                      void *ext_scratch) {
   CHFSI_ARG(!bf16 || sizeof(T) == 4, "the BF16 apply works on FP32 multivectors");
   CHFSI_ARG(!R || (!bf16 && sizeof(T) == 4 && rcoef && ldR >= B), "bad defect source");
   if (!op->has_H) {
@@ -437,10 +474,22 @@ chfsi_status hx_fused(chfsi_operator *op, const T *src, int64_t ld_src, T *dst, 
   const T *sc = scaled ? OpData<T>::s(op) : nullptr;
   T *W = nullptr;
   if (op->n_slots > 0) {
-    Scratch &P = OpData<T>::partial(op);
-    Scratch &Wb = OpData<T>::W(op);
-    CHFSI_TRY(P.ensure(sizeof(T) * size_t(op->n_slots) * B));
-    CHFSI_TRY(Wb.ensure(sizeof(T) * size_t(op->n_proj) * B));
+    // projection partials (n_slots x B) and V-scaled projections (n_proj x B):
+    // the operator's own buffers, or a caller-provided scratch laid out as
+    // [partials | W] with W at the next 256-byte boundary
+    T *Pp = nullptr;
+    if (ext_scratch) {
+      Pp = static_cast<T *>(ext_scratch);
+      W = reinterpret_cast<T *>(static_cast<char *>(ext_scratch) +
+                                noalloc_w_offset(sizeof(T), op->n_slots, B));
+    } else {
+      Scratch &P = OpData<T>::partial(op);
+      Scratch &Wb = OpData<T>::W(op);
+      CHFSI_TRY(P.ensure(sizeof(T) * size_t(op->n_slots) * B));
+      CHFSI_TRY(Wb.ensure(sizeof(T) * size_t(op->n_proj) * B));
+      Pp = static_cast<T *>(P.ptr);
+      W = static_cast<T *>(Wb.ptr);
+    }
     ProjLaunch<T> pl{};
     pl.cells = op->d_nl_cells;
     pl.ncells = op->n_nl_cells;
@@ -452,15 +501,14 @@ chfsi_status hx_fused(chfsi_operator *op, const T *src, int64_t ld_src, T *dst, 
     pl.src = src;
     pl.ld_src = ld_src;
     pl.s_in = sc;
-    pl.partial = static_cast<T *>(P.ptr);
+    pl.partial = Pp;
     pl.ldP = B;
     pl.B = B;
     pl.max_srb = op->max_srb;
     pl.variant = sizeof(T) == 8 ? op->proj_variant64 : op->proj_variant32;
     CHFSI_HIP(launch_proj<T>(pl, s));
-    W = static_cast<T *>(Wb.ptr);
     CHFSI_HIP(launch_proj_reduce<T>(op->d_proj_ptr, op->d_proj_slots, op->d_proj_V, op->n_proj,
-                                    static_cast<T *>(P.ptr), B, W, B, B, s));
+                                    Pp, B, W, B, B, s));
   }
   HxLaunch<T> a{};
   a.map = op->d_map;
@@ -506,10 +554,10 @@ chfsi_status hx_fused(chfsi_operator *op, const T *src, int64_t ld_src, T *dst, 
 template chfsi_status hx_fused<double>(chfsi_operator *, const double *, int64_t, double *,
                                        int64_t, int, double, double, double, int, bool,
                                        hipStream_t, bool, const double *, int64_t,
-                                       const double *);
+                                       const double *, void *);
 template chfsi_status hx_fused<float>(chfsi_operator *, const float *, int64_t, float *, int64_t,
                                       int, float, float, float, int, bool, hipStream_t, bool,
-                                      const float *, int64_t, const float *);
+                                      const float *, int64_t, const float *, void *);
 
 }  // namespace chfsi
 
@@ -520,6 +568,27 @@ extern "C" chfsi_status chfsi_hx(chfsi_operator *op, const double *src, int64_t 
   // dst <- (scale_flag ? dst : S dst) + scalar * S H S src
   CHFSI_TRY(hx_fused<double>(op, src, ld_src, dst, ld_dst, B, scalar, 1.0, 0.0,
                              scale_flag ? 0 : 2, true, as_stream(stream)));
+  if (op->nc > 0) CHFSI_HIP(launch_zero_rows(op->d_c_rows, op->nc, dst, ld_dst, B, as_stream(stream)));
+  return CHFSI_OK;
+}
+
+extern "C" chfsi_status chfsi_hx_noalloc(chfsi_operator *op, const double *src, int64_t ld_src,
+                                         double *dst, int64_t ld_dst, int32_t B,
+                                         int32_t scale_flag, double scalar, void *stream) {
+  CHFSI_ARG(op && src && dst, "null argument");
+  CHFSI_ARG(B > 0, "bad block width");
+  void *ws = nullptr;
+  if (op->n_slots > 0) {
+    const size_t need = noalloc_scratch_bytes(sizeof(double), op->n_slots, op->n_proj, B);
+    CHFSI_ARG(need <= kNoAllocScratchBytes,
+              "chfsi_hx_noalloc: %d columns need %zu bytes of projection scratch "
+              "(%d slots, %d projectors); the fixed scratch holds %zu",
+              B, need, op->n_slots, op->n_proj, kNoAllocScratchBytes);
+    CHFSI_TRY(noalloc_scratch(&ws));
+  }
+  CHFSI_TRY(hx_fused<double>(op, src, ld_src, dst, ld_dst, B, scalar, 1.0, 0.0,
+                             scale_flag ? 0 : 2, true, as_stream(stream), false, nullptr, 0,
+                             nullptr, ws));
   if (op->nc > 0) CHFSI_HIP(launch_zero_rows(op->d_c_rows, op->nc, dst, ld_dst, B, as_stream(stream)));
   return CHFSI_OK;
 }

@@ -23,11 +23,20 @@ struct Scratch {
 // (see HxLaunch for base / beta_mode).  bf16 selects the BF16-input kernel
 // (T = float only).  R (nullable, ld ldR) adds rcoef[v] * R[:, v] to every row
 // once (the defect form of the filter; FP32, not with bf16).
+// Layout of a caller-provided HX scratch: projection partials (n_slots x B),
+// then the V-scaled projections (n_proj x B) at the next 256-byte boundary.
+inline size_t noalloc_w_offset(size_t elem, int n_slots, int B) {
+  return (elem * size_t(n_slots) * size_t(B) + 255) / 256 * 256;
+}
+inline size_t noalloc_scratch_bytes(size_t elem, int n_slots, int n_proj, int B) {
+  return noalloc_w_offset(elem, n_slots, B) + elem * size_t(n_proj) * size_t(B);
+}
+
 template <typename T>
 chfsi_status hx_fused(chfsi_operator *op, const T *src, int64_t ld_src, T *dst, int64_t ld_dst,
                       int B, T alpha, T beta, T gamma, int beta_mode, bool scaled, hipStream_t s,
                       bool bf16 = false, const T *R = nullptr, int64_t ldR = 0,
-                      const T *rcoef = nullptr);
+                      const T *rcoef = nullptr, void *ext_scratch = nullptr);
 
 }  // namespace chfsi
 

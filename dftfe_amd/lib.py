@@ -96,6 +96,8 @@ SIGNATURES = {
     "chfsi_operator_n_dofs": (ct.c_int64, [_VP]),
     "chfsi_hx": (ct.c_int, [_VP, _VP, ct.c_int64, _VP, ct.c_int64, ct.c_int32, ct.c_int32,
                             ct.c_double, _VP]),
+    "chfsi_hx_noalloc": (ct.c_int, [_VP, _VP, ct.c_int64, _VP, ct.c_int64, ct.c_int32,
+                                    ct.c_int32, ct.c_double, _VP]),
     "chfsi_hx_fp32": (ct.c_int, [_VP, _VP, ct.c_int64, _VP, ct.c_int64, ct.c_int32, ct.c_int32,
                                  ct.c_float, _VP]),
     "chfsi_hx_bf16": (ct.c_int, [_VP, _VP, ct.c_int64, _VP, ct.c_int64, ct.c_int32, ct.c_int32,

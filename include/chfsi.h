@@ -253,6 +253,18 @@ int64_t chfsi_operator_n_dofs(const chfsi_operator *op);
 chfsi_status chfsi_hx(chfsi_operator *op, const double *src, int64_t ld_src, double *dst,
                       int64_t ld_dst, int32_t B, int32_t scale_flag, double scalar, void *stream);
 
+/* chfsi_hx without any device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, kernels
+ * and results (bit for bit) as chfsi_hx; the nonlocal projection scratch,
+ * n_slots*B + n_proj*B doubles (plus 256-byte alignment), lives in a fixed
+ * 64 MiB device array per GPU instead of operator-owned buffers, and larger
+ * blocks are rejected with CHFSI_ERR_ARG.  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (other streams, other threads) race. */
+chfsi_status chfsi_hx_noalloc(chfsi_operator *op, const double *src, int64_t ld_src,
+                              double *dst, int64_t ld_dst, int32_t B, int32_t scale_flag,
+                              double scalar, void *stream);
+
 /* Same operator with FP32 multivectors, FP32 cell matrices, FP32 MFMA. */
 chfsi_status chfsi_hx_fp32(chfsi_operator *op, const float *src, int64_t ld_src, float *dst,
                            int64_t ld_dst, int32_t B, int32_t scale_flag, float scalar,
