@@ -125,8 +125,8 @@ def band_energy(op, V: torch.Tensor, X0: torch.Tensor, bounds, m: int, occ: torc
 def hellmann_feynman_gradient(op, X_fe: torch.Tensor, occ: torch.Tensor, spin: float = 2.0):
     """sum_i s f_i |psi_i(q)|^2 JxW(c, q) from FE-basis eigenvectors (density route)."""
     rho = op.density(X_fe, spin * occ)
-    jxw = torch.as_tensor(op.prob.JxW, device=rho.device)
-    return rho * jxw[None, :]
+    jxw = torch.as_tensor(op.prob.jxw_cells, device=rho.device)
+    return rho * jxw
 
 
 __all__ = ["HXFunction", "FilterFunction", "band_energy", "fermi_occupations",

@@ -61,7 +61,19 @@ class KohnShamOperatorDevice:
         d.inv_sqrt_mass = arr("isq", prob.inv_sqrt_mass, np.float64, ct.c_double)
         d.shape_values = arr("N", prob.Nq, np.float64, ct.c_double)
         d.stiffness = arr("K", prob.K_e, np.float64, ct.c_double)
-        if prob.Gq is not None:
+        graded = getattr(prob, "cell_size", None) is not None
+        if graded:
+            # per-cell geometry: each cell's stiffness integral, and the
+            # reference-cell gradients with the cells' (Cartesian) inverse Jacobians
+            sc = prob.cell_scale
+            d.cell_stiffness = arr("Kc", sc[:, None, None] * prob.K_e[None], np.float64,
+                                   ct.c_double)
+            if prob.Gq is not None:
+                d.shape_gradients = arr("Gq", prob.Gq * prob.h, np.float64, ct.c_double)
+                d.inverse_jacobians = arr("iJ", np.repeat((1.0 / prob.cell_size)[:, None], 3,
+                                                          axis=1), np.float64, ct.c_double)
+                d.jacobian_kind = 3
+        elif prob.Gq is not None:
             d.shape_gradients = arr("Gq", prob.Gq, np.float64, ct.c_double)
         d.n_proj = prob.n_proj
         d.n_pairs = len(prob.pair_cell)
@@ -72,13 +84,16 @@ class KohnShamOperatorDevice:
             d.pair_C = arr("pC", prob.pair_C, np.float64, ct.c_double)
             d.proj_V = arr("pV", prob.proj_V, np.float64, ct.c_double)
         if len(prob.slave_rows):
-            ns = len(prob.slave_rows)
-            d.n_constrained = ns
-            d.constraint_rows = arr("cr", prob.slave_rows, np.int64, ct.c_int64)
-            d.constraint_ptr = arr("cp", np.arange(ns + 1), np.int64, ct.c_int64)
-            d.constraint_cols = arr("cc", prob.master_rows, np.int64, ct.c_int64)
-            d.constraint_vals = arr("cv", np.ones(ns), np.float64, ct.c_double)
-        d.jxw = arr("jxw", np.tile(prob.JxW, prob.C), np.float64, ct.c_double)
+            rows, ptr, cols, vals = prob.constraint_csr()
+            d.n_constrained = len(rows)
+            d.constraint_rows = arr("cr", rows, np.int64, ct.c_int64)
+            d.constraint_ptr = arr("cp", ptr, np.int64, ct.c_int64)
+            d.constraint_cols = arr("cc", cols, np.int64, ct.c_int64)
+            d.constraint_vals = arr("cv", vals, np.float64, ct.c_double)
+        if graded:
+            d.jxw = arr("jxw", prob.jxw_cells, np.float64, ct.c_double)
+        else:
+            d.jxw = arr("jxw", np.tile(prob.JxW, prob.C), np.float64, ct.c_double)
         self.kpoints = None
         if kpoints is not None:
             from . import fe
@@ -466,7 +481,7 @@ class ChebyshevSolverDevice:
                  filter_columns: int = 0, mixed_prec_switch_residual: float = 0.0,
                  mixed_prec_defect_filter: bool = False, use_shep: bool = True,
                  lanczos_full_residual_margin: bool = False,
-                 is_pseudopotential: bool = False):
+                 is_pseudopotential: bool = False, mixed_prec_fp64_top_columns: int = 0):
         """mixed_dense: FP32 off-diagonal blocks in the overlap, CholGS rotation,
         X^T H X and Rayleigh-Ritz rotation when a pass runs with
         use_mixed_prec_overall (the reference's USE MIXED PREC CGS O / CGS SR /
@@ -482,7 +497,9 @@ class ChebyshevSolverDevice:
         lanczos_full_residual_margin: bUp = ceil(theta_max + |f|) (the
         reference's reproducible-output margin) instead of |f| / 10.
         is_pseudopotential: gates the first-SCF degree factor and the halving
-        of a looked-up degree for all-electron runs."""
+        of a looked-up degree for all-electron runs.
+        mixed_prec_fp64_top_columns: with a lower filter precision, the last
+        k columns (highest Ritz values) are filtered in FP64."""
         self.lib = L.load()
         p = L.Params()
         self.lib.chfsi_default_params(ct.byref(p))
@@ -500,6 +517,7 @@ class ChebyshevSolverDevice:
         p.use_shep = int(use_shep)
         p.lanczos_full_residual_margin = int(lanczos_full_residual_margin)
         p.is_pseudopotential = int(is_pseudopotential)
+        p.mixed_prec_fp64_top_columns = int(mixed_prec_fp64_top_columns)
         for f in ("mixed_prec_overlap", "mixed_prec_cgs_rotation", "mixed_prec_xthx",
                   "mixed_prec_rr_rotation", "mixed_prec_xthx_spectrum_split"):
             setattr(p, f, int(mixed_dense))

@@ -177,7 +177,7 @@ chfsi_status zapply_round(chfsi_operator *op, const ZHxItem<T> *dev_items,
     a.nl_off = op->d_nl_off;
     a.slot_proj = op->d_slot_proj;
   }
-  for (int c = 0; c < 8; ++c) {
+  for (int c = 0; c < op->ncolors; ++c) {
     a.cells = op->d_cells_by_color + op->color_off[c];
     a.ncells = op->color_off[c + 1] - op->color_off[c];
     if (a.ncells == 0) continue;

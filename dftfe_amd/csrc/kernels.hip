@@ -637,19 +637,23 @@ __global__ __launch_bounds__(64 * NW) void hx_bf16_kernel(HxLaunch<float> a, con
         const uint32_t w = smap[row];
         const int64_t g = w & 0x7fffffffu;
         const float so = a.s_out ? a.s_out[g] : 1.0f;
+        // zero output scale: constrained row, plain cell sums (no combine)
+        const bool cons = so == 0.0f;
         const int64_t addr = g * a.ld_dst + v;
         float base;
         if (w >> 31) {
           base = 0.0f;
-          if (a.beta_mode == 0)
-            base = a.beta * a.dst[addr];
-          else if (a.beta_mode == 2)
-            base = a.beta * so * a.dst[addr];
-          if (a.gamma != 0.0f) base += a.gamma * a.src[g * a.ld_src + v];
+          if (!cons) {
+            if (a.beta_mode == 0)
+              base = a.beta * a.dst[addr];
+            else if (a.beta_mode == 2)
+              base = a.beta * so * a.dst[addr];
+            if (a.gamma != 0.0f) base += a.gamma * a.src[g * a.ld_src + v];
+          }
         } else {
           base = a.dst[addr];
         }
-        a.dst[addr] = base + a.alpha * so * acc[j][cb][r];
+        a.dst[addr] = base + a.alpha * (cons ? 1.0f : so) * acc[j][cb][r];
       }
     }
   }
@@ -840,11 +844,13 @@ __global__ void proj_reduce_kernel(const int *proj_ptr, const int *proj_slots,
 // transposed entries, so each pair's quadrature sum is formed once.
 // ---------------------------------------------------------------------------
 template <bool GGA>
-__global__ __launch_bounds__(256, 3) void build_h_kernel(const double *K, const double *Nq,
-                                                      const double *G, const double *vJxW,
-                                                      const double *gJxW, int C, int ne,
-                                                      int ne_pad, int nq, double *Ht64,
-                                                      float *Ht32, uint16_t *Ht16) {
+__global__ __launch_bounds__(256, 3) void build_h_kernel(const double *K, int64_t K_stride,
+                                                      const double *Nq, const double *G,
+                                                      const double *invJ, int jac,
+                                                      const double *vJxW, const double *gJxW,
+                                                      int C, int ne, int ne_pad, int nq,
+                                                      double *Ht64, float *Ht32,
+                                                      uint16_t *Ht16) {
   constexpr int QC = GGA ? 16 : 32;            // quadrature points per LDS chunk
   constexpr int CT = 64;                       // cells per workgroup
   constexpr int NS = 18;                       // padded row of 16 nodes
@@ -900,10 +906,29 @@ __global__ __launch_bounds__(256, 3) void build_h_kernel(const double *K, const 
         sW[qq][c] = v;
     }
     if constexpr (GGA) {
-      for (int e = tid; e < CT * QC * 3; e += 256) {
-        const int c = e / (QC * 3), r = e % (QC * 3), qq = r / 3, d = r % 3;
-        const int q = q0 + qq, cell = c0 + c;
-        sW[qq][c * 4 + 1 + d] = (q < nq && cell < C) ? gJxW[((int64_t)cell * nq + q) * 3 + d] : 0.0;
+      // with per-cell inverse Jacobians the shape gradients are the reference
+      // cell's, and the Jacobian moves onto the weights:
+      // sum_d g_d (J grad_ref N)_d = sum_e (sum_d g_d J[d][e]) d_e N_ref
+      for (int e = tid; e < CT * QC; e += 256) {
+        const int c = e / QC, qq = e % QC, q = q0 + qq, cell = c0 + c;
+        const bool ok = q < nq && cell < C;
+        const double *gp = gJxW + ((int64_t)(ok ? cell : 0) * nq + (ok ? q : 0)) * 3;
+        const double g0 = ok ? gp[0] : 0.0, g1 = ok ? gp[1] : 0.0, g2 = ok ? gp[2] : 0.0;
+        double w0 = g0, w1 = g1, w2 = g2;
+        if (jac == 3 && ok) {
+          const double *J = invJ + (int64_t)cell * 3;
+          w0 = g0 * J[0];
+          w1 = g1 * J[1];
+          w2 = g2 * J[2];
+        } else if (jac == 9 && ok) {
+          const double *J = invJ + (int64_t)cell * 9;
+          w0 = g0 * J[0] + g1 * J[3] + g2 * J[6];
+          w1 = g0 * J[1] + g1 * J[4] + g2 * J[7];
+          w2 = g0 * J[2] + g1 * J[5] + g2 * J[8];
+        }
+        sW[qq][c * 4 + 1] = w0;
+        sW[qq][c * 4 + 2] = w1;
+        sW[qq][c * 4 + 3] = w2;
       }
     }
     __syncthreads();
@@ -956,10 +981,11 @@ __global__ __launch_bounds__(256, 3) void build_h_kernel(const double *K, const 
       const int cell = c0 + cb * 16 + c;
       if (cell >= C) break;  // e grows with c: no later element has a cell
       const int64_t base = (int64_t)cell * per;
+      const double *Kc = K + (int64_t)cell * K_stride;
       {
         const int il = wi >> 4, jl = wi & 15, i = ib * 16 + il, j = jb * 16 + jl;
         const double q = smem[(c * 16 + il) * OS + jl];
-        const double h = (i < ne && j < ne) ? 0.5 * K[(int64_t)i * ne + j] + q : 0.0;
+        const double h = (i < ne && j < ne) ? 0.5 * Kc[(int64_t)i * ne + j] + q : 0.0;
         const int64_t o = base + (int64_t)jb * ne_pad * 16 + (int64_t)i * 16 + jl;
         Ht64[o] = h;
         if (Ht32) Ht32[o] = (float)h;
@@ -968,7 +994,7 @@ __global__ __launch_bounds__(256, 3) void build_h_kernel(const double *K, const 
       if (ib != jb) {
         const int jl = wi >> 4, il = wi & 15, i = ib * 16 + il, j = jb * 16 + jl;
         const double q = smem[(c * 16 + il) * OS + jl];
-        const double h = (i < ne && j < ne) ? 0.5 * K[(int64_t)j * ne + i] + q : 0.0;
+        const double h = (i < ne && j < ne) ? 0.5 * Kc[(int64_t)j * ne + i] + q : 0.0;
         const int64_t o = base + (int64_t)ib * ne_pad * 16 + (int64_t)j * 16 + il;
         Ht64[o] = h;
         if (Ht32) Ht32[o] = (float)h;
@@ -1183,35 +1209,41 @@ __global__ void zero_upper_kernel(double *A, int n, int64_t lda) {
   if (i < j) A[j * lda + i] = 0.0;
 }
 
+// x[rows[r], v] = sum_j vals[j] g[cols[j]] x[cols[j], v]  (g nullable: 1)
+template <typename T>
 __global__ void distribute_kernel(const int64_t *rows, const int64_t *ptr, const int64_t *cols,
-                                  const double *vals, int64_t nrows, double *X, int64_t ld,
-                                  int B) {
+                                  const double *vals, int64_t nrows, const T *g, T *X,
+                                  int64_t ld, int B) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= nrows * B) return;
   const int64_t r = idx / B;
   const int v = (int)(idx % B);
-  double s = 0.0;
-  for (int64_t j = ptr[r]; j < ptr[r + 1]; ++j) s += vals[j] * X[cols[j] * ld + v];
+  T s = T(0);
+  for (int64_t j = ptr[r]; j < ptr[r + 1]; ++j) {
+    const int64_t c = cols[j];
+    s += T(vals[j]) * (g ? g[c] * X[c * ld + v] : X[c * ld + v]);
+  }
   X[rows[r] * ld + v] = s;
 }
-
+// x[m, v] += so[m] sum_j vals[j] x[slave_j, v] over the master-major CSR (so nullable: 1)
+template <typename T>
 __global__ void slave_to_master_kernel(const int64_t *m_rows, const int64_t *m_ptr,
                                        const int64_t *m_slaves, const double *m_vals, int64_t nm,
-                                       double *X, int64_t ld, int B) {
+                                       const T *so, T *X, int64_t ld, int B) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= nm * B) return;
   const int64_t r = idx / B;
   const int v = (int)(idx % B);
-  double s = 0.0;
-  for (int64_t j = m_ptr[r]; j < m_ptr[r + 1]; ++j) s += m_vals[j] * X[m_slaves[j] * ld + v];
-  X[m_rows[r] * ld + v] += s;
+  T s = T(0);
+  for (int64_t j = m_ptr[r]; j < m_ptr[r + 1]; ++j) s += T(m_vals[j]) * X[m_slaves[j] * ld + v];
+  const int64_t m = m_rows[r];
+  X[m * ld + v] += (so ? so[m] : T(1)) * s;
 }
-
-__global__ void zero_rows_kernel(const int64_t *rows, int64_t nrows, double *X, int64_t ld,
-                                 int B) {
+template <typename T>
+__global__ void zero_rows_kernel(const int64_t *rows, int64_t nrows, T *X, int64_t ld, int B) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= nrows * B) return;
-  X[rows[idx / B] * ld + idx % B] = 0.0;
+  X[rows[idx / B] * ld + idx % B] = T(0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1521,10 +1553,13 @@ __global__ __launch_bounds__(256) void hx1_kernel(HxLaunch<T> a) {
     const int64_t gd = w & 0x7fffffffu;
     const bool first = (w >> 31) != 0;
     const T so = sout[row];
+    // a zero output scale marks a constrained row: it collects the plain cell
+    // sums (no combine) for the slave -> master pass
+    const bool cons = so == T(0);
     const T d = a.dst[gd * a.ld_dst];
     const T keep = a.beta_mode == 0 ? a.beta * d : (a.beta_mode == 2 ? (a.beta * so) * d : T(0));
-    const T base = first ? fmadd(a.gamma, a.src[gd * a.ld_src], keep) : d;
-    a.dst[gd * a.ld_dst] = fmadd(a.alpha * so, acc[g], base);
+    const T base = first ? (cons ? T(0) : fmadd(a.gamma, a.src[gd * a.ld_src], keep)) : d;
+    a.dst[gd * a.ld_dst] = fmadd(a.alpha * (cons ? T(1) : so), acc[g], base);
   }
 }
 
@@ -1899,17 +1934,20 @@ template hipError_t launch_proj_reduce<double>(const int *, const int *, const d
 
 hipError_t launch_build_h(const double *K, const double *Nq, const double *vJxW, const double *G,
                           const double *gJxW, int C, int ne, int ne_pad, int nq, double *Ht64,
-                          float *Ht32, uint16_t *Ht16, hipStream_t s) {
+                          float *Ht32, uint16_t *Ht16, hipStream_t s, bool K_per_cell,
+                          const double *invJ, int jac) {
   if ((gJxW && !G) || ne_pad % 16 != 0 || ne_pad < ne || C <= 0) return hipErrorInvalidValue;
+  if (!(jac == 0 || ((jac == 3 || jac == 9) && invJ))) return hipErrorInvalidValue;
   const int nb = ne_pad / 16;
   const dim3 grid(nb * (nb + 1) / 2, (C + 63) / 64);
   if (grid.y > 65535u) return hipErrorInvalidValue;
+  const int64_t Ks = K_per_cell ? (int64_t)ne * ne : 0;
   if (gJxW)
-    hipLaunchKernelGGL(build_h_kernel<true>, grid, dim3(256), 0, s, K, Nq, G, vJxW, gJxW, C, ne,
-                       ne_pad, nq, Ht64, Ht32, Ht16);
+    hipLaunchKernelGGL(build_h_kernel<true>, grid, dim3(256), 0, s, K, Ks, Nq, G, invJ, jac, vJxW,
+                       gJxW, C, ne, ne_pad, nq, Ht64, Ht32, Ht16);
   else
-    hipLaunchKernelGGL(build_h_kernel<false>, grid, dim3(256), 0, s, K, Nq, G, vJxW, gJxW, C, ne,
-                       ne_pad, nq, Ht64, Ht32, Ht16);
+    hipLaunchKernelGGL(build_h_kernel<false>, grid, dim3(256), 0, s, K, Ks, Nq, G, invJ, jac,
+                       vJxW, gJxW, C, ne, ne_pad, nq, Ht64, Ht32, Ht16);
   return hipGetLastError();
 }
 
@@ -2048,30 +2086,71 @@ hipError_t launch_zero_upper(double *A, int n, int64_t lda, hipStream_t st) {
   return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_distribute_scaled(const int64_t *rows, const int64_t *ptr, const int64_t *cols,
+                                    const double *vals, int64_t nrows, const T *g, T *X,
+                                    int64_t ld, int B, hipStream_t st) {
+  if (nrows == 0) return hipSuccess;
+  hipLaunchKernelGGL(distribute_kernel<T>, dim3(grid_for(nrows * B, 256, 1u << 30)), dim3(256),
+                     0, st, rows, ptr, cols, vals, nrows, g, X, ld, B);
+  return hipGetLastError();
+}
+template hipError_t launch_distribute_scaled<double>(const int64_t *, const int64_t *,
+                                                     const int64_t *, const double *, int64_t,
+                                                     const double *, double *, int64_t, int,
+                                                     hipStream_t);
+template hipError_t launch_distribute_scaled<float>(const int64_t *, const int64_t *,
+                                                    const int64_t *, const double *, int64_t,
+                                                    const float *, float *, int64_t, int,
+                                                    hipStream_t);
+
+template <typename T>
+hipError_t launch_slave_to_master_scaled(const int64_t *m_rows, const int64_t *m_ptr,
+                                         const int64_t *m_slaves, const double *m_vals,
+                                         int64_t nm, const T *so, T *X, int64_t ld, int B,
+                                         hipStream_t st) {
+  if (nm == 0) return hipSuccess;
+  hipLaunchKernelGGL(slave_to_master_kernel<T>, dim3(grid_for(nm * B, 256, 1u << 30)), dim3(256),
+                     0, st, m_rows, m_ptr, m_slaves, m_vals, nm, so, X, ld, B);
+  return hipGetLastError();
+}
+template hipError_t launch_slave_to_master_scaled<double>(const int64_t *, const int64_t *,
+                                                          const int64_t *, const double *,
+                                                          int64_t, const double *, double *,
+                                                          int64_t, int, hipStream_t);
+template hipError_t launch_slave_to_master_scaled<float>(const int64_t *, const int64_t *,
+                                                         const int64_t *, const double *,
+                                                         int64_t, const float *, float *,
+                                                         int64_t, int, hipStream_t);
+
+template <typename T>
+hipError_t launch_zero_rows_t(const int64_t *rows, int64_t nrows, T *X, int64_t ld, int B,
+                              hipStream_t st) {
+  if (nrows == 0) return hipSuccess;
+  hipLaunchKernelGGL(zero_rows_kernel<T>, dim3(grid_for(nrows * B, 256, 1u << 30)), dim3(256), 0,
+                     st, rows, nrows, X, ld, B);
+  return hipGetLastError();
+}
+template hipError_t launch_zero_rows_t<double>(const int64_t *, int64_t, double *, int64_t, int,
+                                               hipStream_t);
+template hipError_t launch_zero_rows_t<float>(const int64_t *, int64_t, float *, int64_t, int,
+                                              hipStream_t);
+
 hipError_t launch_distribute(const int64_t *rows, const int64_t *ptr, const int64_t *cols,
                              const double *vals, int64_t nrows, double *X, int64_t ld, int B,
                              hipStream_t st) {
-  if (nrows == 0) return hipSuccess;
-  hipLaunchKernelGGL(distribute_kernel, dim3(grid_for(nrows * B, 256, 1u << 30)), dim3(256), 0,
-                     st, rows, ptr, cols, vals, nrows, X, ld, B);
-  return hipGetLastError();
+  return launch_distribute_scaled<double>(rows, ptr, cols, vals, nrows, nullptr, X, ld, B, st);
 }
 
 hipError_t launch_slave_to_master(const int64_t *m_rows, const int64_t *m_ptr,
                                   const int64_t *m_slaves, const double *m_vals, int64_t nm,
                                   double *X, int64_t ld, int B, hipStream_t st) {
-  if (nm == 0) return hipSuccess;
-  hipLaunchKernelGGL(slave_to_master_kernel, dim3(grid_for(nm * B, 256, 1u << 30)), dim3(256), 0,
-                     st, m_rows, m_ptr, m_slaves, m_vals, nm, X, ld, B);
-  return hipGetLastError();
+  return launch_slave_to_master_scaled<double>(m_rows, m_ptr, m_slaves, m_vals, nm, nullptr, X,
+                                               ld, B, st);
 }
 
 hipError_t launch_zero_rows(const int64_t *rows, int64_t nrows, double *X, int64_t ld, int B,
                             hipStream_t st) {
-  if (nrows == 0) return hipSuccess;
-  hipLaunchKernelGGL(zero_rows_kernel, dim3(grid_for(nrows * B, 256, 1u << 30)), dim3(256), 0, st,
-                     rows, nrows, X, ld, B);
-  return hipGetLastError();
+  return launch_zero_rows_t<double>(rows, nrows, X, ld, B, st);
 }
-
 }  // namespace chfsi

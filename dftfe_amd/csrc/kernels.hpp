@@ -18,6 +18,9 @@ constexpr int kTile = 16;
 //   base = (first touch of g in colour order)
 //            ? beta' * dst[g,v] + gamma * src[g,v]    (beta' = beta or beta*s_out[g])
 //            : dst[g,v]
+// except on a constrained row (s_out[g] == 0, a hanging node), which collects
+// the plain sums alpha * sum_k Htile[i,k] xin(k,v) (base 0 at first touch)
+// for the slave -> master pass after the colour launches
 // and xin(k,v) = s_in[g_k] * src[g_k, v] for the n_e nodes, followed by the
 // cell's nonlocal slots whose rows come from W[proj(slot), v] (already scaled
 // by the coupling constants).
@@ -88,13 +91,18 @@ hipError_t launch_proj_reduce(const int *proj_ptr, const int *proj_slots, const 
                               int n_proj, const T *partial, int64_t ldP, T *W, int64_t ldW,
                               int B, hipStream_t s);
 
-// H_e = 1/2 K + N^T diag(vJxW_c) N, written tiled/padded in FP64, FP32 and BF16
-// (Ht32, Ht16 nullable).
-// With gJxW (C*n_q*3, nullable) and the physical shape gradients G (n_q*n_e*3)
-// the GGA term 2 sum_q gJxW(q).(grad N_i(q) N_j(q) + N_i(q) grad N_j(q)) is added.
+// H_e = 1/2 K_e + N^T diag(vJxW_c) N, written tiled/padded in FP64, FP32 and BF16
+// (Ht32, Ht16 nullable); K is one n_e x n_e stiffness for all cells, or with
+// K_per_cell one per cell (C x n_e x n_e, the cells' gradient integrals).
+// With gJxW (C*n_q*3, nullable) and the shape gradients G (n_q*n_e*3) the GGA
+// term 2 sum_q gJxW(q).(grad N_i(q) N_j(q) + N_i(q) grad N_j(q)) is added; G
+// holds the physical gradients (jac = 0), or the reference cell's with
+// per-cell inverse Jacobians invJ: C x 3 diagonals (jac = 3, Cartesian cells)
+// or C x 9 row-major matrices (jac = 9), grad N = J grad_ref N.
 hipError_t launch_build_h(const double *K, const double *Nq, const double *vJxW, const double *G,
                           const double *gJxW, int C, int ne, int ne_pad, int nq, double *Ht64,
-                          float *Ht32, uint16_t *Ht16, hipStream_t s);
+                          float *Ht32, uint16_t *Ht16, hipStream_t s, bool K_per_cell = false,
+                          const double *invJ = nullptr, int jac = 0);
 // tiled padded -> dense C x ne x ne (row-major) and back
 hipError_t launch_untile_h(const double *Ht, int C, int ne, int ne_pad, double *H, hipStream_t s);
 hipError_t launch_tile_h(const double *H, int C, int ne, int ne_pad, double *Ht64, float *Ht32,
@@ -169,5 +177,19 @@ hipError_t launch_slave_to_master(const int64_t *m_rows, const int64_t *m_ptr,
                                   double *X, int64_t ld, int B, hipStream_t st);
 hipError_t launch_zero_rows(const int64_t *rows, int64_t nrows, double *X, int64_t ld, int B,
                             hipStream_t st);
+// x[rows[r], v] = sum_j vals[j] g[cols[j]] x[cols[j], v]  (g nullable: 1)
+template <typename T>
+hipError_t launch_distribute_scaled(const int64_t *rows, const int64_t *ptr, const int64_t *cols,
+                                    const double *vals, int64_t nrows, const T *g, T *X,
+                                    int64_t ld, int B, hipStream_t st);
+// x[m, v] += so[m] sum_j vals[j] x[slave_j, v]  (master-major CSR; so nullable: 1)
+template <typename T>
+hipError_t launch_slave_to_master_scaled(const int64_t *m_rows, const int64_t *m_ptr,
+                                         const int64_t *m_slaves, const double *m_vals,
+                                         int64_t nm, const T *so, T *X, int64_t ld, int B,
+                                         hipStream_t st);
+template <typename T>
+hipError_t launch_zero_rows_t(const int64_t *rows, int64_t nrows, T *X, int64_t ld, int B,
+                              hipStream_t st);
 
 }  // namespace chfsi

@@ -78,7 +78,8 @@ chfsi_operator::~chfsi_operator() {
                   d_nlCt64, d_nlCt32, d_slot_proj, d_proj_ptr, d_proj_slots, d_proj_V,
                   d_nl_cells, d_c_rows, d_c_ptr, d_c_cols, d_c_vals, d_m_rows, d_m_ptr,
                   d_m_slaves, d_m_vals, d_Mc, d_Gd, d_Hk64, d_Hk32, d_nlCk64, d_nlCtk64,
-                  d_nlCk32, d_nlCtk32};
+                  d_nlCk32, d_nlCtk32, d_invJ, d_s_gather64, d_ones_cons64, d_s_gather32,
+                  d_ones_cons32};
   for (void *p : ptrs)
     if (p) hipFree(p);
   ztab_dev.release();
@@ -137,8 +138,26 @@ extern "C" chfsi_status chfsi_operator_create(const chfsi_operator_desc *d, chfs
       else
         all_identity = false;
     }
-    CHFSI_ARG(all_identity,
-              "only periodic (single-master, unit-coefficient) constraints are supported by HX");
+    if (!all_identity) {
+      // general rows: nothing is folded; masters must be unconstrained
+      op->general_constraints = true;
+      for (int64_t g = 0; g < M; ++g) fold[g] = g;
+      std::vector<char> is_c(M, 0);
+      for (int64_t r = 0; r < op->nc; ++r) is_c[d->constraint_rows[r]] = 1;
+      for (int64_t j = 0; j < d->constraint_ptr[op->nc]; ++j)
+        CHFSI_ARG(!is_c[d->constraint_cols[j]], "master %lld of a constraint is itself constrained",
+                  (long long)d->constraint_cols[j]);
+      std::vector<double> sg(d->inv_sqrt_mass, d->inv_sqrt_mass + M), oc(M, 1.0);
+      for (int64_t r = 0; r < op->nc; ++r) {
+        sg[d->constraint_rows[r]] = 1.0;
+        oc[d->constraint_rows[r]] = 0.0;
+      }
+      std::vector<float> sg32(sg.begin(), sg.end()), oc32(oc.begin(), oc.end());
+      CHFSI_TRY(dev_upload(&op->d_s_gather64, sg.data(), M));
+      CHFSI_TRY(dev_upload(&op->d_ones_cons64, oc.data(), M));
+      CHFSI_TRY(dev_upload(&op->d_s_gather32, sg32.data(), M));
+      CHFSI_TRY(dev_upload(&op->d_ones_cons32, oc32.data(), M));
+    }
     const int64_t nnz = d->constraint_ptr[op->nc];
     std::vector<int64_t> ptr(d->constraint_ptr, d->constraint_ptr + op->nc + 1);
     CHFSI_TRY(dev_upload(&op->d_c_rows, d->constraint_rows, op->nc));
@@ -168,22 +187,24 @@ extern "C" chfsi_status chfsi_operator_create(const chfsi_operator_desc *d, chfs
   }
 
   // ---- colour classes and first-touch flags
-  std::vector<std::vector<int>> by_color(8);
+  constexpr int KC = chfsi_operator::kMaxColors;
+  std::vector<std::vector<int>> by_color(KC);
   for (int c = 0; c < C; ++c) {
     const int col = d->cell_color[c];
-    CHFSI_ARG(col >= 0 && col < 8, "cell colour must be in [0, 8)");
+    CHFSI_ARG(col >= 0 && col < KC, "cell colour must be in [0, %d)", KC);
     by_color[col].push_back(c);
+    op->ncolors = std::max(op->ncolors, col + 1);
   }
   std::vector<int> cells_sorted;
   op->color_off[0] = 0;
-  for (int k = 0; k < 8; ++k) {
+  for (int k = 0; k < op->ncolors; ++k) {
     cells_sorted.insert(cells_sorted.end(), by_color[k].begin(), by_color[k].end());
     op->color_off[k + 1] = (int)cells_sorted.size();
   }
   std::vector<uint32_t> map(size_t(C) * ne);
   std::vector<int> touched_color(M, -1);
   std::vector<int> touched_cell(M, -1);
-  for (int k = 0; k < 8; ++k) {
+  for (int k = 0; k < op->ncolors; ++k) {
     for (int c : by_color[k]) {
       for (int i = 0; i < ne; ++i) {
         const int64_t g0 = d->cell_map[size_t(c) * ne + i];
@@ -215,17 +236,33 @@ extern "C" chfsi_status chfsi_operator_create(const chfsi_operator_desc *d, chfs
   CHFSI_TRY(dev_upload(&op->d_inv_sqrt_mass32, f32.data(), M));
 
   // ---- shape data for the cell Hamiltonian build
-  if (d->shape_values && d->stiffness) {
+  if (d->shape_values && (d->stiffness || d->cell_stiffness)) {
     CHFSI_TRY(dev_upload(&op->d_Nq, d->shape_values, size_t(op->nq) * ne));
-    CHFSI_TRY(dev_upload(&op->d_K, d->stiffness, size_t(ne) * ne));
+    if (d->cell_stiffness) {
+      op->K_per_cell = true;
+      CHFSI_TRY(dev_upload(&op->d_K, d->cell_stiffness, size_t(C) * ne * ne));
+    } else {
+      CHFSI_TRY(dev_upload(&op->d_K, d->stiffness, size_t(ne) * ne));
+    }
     if (d->shape_gradients)
       CHFSI_TRY(dev_upload(&op->d_Gq, d->shape_gradients, size_t(op->nq) * ne * 3));
+  }
+  if (d->inverse_jacobians) {
+    CHFSI_ARG(d->jacobian_kind == 3 || d->jacobian_kind == 9,
+              "jacobian_kind must be 3 (Cartesian) or 9 (affine) with inverse_jacobians");
+    op->jac = d->jacobian_kind;
+    CHFSI_TRY(dev_upload(&op->d_invJ, d->inverse_jacobians, size_t(C) * op->jac));
+  } else {
+    CHFSI_ARG(d->jacobian_kind == 0, "jacobian_kind set without inverse_jacobians");
   }
   if (d->jxw) CHFSI_TRY(dev_upload(&op->d_jxw, d->jxw, size_t(C) * op->nq));
   if (d->n_kpoints > 0) {
     CHFSI_ARG(d->kpoints && d->cell_mass && d->cell_grad, "k-points need kpoints, cell_mass, cell_grad");
     CHFSI_ARG(d->n_pairs == 0 || d->pair_C_kpoints, "k-points need pair_C_kpoints");
     CHFSI_ARG(d->shape_values && d->stiffness, "k-points need shape data for the cell matrices");
+    CHFSI_ARG(!d->cell_stiffness && !d->inverse_jacobians,
+              "the k-point cell terms are built for one cell geometry (uniform meshes)");
+    CHFSI_ARG(!op->general_constraints, "k-points support periodic constraints only");
     op->nk = d->n_kpoints;
     op->kpts.assign(d->kpoints, d->kpoints + 3 * size_t(op->nk));
     CHFSI_TRY(dev_upload(&op->d_Mc, d->cell_mass, size_t(ne) * ne));
@@ -354,7 +391,8 @@ extern "C" chfsi_status chfsi_operator_set_potential(chfsi_operator *op, const d
   CHFSI_ARG(op && vJxW, "null argument");
   CHFSI_ARG(op->d_Nq && op->d_K, "operator was created without shape data");
   CHFSI_HIP(launch_build_h(op->d_K, op->d_Nq, vJxW, nullptr, nullptr, op->C, op->ne, op->ne_pad,
-                           op->nq, op->d_Ht64, op->d_Ht32, op->d_Ht16, as_stream(stream)));
+                           op->nq, op->d_Ht64, op->d_Ht32, op->d_Ht16, as_stream(stream),
+                           op->K_per_cell));
   op->has_H = true;
   return rebuild_kpoint_matrices(op, as_stream(stream));
 }
@@ -365,7 +403,8 @@ extern "C" chfsi_status chfsi_operator_set_potential_gga(chfsi_operator *op, con
   CHFSI_ARG(op->d_Nq && op->d_K, "operator was created without shape data");
   CHFSI_ARG(op->d_Gq, "operator was created without shape gradients");
   CHFSI_HIP(launch_build_h(op->d_K, op->d_Nq, vJxW, op->d_Gq, gJxW, op->C, op->ne, op->ne_pad,
-                           op->nq, op->d_Ht64, op->d_Ht32, op->d_Ht16, as_stream(stream)));
+                           op->nq, op->d_Ht64, op->d_Ht32, op->d_Ht16, as_stream(stream),
+                           op->K_per_cell, op->d_invJ, op->jac));
   op->has_H = true;
   return rebuild_kpoint_matrices(op, as_stream(stream));
 }
@@ -447,6 +486,8 @@ struct OpData<double> {
   static const double *s(chfsi_operator *op) { return op->d_inv_sqrt_mass; }
   static Scratch &partial(chfsi_operator *op) { return op->partial64; }
   static Scratch &W(chfsi_operator *op) { return op->W64; }
+  static const double *s_gather(chfsi_operator *op) { return op->d_s_gather64; }
+  static const double *ones_cons(chfsi_operator *op) { return op->d_ones_cons64; }
 };
 template <>
 struct OpData<float> {
@@ -456,6 +497,8 @@ struct OpData<float> {
   static const float *s(chfsi_operator *op) { return op->d_inv_sqrt_mass32; }
   static Scratch &partial(chfsi_operator *op) { return op->partial32; }
   static Scratch &W(chfsi_operator *op) { return op->W32; }
+  static const float *s_gather(chfsi_operator *op) { return op->d_s_gather32; }
+  static const float *ones_cons(chfsi_operator *op) { return op->d_ones_cons32; }
 };
 
 template <typename T>
@@ -472,6 +515,21 @@ chfsi_status hx_fused(chfsi_operator *op, const T *src, int64_t ld_src, T *dst, 
   CHFSI_ARG(B > 0 && ld_src >= B && ld_dst >= B, "bad block width / leading dimension");
   CHFSI_ARG(src != dst, "HX needs distinct source and destination");
   const T *sc = scaled ? OpData<T>::s(op) : nullptr;
+  // input (gather) and output (scatter) scales; the same M^{-1/2} unless the
+  // operator has general constraints
+  const T *s_in = sc, *s_out = sc;
+  T *src_w = const_cast<T *>(src);
+  if (op->general_constraints) {
+    // constraintMatrixInfoDevice::distribute before the cell products
+    // (kohnShamDFTOperatorDevice.cc:3806-3807): the constrained rows of src
+    // take sum_j c_j s_j src_j (scaled masters) and are gathered with scale 1;
+    // in the scatter a zero output scale marks them, so they collect the plain
+    // cell sums for the slave -> master pass below
+    CHFSI_HIP(launch_distribute_scaled<T>(op->d_c_rows, op->d_c_ptr, op->d_c_cols, op->d_c_vals,
+                                          op->nc, sc, src_w, ld_src, B, s));
+    s_in = scaled ? OpData<T>::s_gather(op) : nullptr;
+    s_out = scaled ? sc : OpData<T>::ones_cons(op);
+  }
   T *W = nullptr;
   if (op->n_slots > 0) {
     // projection partials (n_slots x B) and V-scaled projections (n_proj x B):
@@ -500,7 +558,7 @@ chfsi_status hx_fused(chfsi_operator *op, const T *src, int64_t ld_src, T *dst, 
     pl.nl_Ct = OpData<T>::nlCt(op);
     pl.src = src;
     pl.ld_src = ld_src;
-    pl.s_in = sc;
+    pl.s_in = s_in;
     pl.partial = Pp;
     pl.ldP = B;
     pl.B = B;
@@ -519,8 +577,8 @@ chfsi_status hx_fused(chfsi_operator *op, const T *src, int64_t ld_src, T *dst, 
   a.ld_src = ld_src;
   a.dst = dst;
   a.ld_dst = ld_dst;
-  a.s_in = sc;
-  a.s_out = sc;
+  a.s_in = s_in;
+  a.s_out = s_out;
   a.alpha = alpha;
   a.beta = beta;
   a.gamma = gamma;
@@ -537,9 +595,10 @@ chfsi_status hx_fused(chfsi_operator *op, const T *src, int64_t ld_src, T *dst, 
     a.W = W;
     a.ldW = B;
   }
-  for (int k = 0; k < 8; ++k) {
+  for (int k = 0; k < op->ncolors; ++k) {
     a.cells = op->d_cells_by_color + op->color_off[k];
     a.ncells = op->color_off[k + 1] - op->color_off[k];
+    if (a.ncells == 0) continue;
     if constexpr (sizeof(T) == 4) {
       if (bf16) {
         CHFSI_HIP(launch_hx_bf16(a, op->d_Ht16, op->d_nlC16, s));
@@ -547,6 +606,15 @@ chfsi_status hx_fused(chfsi_operator *op, const T *src, int64_t ld_src, T *dst, 
       }
     }
     CHFSI_HIP(launch_hx<T>(a, s));
+  }
+  if (op->general_constraints) {
+    // distribute_slave_to_master (kohnShamDFTOperatorDevice.cc:3832-3838):
+    // dst[m] += s_out[m] sum_s c_sm dst[s], then the constrained rows of dst
+    // and src are cleared
+    CHFSI_HIP(launch_slave_to_master_scaled<T>(op->d_m_rows, op->d_m_ptr, op->d_m_slaves,
+                                               op->d_m_vals, op->nm, s_out, dst, ld_dst, B, s));
+    CHFSI_HIP(launch_zero_rows_t<T>(op->d_c_rows, op->nc, dst, ld_dst, B, s));
+    CHFSI_HIP(launch_zero_rows_t<T>(op->d_c_rows, op->nc, src_w, ld_src, B, s));
   }
   return CHFSI_OK;
 }
@@ -597,18 +665,22 @@ extern "C" chfsi_status chfsi_hx_fp32(chfsi_operator *op, const float *src, int6
                                       float *dst, int64_t ld_dst, int32_t B, int32_t scale_flag,
                                       float scalar, void *stream) {
   CHFSI_ARG(op && src && dst, "null argument");
-  CHFSI_ARG(op->nc == 0, "FP32 HX supports unconstrained (periodic-folded) layouts only");
-  return hx_fused<float>(op, src, ld_src, dst, ld_dst, B, scalar, 1.0f, 0.0f,
-                         scale_flag ? 0 : 2, true, as_stream(stream));
+  CHFSI_TRY(hx_fused<float>(op, src, ld_src, dst, ld_dst, B, scalar, 1.0f, 0.0f,
+                            scale_flag ? 0 : 2, true, as_stream(stream)));
+  if (op->nc > 0)
+    CHFSI_HIP(launch_zero_rows_t<float>(op->d_c_rows, op->nc, dst, ld_dst, B, as_stream(stream)));
+  return CHFSI_OK;
 }
 
 extern "C" chfsi_status chfsi_hx_bf16(chfsi_operator *op, const float *src, int64_t ld_src,
                                       float *dst, int64_t ld_dst, int32_t B, int32_t scale_flag,
                                       float scalar, void *stream) {
   CHFSI_ARG(op && src && dst, "null argument");
-  CHFSI_ARG(op->nc == 0, "BF16 HX supports unconstrained (periodic-folded) layouts only");
-  return hx_fused<float>(op, src, ld_src, dst, ld_dst, B, scalar, 1.0f, 0.0f,
-                         scale_flag ? 0 : 2, true, as_stream(stream), true);
+  CHFSI_TRY(hx_fused<float>(op, src, ld_src, dst, ld_dst, B, scalar, 1.0f, 0.0f,
+                            scale_flag ? 0 : 2, true, as_stream(stream), true));
+  if (op->nc > 0)
+    CHFSI_HIP(launch_zero_rows_t<float>(op->d_c_rows, op->nc, dst, ld_dst, B, as_stream(stream)));
+  return CHFSI_OK;
 }
 
 extern "C" chfsi_status chfsi_hx_cheby(chfsi_operator *op, const double *src, int64_t ld_src,
@@ -750,14 +822,50 @@ extern "C" chfsi_status chfsi_operator_autotune(chfsi_operator *op, int32_t B, i
   return CHFSI_OK;
 }
 
+namespace chfsi {
+
+template <typename T>
+chfsi_status quad_product(chfsi_operator *op, const T *a, int64_t lda, const T *b, int64_t ldb,
+                          int B, bool scaled, const double *jxw, const double *w, double wscale,
+                          double *g, int accumulate, hipStream_t st, int wshift) {
+  const double *s_in = scaled ? op->d_inv_sqrt_mass : nullptr;
+  T *aw = const_cast<T *>(a), *bw = const_cast<T *>(b);
+  if (op->general_constraints) {
+    // interpolation needs the constrained rows' values: distributed from the
+    // (scaled) masters into the rows, gathered with scale 1, cleared after
+    const T *sc = scaled ? OpData<T>::s(op) : nullptr;
+    CHFSI_HIP(launch_distribute_scaled<T>(op->d_c_rows, op->d_c_ptr, op->d_c_cols, op->d_c_vals,
+                                          op->nc, sc, aw, lda, B, st));
+    if (bw != aw)
+      CHFSI_HIP(launch_distribute_scaled<T>(op->d_c_rows, op->d_c_ptr, op->d_c_cols,
+                                            op->d_c_vals, op->nc, sc, bw, ldb, B, st));
+    if (scaled) s_in = op->d_s_gather64;
+  }
+  CHFSI_HIP(launch_quad_product(op->d_map, op->C, op->ne, op->nq, op->d_Nq, jxw, a, lda, b, ldb,
+                                B, s_in, w, wscale, g, accumulate, st, wshift));
+  if (op->general_constraints) {
+    CHFSI_HIP(launch_zero_rows_t<T>(op->d_c_rows, op->nc, aw, lda, B, st));
+    if (bw != aw) CHFSI_HIP(launch_zero_rows_t<T>(op->d_c_rows, op->nc, bw, ldb, B, st));
+  }
+  return CHFSI_OK;
+}
+template chfsi_status quad_product<double>(chfsi_operator *, const double *, int64_t,
+                                           const double *, int64_t, int, bool, const double *,
+                                           const double *, double, double *, int, hipStream_t,
+                                           int);
+template chfsi_status quad_product<float>(chfsi_operator *, const float *, int64_t, const float *,
+                                          int64_t, int, bool, const double *, const double *,
+                                          double, double *, int, hipStream_t, int);
+
+}  // namespace chfsi
+
 extern "C" chfsi_status chfsi_density(chfsi_operator *op, const double *X, int64_t ld, int32_t N,
                                       const double *w_dev, double *rho_dev, int32_t accumulate,
                                       void *stream) {
   CHFSI_ARG(op && X && rho_dev && N > 0 && ld >= N, "bad argument");
   CHFSI_ARG(op->d_Nq, "operator was created without shape data");
-  CHFSI_HIP(launch_quad_product(op->d_map, op->C, op->ne, op->nq, op->d_Nq, nullptr, X, ld, X, ld,
-                                N, nullptr, w_dev, 1.0, rho_dev, accumulate, as_stream(stream)));
-  return CHFSI_OK;
+  return quad_product<double>(op, X, ld, X, ld, N, false, nullptr, w_dev, 1.0, rho_dev,
+                              accumulate, as_stream(stream), 0);
 }
 
 extern "C" chfsi_status chfsi_density_complex(chfsi_operator *op, const void *X, int64_t ld,
@@ -768,10 +876,8 @@ extern "C" chfsi_status chfsi_density_complex(chfsi_operator *op, const void *X,
   // |psi|^2 = Re^2 + Im^2: the real contraction over the 2N interleaved real
   // columns, each weight shared by the pair
   const double *Xr = static_cast<const double *>(X);
-  CHFSI_HIP(launch_quad_product(op->d_map, op->C, op->ne, op->nq, op->d_Nq, nullptr, Xr, 2 * ld,
-                                Xr, 2 * ld, 2 * N, nullptr, w_dev, 1.0, rho_dev, accumulate,
-                                as_stream(stream), 1));
-  return CHFSI_OK;
+  return quad_product<double>(op, Xr, 2 * ld, Xr, 2 * ld, 2 * N, false, nullptr, w_dev, 1.0,
+                              rho_dev, accumulate, as_stream(stream), 1);
 }
 
 extern "C" chfsi_status chfsi_hx_adjoint_v(chfsi_operator *op, const double *Ybar, int64_t ldy,
@@ -780,8 +886,6 @@ extern "C" chfsi_status chfsi_hx_adjoint_v(chfsi_operator *op, const double *Yba
                                            void *stream) {
   CHFSI_ARG(op && Ybar && X && g_dev && B > 0 && ldy >= B && ldx >= B, "bad argument");
   CHFSI_ARG(op->d_Nq && op->d_jxw, "operator was created without shape data / JxW");
-  CHFSI_HIP(launch_quad_product(op->d_map, op->C, op->ne, op->nq, op->d_Nq, op->d_jxw, Ybar, ldy,
-                                X, ldx, B, op->d_inv_sqrt_mass, nullptr, scalar, g_dev,
-                                accumulate, as_stream(stream)));
-  return CHFSI_OK;
+  return quad_product<double>(op, Ybar, ldy, X, ldx, B, true, op->d_jxw, nullptr, scalar, g_dev,
+                              accumulate, as_stream(stream), 0);
 }

@@ -38,6 +38,15 @@ chfsi_status hx_fused(chfsi_operator *op, const T *src, int64_t ld_src, T *dst, 
                       bool bf16 = false, const T *R = nullptr, int64_t ldR = 0,
                       const T *rcoef = nullptr, void *ext_scratch = nullptr);
 
+// g (+)= wscale JxW sum_v w_v (N.(s a_v))(N.(s b_v)) per (cell, quadrature
+// point), s = M^{-1/2} when scaled; with general constraints the rows of a and
+// b are distributed for the interpolation and cleared after (a and b are
+// written).  See launch_quad_product.
+template <typename T>
+chfsi_status quad_product(chfsi_operator *op, const T *a, int64_t lda, const T *b, int64_t ldb,
+                          int B, bool scaled, const double *jxw, const double *w, double wscale,
+                          double *g, int accumulate, hipStream_t st, int wshift);
+
 }  // namespace chfsi
 
 struct chfsi_operator {
@@ -47,12 +56,19 @@ struct chfsi_operator {
   // cell -> DOF map with first-touch flag (bit 31), cells sorted by colour
   uint32_t *d_map = nullptr;
   int *d_cells_by_color = nullptr;
-  int color_off[9] = {0};
+  static constexpr int kMaxColors = 32;
+  int ncolors = 0;
+  int color_off[kMaxColors + 1] = {0};
   // Loewdin scalings
   double *d_sqrt_mass = nullptr, *d_inv_sqrt_mass = nullptr;
   float *d_sqrt_mass32 = nullptr, *d_inv_sqrt_mass32 = nullptr;
   // shape data and cell Hamiltonians (padded K-tiled layout)
   double *d_K = nullptr, *d_Nq = nullptr, *d_Gq = nullptr, *d_jxw = nullptr;
+  // per-cell geometry: stiffness per cell (d_K then holds C*n_e*n_e) and the
+  // cells' inverse Jacobians (jac = 3 diagonal, 9 full; 0: none)
+  bool K_per_cell = false;
+  double *d_invJ = nullptr;
+  int jac = 0;
   double *d_Ht64 = nullptr;
   float *d_Ht32 = nullptr;
   uint16_t *d_Ht16 = nullptr, *d_nlC16 = nullptr;  // BF16 copies
@@ -74,6 +90,14 @@ struct chfsi_operator {
   double *d_c_vals = nullptr;
   int64_t *d_m_rows = nullptr, *d_m_ptr = nullptr, *d_m_slaves = nullptr;
   double *d_m_vals = nullptr;
+  // general (hanging-node) rows: not folded into the map; every apply
+  // distributes into the constrained rows of its source (gathered with scale
+  // 1: s_gather = M^{-1/2} with 1 on constrained rows) and sends the rows' sums
+  // to their masters (output scale 0 marks them; ones_cons = 1 with 0 on
+  // constrained rows for the unscaled apply)
+  bool general_constraints = false;
+  double *d_s_gather64 = nullptr, *d_ones_cons64 = nullptr;
+  float *d_s_gather32 = nullptr, *d_ones_cons32 = nullptr;
   // Bloch k-points: complex cell matrices and projector tiles per k-point in
   // the paired (Re, Im) tile layout of the complex kernels
   int nk = 0, active_k = 0;

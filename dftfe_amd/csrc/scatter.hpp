@@ -35,7 +35,7 @@ __device__ __forceinline__ void gstore(T *p, int64_t i, T v) {
 // P is the launch (or batch-entry) description providing dst, src, ld_dst,
 // ld_src, alpha, beta, gamma, beta_mode and B (real columns); ne is the
 // number of local rows, smap the DOF and first-touch bit of each, sout its
-// output scale.  The destination values of a batch of row blocks (ELEMS
+// output scale (zero for a constrained row: see the store below).  The destination values of a batch of row blocks (ELEMS
 // elements per thread) are loaded from clamped, always-valid addresses before
 // any is used, so their latencies overlap; the gamma term reads the source
 // rows of first-touch rows only.  Loads are never skipped by a branch; the
@@ -145,6 +145,11 @@ __device__ __forceinline__ void hx_scatter(const P &a, int ne,
         const int rowc = rok ? row : 0;
         const bool first = (smap[rowc] >> 31) != 0;
         const T so = sout[rowc];
+        // a zero output scale marks a constrained row (hanging node): it
+        // collects the plain cell sums, without the combine, for the
+        // slave -> master pass that follows the colour launches
+        const bool cons = so == T(0);
+        const T sa = a.alpha * (cons ? T(1) : so);
 #pragma unroll
         for (int cb = 0; cb < CBW; ++cb) {
           const int v = v0 + (wc * CBW + cb) * 16 + (lane & 15);
@@ -154,8 +159,9 @@ __device__ __forceinline__ void hx_scatter(const P &a, int ne,
           if constexpr (HasDefect<P>::value && sizeof(T) == 4) {
             if (any_r) base = first ? sv[jj][cb][r] + keep : d;
           }
+          base = (first && cons) ? T(0) : base;
           if (rok && v < a.B)
-            gstore(a.dst, (int64_t)gg[jj][r] * a.ld_dst + v, fmadd(a.alpha * so, acc[j][cb][r], base));
+            gstore(a.dst, (int64_t)gg[jj][r] * a.ld_dst + v, fmadd(sa, acc[j][cb][r], base));
         }
       }
     }

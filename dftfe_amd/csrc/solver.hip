@@ -798,6 +798,7 @@ extern "C" void chfsi_default_params(chfsi_params *p) {
   p->mixed_prec_defect_filter = 0;
   p->is_pseudopotential = 0;            // utils/dftParameters.cc:1173
   p->lanczos_full_residual_margin = 0;
+  p->mixed_prec_fp64_top_columns = 0;
 }
 
 extern "C" chfsi_status chfsi_chebyshev_degree(const chfsi_params *params, double bUp,
@@ -962,8 +963,8 @@ chfsi_status filter_backward_impl(chfsi_operator *op, const double *X0, int64_t 
       }
     }
     for (int k = k1; k >= k0 + 1; --k) {
-      CHFSI_HIP(launch_quad_product(op->d_map, op->C, op->ne, op->nq, op->d_Nq, op->d_jxw, Lk, B,
-                                    X(k - 1), B, B, op->d_inv_sqrt_mass, nullptr, a1[k], g, 1, st));
+      CHFSI_TRY(quad_product<T>(op, Lk, B, X(k - 1), B, B, true, op->d_jxw, nullptr, a1[k], g, 1,
+                                st, 0));
       CHFSI_TRY(hx_fused<T>(op, Lk, B, Lm1, B, B, T(a1[k]), T(1), T(-c * a1[k]), 0, true, st));
       if (k >= 2) CHFSI_HIP(launch_axpby<T>((int64_t)vec, T(a2[k]), Lk, T(1), Lm2, st));
       // shift: (L_k, L_{k-1}, L_{k-2}) <- (L_{k-1}, L_{k-2}, 0)
@@ -993,7 +994,6 @@ extern "C" chfsi_status chfsi_chebyshev_filter_backward(
     return filter_backward_impl<double>(op, X0, ldx, Ybar, ldy, B, m, a, b, a0, every, Xbar, ldxb,
                                         g, st);
   CHFSI_ARG(precision == CHFSI_FP32, "filter backward precision must be CHFSI_FP64 or CHFSI_FP32");
-  CHFSI_ARG(op->nc == 0, "FP32 filter backward supports unconstrained layouts only");
   return filter_backward_impl<float>(op, X0, ldx, Ybar, ldy, B, m, a, b, a0, every, Xbar, ldxb, g,
                                      st);
 }
@@ -1079,10 +1079,17 @@ chfsi_status filter_all(chfsi_solver *s, chfsi_operator *op, double *X, int64_t 
   const int prec = use_mixed_prec_overall && !switched ? P.filter_precision : CHFSI_FP64;
   s->last_precision = prec;
   const int G = filter_group_columns(P, M, N, B);
-  for (int j = 0; j < N; j += G) {
-    const int bw = std::min(G, N - j);
+  // the top columns in FP64 when the filter runs in a lower precision
+  const int ktop = prec == CHFSI_FP64 ? 0 : std::max(0, std::min(P.mixed_prec_fp64_top_columns, N));
+  const int nlow = N - ktop;
+  for (int j = 0; j < nlow; j += G) {
+    const int bw = std::min(G, nlow - j);
     CHFSI_TRY(filter_block(s, op, X + j, N, bw, m, s->bLow, s->bUp, s->a0, prec,
                            use_mixed_prec_overall ? P.fp64_tail_degrees : 0, st));
+  }
+  for (int j = nlow; j < N; j += G) {
+    const int bw = std::min(G, N - j);
+    CHFSI_TRY(filter_block(s, op, X + j, N, bw, m, s->bLow, s->bUp, s->a0, CHFSI_FP64, 0, st));
   }
   return CHFSI_OK;
 }

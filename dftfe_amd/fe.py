@@ -170,10 +170,47 @@ class FEProblem:
     M_c: Optional[np.ndarray] = None      # (ne, ne) consistent cell mass  int N_i N_j
     G_c: Optional[np.ndarray] = None      # (3, ne, ne) int N_i dN_j/dx_d
     Gq: Optional[np.ndarray] = None       # (nqc, ne, 3) physical shape gradients
+    # graded meshes (build_graded_problem): per-cell edge length and origin;
+    # JxW, K_e and Gq above then describe a cell of edge h, and a cell of edge
+    # a scales them by (a/h)^3, a/h and h/a
+    cell_size: Optional[np.ndarray] = None    # (C,)
+    cell_origin: Optional[np.ndarray] = None  # (C, 3)
+    # general affine constraints (hanging nodes) as CSR over slave_rows:
+    # x[slave_rows[r]] = sum_j c_vals[j] x[c_cols[j]], j in c_ptr[r]..c_ptr[r+1]
+    c_ptr: Optional[np.ndarray] = None
+    c_cols: Optional[np.ndarray] = None
+    c_vals: Optional[np.ndarray] = None
+
+    @property
+    def graded(self) -> bool:
+        return self.cell_size is not None
+
+    @property
+    def cell_scale(self) -> np.ndarray:
+        """(C,) edge length of each cell relative to h."""
+        if self.cell_size is None:
+            return np.ones(self.C)
+        return self.cell_size / self.h
+
+    @property
+    def jxw_cells(self) -> np.ndarray:
+        """(C, nqc) quadrature weights times Jacobian determinants of every cell."""
+        return self.cell_scale[:, None] ** 3 * self.JxW[None, :]
 
     @property
     def vJxW(self) -> np.ndarray:
+        if self.cell_size is not None:
+            return self.V_q * self.jxw_cells
         return self.V_q * self.JxW[None, :]
+
+    def constraint_csr(self):
+        """(rows, ptr, cols, vals) of the constraints (periodic identities in
+        the dealii layout, hanging-node rows on graded meshes)."""
+        if self.c_ptr is not None:
+            return self.slave_rows, self.c_ptr, self.c_cols, self.c_vals
+        ns = len(self.slave_rows)
+        return (self.slave_rows, np.arange(ns + 1, dtype=np.int64),
+                self.master_rows.astype(np.int64), np.ones(ns))
 
     @property
     def lanczos_start(self) -> np.ndarray:
@@ -198,6 +235,12 @@ class FEProblem:
         if cells is None:
             cells = np.arange(self.C)
         nq = self.wl.nq
+        if self.cell_size is not None:
+            g = self.gauss_x
+            local = np.stack([np.tile(g, nq * nq), np.tile(np.repeat(g, nq), nq),
+                              np.repeat(g, nq * nq)], axis=1)
+            a = self.cell_size[cells]
+            return self.cell_origin[cells][:, None, :] + a[:, None, None] * local[None, :, :]
         g = self.gauss_x * self.h
         qx = np.tile(g, nq * nq)
         qy = np.tile(np.repeat(g, nq), nq)
@@ -358,13 +401,18 @@ def projectors(prob: FEProblem, kpoint=None):
     ne, nqc = prob.ne, prob.nqc
     kvec = None if kpoint is None else np.asarray(kpoint, dtype=np.float64)
     NqJ = (prob.Nq * prob.JxW[:, None]).T                               # (ne, nqc)
-    centres = (prob.cell_coords.astype(np.float64) + 0.5) * h
+    if prob.cell_size is not None:
+        centres = prob.cell_origin + 0.5 * prob.cell_size[:, None]
+        half_diag = prob.cell_size * math.sqrt(3.0) / 2.0
+    else:
+        centres = (prob.cell_coords.astype(np.float64) + 0.5) * h
+        half_diag = h * math.sqrt(3.0) / 2.0
     pair_atom, pair_cell, pair_C = [], [], []
     for a in range(wl.n_atoms):
         rc = prob.atom_width[a]
         rcut = wl.rcut_factor * rc
         dc = _min_image(centres - prob.atom_pos[a], L)
-        sel = np.nonzero(np.sqrt((dc ** 2).sum(1)) <= rcut + h * math.sqrt(3.0) / 2.0)[0]
+        sel = np.nonzero(np.sqrt((dc ** 2).sum(1)) <= rcut + half_diag)[0]
         if len(sel) == 0:
             continue
         xq = prob.quad_points(sel)                       # (s, nqc, 3)
@@ -375,6 +423,9 @@ def projectors(prob: FEProblem, kpoint=None):
         if kvec is not None:
             phase = np.exp(-1j * ((prob.atom_pos[a][None, None, :] + d) @ kvec))
             beta = beta * phase[..., None]
+        if prob.cell_size is not None:
+            # per-cell Jacobian determinant (a/h)^3 of the weights
+            beta = beta * (prob.cell_scale[sel] ** 3)[:, None, None]
         B2 = beta.transpose(1, 0, 2).reshape(nqc, -1)                   # (nqc, s*4)
         Ce = (NqJ @ B2).reshape(ne, len(sel), 4).transpose(1, 0, 2)
         keep = np.abs(Ce).reshape(len(sel), -1).max(axis=1) > 0.0
@@ -386,6 +437,172 @@ def projectors(prob: FEProblem, kpoint=None):
         return np.zeros(0, np.int64), np.zeros(0, np.int64), np.zeros((0, ne, 4), dt)
     return (np.concatenate(pair_atom), np.concatenate(pair_cell),
             np.concatenate(pair_C, axis=0))
+
+
+def greedy_colors(cell_map: np.ndarray, M: int) -> np.ndarray:
+    """Colour classes of cells such that cells of one colour share no DOF
+    (greedy, cells in order; each cell takes the lowest colour unused by the
+    cells it shares a DOF with)."""
+    C = cell_map.shape[0]
+    cells_of = [[] for _ in range(M)]
+    for c in range(C):
+        for g in np.unique(cell_map[c]):
+            cells_of[g].append(c)
+    color = np.full(C, -1, dtype=np.int64)
+    for c in range(C):
+        used = {int(color[o]) for g in np.unique(cell_map[c]) for o in cells_of[g] if color[o] >= 0}
+        k = 0
+        while k in used:
+            k += 1
+        color[c] = k
+    return color.astype(np.int8)
+
+
+def build_graded_problem(wl: Workload, refine_lo: int, refine_hi: int) -> FEProblem:
+    """Periodic cube with the coarse cells [refine_lo, refine_hi)^3 split into
+    2x2x2 children: a graded mesh with two cell sizes and hanging nodes on the
+    coarse-fine interface (2:1, as deal.II's one-level refinement).
+
+    * DOFs are the distinct node positions (periodicity folded); a fine-cell
+      node on the boundary of an unrefined coarse cell that is not one of that
+      cell's nodes is a hanging node: a constrained row
+      x_s = sum_j N_j^coarse(x_s) x_j over the coarse cell's nodes whose basis
+      function is nonzero there (its face or edge nodes), the interpolation
+      deal.II's hanging-node constraints express (src/dft/initUnmovedTriangulation.cc:136-211
+      builds them with make_hanging_node_constraints).
+    * Each cell keeps its own size a: JxW scales by (a/h)^3, the stiffness by
+      a/h and the shape gradients by h/a (Cartesian cells).
+    * The lumped mass is condensed through the constraints as
+      distribute_local_to_global does (kohnShamDFTOperatorDevice.cc:977-983):
+      a slave's local mass goes to its masters times the coefficients; slave
+      rows have zero mass and zero M^{+-1/2}.
+    * Cells are coloured greedily (no two cells of a colour share a DOF)."""
+    n, p, nq, L = wl.n, wl.p, wl.nq, wl.L
+    if not (0 <= refine_lo < refine_hi <= n):
+        raise ValueError("refined block must lie inside the coarse grid")
+    h = L / n
+    pp = p + 1
+    ne = pp ** 3
+    nqc = nq ** 3
+    base = build_problem(dataclasses.replace(wl, n_atoms=0), "unique")
+    gll_x = base.gll_x
+    # ---- cells: coarse cells outside the block, 8 children inside
+    origins, sizes, coords = [], [], []
+    for cz in range(n):
+        for cy in range(n):
+            for cx in range(n):
+                inside = all(refine_lo <= c < refine_hi for c in (cx, cy, cz))
+                if not inside:
+                    origins.append((cx * h, cy * h, cz * h))
+                    sizes.append(h)
+                    coords.append((cx, cy, cz))
+                    continue
+                for dz in range(2):
+                    for dy in range(2):
+                        for dx in range(2):
+                            origins.append(((cx + 0.5 * dx) * h, (cy + 0.5 * dy) * h,
+                                            (cz + 0.5 * dz) * h))
+                            sizes.append(0.5 * h)
+                            coords.append((cx, cy, cz))
+    origin = np.asarray(origins)
+    size = np.asarray(sizes)
+    C = len(size)
+    lc, lb, la = np.meshgrid(np.arange(pp), np.arange(pp), np.arange(pp), indexing="ij")
+    la, lb, lc = la.ravel(), lb.ravel(), lc.ravel()
+    local = np.stack([gll_x[la], gll_x[lb], gll_x[lc]], axis=1)          # (ne, 3)
+    pos = origin[:, None, :] + size[:, None, None] * local[None, :, :]    # (C, ne, 3)
+    # ---- distinct positions (periodic) -> DOFs, ordered z, y, x
+    Q = 1 << 36
+    key = np.rint(pos / L * Q).astype(np.int64) % Q
+    flat = key.reshape(-1, 3)[:, ::-1]
+    uniq, inv = np.unique(flat, axis=0, return_inverse=True)
+    cell_map = inv.reshape(C, ne).astype(np.int64)
+    M = len(uniq)
+    dof_pos = uniq[:, ::-1].astype(np.float64) / Q * L                     # (M, 3) in [0, L)
+    # ---- hanging nodes: on the closed box of an unrefined coarse cell, not its node
+    refined = np.zeros((n, n, n), dtype=bool)
+    refined[refine_lo:refine_hi, refine_lo:refine_hi, refine_lo:refine_hi] = True
+    coarse_cell = {}
+    for c in range(C):
+        if size[c] == h:
+            coarse_cell[coords[c]] = c
+    rows, ptr, cols, vals = [], [0], [], []
+    tol = 1e-9
+    for g in range(M):
+        u = dof_pos[g] / h
+        cand = []
+        for d in range(3):
+            k = int(round(u[d]))
+            cand.append([(k - 1) % n, k % n] if abs(u[d] - k) < tol else [int(np.floor(u[d])) % n])
+        found = False
+        for cz in cand[2]:
+            for cy in cand[1]:
+                for cx in cand[0]:
+                    if found or refined[cz, cy, cx]:
+                        continue
+                    xi = _min_image(dof_pos[g] - np.array([cx, cy, cz]) * h, L) / h
+                    # the image of the node inside this cell's closed box
+                    xi = np.where(xi < -tol, xi + L / h, xi)
+                    if np.any(xi < -tol) or np.any(xi > 1 + tol):
+                        continue
+                    xi = np.clip(xi, 0.0, 1.0)
+                    on_node = all(np.min(np.abs(gll_x - xi[d])) < tol for d in range(3))
+                    if on_node:
+                        continue
+                    ph = [lagrange_tables(gll_x, np.array([xi[d]]))[0][0] for d in range(3)]
+                    w = ph[0][la] * ph[1][lb] * ph[2][lc]
+                    keep = np.nonzero(np.abs(w) > 1e-13)[0]
+                    cc = coarse_cell[(cx, cy, cz)]
+                    rows.append(g)
+                    cols.extend(cell_map[cc][keep].tolist())
+                    vals.extend(w[keep].tolist())
+                    ptr.append(len(cols))
+                    found = True
+    slave_rows = np.asarray(rows, dtype=np.int64)
+    c_ptr = np.asarray(ptr, dtype=np.int64)
+    c_cols = np.asarray(cols, dtype=np.int64)
+    c_vals = np.asarray(vals)
+    constrained = np.zeros(M, dtype=bool)
+    constrained[slave_rows] = True
+    if np.any(constrained[c_cols]):
+        raise RuntimeError("a master of a hanging node is itself constrained")
+    scale = size / h
+
+    prob = dataclasses.replace(
+        base, wl=wl, C=C, M=M, cell_coords=np.asarray(coords, dtype=np.int64), cell_map=cell_map,
+        cell_color=greedy_colors(cell_map, M), V_q=np.zeros((C, nqc)),
+        mass=np.zeros(M), sqrt_mass=np.zeros(M), inv_sqrt_mass=np.zeros(M),
+        constrained=constrained, slave_rows=slave_rows,
+        master_rows=np.zeros(0, dtype=np.int64), cell_size=size, cell_origin=origin,
+        c_ptr=c_ptr, c_cols=c_cols, c_vals=c_vals)
+    # ---- atoms, wells (same seeded draws as build_problem)
+    rng = np.random.default_rng(wl.seed)
+    prob.atom_pos = rng.uniform(0.0, L, size=(wl.n_atoms, 3))
+    prob.atom_depth = rng.uniform(*wl.depth, size=wl.n_atoms)
+    prob.atom_width = rng.uniform(*wl.width, size=wl.n_atoms)
+    prob.proj_V = rng.uniform(*wl.vnl, size=4 * wl.n_atoms)
+    prob.n_proj = 4 * wl.n_atoms
+    xq = prob.quad_points()
+    V = np.zeros((C, nqc))
+    for a in range(wl.n_atoms):
+        d = _min_image(xq - prob.atom_pos[a], L)
+        V -= prob.atom_depth[a] * np.exp(-(d ** 2).sum(-1) / prob.atom_width[a] ** 2)
+    prob.V_q = V
+    # ---- lumped mass, condensed through the constraints
+    m_loc = h ** 3 * base.gll_w[la] * base.gll_w[lb] * base.gll_w[lc]
+    mass = np.zeros(M)
+    np.add.at(mass, cell_map.ravel(), (scale[:, None] ** 3 * m_loc[None, :]).ravel())
+    for r, srow in enumerate(slave_rows):
+        js = slice(c_ptr[r], c_ptr[r + 1])
+        np.add.at(mass, c_cols[js], c_vals[js] * mass[srow])
+    mass[slave_rows] = 0.0
+    prob.mass = mass
+    ok = ~constrained
+    prob.sqrt_mass = np.where(ok, np.sqrt(np.maximum(mass, 0.0)), 0.0)
+    prob.inv_sqrt_mass = np.where(ok & (mass > 1e-15),
+                                  1.0 / np.sqrt(np.where(mass > 0, mass, 1.0)), 0.0)
+    prob.pair_atom, prob.pair_cell, prob.pair_C = projectors(prob)
+    return prob
 
 
 def with_kpoint(prob: FEProblem, kpoint) -> FEProblem:

@@ -46,6 +46,8 @@ class OperatorDesc(ct.Structure):
         ("n_kpoints", ct.c_int32), ("kpoints", c_double_p), ("cell_mass", c_double_p),
         ("cell_grad", c_double_p), ("pair_C_kpoints", c_double_p),
         ("shape_gradients", c_double_p),
+        ("cell_stiffness", c_double_p), ("inverse_jacobians", c_double_p),
+        ("jacobian_kind", ct.c_int32),
     ]
 
 
@@ -71,6 +73,7 @@ class Params(ct.Structure):
         ("mixed_prec_switch_residual", ct.c_double),
         ("mixed_prec_defect_filter", ct.c_int32),
         ("is_pseudopotential", ct.c_int32), ("lanczos_full_residual_margin", ct.c_int32),
+        ("mixed_prec_fp64_top_columns", ct.c_int32),
     ]
 
 

@@ -65,7 +65,7 @@ typedef struct {
   int32_t quad_per_cell;     /* n_q                                            */
   const int64_t *cell_map;   /* C*n_e: DOF of local node i of cell c
                                 (vectorUtilities.cc:398-500)                   */
-  const int8_t *cell_color;  /* C: colour 0..7; cells of one colour share no DOF */
+  const int8_t *cell_color;  /* C: colour 0..31; cells of one colour share no DOF */
   const double *sqrt_mass;   /* M: M^{1/2}, 0 on constrained rows              */
   const double *inv_sqrt_mass; /* M: M^{-1/2}, 0 on constrained rows           */
   const double *shape_values;  /* n_q*n_e: N_i(q), row q                        */
@@ -79,7 +79,12 @@ typedef struct {
   const double *pair_C;      /* n_pairs*n_e*proj_per_pair: C_e[i][alpha]        */
   const double *proj_V;      /* n_proj: coupling constants                     */
   /* affine constraints, CSR over constrained rows (constraintMatrixInfoDevice.cc:446-542).
-   * n_constrained == 0 means none. */
+   * n_constrained == 0 means none.  Rows that are all single-master, unit
+   * coefficient (periodic identities) are folded into the cell map; any other
+   * row (hanging nodes: several masters, fractional coefficients) makes every
+   * apply distribute before the cell products and send the constrained rows'
+   * sums to their masters after them (kohnShamDFTOperatorDevice.cc:3806-3838).
+   * Masters must not be constrained themselves. */
   int64_t n_constrained;
   const int64_t *constraint_rows;  /* n_constrained slave rows                  */
   const int64_t *constraint_ptr;   /* n_constrained+1                           */
@@ -104,6 +109,18 @@ typedef struct {
    * cells, hamiltonianMatrixCalculatorFlattenedDevice.cc:415-427); nullable,
    * needed only by chfsi_operator_set_potential_gga */
   const double *shape_gradients;
+  /* per-cell geometry (graded or non-Cartesian meshes); nullable.
+   * cell_stiffness: C*n_e*n_e, each cell's int grad N_i . grad N_j (the
+   * reference's cellShapeFunctionGradientIntegral,
+   * hamiltonianMatrixCalculatorFlattenedDevice.cc:110-111); when set it
+   * replaces `stiffness`.
+   * inverse_jacobians with jacobian_kind 3 (C*3 diagonals, Cartesian cells)
+   * or 9 (C*9 row-major matrices): shape_gradients are then the reference
+   * cell's and the GGA build applies each cell's inverse Jacobian
+   * (:402-427); jacobian_kind 0: shape_gradients are physical and shared. */
+  const double *cell_stiffness;
+  const double *inverse_jacobians;
+  int32_t jacobian_kind;
 } chfsi_operator_desc;
 
 /* Batch of k-point blocks filtered together (complex form). */
@@ -189,6 +206,13 @@ typedef struct {
                                       leave bUp below the top of the spectrum, and
                                       the filter then amplifies the states above
                                       bUp every pass                               */
+  int32_t mixed_prec_fp64_top_columns; /* with a lower filter precision: the last k
+                                      columns (the highest Ritz values of the
+                                      previous pass) are filtered in FP64.  Their
+                                      filter gain is the smallest, so in the FP32
+                                      forms their rounding, relative to the
+                                      column, is the largest.  No reference
+                                      counterpart; 0: none                         */
 } chfsi_params;
 
 const char *chfsi_last_error(void);

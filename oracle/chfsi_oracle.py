@@ -62,16 +62,24 @@ def cell_hamiltonians(prob, V_q: Optional[np.ndarray] = None, cells=None,
     cells = np.arange(prob.C) if cells is None else np.asarray(cells)
     G = shape_gradients(prob)
     K = np.einsum("q,qid,qjd->ij", prob.JxW, G, G)
-    vJxW = V_q[cells] * prob.JxW[None, :]
+    # graded meshes: a cell of edge a = s h has JxW s^3 JxW_h and gradients
+    # grad_h / s, so its stiffness integral is s K_h
+    # (per-cell cellShapeFunctionGradientIntegral, hamiltonianMatrixCalculatorFlattenedDevice.cc:110-111;
+    #  inverse Jacobian of Cartesian cells, :408-420)
+    scale = prob.cell_scale if getattr(prob, "cell_size", None) is not None else np.ones(prob.C)
+    vJxW = V_q[cells] * prob.JxW[None, :] * (scale[cells] ** 3)[:, None]
     Nt = prob.Nq.T
     shift = 0.0
     if prob.kpoint is not None:
+        if getattr(prob, "cell_size", None) is not None:
+            raise ValueError("Bloch cell terms are built for uniform meshes only")
         shift = bloch_cell_terms(prob, G)
     H = np.empty((len(cells), prob.ne, prob.ne), dtype=np.result_type(np.float64, shift))
     for k in range(len(cells)):
-        H[k] = 0.5 * K + (Nt * vJxW[k][None, :]) @ prob.Nq + shift
+        sc = scale[cells[k]]
+        H[k] = 0.5 * sc * K + (Nt * vJxW[k][None, :]) @ prob.Nq + shift
         if gJxW is not None:
-            U = np.einsum("qd,qid->qi", gJxW[cells[k]], G)      # g . grad N_i at q
+            U = np.einsum("qd,qid->qi", gJxW[cells[k]], G / sc)  # g . grad N_i at q
             H[k] += 2.0 * (U.T @ prob.Nq + prob.Nq.T @ U)
     return H
 
@@ -144,17 +152,29 @@ class OracleOperator:
 
     # -- constraints (utils/constraintMatrixInfo.cc; device
     #    utils/constraintMatrixInfoDevice.cc:544-588 distribute, :596-816 slave->master,
-    #    :818-855 set_zero).  Periodic constraints: one master, coefficient 1.
+    #    :818-855 set_zero).  Affine rows x_s = sum_j c_sj x_j (periodic: one
+    #    master, coefficient 1; hanging nodes: the coarse face or edge nodes
+    #    with the coarse basis values as coefficients).
+    def _constraint_matrix(self):
+        """Sparse (S x M) matrix of the constraint coefficients, row r for slave_rows[r]."""
+        if not hasattr(self, "_Pc"):
+            p = self.prob
+            rows, ptr, cols, vals = p.constraint_csr()
+            r = np.repeat(np.arange(len(rows)), np.diff(ptr))
+            self._Pc = sp.csr_matrix((vals, (r, cols)), shape=(len(rows), p.M))
+        return self._Pc
+
     def distribute(self, X):
         p = self.prob
         if len(p.slave_rows):
-            X[p.slave_rows] = X[p.master_rows]
+            X[p.slave_rows] = self._constraint_matrix() @ X
 
     def distribute_slave_to_master(self, Y):
         p = self.prob
         if len(p.slave_rows):
-            np.add.at(Y, p.master_rows, Y[p.slave_rows])
+            Ys = Y[p.slave_rows].copy()
             Y[p.slave_rows] = 0.0
+            Y += self._constraint_matrix().T @ Ys
 
     def set_zero(self, X):
         p = self.prob
@@ -223,11 +243,15 @@ class OracleOperator:
             Cg = sp.coo_matrix((p.pair_C.ravel(), (rows, cols)), shape=(p.M, p.n_proj)).tocsr()
             H = H + Cg @ sp.diags(p.proj_V) @ Cg.conj().T
         if len(p.slave_rows):
-            # P^T H P with P the periodic prolongation (slaves copy masters)
+            # P^T H P with P the prolongation: identity on unconstrained rows,
+            # the constraint coefficients on slave rows (assembled from the
+            # CSR arrays directly, not through distribute)
             keep = np.nonzero(~p.constrained)[0]
-            src = np.arange(p.M).copy()
-            src[p.slave_rows] = p.master_rows
-            P = sp.csr_matrix((np.ones(p.M), (np.arange(p.M), src)), shape=(p.M, p.M))
+            rows, ptr, cols, vals = p.constraint_csr()
+            pr = np.concatenate([keep, np.repeat(rows, np.diff(ptr))])
+            pc = np.concatenate([keep, cols])
+            pv = np.concatenate([np.ones(len(keep)), vals])
+            P = sp.csr_matrix((pv, (pr, pc)), shape=(p.M, p.M))
             H = (P.T @ H @ P).tocsr()
             mask = np.zeros(p.M)
             mask[keep] = 1.0
@@ -512,9 +536,13 @@ def hx_adjoint_v(op, Ybar: np.ndarray, X: np.ndarray, scalar: float = 1.0) -> np
     """dL/dV(c, q) for Y = scalar * H~(V) X and Ybar = dL/dY:
     scalar * JxW_q * sum_v (N S Ybar_v)(q) (N S X_v)(q)."""
     p = op.prob
-    a = quad_values(p, op.s[:, None] * Ybar)
-    b = quad_values(p, op.s[:, None] * X)
-    return scalar * p.JxW[None, :] * (a * b).sum(axis=2)
+    Ys = op.s[:, None] * Ybar
+    Xs = op.s[:, None] * X
+    op.distribute(Ys)
+    op.distribute(Xs)
+    a = quad_values(p, Ys)
+    b = quad_values(p, Xs)
+    return scalar * p.jxw_cells * (a * b).sum(axis=2)
 
 
 def filter_backward(op, X0: np.ndarray, Ybar: np.ndarray, m: int, a: float, b: float, a0: float):
@@ -569,10 +597,16 @@ def fermi_dirac(eps: np.ndarray, mu: float, kT: float) -> np.ndarray:
 
 def density_at_quadrature(prob, X: np.ndarray, occ: np.ndarray, spin_factor: float = 2.0,
                           kweight: float = 1.0) -> np.ndarray:
-    """rho(q) = sum_i f_i w_k s |psi_i(q)|^2, psi interpolated from the nodes.
+    """rho(q) = sum_i f_i w_k s |psi_i(q)|^2, psi interpolated from the nodes
+    after the constrained rows are distributed from their masters.
     Reference: src/dft/densityCalculator.cc:212-254, :622-689.  Shape (C, nqc)."""
     rho = np.zeros((prob.C, prob.nqc))
     w = spin_factor * kweight * occ
+    if len(prob.slave_rows):
+        rows, ptr, cols, vals = prob.constraint_csr()
+        r = np.repeat(np.arange(len(rows)), np.diff(ptr))
+        X = X.copy()
+        X[rows] = sp.csr_matrix((vals, (r, cols)), shape=(len(rows), prob.M)) @ X
     for c0 in range(0, prob.C, 256):
         c1 = min(prob.C, c0 + 256)
         Xc = X[prob.cell_map[c0:c1]]                      # (k, ne, N)
