@@ -25,6 +25,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <complex>
 #include <cstdint>
 #include <memory>
 #include <stdexcept>
@@ -53,9 +54,18 @@ template <utils::MemorySpace memorySpace>
 class BLASWrapper {};
 }  // namespace linearAlgebra
 
+/* dataTypes::number is real, or std::complex<double> when the including code
+ * defines USE_COMPLEX, as in the reference (include/dftfeDataTypes.h:38-48).
+ * The complex form runs on the operator's active k-point
+ * (reinitkPointSpinIndex). */
 namespace dataTypes {
+#ifdef USE_COMPLEX
+typedef std::complex<double> number;
+typedef std::complex<float> numberFP32;
+#else
 typedef double number;
 typedef float numberFP32;
+#endif
 }  // namespace dataTypes
 
 /* the projected dense eigensolve runs on the GPU (rocSOLVER) instead of ELPA */
@@ -69,6 +79,35 @@ inline void chfsiCheck(chfsi_status st, const char *what) {
 inline void hipCheck(hipError_t e, const char *what) {
   if (e != hipSuccess)
     throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+/* the library call of each number type: real entry points, or the complex
+ * ones at the operator's active k-point (interleaved std::complex layout) */
+inline chfsi_status hxCall(chfsi_operator *op, const double *x, int64_t ldx, double *y,
+                           int64_t ldy, int32_t B, int32_t scaleFlag, double scalar, void *st) {
+  return chfsi_hx(op, x, ldx, y, ldy, B, scaleFlag, scalar, st);
+}
+inline chfsi_status hxCall(chfsi_operator *op, const std::complex<double> *x, int64_t ldx,
+                           std::complex<double> *y, int64_t ldy, int32_t B, int32_t scaleFlag,
+                           double scalar, void *st) {
+  return chfsi_hx_complex(op, x, ldx, y, ldy, B, scaleFlag, scalar, CHFSI_FP64, st);
+}
+inline chfsi_status solveCall(chfsi_solver *s, chfsi_operator *op, double *X, int64_t M,
+                              int32_t N, double *eig, double *res, int32_t first,
+                              int32_t computeResidual, int32_t mixed, int32_t firstScf,
+                              double *bUp, double *timings, void *st) {
+  return chfsi_solve(s, op, X, M, N, eig, res, first, computeResidual, mixed, firstScf, bUp,
+                     timings, st);
+}
+inline chfsi_status solveCall(chfsi_solver *s, chfsi_operator *op, std::complex<double> *X,
+                              int64_t M, int32_t N, double *eig, double *res, int32_t first,
+                              int32_t computeResidual, int32_t mixed, int32_t firstScf,
+                              double *bUp, double *timings, void *st) {
+  return chfsi_solve_complex(s, op, X, M, N, eig, res, first, computeResidual, mixed, firstScf,
+                             bUp, timings, st);
+}
+constexpr bool isComplexNumber() {
+  return sizeof(dataTypes::number) == 2 * sizeof(double);
 }
 }  // namespace internal
 
@@ -296,9 +335,10 @@ class kohnShamDFTOperatorDeviceClass : public operatorDFTDeviceClass {
           const bool onlyHPrimePartForFirstOrderDensityMatResponse = false) override {
     if (onlyHPrimePartForFirstOrderDensityMatResponse)
       throw std::runtime_error("HX: H' only is not supported");
-    internal::chfsiCheck(chfsi_hx(d_op, X.begin(), numberComponents, Y.begin(), numberComponents,
-                                  (int32_t)numberComponents, scaleFlag ? 1 : 0, scalar, d_stream),
-                         "chfsi_hx");
+    internal::chfsiCheck(internal::hxCall(d_op, X.begin(), numberComponents, Y.begin(),
+                                          numberComponents, (int32_t)numberComponents,
+                                          scaleFlag ? 1 : 0, scalar, d_stream),
+                         "HX");
   }
   /* Y <- Y + H X with X already scaled (kohnShamDFTOperatorDevice.cc:3874-3997) */
   void HXCheby(distributedDeviceVec<dataTypes::number> &X,
@@ -309,9 +349,16 @@ class kohnShamDFTOperatorDeviceClass : public operatorDFTDeviceClass {
                bool returnBeforeCompressSkipUpdateSkipLocal = false) override {
     if (returnBeforeCompressSkipUpdateSkipNonLocal || returnBeforeCompressSkipUpdateSkipLocal)
       throw std::runtime_error("HXCheby: the split local / nonlocal forms are not supported");
+#ifdef USE_COMPLEX
+    (void)X;
+    (void)numberComponents;
+    (void)Y;
+    throw std::runtime_error("HXCheby: the complex form is applied inside the solver's filter");
+#else
     internal::chfsiCheck(chfsi_hx_cheby(d_op, X.begin(), numberComponents, Y.begin(),
                                         numberComponents, (int32_t)numberComponents, d_stream),
                          "chfsi_hx_cheby");
+#endif
   }
   chfsi_operator *chfsiOperator() override { return d_op; }
   hipStream_t stream() const { return d_stream; }
@@ -370,15 +417,19 @@ class chebyshevOrthogonalizedSubspaceIterationSolverDevice {
     const std::int64_t M = flattenedSize / N;
     double bUp = 0.0;
     if (!eigenValues.empty() && eigenValues.size() != N) {
+      if (internal::isComplexNumber())
+        throw std::runtime_error("solve: the spectrum split is built for real wavefunctions");
       const unsigned int nfr = (unsigned int)eigenValues.size();
       if (nfr > N || !eigenVectorsRotFracDensityFlattenedDevice)
         throw std::runtime_error("solve: spectrum split needs n <= N and the XFrac array");
       residuals.assign(nfr, 0.0);
       internal::chfsiCheck(
           chfsi_solve_spectrum_split(d_solver, operatorMatrix.chfsiOperator(),
-                                     eigenVectorsFlattenedDevice, M, (int32_t)N,
-                                     (int32_t)(N - nfr),
-                                     eigenVectorsRotFracDensityFlattenedDevice, nfr,
+                                     reinterpret_cast<double *>(eigenVectorsFlattenedDevice), M,
+                                     (int32_t)N, (int32_t)(N - nfr),
+                                     reinterpret_cast<double *>(
+                                         eigenVectorsRotFracDensityFlattenedDevice),
+                                     nfr,
                                      eigenValues.data(), residuals.data(), isFirstFilteringCall,
                                      computeResidual, useMixedPrecOverall, isFirstScf, &bUp,
                                      d_timings, streamOf(operatorMatrix)),
@@ -387,11 +438,12 @@ class chebyshevOrthogonalizedSubspaceIterationSolverDevice {
       eigenValues.assign(N, 0.0);
       residuals.assign(N, 0.0);
       internal::chfsiCheck(
-          chfsi_solve(d_solver, operatorMatrix.chfsiOperator(), eigenVectorsFlattenedDevice, M,
-                      (int32_t)N, eigenValues.data(), residuals.data(), isFirstFilteringCall,
-                      computeResidual, useMixedPrecOverall, isFirstScf, &bUp, d_timings,
-                      streamOf(operatorMatrix)),
-          "chfsi_solve");
+          internal::solveCall(d_solver, operatorMatrix.chfsiOperator(),
+                              eigenVectorsFlattenedDevice, M, (int32_t)N, eigenValues.data(),
+                              residuals.data(), isFirstFilteringCall, computeResidual,
+                              useMixedPrecOverall, isFirstScf, &bUp, d_timings,
+                              streamOf(operatorMatrix)),
+          "solve");
     }
     return bUp;
   }
@@ -407,8 +459,11 @@ class chebyshevOrthogonalizedSubspaceIterationSolverDevice {
     const unsigned int N = totalNumberWaveFunctions;
     if (N == 0 || flattenedSize % N != 0)
       throw std::runtime_error("solveNoRR: flattenedSize is not a multiple of the wavefunction count");
+    if (internal::isComplexNumber())
+      throw std::runtime_error("solveNoRR: built for real wavefunctions");
     internal::chfsiCheck(
-        chfsi_solve_no_rr(d_solver, operatorMatrix.chfsiOperator(), eigenVectorsFlattenedDevice,
+        chfsi_solve_no_rr(d_solver, operatorMatrix.chfsiOperator(),
+                          reinterpret_cast<double *>(eigenVectorsFlattenedDevice),
                           flattenedSize / N, (int32_t)N, (int32_t)numberPasses,
                           d_dftParams.isPseudopotential ? 1 : 0, useMixedPrecOverall,
                           streamOf(operatorMatrix)),

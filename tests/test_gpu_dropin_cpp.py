@@ -146,3 +146,62 @@ def test_dropin_gga_rebuild(run):
     prob, _, _, X0, g, d = run
     ref_g = O.OracleOperator(prob, H_e=O.cell_hamiltonians(prob, gJxW=g))
     assert _rel(_read(d, "hx_gga.f64", (prob.M, N)), ref_g.apply(X0)) < 1e-13
+
+
+DRIVER_C = os.path.join(ROOT, "dftfe_amd", "bin", "dropin_solve_complex")
+
+
+def test_dropin_complex_kpoint(tmp_path):
+    """The classes built with USE_COMPLEX (dataTypes::number = std::complex<double>)
+    on the 20 bohr / p=4 operator with 6 seeded k-points: HX at k-point 3 and
+    two solves with the caller's bound update against the oracle's complex
+    passes (32 states, blocks of 16, degree 20)."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    if not os.path.exists(DRIVER_C):
+        raise RuntimeError(f"{DRIVER_C} is not built (run dftfe_amd.build)")
+    prob = fe.build_workload("cube20_p4_n64")
+    kpts, _ = fe.kpoint_set(prob, n=6)
+    kidx, Nc, Bc, m = 3, 32, 16, 20
+    ref = O.OracleOperator(fe.with_kpoint(prob, kpts[kidx]))
+    lo, up, _, _ = O.lanczos_bounds(ref, prob.lanczos_start.astype(np.complex128), 10)
+    bounds = O.first_call_bounds(lo, up, Nc, prob.M)
+    X0 = fe.complex_guess(prob, Nc, seed_offset=17)
+    d = str(tmp_path)
+    meta = dict(M=prob.M, C=prob.C, ne=prob.ne, nq=prob.nqc, n_proj=prob.n_proj,
+                n_pairs=len(prob.pair_cell), N=Nc, B=Bc, m=m, nk=len(kpts), kidx=kidx,
+                a0=bounds[0], bLow=bounds[1], bUp=bounds[2])
+    with open(os.path.join(d, "meta.txt"), "w") as f:
+        for k, v in meta.items():
+            f.write(f"{k} {v!r}\n")
+    _write(d, "map.i64", prob.cell_map, np.int64)
+    _write(d, "color.i8", prob.cell_color, np.int8)
+    _write(d, "sqrt_mass.f64", prob.sqrt_mass, np.float64)
+    _write(d, "inv_sqrt_mass.f64", prob.inv_sqrt_mass, np.float64)
+    _write(d, "Nq.f64", prob.Nq, np.float64)
+    _write(d, "K.f64", prob.K_e, np.float64)
+    _write(d, "jxw.f64", np.tile(prob.JxW, prob.C), np.float64)
+    _write(d, "pair_cell.i32", prob.pair_cell, np.int32)
+    _write(d, "pair_proj0.i32", 4 * prob.pair_atom, np.int32)
+    _write(d, "pair_C.f64", prob.pair_C, np.float64)
+    _write(d, "proj_V.f64", prob.proj_V, np.float64)
+    _write(d, "vJxW.f64", prob.vJxW, np.float64)
+    _write(d, "kpts.f64", kpts, np.float64)
+    _write(d, "Mc.f64", prob.M_c, np.float64)
+    _write(d, "Gc.f64", prob.G_c, np.float64)
+    zc = np.stack([fe.projectors(prob, k)[2] for k in kpts]).astype(np.complex128)
+    _write(d, "pair_C_k.f64", zc.view(np.float64), np.float64)
+    _write(d, "Xc0.c128", X0, np.complex128)
+    print(f"dropin_solve_complex on cube20_p4_n64 k={kidx} N={Nc}", flush=True)
+    res = subprocess.run([DRIVER_C, d], capture_output=True, text=True, timeout=120)
+    if res.returncode != 0:
+        raise RuntimeError(f"dropin_solve_complex failed ({res.returncode}):\n{res.stdout}\n"
+                           f"{res.stderr}")
+    hx = np.fromfile(os.path.join(d, "hx_c.c128"), dtype=np.complex128).reshape(prob.M, Nc)
+    assert _rel(hx, ref.apply(X0)) < 1e-13
+    lam_o, res_o, X_o = O.chfsi_pass(ref, X0, *bounds, m, Bc)
+    assert np.abs(_read(d, "eigc1.f64", (Nc,)) - lam_o).max() <= 1e-10
+    assert np.abs(_read(d, "resc1.f64", (Nc,)) - res_o).max() <= 1e-8
+    lam2_o, res2_o, _ = O.chfsi_pass(ref, X_o, *O.next_bounds(lam_o), bounds[2], m, Bc)
+    assert np.abs(_read(d, "eigc2.f64", (Nc,)) - lam2_o).max() <= 1e-10
+    assert np.abs(_read(d, "resc2.f64", (Nc,)) - res2_o).max() <= 1e-8
