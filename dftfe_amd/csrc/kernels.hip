@@ -1655,6 +1655,231 @@ hipError_t launch_hx<double>(const HxLaunch<double> &a, hipStream_t s) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// BF16 HX on K = 32 MFMA (v_mfma_f32_16x16x32_bf16) fed by the direct-to-LDS
+// ring of hxp_kernel.  A stage holds two 16-wide K slices of the BF16 cell
+// matrix (rows of 32 bytes; the two 16-byte units of row r are swapped when
+// (r >> 3) & 1, so that the A-operand reads of 16 rows fall in distinct banks)
+// and the 32 gathered FP32 rows of the stage ([k][column] groups of RPO rows
+// per 1 KB copy, the copies 32 bytes apart so that the four k rows of a
+// B-operand read fall in distinct bank windows).  The B operand is read in
+// FP32, scaled and rounded to BF16 in registers; one MFMA covers 32 K.  An odd
+// count of 16-wide slices ends with a half stage whose second slice reads
+// clamped data multiplied by zero.  Same work items, colouring and epilogue
+// as hx_kernel; requires an even number of row blocks.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ bf16x8v pack_bf16x8(const float (&x)[8]) {
+  uint32_t w[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e)
+    w[e] = (uint32_t)to_bf16(x[2 * e]) | ((uint32_t)to_bf16(x[2 * e + 1]) << 16);
+  bf16x8v r;
+  __builtin_memcpy(&r, w, sizeof(r));
+  return r;
+}
+
+template <int NW, int TN, int WC, int NRB, int S>
+__global__ __launch_bounds__(64 * NW)
+__attribute__((amdgpu_waves_per_eu(NW == 8 && S == 2 ? 4 : 1))) void hxb_kernel(
+    HxLaunch<float> a, const uint16_t *Ht16, const uint16_t *nlC16) {
+  using v4 = f32x4;
+  constexpr int NT = 64 * NW;
+  constexpr int WR = NW / WC;
+  constexpr int RBW = (NRB + WR - 1) / WR;
+  constexpr int ROWS_MAX = NRB * 16;
+  constexpr int CBW = TN / 16 / WC;
+  static_assert(TN == 64 && CBW >= 1 && TN % (16 * WC) == 0, "bad tile");
+  static_assert(NRB % 2 == 0, "even row-block count");
+  static_assert(S >= 2, "at least two stages");
+  constexpr int NLMAX = 128;
+  constexpr int KMAX = ROWS_MAX + NLMAX + 16;  // + the zero half of an odd last stage
+  constexpr int HSB = ROWS_MAX * 32;           // bytes of one 16-wide BF16 K slice
+  constexpr int HCP = HSB / 1024;              // 1 KB copies per slice
+  constexpr int RPO = 256 / TN;                // gathered rows per copy
+  constexpr int LPR = TN / 4;                  // lanes per gathered row
+  constexpr int XG = 1024 + 32;                // bytes per gathered copy incl. spacing
+  constexpr int NXC = 32 / RPO;                // gathered copies per stage
+  constexpr int HB = 2 * HSB;
+  constexpr int SB = HB + NXC * XG;
+  constexpr int NOP = 2 * HCP + NXC;
+  constexpr int D = (NOP + NW - 1) / NW;
+  static_assert((S - 2) * D < 64, "vmcnt range");
+
+  __shared__ __align__(16) unsigned char ring[S * SB];
+  __shared__ uint32_t smap[ROWS_MAX];
+  __shared__ float sout[ROWS_MAX];
+  __shared__ __align__(16) int64_t soff[KMAX];
+  __shared__ __align__(16) float sscl[KMAX];
+
+  const int wid = blockIdx.x;
+  const int local = wid >> 3;
+  if (local >= a.per_xcd) return;
+  const int item = (wid & 7) * a.per_xcd + local;
+  if (item >= a.ncells * a.ntiles) return;
+  const int cell = a.cells[item / a.ntiles];
+  const int v0 = (item % a.ntiles) * TN;
+
+  const int tid = threadIdx.x;
+  __builtin_assume(tid < NT);
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wave_s = __builtin_amdgcn_readfirstlane(wave);
+  const int wc = wave % WC;
+  const int wr = wave / WC;
+
+  int nl_begin = 0, nslot = 0;
+  if (a.nl_off) {
+    nl_begin = a.nl_off[cell];
+    nslot = a.nl_off[cell + 1] - nl_begin;
+  }
+  const int nt16 = NRB + (nslot >> 4);  // 16-wide K slices
+  const int nst = (nt16 + 1) >> 1;      // stages
+  const uint32_t *cmap = a.map + (int64_t)cell * a.ne;
+  const uint16_t *Hcell = Ht16 + (int64_t)cell * a.ne_pad * a.ne_pad;
+  const uint16_t *NLcell = nlC16 ? nlC16 + (int64_t)nl_begin * a.ne_pad : Hcell;
+
+  // gather tables: K row t*16 + kk of slice t (cell nodes, then projector
+  // slots); padding rows, empty slots and the zero half read row 0 with scale 0
+  for (int i = tid; i < KMAX; i += NT) {
+    int64_t off = 0;
+    float sc = 0.0f;
+    if (i < ROWS_MAX) {
+      uint32_t w = 0;
+      float so = 1.0f;
+      if (i < a.ne) {
+        w = cmap[i];
+        const uint32_t r = w & 0x7fffffffu;
+        off = (int64_t)r * a.ld_src;
+        sc = a.s_in ? a.s_in[r] : 1.0f;
+        if (a.s_out) so = a.s_out[r];
+      }
+      smap[i] = w;
+      sout[i] = so;
+    } else {
+      const int sl = i - ROWS_MAX;
+      const int pj = sl < nslot ? a.slot_proj[nl_begin + sl] : -1;
+      if (pj >= 0) {
+        off = (int64_t)pj * a.ldW;
+        sc = 1.0f;
+      }
+    }
+    soff[i] = off;
+    sscl[i] = sc;
+  }
+  __syncthreads();
+
+  const uint32_t ring0 = lds_address(ring);
+  // H copy: lane L fills LDS unit L of 64 (row L/2, unit L&1) with the global
+  // unit (L&1) ^ ((row >> 3) & 1) of that row
+  const int hr = lane >> 1;
+  const int hoff = hr * 16 + (((lane & 1) ^ ((hr >> 3) & 1)) * 8);
+  const int xsub = lane / LPR;
+  const int xc = v0 + 4 * (lane % LPR);
+  const int xcl = xc < a.B - 4 ? xc : a.B - 4;  // columns >= B are never stored
+
+  auto slice_ptr = [&](int t) -> const uint16_t * {
+    return t < NRB ? Hcell + (int64_t)t * ROWS_MAX * 16
+                   : NLcell + (int64_t)(t - NRB) * ROWS_MAX * 16;
+  };
+  auto issue = [&](int st, int stage) {
+    const int t0 = 2 * st;
+    const int t1 = 2 * st + 1;
+    const uint16_t *h0 = slice_ptr(t0);
+    const uint16_t *h1 = t1 < nt16 ? slice_ptr(t1) : h0;  // zero half: any finite data
+    const float *x0 = (t0 < NRB ? a.src : a.W) + xcl;
+    const float *x1 = (t1 < NRB ? a.src : a.W) + xcl;
+    const uint32_t sbase = ring0 + stage * SB;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const int o = (wave_s + NW * j) % NOP;  // wave-uniform
+      // LDS destination of the copy: wave-uniform (M0)
+      const uint32_t dst = __builtin_amdgcn_readfirstlane(
+          o < HCP ? sbase + o * 1024
+                  : (o < 2 * HCP ? sbase + HSB + (o - HCP) * 1024 : sbase + HB + (o - 2 * HCP) * XG));
+      if (o < HCP) {
+        copy16_to_lds(h0 + o * 512 + hoff, dst);
+      } else if (o < 2 * HCP) {
+        copy16_to_lds(h1 + (o - HCP) * 512 + hoff, dst);
+      } else {
+        const int x = o - 2 * HCP;
+        const int k = RPO * x + xsub;  // K row within the stage
+        const int ti = k < 16 ? t0 * 16 + k : t1 * 16 + (k - 16);
+        copy16_to_lds((k < 16 ? x0 : x1) + soff[ti], dst);
+      }
+    }
+  };
+
+  v4 acc[RBW][CBW];
+#pragma unroll
+  for (int j = 0; j < RBW; ++j)
+#pragma unroll
+    for (int cb = 0; cb < CBW; ++cb) acc[j][cb] = v4{0, 0, 0, 0};
+
+  const int arow = lane & 15;
+  const int g = lane >> 4;  // K group: k = 8 g .. 8 g + 7
+  // A operand: slice g >> 1, unit g & 1 of row 16 rb + arow, swizzled
+  const int aoff = (g >> 1) * HSB + arow * 32 + (((g & 1) ^ ((arow >> 3) & 1)) * 16);
+
+  auto compute = [&](int stage, int st) {
+    const unsigned char *Hs = ring + stage * SB;
+    const float *Xs = reinterpret_cast<const float *>(ring + stage * SB + HB);
+    const int tk = (g < 2 ? 2 * st : 2 * st + 1) * 16 + (g & 1) * 8;  // table row of k = 8 g
+    float sc[8];
+    *reinterpret_cast<v4 *>(&sc[0]) = *reinterpret_cast<const v4 *>(&sscl[tk]);
+    *reinterpret_cast<v4 *>(&sc[4]) = *reinterpret_cast<const v4 *>(&sscl[tk + 4]);
+    bf16x8v b8[CBW];
+#pragma unroll
+    for (int cb = 0; cb < CBW; ++cb) {
+      const int col = (wc * CBW + cb) * 16 + arow;
+      float x[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int k = 8 * g + e;
+        x[e] = Xs[(k / RPO) * (XG / 4) + (k % RPO) * TN + col] * sc[e];
+      }
+      b8[cb] = pack_bf16x8(x);
+    }
+#pragma unroll
+    for (int j = 0; j < RBW; ++j) {
+      const int rb = wr + WR * j;
+      if (rb < NRB) {
+        const bf16x8v a8 = *reinterpret_cast<const bf16x8v *>(Hs + aoff + rb * 512);
+#pragma unroll
+        for (int cb = 0; cb < CBW; ++cb)
+          acc[j][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a8, b8[cb], acc[j][cb], 0, 0, 0);
+      }
+    }
+  };
+
+#pragma unroll
+  for (int s = 0; s < S - 1; ++s) issue(s < nst ? s : nst - 1, s);
+  for (int st = 0; st < nst; ++st) {
+    wait_vmcnt<(S - 2) * D>();
+    __syncthreads();
+    const int sn = st + S - 1;
+    issue(sn < nst ? sn : nst - 1, sn % S);
+    compute(st % S, st);
+  }
+  wait_vmcnt<0>();  // no copy may still target LDS when the workgroup ends
+
+  hx_scatter<float, NRB, WR, RBW, CBW, 8>(a, a.ne, acc, smap, sout, wr, wc, lane, v0);
+}
+
+template <int NRB>
+static hipError_t hxb_go(HxLaunch<float> a, const uint16_t *Ht16, const uint16_t *nlC16,
+                         hipStream_t s) {
+  constexpr int NW = 8, TN = 64, WC = 4, S = 2;
+  a.ntiles = (a.B + TN - 1) / TN;
+  const int items = a.ncells * a.ntiles;
+  if (items == 0) return hipSuccess;
+  a.per_xcd = (items + 7) / 8;
+  hipLaunchKernelGGL((hxb_kernel<NW, TN, WC, NRB, S>), dim3(8 * a.per_xcd), dim3(64 * NW), 0, s,
+                     a, Ht16, nlC16);
+  return hipGetLastError();
+}
+
 template <int NRB>
 static hipError_t hx_bf16_go(HxLaunch<float> a, const uint16_t *Ht16, const uint16_t *nlC16,
                              hipStream_t s) {
@@ -1670,6 +1895,23 @@ static hipError_t hx_bf16_go(HxLaunch<float> a, const uint16_t *Ht16, const uint
 
 hipError_t launch_hx_bf16(const HxLaunch<float> &a, const uint16_t *Ht16, const uint16_t *nlC16,
                           hipStream_t s) {
+  // variant 0: the K = 32 ring (16-byte aligned rows and blocks, B a multiple
+  // of 4, an even number of row blocks); 1 or otherwise: the register-staged
+  // K = 16 kernel
+  auto al = [](const void *p) { return ((uintptr_t)p & 15) == 0; };
+  bool ring = a.variant == 0 && a.B >= 4 && a.B % 4 == 0 && a.ld_src % 4 == 0 && al(a.src) &&
+              al(Ht16) && ((a.ne_pad >> 4) % 2 == 0);
+  if (a.nl_off) ring = ring && a.ldW % 4 == 0 && al(a.W) && al(nlC16);
+  if (ring) {
+    switch (a.ne_pad >> 4) {
+      case 2: return hxb_go<2>(a, Ht16, nlC16, s);
+      case 4: return hxb_go<4>(a, Ht16, nlC16, s);
+      case 8: return hxb_go<8>(a, Ht16, nlC16, s);
+      case 14: return hxb_go<14>(a, Ht16, nlC16, s);
+      case 22: return hxb_go<22>(a, Ht16, nlC16, s);
+      default: break;
+    }
+  }
   switch (a.ne_pad >> 4) {
     case 1: return hx_bf16_go<1>(a, Ht16, nlC16, s);
     case 2: return hx_bf16_go<2>(a, Ht16, nlC16, s);

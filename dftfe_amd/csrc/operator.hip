@@ -584,7 +584,7 @@ chfsi_status hx_fused(chfsi_operator *op, const T *src, int64_t ld_src, T *dst, 
   a.gamma = gamma;
   a.beta_mode = beta_mode;
   a.B = B;
-  a.variant = sizeof(T) == 8 ? op->hx_variant64 : op->hx_variant32;
+  a.variant = sizeof(T) == 8 ? op->hx_variant64 : (bf16 ? op->hx_variant16 : op->hx_variant32);
   a.R = R;
   a.ldR = ldR;
   a.rcoef = rcoef;
@@ -717,6 +717,11 @@ extern "C" chfsi_status chfsi_constraints_set_zero(chfsi_operator *op, double *X
 extern "C" chfsi_status chfsi_operator_set_hx_variant(chfsi_operator *op, int32_t precision,
                                                      int32_t variant) {
   CHFSI_ARG(op, "null operator");
+  if (precision == CHFSI_BF16) {
+    CHFSI_ARG(variant == 0 || variant == 1, "bad BF16 variant (0: K = 32 ring, 1: K = 16)");
+    op->hx_variant16 = variant;
+    return CHFSI_OK;
+  }
   CHFSI_ARG(precision == CHFSI_FP64 || precision == CHFSI_FP32, "bad precision");
   CHFSI_ARG(variant >= 0 && variant < hx_variants(precision == CHFSI_FP64), "bad variant");
   if (precision == CHFSI_FP64)

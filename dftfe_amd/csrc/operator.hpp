@@ -82,6 +82,7 @@ struct chfsi_operator {
   chfsi::Scratch partial64, partial32, W64, W32, tmp;
   // HX tile and projection variants per precision (chosen by chfsi_operator_autotune)
   int hx_variant64 = 0, hx_variant32 = 0;
+  int hx_variant16 = 0;  // BF16 apply: 0 K = 32 ring, 1 register-staged K = 16
   int proj_variant64 = 0, proj_variant32 = 0;
   int max_srb = 1;  // largest slot count of any cell / 16
   // constraints (slave-major and master-major CSR)

@@ -211,6 +211,30 @@ def test_hx_bf16(small):
     assert 1e-5 < err < 3e-2
 
 
+@pytest.mark.parametrize("B", [64, 72, 8])
+def test_hx_bf16_ring_matches_register_kernel(small, B):
+    """K = 32 BF16 ring (variant 0) against the register-staged K = 16 form
+    (variant 1): FP32 accumulation rounding apart.  Cells here hold 16
+    projector slots (one 16-wide slice), so every stage count is odd and the
+    last stage's second half is the zero half."""
+    from dftfe_amd import lib as L
+
+    prob, ref, op = small
+    X = prob.initial_guess(N=B, seed_offset=B + 11)
+    Xd = torch.tensor(X, device="cuda", dtype=torch.float32)
+    out = {}
+    for v in (0, 1):
+        Yd = torch.zeros_like(Xd)
+        op.set_hx_variant(L.BF16, v)
+        try:
+            op.HX(Xd, False, 1.0, Yd, bf16=True)
+        finally:
+            op.set_hx_variant(L.BF16, 0)
+        out[v] = Yd.double().cpu().numpy()
+    assert np.isfinite(out[0]).all()
+    assert _rel(out[0], out[1]) < 1e-5
+
+
 def test_filter_mixed_with_fp64_tail(small):
     """FP32 / BF16 filters, optionally finished with FP64 degrees: eigenvalue
     differences to the FP64 oracle after one pass stay at the precision's level
@@ -345,7 +369,7 @@ def test_pass_mixed_fp32(small):
     lam, res, _ = solver.solve(op, Xd, use_mixed_prec_overall=True)
     err = np.abs(lam - lam_o).max()
     print(f"mixed-precision single pass max|dlambda| = {err:.3e} Ha")
-    assert err < 1e-5
+    assert err < 1e-7
 
 
 def test_pass_blocks_and_mixed_dense(small):

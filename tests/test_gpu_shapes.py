@@ -130,6 +130,32 @@ def test_hx_fp32_and_bf16(shp):
     assert _rel(Y1.double().cpu().numpy(), ref.apply(X[:, :1].astype(np.float32).astype(np.float64))) < 2e-6
 
 
+@pytest.mark.parametrize("B", [64, 200])
+def test_hx_bf16_ring_matches_register_kernel(shp, B):
+    """The K = 32 BF16 ring (variant 0) and the register-staged K = 16 kernel
+    (variant 1) round the same operands to BF16 and accumulate in FP32 in
+    different orders: equal to FP32 accumulation rounding, and both at the
+    BF16 level of the oracle."""
+    from dftfe_amd import lib as L
+
+    prob, ref, op = shp
+    X = prob.initial_guess(N=B, seed_offset=B + 9)
+    Xd = torch.tensor(X, device="cuda", dtype=torch.float32)
+    out = {}
+    for v in (0, 1):
+        Yd = torch.zeros_like(Xd)
+        op.set_hx_variant(L.BF16, v)
+        try:
+            op.HX(Xd, False, 1.0, Yd, bf16=True)
+        finally:
+            op.set_hx_variant(L.BF16, 0)
+        out[v] = Yd.double().cpu().numpy()
+    assert np.isfinite(out[0]).all()
+    assert _rel(out[0], out[1]) < 1e-5
+    want = ref.apply(X.astype(np.float32).astype(np.float64))
+    assert 1e-5 < _rel(out[0], want) < 3e-2
+
+
 def test_hx_strided_flags_and_cheby(shp):
     prob, ref, op = shp
     X = prob.initial_guess(N=96, seed_offset=5)
