@@ -15,10 +15,14 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libchfsi.so")
-SOURCES = ["kernels.hip", "kernels_bloch.hip", "operator.hip", "solver.hip", "bloch.hip"]
+SOURCES = ["kernels.hip", "kernels_bloch.hip", "operator.hip", "solver.hip", "bloch.hip",
+           "dense.hip"]
 ROOT = os.path.dirname(HERE)
 DROPIN_SRC = os.path.join(ROOT, "tests", "cpp", "dropin_solve.cpp")
 DROPIN = os.path.join(HERE, "bin", "dropin_solve")
+# the same classes built with USE_COMPLEX (complex wavefunctions, k-points)
+DROPIN_C_SRC = os.path.join(ROOT, "tests", "cpp", "dropin_solve_complex.cpp")
+DROPIN_C = os.path.join(HERE, "bin", "dropin_solve_complex")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
@@ -57,19 +61,24 @@ def build(verbose: bool = False) -> str:
 
 
 def build_dropin(verbose: bool = False) -> str:
-    """The C++ drop-in driver, linked against the in-tree libchfsi.so."""
+    """The C++ drop-in drivers (real, and with USE_COMPLEX), linked against the
+    in-tree libchfsi.so."""
     os.makedirs(os.path.dirname(DROPIN), exist_ok=True)
-    cmd = [HIPCC, "-O2", "-std=c++17", "--offload-arch=gfx950",
-           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-I", os.path.join(ROOT, "include"), DROPIN_SRC, "-o", DROPIN + ".tmp",
-           "-L", HERE, "-lchfsi", "-Wl,-rpath,$ORIGIN/..",
-           "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib"]
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError(f"drop-in driver build failed:\n{res.stderr}")
-    os.replace(DROPIN + ".tmp", DROPIN)
-    if verbose:
-        print("built", DROPIN)
+    for src, out, defs in ((DROPIN_SRC, DROPIN, []),
+                           (DROPIN_C_SRC, DROPIN_C, ["-DUSE_COMPLEX"])):
+        if not os.path.exists(src):
+            continue
+        cmd = [HIPCC, "-O2", "-std=c++17", "--offload-arch=gfx950",
+               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", *defs,
+               "-I", os.path.join(ROOT, "include"), src, "-o", out + ".tmp",
+               "-L", HERE, "-lchfsi", "-Wl,-rpath,$ORIGIN/..",
+               "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib"]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"drop-in driver build failed ({src}):\n{res.stderr}")
+        os.replace(out + ".tmp", out)
+        if verbose:
+            print("built", out)
     return DROPIN
 
 

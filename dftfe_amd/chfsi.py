@@ -481,7 +481,8 @@ class ChebyshevSolverDevice:
                  filter_columns: int = 0, mixed_prec_switch_residual: float = 0.0,
                  mixed_prec_defect_filter: bool = False, use_shep: bool = True,
                  lanczos_full_residual_margin: bool = False,
-                 is_pseudopotential: bool = False, mixed_prec_fp64_top_columns: int = 0):
+                 is_pseudopotential: bool = False, mixed_prec_fp64_top_columns: int = 0,
+                 dense_rocblas: bool = False):
         """mixed_dense: FP32 off-diagonal blocks in the overlap, CholGS rotation,
         X^T H X and Rayleigh-Ritz rotation when a pass runs with
         use_mixed_prec_overall (the reference's USE MIXED PREC CGS O / CGS SR /
@@ -499,7 +500,9 @@ class ChebyshevSolverDevice:
         is_pseudopotential: gates the first-SCF degree factor and the halving
         of a looked-up degree for all-electron runs.
         mixed_prec_fp64_top_columns: with a lower filter precision, the last
-        k columns (highest Ritz values) are filtered in FP64."""
+        k columns (highest Ritz values) are filtered in FP64.
+        dense_rocblas: FP64 overlap, X^T H X and rotations through rocBLAS
+        instead of the library's MFMA kernels."""
         self.lib = L.load()
         p = L.Params()
         self.lib.chfsi_default_params(ct.byref(p))
@@ -518,6 +521,7 @@ class ChebyshevSolverDevice:
         p.lanczos_full_residual_margin = int(lanczos_full_residual_margin)
         p.is_pseudopotential = int(is_pseudopotential)
         p.mixed_prec_fp64_top_columns = int(mixed_prec_fp64_top_columns)
+        p.dense_rocblas = int(dense_rocblas)
         for f in ("mixed_prec_overlap", "mixed_prec_cgs_rotation", "mixed_prec_xthx",
                   "mixed_prec_rr_rotation", "mixed_prec_xthx_spectrum_split"):
             setattr(p, f, int(mixed_dense))

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "dense.hpp"
 #include "kernels.hpp"
 #include "operator.hpp"
 #include "solver.hpp"
@@ -431,8 +432,39 @@ int64_t panel_rows(int64_t M, int N) {
 // (subspaceRotationScalapack, linearAlgebraOperationsDevice.cc:1832-2242).
 // Mixed (subspaceRotationRRMixedPrecScalapack, :2660-3077): the bs x bs diagonal
 // blocks of Q in FP64, the rest from FP32 copies of Q and X.
+// C = A^T B over M rows with the library's FP64 MFMA kernel (lower-triangle
+// tiles i >= j + diag_off, or all with diag_off = -1)
+chfsi_status gram_fp64(chfsi_solver *s, const double *A, int64_t lda, const double *B,
+                       int64_t ldb, int m, int n, int64_t M, int diag_off, double *C,
+                       int64_t ldc, hipStream_t st) {
+  const size_t pb = gram_partial_bytes(M, m, n, diag_off);
+  if (pb == 0) return CHFSI_OK;
+  CHFSI_TRY(s->split.ensure(pb));
+  CHFSI_TRY(s->gtiles.ensure(sizeof(int2) * kGramMaxTiles));
+  CHFSI_HIP(launch_gram(A, lda, B, ldb, m, n, M, diag_off, C, ldc, sptr<double>(s->split),
+                        static_cast<int2 *>(s->gtiles.ptr), st));
+  return CHFSI_OK;
+}
+
+// X <- X R in row panels with the library's FP64 MFMA kernel; R(i, j) =
+// Rm[i + j ldr] (trans 0) or Rm[j + i ldr] (trans 1), upper triangular when upper
+chfsi_status rotate_fp64(chfsi_solver *s, double *X, int64_t M, int N, const double *Rm,
+                         int64_t ldr, int trans, int upper, hipStream_t st) {
+  const int64_t chunk = panel_rows(M, N);
+  CHFSI_TRY(s->rot.ensure(sizeof(double) * chunk * N));
+  double *tmp = sptr<double>(s->rot);
+  for (int64_t n0 = 0; n0 < M; n0 += chunk) {
+    const int64_t nc = std::min(chunk, M - n0);
+    CHFSI_HIP(launch_rotate(X + n0 * N, N, Rm, ldr, trans, upper, N, nc, tmp, N, st));
+    CHFSI_HIP(hipMemcpyAsync(X + n0 * N, tmp, sizeof(double) * nc * N, hipMemcpyDeviceToDevice,
+                             st));
+  }
+  return CHFSI_OK;
+}
+
 chfsi_status rotate_dense(chfsi_solver *s, double *X, const float *Xf, int64_t M, int N,
                           const double *Q, int bs, hipStream_t st) {
+  if (!Xf && !s->params.dense_rocblas) return rotate_fp64(s, X, M, N, Q, N, 0, 0, st);
   const int64_t chunk = panel_rows(M, N);
   CHFSI_TRY(s->rot.ensure(sizeof(double) * chunk * N));
   double *tmp = sptr<double>(s->rot);
@@ -474,6 +506,8 @@ chfsi_status rotate_dense(chfsi_solver *s, double *X, const float *Xf, int64_t M
 // FP64, the blocks left of them from FP32 copies.
 chfsi_status rotate_lower(chfsi_solver *s, double *X, const float *Xf, int64_t M, int N,
                           const double *Linv, int bs, hipStream_t st) {
+  // X L^{-T}: R(i, j) = Linv[j + i N], upper triangular
+  if (!Xf && !s->params.dense_rocblas) return rotate_fp64(s, X, M, N, Linv, N, 1, 1, st);
   const int64_t chunk = panel_rows(M, N);
   CHFSI_TRY(s->rot.ensure(sizeof(double) * chunk * N));
   double *tmp = sptr<double>(s->rot);
@@ -518,6 +552,7 @@ chfsi_status rotate_lower(chfsi_solver *s, double *X, const float *Xf, int64_t M
 // in FP64, the blocks below them from the FP32 copy.
 chfsi_status overlap_lower(chfsi_solver *s, const double *X, const float *Xf, int64_t M, int N,
                            int bs, double *S, hipStream_t st) {
+  if (!Xf && !s->params.dense_rocblas) return gram_fp64(s, X, N, X, N, N, N, M, 0, S, N, st);
   for (int j = 0; j < N; j += bs) {
     const int bw = std::min(bs, N - j);
     double *Sj = S + j + (int64_t)j * N;
@@ -575,6 +610,17 @@ chfsi_status xthx_lower(chfsi_solver *s, chfsi_operator *op, const double *X, in
     HXf = sptr<float>(s->HXf);
   }
   int g0 = 0, gw = 0;  // columns [g0, g0 + gw) of H~X are in HXg (ld gw)
+  if (!Xf && !s->params.dense_rocblas) {
+    // per group of columns: Hp[:, g0:g0+gw] = X^T (H~X)[:, g0:g0+gw], the
+    // tiles that reach the lower triangle
+    for (g0 = 0; g0 < N; g0 += G) {
+      gw = std::min(G, N - g0);
+      CHFSI_TRY(hx_fused<double>(op, X + g0, N, HXg, gw, gw, 1.0, 0.0, 0.0, 1, true, st));
+      if (op->nc) CHFSI_HIP(launch_zero_rows(op->d_c_rows, op->nc, HXg, gw, gw, st));
+      CHFSI_TRY(gram_fp64(s, X, N, HXg, gw, N, gw, M, g0, Hp + (int64_t)g0 * N, N, st));
+    }
+    return CHFSI_OK;
+  }
   for (int j = 0; j < N; j += B) {
     const int bw = std::min(B, N - j);
     if (j >= g0 + gw) {
@@ -799,6 +845,7 @@ extern "C" void chfsi_default_params(chfsi_params *p) {
   p->is_pseudopotential = 0;            // utils/dftParameters.cc:1173
   p->lanczos_full_residual_margin = 0;
   p->mixed_prec_fp64_top_columns = 0;
+  p->dense_rocblas = 0;
 }
 
 extern "C" chfsi_status chfsi_chebyshev_degree(const chfsi_params *params, double bUp,

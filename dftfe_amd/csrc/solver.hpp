@@ -30,6 +30,8 @@ struct chfsi_solver {
   chfsi::Scratch R32, dcoef;
   // generalised Rayleigh-Ritz: L^{-1} and a product buffer (2 N x N)
   chfsi::Scratch gep;
+  // tile table of the FP64 gram kernel
+  chfsi::Scratch gtiles;
   bool have_v0 = false;
   // largest residual norm over the lowest 90% of states of the last pass that
   // computed residuals, and the filter precision the last pass used
@@ -38,7 +40,8 @@ struct chfsi_solver {
   ~chfsi_solver() {
     if (blas) rocblas_destroy_handle(blas);
     for (chfsi::Scratch *s : {&Y64, &X32, &Y32, &HXb, &Hp, &rot, &evals, &info, &part, &lam, &v0,
-                              &split, &Xf, &HXf, &Qf, &split32, &rot32, &R32, &dcoef, &gep})
+                              &split, &Xf, &HXf, &Qf, &split32, &rot32, &R32, &dcoef, &gep,
+                              &gtiles})
       s->release();
   }
 };

@@ -73,7 +73,7 @@ class Params(ct.Structure):
         ("mixed_prec_switch_residual", ct.c_double),
         ("mixed_prec_defect_filter", ct.c_int32),
         ("is_pseudopotential", ct.c_int32), ("lanczos_full_residual_margin", ct.c_int32),
-        ("mixed_prec_fp64_top_columns", ct.c_int32),
+        ("mixed_prec_fp64_top_columns", ct.c_int32), ("dense_rocblas", ct.c_int32),
     ]
 
 

@@ -213,6 +213,11 @@ typedef struct {
                                       forms their rounding, relative to the
                                       column, is the largest.  No reference
                                       counterpart; 0: none                         */
+  int32_t dense_rocblas;           /* 1: the FP64 overlap, X^T H~ X and rotations
+                                      through rocBLAS dgemm; 0: the library's own
+                                      FP64 MFMA kernels (split-K lower-triangle
+                                      gram, row-panel rotation).  The mixed-
+                                      precision dense options use rocBLAS either way */
 } chfsi_params;
 
 const char *chfsi_last_error(void);
@@ -258,7 +263,9 @@ chfsi_status chfsi_operator_get_cell_matrices(chfsi_operator *op, double *H_dev,
  * then times the projection-kernel variants with it and keeps the fastest.
  * times_ms (40 entries) [0..15]: FP64 tile variants, [16..19]: FP64 projection
  * variants, [20..35]: FP32 tile variants, [36..39]: FP32 projection variants
- * (ms per HX; -1 where a variant does not exist). */
+ * (ms per HX; -1 where a variant does not exist).  precision CHFSI_BF16 selects
+ * the BF16 apply's kernel: 0 (default) K = 32 MFMA on the direct-to-LDS ring,
+ * 1 the register-staged K = 16 form. */
 chfsi_status chfsi_operator_set_hx_variant(chfsi_operator *op, int32_t precision, int32_t variant);
 chfsi_status chfsi_operator_set_proj_variant(chfsi_operator *op, int32_t precision,
                                              int32_t variant);
