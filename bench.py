@@ -37,7 +37,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
 # mixed passes: the columns of the top quarter of the states keep an FP64
 # filter (their filter gain is the smallest, so the FP32 rounding of the
 # defect form is largest relative to them); -1 = N / 4
@@ -125,6 +125,7 @@ def main():
 
     import torch
     from dftfe_amd import fe
+    from dftfe_amd import lib as L
     from dftfe_amd.chfsi import ChebyshevSolverDevice, KohnShamOperatorDevice
 
     dev = torch.device("cuda:0")
@@ -152,23 +153,34 @@ def main():
     v0 = torch.tensor(prob.lanczos_start, device=dev)
 
     # ---- fused HX timing with HIP events on the stream the library launches on
-    def time_hx(dtype, width):
+    def time_hx(dtype, width, bf16=False):
         Xb = torch.randn(prob.M, width, device=dev, dtype=dtype)
         Yb = torch.zeros_like(Xb)
         for _ in range(3):
-            op.HX(Xb, False, 1.0, Yb)
+            op.HX(Xb, False, 1.0, Yb, bf16=bf16)
         st = torch.cuda.current_stream()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(st)
         for _ in range(args.hx_reps):
-            op.HX(Xb, True, 1.0, Yb)
+            op.HX(Xb, True, 1.0, Yb, bf16=bf16)
         e1.record(st)
         e1.synchronize()
         return e0.elapsed_time(e1) / args.hx_reps
 
+    def time_hx_bf16():
+        """BF16 apply on W columns: the LDS-ring kernel (variant 0) and the
+        register-staged one (variant 1); the faster one stays selected."""
+        t = []
+        for v in (0, 1):
+            op.set_hx_variant(L.BF16, v)
+            t.append(time_hx(torch.float32, W, bf16=True))
+        op.set_hx_variant(L.BF16, int(np.argmin(t)))
+        return t
+
     if args.hx_only:
         hx32_ms = time_hx(torch.float32, W)
         hx64_ms = time_hx(torch.float64, W)
+        hxb_ms = time_hx_bf16()
         src = torch.empty(1 << 28, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
         dst.copy_(src)
@@ -176,6 +188,7 @@ def main():
         F = hx_flops(prob, W)
         print(json.dumps({"width": W, "hx_fp32_ms": round(hx32_ms, 4),
                           "hx_fp64_ms": round(hx64_ms, 4),
+                          "hx_bf16_ms_ring_register": [round(x, 4) for x in hxb_ms],
                           "hx_fp32_tflops": round(F / hx32_ms / 1e9, 2),
                           "hx_fp64_tflops": round(F / hx64_ms / 1e9, 2),
                           "calibration_copy_bytes": 4 << 28, "autotune": tune}), flush=True)
@@ -293,6 +306,10 @@ def main():
     hx64_gf = F / hx64_ms / 1e6
     log(f"HX {W} columns: fp32 {hx32_ms:.3f} ms = {hx32_gf:.0f} GFLOP/s; "
         f"fp64 {hx64_ms:.3f} ms = {hx64_gf:.0f} GFLOP/s")
+    hxb_ms = time_hx_bf16()
+    hxb_best = min(hxb_ms)
+    log(f"HX {W} columns bf16: ring {hxb_ms[0]:.3f} ms, register {hxb_ms[1]:.3f} ms "
+        f"({hx32_ms / hxb_best:.2f}x fp32)")
     hx32_blk = time_hx(torch.float32, B)
     hx64_blk = time_hx(torch.float64, B)
     log(f"HX {B} columns: fp32 {hx32_blk:.3f} ms, fp64 {hx64_blk:.3f} ms")
@@ -370,6 +387,10 @@ def main():
         "algorithmic_bytes": hx_bytes(prob, W, 4),
         "fp64": {"achieved": round(F / hx64_ms / 1e9, 2), "peak": PEAK_TFLOPS["fp64"],
                  "frac": round(F / hx64_ms / 1e9 / PEAK_TFLOPS["fp64"], 4)},
+        "bf16": {"achieved": round(F / hxb_best / 1e9, 2), "peak": PEAK_TFLOPS["bf16"],
+                 "frac": round(F / hxb_best / 1e9 / PEAK_TFLOPS["bf16"], 4),
+                 "ms_ring_register": [round(x, 4) for x in hxb_ms],
+                 "speedup_vs_fp32": round(hx32_ms / hxb_best, 3)},
     }
 
     if errors:

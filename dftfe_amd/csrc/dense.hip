@@ -27,7 +27,6 @@ constexpr int TT = 128;   // output tile edge
 constexpr int KS = 16;    // K rows per LDS slab
 constexpr int NT = 256;   // 4 waves, each a 64 x 64 quarter of the tile
 constexpr int PADW = 16;  // LDS row padding of [k][128] slabs (doubles)
-constexpr int RSX = 18;   // LDS row stride of [128][16] slabs (doubles)
 
 // Work items are spread over the 8 XCDs so that consecutive items (the tiles
 // of one K chunk in gram, the column tiles of one row panel in rotate) share
@@ -170,15 +169,22 @@ __global__ void gram_reduce_kernel(const double *P, const int2 *tiles, int ntile
 }
 
 // Y[k, J + j] = sum_i X[k, i] R[i, J + j] for the rows [r0, r0 + rows) of the
-// panel; R[i][j] = Rm[i + j ldr] (trans = 0) or Rm[j + i ldr] (trans = 1).
+// panel; R[i][j] = Rm[i + j ldr] (TR = 0) or Rm[j + i ldr] (TR = 1).
 // upper: R is upper triangular, slabs below the tile's last column are skipped.
+// LDS slabs: X as [128 rows][KS + 1] (odd stride: the 16 rows of an A-operand
+// read fall in distinct banks); R as [128 columns][KS + 1] when its columns
+// are contiguous in memory (TR = 0), else [KS rows][128 + PADW].
+template <int TR>
 __global__ __launch_bounds__(NT, 2) void rotate_kernel(const double *X, int64_t ldx,
-                                                       const double *Rm, int64_t ldr, int trans,
-                                                       int upper, int N, int64_t rows,
-                                                       double *Y, int64_t ldy, int ntj,
-                                                       int per_xcd, int nitems) {
-  __shared__ __align__(16) double Xs[2][TT * RSX];
-  __shared__ __align__(16) double Rs[2][KS][TT + PADW];
+                                                       const double *Rm, int64_t ldr, int upper,
+                                                       int N, int64_t rows, double *Y,
+                                                       int64_t ldy, int ntj, int per_xcd,
+                                                       int nitems) {
+  constexpr int XS = KS + 1;
+  constexpr int RSZ = TR ? KS * (TT + PADW) : TT * XS;
+  constexpr int HK = KS / 2;  // doubles per thread and row of a slab
+  __shared__ __align__(16) double Xs[2][TT * XS];
+  __shared__ __align__(16) double Rs[2][RSZ];
   const int item = xcd_item(per_xcd);
   if ((blockIdx.x >> 3) >= per_xcd || item >= nitems) return;
   const int64_t R0 = (int64_t)(item / ntj) * TT;
@@ -188,66 +194,87 @@ __global__ __launch_bounds__(NT, 2) void rotate_kernel(const double *X, int64_t 
   const int iend = upper ? (J + TT < N ? J + TT : N) : N;
   const int nslab = (iend + KS - 1) / KS;
 
-  // X slab: 128 rows x 16 columns, thread t: row t/2, 8 columns from 8*(t%2)
-  const int xr = tid >> 1, xc = (tid & 1) * 8;
-  // R slab: 16 rows (i) x 128 columns (j)
-  double rx[8], rr[8];
-  const bool xvec = (ldx % 2 == 0) && (((uintptr_t)X & 15) == 0) && (N % 16 == 0);
+  // X slab: thread t takes row t/2, HK consecutive columns from HK*(t%2)
+  const int xr = tid >> 1, xc = (tid & 1) * HK;
+  double rx[HK], rr[HK];
+  // 16-byte loads when rows are aligned and every slab lies inside the matrix
+  const bool vec = (ldx % 2 == 0) && (ldr % 2 == 0) && (((uintptr_t)X & 15) == 0) &&
+                   (((uintptr_t)Rm & 15) == 0) && (N % KS == 0);
   auto load = [&](int i0) {
     const int64_t k = R0 + xr;
     const bool kok = k < rows;
     const double *xp = X + (kok ? k : 0) * ldx;
-    if (xvec) {
-      // N % 16 == 0: the slab [i0, i0 + 16) lies inside every row
+    if (vec) {
       const double z = kok ? 1.0 : 0.0;
       const double2 *p = reinterpret_cast<const double2 *>(xp + i0 + xc);
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
+      for (int e = 0; e < HK / 2; ++e) {
         const double2 v = p[e];
         rx[2 * e] = z * v.x;
         rx[2 * e + 1] = z * v.y;
       }
     } else {
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
+      for (int e = 0; e < HK; ++e) {
         const int i = i0 + xc + e;
         const double v = xp[i < N ? i : 0];
         rx[e] = (kok && i < N) ? v : 0.0;
       }
     }
-    if (trans) {
+    if (TR) {
       // (i, j) at Rm[j + i ldr]: thread t takes row i0 + t/16, 8 consecutive j
       const int ii = i0 + (tid >> 4), jj = J + (tid & 15) * 8;
+      if (vec && jj + 8 <= N) {
+        const double z = ii < N ? 1.0 : 0.0;
+        const double2 *p = reinterpret_cast<const double2 *>(Rm + (int64_t)(ii < N ? ii : 0) * ldr + jj);
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int j = jj + e;
-        const double v = Rm[(int64_t)(ii < N ? ii : 0) * ldr + (j < N ? j : 0)];
-        rr[e] = (ii < N && j < N) ? v : 0.0;
+        for (int e = 0; e < 4; ++e) {
+          const double2 v = p[e];
+          rr[2 * e] = z * v.x;
+          rr[2 * e + 1] = z * v.y;
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int j = jj + e;
+          const double v = Rm[(int64_t)(ii < N ? ii : 0) * ldr + (j < N ? j : 0)];
+          rr[e] = (ii < N && j < N) ? v : 0.0;
+        }
       }
     } else {
-      // (i, j) at Rm[i + j ldr]: thread t takes column J + t/2, 8 consecutive i
-      const int j = J + (tid >> 1), ib = i0 + (tid & 1) * 8;
+      // (i, j) at Rm[i + j ldr]: thread t takes column J + t/2, HK consecutive i
+      const int j = J + (tid >> 1), ib = i0 + (tid & 1) * HK;
+      const double *rp = Rm + (int64_t)(j < N ? j : 0) * ldr;
+      if (vec && j < N) {
+        const double2 *p = reinterpret_cast<const double2 *>(rp + ib);
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int i = ib + e;
-        const double v = Rm[(int64_t)(j < N ? j : 0) * ldr + (i < N ? i : 0)];
-        rr[e] = (i < N && j < N) ? v : 0.0;
+        for (int e = 0; e < HK / 2; ++e) {
+          const double2 v = p[e];
+          rr[2 * e] = v.x;
+          rr[2 * e + 1] = v.y;
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < HK; ++e) {
+          const int i = ib + e;
+          const double v = rp[i < N ? i : 0];
+          rr[e] = (i < N && j < N) ? v : 0.0;
+        }
       }
     }
   };
   auto store = [&](int buf) {
 #pragma unroll
-    for (int e = 0; e < 8; e += 2)
-      *reinterpret_cast<double2 *>(&Xs[buf][xr * RSX + xc + e]) = double2{rx[e], rx[e + 1]};
-    if (trans) {
+    for (int e = 0; e < HK; ++e) Xs[buf][xr * XS + xc + e] = rx[e];
+    if (TR) {
       const int ii = tid >> 4, jj = (tid & 15) * 8;
 #pragma unroll
       for (int e = 0; e < 8; e += 2)
-        *reinterpret_cast<double2 *>(&Rs[buf][ii][jj + e]) = double2{rr[e], rr[e + 1]};
+        *reinterpret_cast<double2 *>(&Rs[buf][ii * (TT + PADW) + jj + e]) = double2{rr[e], rr[e + 1]};
     } else {
-      const int jl = tid >> 1, ib = (tid & 1) * 8;
+      const int jl = tid >> 1, ib = (tid & 1) * HK;
 #pragma unroll
-      for (int e = 0; e < 8; ++e) Rs[buf][ib + e][jl] = rr[e];
+      for (int e = 0; e < HK; ++e) Rs[buf][jl * XS + ib + e] = rr[e];
     }
   };
 
@@ -268,8 +295,9 @@ __global__ __launch_bounds__(NT, 2) void rotate_kernel(const double *X, int64_t 
       double av[4], bv[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        av[q] = Xs[buf][(wi * 64 + q * 16 + (lane & 15)) * RSX + k];
-        bv[q] = Rs[buf][k][wj * 64 + q * 16 + (lane & 15)];
+        const int c = q * 16 + (lane & 15);
+        av[q] = Xs[buf][(wi * 64 + c) * XS + k];
+        bv[q] = TR ? Rs[buf][k * (TT + PADW) + wj * 64 + c] : Rs[buf][(wj * 64 + c) * XS + k];
       }
 #pragma unroll
       for (int a = 0; a < 4; ++a)
@@ -298,9 +326,10 @@ __global__ __launch_bounds__(NT, 2) void rotate_kernel(const double *X, int64_t 
 // ---------------------------------------------------------------------------
 
 static int64_t gram_chunk_rows(int64_t K, int ntiles) {
-  // enough work items to fill the chip twice (512 workgroups), chunks of at
-  // least 4096 rows, at most 256 chunks
-  int64_t nch = (512 + ntiles - 1) / ntiles;
+  // at most one wave of work items over the chip's 512 workgroup slots (256
+  // CUs, two workgroups each): a second, partial wave would double the time.
+  // Chunks of at least 4096 rows, at most 256 chunks.
+  int64_t nch = 512 / ntiles;
   nch = nch < 1 ? 1 : (nch > 256 ? 256 : nch);
   int64_t kc = (K + nch - 1) / nch;
   if (kc < 4096) kc = 4096;
@@ -358,8 +387,12 @@ hipError_t launch_rotate(const double *X, int64_t ldx, const double *Rm, int64_t
   if (nitems64 > (int64_t(1) << 30)) return hipErrorInvalidValue;
   const int nitems = (int)nitems64;
   const int per_xcd = (nitems + 7) / 8;
-  hipLaunchKernelGGL(rotate_kernel, dim3(8 * per_xcd), dim3(NT), 0, s, X, ldx, Rm, ldr, trans,
-                     upper, N, rows, Y, ldy, ntj, per_xcd, nitems);
+  if (trans)
+    hipLaunchKernelGGL(rotate_kernel<1>, dim3(8 * per_xcd), dim3(NT), 0, s, X, ldx, Rm, ldr, upper,
+                       N, rows, Y, ldy, ntj, per_xcd, nitems);
+  else
+    hipLaunchKernelGGL(rotate_kernel<0>, dim3(8 * per_xcd), dim3(NT), 0, s, X, ldx, Rm, ldr, upper,
+                       N, rows, Y, ldy, ntj, per_xcd, nitems);
   return hipGetLastError();
 }
 

@@ -423,7 +423,8 @@ chfsi_status cast_x(chfsi_solver *s, const double *X, int64_t M, int N, float **
   return CHFSI_OK;
 }
 
-// Rotation panels: rows of DOFs per panel so that one panel is about 128 MB.
+// Rotation panels: rows of DOFs per panel so that one panel holds 2^27
+// entries (1 GiB in FP64).
 int64_t panel_rows(int64_t M, int N) {
   return std::max<int64_t>(1, std::min<int64_t>(M, (int64_t(1) << 27) / N));
 }
@@ -448,17 +449,36 @@ chfsi_status gram_fp64(chfsi_solver *s, const double *A, int64_t lda, const doub
 
 // X <- X R in row panels with the library's FP64 MFMA kernel; R(i, j) =
 // Rm[i + j ldr] (trans 0) or Rm[j + i ldr] (trans 1), upper triangular when upper
+// Panel p is rotated into buffer p % 2 on st; its copy back into X runs on the
+// solver's copy stream while panel p + 1 is rotated (the rotation reads only
+// its own panel of X).  Before panel p + 2 reuses the buffer, st waits for
+// panel p's copy; at the end st waits for every copy.
 chfsi_status rotate_fp64(chfsi_solver *s, double *X, int64_t M, int N, const double *Rm,
                          int64_t ldr, int trans, int upper, hipStream_t st) {
   const int64_t chunk = panel_rows(M, N);
-  CHFSI_TRY(s->rot.ensure(sizeof(double) * chunk * N));
-  double *tmp = sptr<double>(s->rot);
-  for (int64_t n0 = 0; n0 < M; n0 += chunk) {
-    const int64_t nc = std::min(chunk, M - n0);
-    CHFSI_HIP(launch_rotate(X + n0 * N, N, Rm, ldr, trans, upper, N, nc, tmp, N, st));
-    CHFSI_HIP(hipMemcpyAsync(X + n0 * N, tmp, sizeof(double) * nc * N, hipMemcpyDeviceToDevice,
-                             st));
+  const int64_t npanel = (M + chunk - 1) / chunk;
+  const int nbuf = npanel > 1 ? 2 : 1;
+  CHFSI_TRY(s->rot.ensure(sizeof(double) * chunk * N * nbuf));
+  if (!s->copy_stream) CHFSI_HIP(hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking));
+  for (int b = 0; b < 2; ++b) {
+    if (!s->rot_done[b])
+      CHFSI_HIP(hipEventCreateWithFlags(&s->rot_done[b], hipEventDisableTiming));
+    if (!s->copy_done[b])
+      CHFSI_HIP(hipEventCreateWithFlags(&s->copy_done[b], hipEventDisableTiming));
   }
+  for (int64_t p = 0; p < npanel; ++p) {
+    const int b = (int)(p & 1);
+    const int64_t n0 = p * chunk, nc = std::min(chunk, M - n0);
+    double *tmp = sptr<double>(s->rot) + (int64_t)b * chunk * N;
+    if (p >= 2) CHFSI_HIP(hipStreamWaitEvent(st, s->copy_done[b], 0));
+    CHFSI_HIP(launch_rotate(X + n0 * N, N, Rm, ldr, trans, upper, N, nc, tmp, N, st));
+    CHFSI_HIP(hipEventRecord(s->rot_done[b], st));
+    CHFSI_HIP(hipStreamWaitEvent(s->copy_stream, s->rot_done[b], 0));
+    CHFSI_HIP(hipMemcpyAsync(X + n0 * N, tmp, sizeof(double) * nc * N, hipMemcpyDeviceToDevice,
+                             s->copy_stream));
+    CHFSI_HIP(hipEventRecord(s->copy_done[b], s->copy_stream));
+  }
+  for (int b = 0; b < nbuf; ++b) CHFSI_HIP(hipStreamWaitEvent(st, s->copy_done[b], 0));
   return CHFSI_OK;
 }
 

@@ -32,6 +32,10 @@ struct chfsi_solver {
   chfsi::Scratch gep;
   // tile table of the FP64 gram kernel
   chfsi::Scratch gtiles;
+  // rotation panels: copies of finished panels back into X run on a second
+  // stream, overlapped with the next panel's rotation (two panel buffers)
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t rot_done[2] = {nullptr, nullptr}, copy_done[2] = {nullptr, nullptr};
   bool have_v0 = false;
   // largest residual norm over the lowest 90% of states of the last pass that
   // computed residuals, and the filter precision the last pass used
@@ -39,6 +43,9 @@ struct chfsi_solver {
   int last_precision = CHFSI_FP64;
   ~chfsi_solver() {
     if (blas) rocblas_destroy_handle(blas);
+    for (hipEvent_t e : {rot_done[0], rot_done[1], copy_done[0], copy_done[1]})
+      if (e) (void)hipEventDestroy(e);
+    if (copy_stream) (void)hipStreamDestroy(copy_stream);
     for (chfsi::Scratch *s : {&Y64, &X32, &Y32, &HXb, &Hp, &rot, &evals, &info, &part, &lam, &v0,
                               &split, &Xf, &HXf, &Qf, &split32, &rot32, &R32, &dcoef, &gep,
                               &gtiles})

@@ -3,7 +3,8 @@ overlap and X^T H~X, row-panel rotations) against the oracle and against the
 rocBLAS route (dense_rocblas), through the C ABI.
 
 Workload: 20 bohr, 8^3 cells, p=4 (32,768 DOFs).  Column counts that are not
-multiples of the 128-wide tiles (72, 200) and that span several tiles (300).
+multiples of the 128-wide tiles (72, 200), that span several tiles (300) and a
+multiple of the 16-wide K slabs (256: the 16-byte load path of the rotation).
 Bars: CholGS 1e-10 relative to the oracle, Ritz values 1e-10 Ha, a whole pass
 on either route within 1e-10 Ha of the oracle.
 """
@@ -32,7 +33,7 @@ def small():
     return prob, O.OracleOperator(prob), KohnShamOperatorDevice(prob)
 
 
-@pytest.mark.parametrize("N", [72, 200, 300])
+@pytest.mark.parametrize("N", [72, 200, 256, 300])
 def test_cholgs_and_rayleigh_ritz(small, N):
     from dftfe_amd.chfsi import cholgs, rayleigh_ritz
 
