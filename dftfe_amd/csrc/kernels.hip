@@ -36,10 +36,13 @@ namespace chfsi {
 // over a 16-wide K slice; result lane l holds column l%16, rows acc_row().
 // Eight-wave FP32 tiles of up to 64 columns fit two workgroups per CU in LDS;
 // with one column block per wave their registers are held to 128 so that both
-// are resident.
-template <typename T, int NW, int TN, int WC, int NRB>
-__global__ __launch_bounds__(64 * NW) __attribute__((
-    amdgpu_waves_per_eu(NW == 8 && TN <= 64 && TN == 16 * WC && sizeof(T) == 4 ? 4 : 1))) void hx_kernel(
+// are resident.  NB = 1 keeps one LDS buffer of each slice (two barriers per
+// slice instead of one): the FP64 tiles then fit two workgroups per CU, so one
+// workgroup's gather and scatter overlap the other's MFMAs.
+template <typename T, int NW, int TN, int WC, int NRB, int NB = 2>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(
+    NB == 1 ? NW / 2
+            : (NW == 8 && TN <= 64 && TN == 16 * WC && sizeof(T) == 4 ? 4 : 1)))) void hx_kernel(
     HxLaunch<T> a) {
   using v4 = typename V4<T>::t;
   constexpr int NT = 64 * NW;
@@ -55,9 +58,10 @@ __global__ __launch_bounds__(64 * NW) __attribute__((
   constexpr int NLMAX = 128;             // projector slots per cell
   constexpr int KMAX = ROWS_MAX + NLMAX;  // K rows: cell nodes, then projector slots
 
-  // double-buffered K slices of the cell matrix and of the gathered block
-  __shared__ __align__(16) T Hs[2][ROWS_MAX * RS];
-  __shared__ __align__(16) T Xs[2][TN * RS];
+  // K slices of the cell matrix and of the gathered block (NB buffers)
+  static_assert(NB == 1 || NB == 2, "one or two slice buffers");
+  __shared__ __align__(16) T Hs[NB][ROWS_MAX * RS];
+  __shared__ __align__(16) T Xs[NB][TN * RS];
   // per-cell tables: DOF and first-touch bit of each local node (epilogue);
   // element offset of the source row and input scale of every K row (a DOF
   // row of src for the cell's nodes, a row of W for its projector slots;
@@ -225,10 +229,11 @@ __global__ __launch_bounds__(64 * NW) __attribute__((
   store_tile(0, hregA, xregA, xsclA);
   __syncthreads();
   for (int t = 0; t < ntot; ++t) {
-    const int buf = t & 1;
+    const int buf = NB == 2 ? (t & 1) : 0;
     if (t + 1 < ntot) load_tile(t + 1, hregA, xregA, xsclA);
     compute(buf);
-    if (t + 1 < ntot) store_tile(buf ^ 1, hregA, xregA, xsclA);
+    if (NB == 1) __syncthreads();  // every wave is done with the one buffer
+    if (t + 1 < ntot) store_tile(NB == 2 ? buf ^ 1 : 0, hregA, xregA, xsclA);
     __syncthreads();
   }
 
@@ -468,6 +473,180 @@ __attribute__((amdgpu_waves_per_eu(NW == 8 && S == 2 ? 4 : 1))) void hxp_kernel(
     const int tn = t + S - 1;
     issue(tn < ntot ? tn : ntot - 1, tn % S);
     compute(t % S, t);
+  }
+  wait_vmcnt<0>();  // no copy may still target LDS when the workgroup ends
+
+  hx_scatter<T, NRB, WR, RBW, CBW, 8>(a, a.ne, acc, smap, sout, wr, wc, lane, v0);
+}
+
+// ---------------------------------------------------------------------------
+// Fused HX (FP64) fed by the direct-to-LDS ring
+// ---------------------------------------------------------------------------
+// The FP64 form of hxp_kernel.  A stage holds half of a 16-wide K slice (K
+// rows 8h .. 8h+7 of slice t): per cell-matrix row 8 doubles = 64 bytes, the
+// byte geometry of an FP32 slice row, so the copies and the unit swizzle
+// q ^ f(r) are the same; a half slice of the cell matrix takes 22.5 KB at
+// p = 6, so two stages and the tables fit two workgroups per CU and one
+// workgroup's gather and scatter overlap the other's MFMAs.  MFMA (16x16x4
+// FP64): lane l supplies A[l%16][k] and B[k][l%16] for k = 2 (l/16) + r in
+// the r-th of the two MFMAs of a half slice.  Gathered half slice: 8 K rows
+// of TN = 64 doubles, two rows per 1 KB copy, the copies XG bytes apart so
+// that the B-operand reads of lane groups 0/1 and 2/3 fall in distinct
+// 128-byte bank windows.  Requirements (checked by the launcher): B even,
+// 16-byte aligned rows of src and W (leading dimensions even).
+template <int NW, int WC, int NRB>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(NW / 2))) void hxd_kernel(
+    HxLaunch<double> a) {
+  using T = double;
+  using v4 = f64x4;
+  constexpr int TN = 64;
+  constexpr int S = 2;
+  constexpr int NT = 64 * NW;
+  constexpr int WR = NW / WC;
+  constexpr int RBW = (NRB + WR - 1) / WR;
+  constexpr int ROWS_MAX = NRB * 16;
+  constexpr int CBW = TN / 16 / WC;
+  static_assert(CBW >= 1 && TN % (16 * WC) == 0, "bad column split");
+  constexpr int NLMAX = 128;
+  constexpr int KMAX = ROWS_MAX + NLMAX;
+  constexpr int HB = ROWS_MAX * 64;   // bytes of a half slice of the cell matrix (NRB copies)
+  constexpr int XG = 1024 + 128;      // bytes per gathered copy incl. spacing
+  constexpr int NX = 4;               // gathered copies per half slice (2 K rows each)
+  constexpr int SB = HB + NX * XG;    // bytes per stage
+  constexpr int NOP = NRB + NX;
+  constexpr int D = (NOP + NW - 1) / NW;  // copies per wave per stage
+
+  __shared__ __align__(16) unsigned char ring[S * SB];
+  __shared__ uint32_t smap[ROWS_MAX];
+  __shared__ T sout[ROWS_MAX];
+  __shared__ __align__(16) int64_t soff[KMAX];
+  __shared__ __align__(16) T sscl[KMAX];
+
+  const int wid = blockIdx.x;
+  const int local = wid >> 3;
+  if (local >= a.per_xcd) return;
+  const int item = (wid & 7) * a.per_xcd + local;
+  if (item >= a.ncells * a.ntiles) return;
+  const int cell = a.cells[item / a.ntiles];
+  const int v0 = (item % a.ntiles) * TN;
+
+  const int tid = threadIdx.x;
+  __builtin_assume(tid < NT);
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wave_s = __builtin_amdgcn_readfirstlane(wave);
+  const int wc = wave % WC;
+  const int wr = wave / WC;
+  constexpr int nkt = NRB;
+
+  int nl_begin = 0, nslot = 0;
+  if (a.nl_off) {
+    nl_begin = a.nl_off[cell];
+    nslot = a.nl_off[cell + 1] - nl_begin;
+  }
+  const int ntot = nkt + (nslot >> 4);
+  const int nh = 2 * ntot;  // half slices
+  const uint32_t *cmap = a.map + (int64_t)cell * a.ne;
+  const T *Hcell = a.Ht + (int64_t)cell * a.ne_pad * a.ne_pad;
+  const T *NLcell = a.nl_C ? a.nl_C + (int64_t)nl_begin * a.ne_pad : Hcell;
+
+  // gather tables (padding rows and empty slots read row 0 with scale 0)
+  for (int i = tid; i < KMAX; i += NT) {
+    int64_t off = 0;
+    T sc = T(0);
+    if (i < ROWS_MAX) {
+      uint32_t w = 0;
+      T so = T(1);
+      if (i < a.ne) {
+        w = cmap[i];
+        const uint32_t r = w & 0x7fffffffu;
+        off = (int64_t)r * a.ld_src;
+        sc = a.s_in ? a.s_in[r] : T(1);
+        if (a.s_out) so = a.s_out[r];
+      }
+      smap[i] = w;
+      sout[i] = so;
+    } else {
+      const int sl = i - ROWS_MAX;
+      const int pj = sl < nslot ? a.slot_proj[nl_begin + sl] : -1;
+      if (pj >= 0) {
+        off = (int64_t)pj * a.ldW;
+        sc = T(1);
+      }
+    }
+    soff[i] = off;
+    sscl[i] = sc;
+  }
+  __syncthreads();
+
+  // per-lane parts of the copy addresses
+  const uint32_t ring0 = lds_address(ring);
+  const int hr = lane >> 2;  // row within a copy's 16 rows
+  const int hoff = hr * 16 + 2 * ((lane & 3) ^ (((hr >> 1) ^ (hr >> 3)) & 3));
+  const int xsub = lane >> 5;  // K row within a gathered copy
+  const int xc = v0 + 2 * (lane & 31);
+  const int xcl = xc < a.B - 2 ? xc : a.B - 2;  // columns >= B are never stored
+
+  auto issue = [&](int h, int stage) {
+    const int t = h >> 1, half = h & 1;
+    const bool own = t < nkt;
+    const T *tile = (own ? Hcell + (int64_t)t * ROWS_MAX * 16
+                         : NLcell + (int64_t)(t - nkt) * ROWS_MAX * 16) + half * 8;
+    const T *xb = (own ? a.src : a.W) + xcl;
+    const uint32_t sbase = ring0 + stage * SB;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const int o = (wave_s + NW * j) % NOP;  // wave-uniform
+      if (o < NRB) {
+        copy16_to_lds(tile + o * 256 + hoff, sbase + o * 1024);
+      } else {
+        const int x = o - NRB;
+        copy16_to_lds(xb + soff[t * 16 + half * 8 + 2 * x + xsub], sbase + HB + x * XG);
+      }
+    }
+  };
+
+  v4 acc[RBW][CBW];
+#pragma unroll
+  for (int j = 0; j < RBW; ++j)
+#pragma unroll
+    for (int cb = 0; cb < CBW; ++cb) acc[j][cb] = v4{0, 0, 0, 0};
+
+  const int arow = lane & 15;
+  const int g = lane >> 4;  // K rows 2g, 2g + 1 of the half slice
+  const int aoff = arow * 64 + ((g ^ (((arow >> 1) ^ (arow >> 3)) & 3)) * 16);  // bytes
+
+  auto compute = [&](int stage, int h) {
+    const unsigned char *Hs = ring + stage * SB;
+    const T *Xs = reinterpret_cast<const T *>(ring + stage * SB + HB + g * XG);
+    const int k0 = (h >> 1) * 16 + (h & 1) * 8 + 2 * g;
+    const T sc0 = sscl[k0], sc1 = sscl[k0 + 1];
+    T b0[CBW], b1[CBW];
+#pragma unroll
+    for (int cb = 0; cb < CBW; ++cb) {
+      const int col = (wc * CBW + cb) * 16 + (lane & 15);
+      b0[cb] = Xs[col] * sc0;
+      b1[cb] = Xs[TN + col] * sc1;
+    }
+#pragma unroll
+    for (int j = 0; j < RBW; ++j) {
+      const int rb = wr + WR * j;
+      if (rb < NRB) {
+        const double2 a2 = *reinterpret_cast<const double2 *>(Hs + rb * 1024 + aoff);
+#pragma unroll
+        for (int cb = 0; cb < CBW; ++cb) acc[j][cb] = mfma16(a2.x, b0[cb], acc[j][cb]);
+#pragma unroll
+        for (int cb = 0; cb < CBW; ++cb) acc[j][cb] = mfma16(a2.y, b1[cb], acc[j][cb]);
+      }
+    }
+  };
+
+  issue(0, 0);
+  for (int h = 0; h < nh; ++h) {
+    wait_vmcnt<0>();
+    __syncthreads();
+    if (h + 1 < nh) issue(h + 1, (h + 1) & 1);
+    compute(h & 1, h);
   }
   wait_vmcnt<0>();  // no copy may still target LDS when the workgroup ends
 
@@ -1409,28 +1588,28 @@ static inline unsigned grid_for(int64_t n, int block, unsigned cap = 1u << 20) {
   return (unsigned)g;
 }
 
-template <typename T, int NW, int TN, int WC, int NRB>
+template <typename T, int NW, int TN, int WC, int NRB, int NB>
 static hipError_t hx_go(HxLaunch<T> a, hipStream_t s) {
   a.ntiles = (a.B + TN - 1) / TN;
   const int items = a.ncells * a.ntiles;
   if (items == 0) return hipSuccess;
   a.per_xcd = (items + 7) / 8;
-  hipLaunchKernelGGL((hx_kernel<T, NW, TN, WC, NRB>), dim3(8 * a.per_xcd), dim3(64 * NW), 0,
+  hipLaunchKernelGGL((hx_kernel<T, NW, TN, WC, NRB, NB>), dim3(8 * a.per_xcd), dim3(64 * NW), 0,
                      s, a);
   return hipGetLastError();
 }
 
 // Row-block count of the padded cell matrix is a compile-time constant so the
 // MFMA loop has no guards: (p+1)^3 padded to 16 for p = 1..6.
-template <typename T, int NW, int TN, int WC>
+template <typename T, int NW, int TN, int WC, int NB = 2>
 static hipError_t hx_dispatch(const HxLaunch<T> &a, hipStream_t s) {
   switch (a.ne_pad >> 4) {
-    case 1: return hx_go<T, NW, TN, WC, 1>(a, s);
-    case 2: return hx_go<T, NW, TN, WC, 2>(a, s);
-    case 4: return hx_go<T, NW, TN, WC, 4>(a, s);
-    case 8: return hx_go<T, NW, TN, WC, 8>(a, s);
-    case 14: return hx_go<T, NW, TN, WC, 14>(a, s);
-    case 22: return hx_go<T, NW, TN, WC, 22>(a, s);
+    case 1: return hx_go<T, NW, TN, WC, 1, NB>(a, s);
+    case 2: return hx_go<T, NW, TN, WC, 2, NB>(a, s);
+    case 4: return hx_go<T, NW, TN, WC, 4, NB>(a, s);
+    case 8: return hx_go<T, NW, TN, WC, 8, NB>(a, s);
+    case 14: return hx_go<T, NW, TN, WC, 14, NB>(a, s);
+    case 22: return hx_go<T, NW, TN, WC, 22, NB>(a, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -1441,8 +1620,18 @@ static hipError_t hxp_go(HxLaunch<float> a, hipStream_t s) {
   const int items = a.ncells * a.ntiles;
   if (items == 0) return hipSuccess;
   a.per_xcd = (items + 7) / 8;
-  hipLaunchKernelGGL((hxp_kernel<NW, TN, WC, NRB, S>), dim3(8 * a.per_xcd), dim3(64 * NW), 0, s,
-                     a);
+  hipLaunchKernelGGL((hxp_kernel<NW, TN, WC, NRB, S>), dim3(8 * a.per_xcd), dim3(64 * NW), 0,
+                     s, a);
+  return hipGetLastError();
+}
+
+template <int NW, int WC, int NRB>
+static hipError_t hxd_go(HxLaunch<double> a, hipStream_t s) {
+  a.ntiles = (a.B + 63) / 64;
+  const int items = a.ncells * a.ntiles;
+  if (items == 0) return hipSuccess;
+  a.per_xcd = (items + 7) / 8;
+  hipLaunchKernelGGL((hxd_kernel<NW, WC, NRB>), dim3(8 * a.per_xcd), dim3(64 * NW), 0, s, a);
   return hipGetLastError();
 }
 
@@ -1594,6 +1783,27 @@ static bool hxp_fits(const HxLaunch<float> &a) {
   return ok;
 }
 
+static bool hxd_fits(const HxLaunch<double> &a) {
+  auto al = [](const void *p) { return ((uintptr_t)p & 15) == 0; };
+  bool ok = a.B >= 2 && a.B % 2 == 0 && a.ld_src % 2 == 0 && al(a.src) && al(a.Ht);
+  if (a.nl_off) ok = ok && a.ldW % 2 == 0 && al(a.W) && al(a.nl_C);
+  return ok;
+}
+
+template <int NW, int WC>
+static hipError_t hxd_dispatch(const HxLaunch<double> &a, hipStream_t s) {
+  if (!hxd_fits(a)) return hx_dispatch<double, 8, 64, 4>(a, s);
+  switch (a.ne_pad >> 4) {
+    case 1: return hxd_go<NW, WC, 1>(a, s);
+    case 2: return hxd_go<NW, WC, 2>(a, s);
+    case 4: return hxd_go<NW, WC, 4>(a, s);
+    case 8: return hxd_go<NW, WC, 8>(a, s);
+    case 14: return hxd_go<NW, WC, 14>(a, s);
+    case 22: return hxd_go<NW, WC, 22>(a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 template <int NW, int TN, int WC, int S>
 static hipError_t hxp_dispatch(const HxLaunch<float> &a, hipStream_t s) {
   if (!hxp_fits(a)) return hx_dispatch<float, 8, 64, 4>(a, s);
@@ -1622,7 +1832,10 @@ static hipError_t hxp_dispatch(const HxLaunch<float> &a, hipStream_t s) {
 //         1: 4 x  32 x 2
 //         2: 16 x 64 x 4 (four row groups, 16 waves per CU)
 //         3: 8 x 64 x 2, 4: 16 x 64 x 2 (fewer column waves)
-int hx_variants(bool fp64) { return fp64 ? 5 : 12; }
+//         5: 4 x 32 x 2 with single-buffered slices (two workgroups per CU)
+//   direct-to-LDS ring of half slices (two workgroups per CU): 6: 4 x 64 x 2,
+//            7: 8 x 64 x 4
+int hx_variants(bool fp64) { return fp64 ? 8 : 12; }
 
 template <>
 hipError_t launch_hx<float>(const HxLaunch<float> &a, hipStream_t s) {
@@ -1651,6 +1864,9 @@ hipError_t launch_hx<double>(const HxLaunch<double> &a, hipStream_t s) {
     case 2: return hx_dispatch<double, 16, 64, 4>(a, s);
     case 3: return hx_dispatch<double, 8, 64, 2>(a, s);
     case 4: return hx_dispatch<double, 16, 64, 2>(a, s);
+    case 5: return hx_dispatch<double, 4, 32, 2, 1>(a, s);
+    case 6: return hxd_dispatch<4, 2>(a, s);
+    case 7: return hxd_dispatch<8, 4>(a, s);
     default: return hx_dispatch<double, 8, 64, 4>(a, s);
   }
 }
@@ -1867,10 +2083,10 @@ __attribute__((amdgpu_waves_per_eu(NW == 8 && S == 2 ? 4 : 1))) void hxb_kernel(
   hx_scatter<float, NRB, WR, RBW, CBW, 8>(a, a.ne, acc, smap, sout, wr, wc, lane, v0);
 }
 
-template <int NRB>
+template <int NRB, int NW, int WC, int S>
 static hipError_t hxb_go(HxLaunch<float> a, const uint16_t *Ht16, const uint16_t *nlC16,
                          hipStream_t s) {
-  constexpr int NW = 8, TN = 64, WC = 4, S = 2;
+  constexpr int TN = 64;
   a.ntiles = (a.B + TN - 1) / TN;
   const int items = a.ncells * a.ntiles;
   if (items == 0) return hipSuccess;
@@ -1893,23 +2109,37 @@ static hipError_t hx_bf16_go(HxLaunch<float> a, const uint16_t *Ht16, const uint
   return hipGetLastError();
 }
 
+template <int NW, int WC, int S>
+static hipError_t hxb_dispatch(const HxLaunch<float> &a, const uint16_t *Ht16,
+                               const uint16_t *nlC16, hipStream_t s) {
+  switch (a.ne_pad >> 4) {
+    case 2: return hxb_go<2, NW, WC, S>(a, Ht16, nlC16, s);
+    case 4: return hxb_go<4, NW, WC, S>(a, Ht16, nlC16, s);
+    case 8: return hxb_go<8, NW, WC, S>(a, Ht16, nlC16, s);
+    case 14: return hxb_go<14, NW, WC, S>(a, Ht16, nlC16, s);
+    case 22: return hxb_go<22, NW, WC, S>(a, Ht16, nlC16, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+int hx_bf16_variants() { return 4; }
+
 hipError_t launch_hx_bf16(const HxLaunch<float> &a, const uint16_t *Ht16, const uint16_t *nlC16,
                           hipStream_t s) {
-  // variant 0: the K = 32 ring (16-byte aligned rows and blocks, B a multiple
-  // of 4, an even number of row blocks); 1 or otherwise: the register-staged
-  // K = 16 kernel
+  // Variants: the K = 32 ring (16-byte aligned rows and blocks, B a multiple
+  // of 4, an even number of row blocks) as waves x column waves x stages
+  // 0: 8 x 4 x 2 (two workgroups per CU), 2: 8 x 4 x 3, 3: 16 x 4 x 3 (one
+  // workgroup per CU); 1, or a layout the ring does not take: the
+  // register-staged K = 16 kernel
   auto al = [](const void *p) { return ((uintptr_t)p & 15) == 0; };
-  bool ring = a.variant == 0 && a.B >= 4 && a.B % 4 == 0 && a.ld_src % 4 == 0 && al(a.src) &&
-              al(Ht16) && ((a.ne_pad >> 4) % 2 == 0);
+  bool ring = a.variant != 1 && a.B >= 4 && a.B % 4 == 0 && a.ld_src % 4 == 0 && al(a.src) &&
+              al(Ht16) && ((a.ne_pad >> 4) % 2 == 0) && (a.ne_pad >> 4) >= 2;
   if (a.nl_off) ring = ring && a.ldW % 4 == 0 && al(a.W) && al(nlC16);
   if (ring) {
-    switch (a.ne_pad >> 4) {
-      case 2: return hxb_go<2>(a, Ht16, nlC16, s);
-      case 4: return hxb_go<4>(a, Ht16, nlC16, s);
-      case 8: return hxb_go<8>(a, Ht16, nlC16, s);
-      case 14: return hxb_go<14>(a, Ht16, nlC16, s);
-      case 22: return hxb_go<22>(a, Ht16, nlC16, s);
-      default: break;
+    switch (a.variant) {
+      case 2: return hxb_dispatch<8, 4, 3>(a, Ht16, nlC16, s);
+      case 3: return hxb_dispatch<16, 4, 3>(a, Ht16, nlC16, s);
+      default: return hxb_dispatch<8, 4, 2>(a, Ht16, nlC16, s);
     }
   }
   switch (a.ne_pad >> 4) {

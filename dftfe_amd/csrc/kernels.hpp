@@ -79,6 +79,7 @@ struct ProjLaunch {
 template <typename T>
 hipError_t launch_hx(const HxLaunch<T> &a, hipStream_t s);
 int hx_variants(bool fp64);
+int hx_bf16_variants();
 // BF16 cell matrices / inputs, FP32 accumulation and multivectors
 hipError_t launch_hx_bf16(const HxLaunch<float> &a, const uint16_t *Ht16, const uint16_t *nlC16,
                           hipStream_t s);

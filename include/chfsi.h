@@ -264,8 +264,9 @@ chfsi_status chfsi_operator_get_cell_matrices(chfsi_operator *op, double *H_dev,
  * times_ms (40 entries) [0..15]: FP64 tile variants, [16..19]: FP64 projection
  * variants, [20..35]: FP32 tile variants, [36..39]: FP32 projection variants
  * (ms per HX; -1 where a variant does not exist).  precision CHFSI_BF16 selects
- * the BF16 apply's kernel: 0 (default) K = 32 MFMA on the direct-to-LDS ring,
- * 1 the register-staged K = 16 form. */
+ * the BF16 apply's kernel: K = 32 MFMA on the direct-to-LDS ring with 0
+ * (default) 8 waves and 2 stages (two workgroups per CU), 2: 8 waves and 3
+ * stages, 3: 16 waves and 3 stages; 1: the register-staged K = 16 form. */
 chfsi_status chfsi_operator_set_hx_variant(chfsi_operator *op, int32_t precision, int32_t variant);
 chfsi_status chfsi_operator_set_proj_variant(chfsi_operator *op, int32_t precision,
                                              int32_t variant);

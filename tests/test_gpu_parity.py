@@ -66,7 +66,7 @@ def test_hx_fp64_matches_oracle(small, B):
     assert _rel(Y, ref.apply(X)) < 1e-13
 
 
-@pytest.mark.parametrize("precision,variant", [(0, v) for v in range(5)] + [(1, v) for v in range(12)])
+@pytest.mark.parametrize("precision,variant", [(0, v) for v in range(8)] + [(1, v) for v in range(12)])
 def test_hx_tile_variants(small, precision, variant):
     """Every HX tile variant gives the oracle's H~X (FP64: 1e-13, FP32: 2e-6)."""
     prob, ref, op = small
@@ -213,17 +213,17 @@ def test_hx_bf16(small):
 
 @pytest.mark.parametrize("B", [64, 72, 8])
 def test_hx_bf16_ring_matches_register_kernel(small, B):
-    """K = 32 BF16 ring (variant 0) against the register-staged K = 16 form
-    (variant 1): FP32 accumulation rounding apart.  Cells here hold 16
-    projector slots (one 16-wide slice), so every stage count is odd and the
-    last stage's second half is the zero half."""
+    """K = 32 BF16 ring (variants 0, 2, 3: stage and wave counts) against the
+    register-staged K = 16 form (variant 1): FP32 accumulation rounding apart.
+    Cells here hold 16 projector slots (one 16-wide slice), so every stage
+    count is odd and the last stage's second half is the zero half."""
     from dftfe_amd import lib as L
 
     prob, ref, op = small
     X = prob.initial_guess(N=B, seed_offset=B + 11)
     Xd = torch.tensor(X, device="cuda", dtype=torch.float32)
     out = {}
-    for v in (0, 1):
+    for v in (0, 1, 2, 3):
         Yd = torch.zeros_like(Xd)
         op.set_hx_variant(L.BF16, v)
         try:
@@ -231,8 +231,9 @@ def test_hx_bf16_ring_matches_register_kernel(small, B):
         finally:
             op.set_hx_variant(L.BF16, 0)
         out[v] = Yd.double().cpu().numpy()
-    assert np.isfinite(out[0]).all()
-    assert _rel(out[0], out[1]) < 1e-5
+    for v in (0, 2, 3):
+        assert np.isfinite(out[v]).all()
+        assert _rel(out[v], out[1]) < 1e-5, v
 
 
 def test_filter_mixed_with_fp64_tail(small):
