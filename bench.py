@@ -38,6 +38,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
+PEAK_HBM_GBPS = 8000.0
 # mixed passes: the columns of the top quarter of the states keep an FP64
 # filter (their filter gain is the smallest, so the FP32 rounding of the
 # defect form is largest relative to them); -1 = N / 4
@@ -74,6 +75,14 @@ def measured_hx_traffic(workload, width, variant, proj_variant, precision="fp32"
     return int(n * (tile["fetch_bytes"] + tile["write_bytes"]) + proj["fetch_bytes"]
                + proj["write_bytes"] + t["reduction"]["fetch_bytes"]
                + t["reduction"]["write_bytes"])
+
+
+def dense_without_hx(phases, hx64_ms):
+    """CholGS + Rayleigh-Ritz + residual of one pass less the two FP64 HX on
+    all columns that Rayleigh-Ritz and the residual apply (ms)."""
+    if "cholgs" not in phases:
+        return None
+    return round(phases["cholgs"] + phases["rayleigh_ritz"] + phases["residual"] - 2 * hx64_ms, 2)
 
 
 def pass_flops(prob, N, B, m, lanczos_steps):
@@ -199,22 +208,23 @@ def main():
     full_margin = args.lanczos_margin == "full"
     head_defect = args.mixed_form == "defect"
 
-    def make_solver(dense=False, defect=False, top=0):
+    def make_solver(dense=False, defect=False, top=0, rocblas=False, shep=True):
         s_ = ChebyshevSolverDevice(cheby_block_size=B, cheby_degree=m,
                                    lanczos_steps=wl.lanczos_steps, mixed_dense=dense,
                                    mixed_prec_defect_filter=defect,
                                    lanczos_full_residual_margin=full_margin,
-                                   mixed_prec_fp64_top_columns=top)
+                                   mixed_prec_fp64_top_columns=top, dense_rocblas=rocblas,
+                                   use_shep=shep)
         s_.set_lanczos_start(v0)
         return s_
 
     def run_passes(mixed: bool, npass: int, dense: bool = False, defect: bool = False,
-                   label=None, top: int = 0):
+                   label=None, top: int = 0, rocblas: bool = False, shep: bool = True):
         """npass passes from the same start with the reference's bookkeeping
         (a0 = lowest, bLow = top Ritz value, kohnShamEigenSolve.cc:483-498).
         With a label, the last Ritz vectors and values are kept for the error
         checks."""
-        solver = make_solver(dense, defect, top)
+        solver = make_solver(dense, defect, top, rocblas, shep)
         X = X0.clone()
         lams, ress, phases = [], [], []
         torch.cuda.synchronize()
@@ -249,6 +259,16 @@ def main():
     if not args.no_fp64:
         fp64_ms, lam_64, res_64, ph_64 = run_passes(False, args.steps, label="fp64")
         log(f"fp64: {fp64_ms:.1f} ms/pass; last max residual {res_64[-1].max():.3e}")
+    # the headline mixed passes with the dense FP64 steps on rocBLAS (Tensile
+    # DGEMM per block) instead of the library's MFMA kernels, and with the
+    # generalised Rayleigh-Ritz (SUBSPACE PROJ SHEP GPU = false: no CholGS,
+    # one rotation by L^-T Q)
+    rb_ms, lam_rb, _, ph_rb = run_passes(True, 2, defect=head_defect, top=args.fp64_top,
+                                         rocblas=True)
+    log(f"mixed, dense steps on rocBLAS: {rb_ms:.1f} ms/pass; phases {ph_rb[-1]}")
+    gep_ms, lam_gep, _, ph_gep = run_passes(True, 2, defect=head_defect, top=args.fp64_top,
+                                            shep=False)
+    log(f"mixed, generalised Rayleigh-Ritz: {gep_ms:.1f} ms/pass; phases {ph_gep[-1]}")
     # the same mixed passes with the reference's opt-in mixed-precision dense
     # steps (FP32 off-diagonal blocks of the overlap, CholGS rotation, X^T H X
     # and Rayleigh-Ritz rotation)
@@ -338,6 +358,15 @@ def main():
                        "lanczos_margin": args.lanczos_margin},
         "mixed_pass_ms": round(mixed_ms, 2),
         "mixed_dense_pass_ms": round(md_ms, 2),
+        "dense_rocblas_pass_ms": round(rb_ms, 2),
+        "phase_ms_dense_rocblas_last": ph_rb[-1],
+        "max_eig_err_dense_rocblas_vs_mixed_Ha": [float(np.abs(a - b).max())
+                                                  for a, b in zip(lam_rb, lam_mx)],
+        "gep_pass_ms": round(gep_ms, 2),
+        "phase_ms_gep_last": ph_gep[-1],
+        "max_eig_err_gep_vs_mixed_Ha": [float(np.abs(a - b).max())
+                                        for a, b in zip(lam_gep, lam_mx)],
+        "dense_fp64_ms_without_hx": dense_without_hx(ph_mx[-1], hx64_ms),
         ("mixed_plain_pass_ms" if head_defect else "mixed_defect_pass_ms"): round(df_ms, 2),
         ("phase_ms_mixed_plain_last" if head_defect else "phase_ms_mixed_defect_last"): ph_df[-1],
         "phase_ms_mixed_dense_last": ph_md[-1],
@@ -372,6 +401,11 @@ def main():
         out["max_residual_fp64"] = [float(r.max()) for r in res_64]
     pm = tune.get("fp32_proj_ms") or [0.0]
     proj32 = int(np.argmin(pm))  # the projection variant the autotuner kept
+    proj64 = int(np.argmin(tune.get("fp64_proj_ms") or [0.0]))
+    vb = int(np.argmin(hxb_ms))
+    # BF16 apply: FP32 multivectors (read, read + write Y), BF16 cell matrices
+    bf16_bytes = 3 * prob.M * W * 4 + prob.C * prob.ne ** 2 * 2 + prob.C * prob.ne * 4
+    bf16_traffic = measured_hx_traffic(args.workload, W, vb, proj32, "bf16")
     out["roofline"] = {
         "kernel": f"fused FP32 HX on {W} columns (8 colour launches of tile variant "
                   f"{tune['fp32']} + projection variant {proj32} + reduction)",
@@ -386,11 +420,18 @@ def main():
                           f"(rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
         "algorithmic_bytes": hx_bytes(prob, W, 4),
         "fp64": {"achieved": round(F / hx64_ms / 1e9, 2), "peak": PEAK_TFLOPS["fp64"],
-                 "frac": round(F / hx64_ms / 1e9 / PEAK_TFLOPS["fp64"], 4)},
-        "bf16": {"achieved": round(F / hxb_best / 1e9, 2), "peak": PEAK_TFLOPS["bf16"],
-                 "frac": round(F / hxb_best / 1e9 / PEAK_TFLOPS["bf16"], 4),
-                 "ms_variants": [round(x, 4) for x in hxb_ms],
-                 "variant": int(np.argmin(hxb_ms)),
+                 "frac": round(F / hx64_ms / 1e9 / PEAK_TFLOPS["fp64"], 4),
+                 "tile_variant": tune["fp64"], "projection_variant": proj64,
+                 "traffic": measured_hx_traffic(args.workload, W, tune["fp64"], proj64, "fp64")},
+        # the BF16 apply moves FP32 multivectors: its bound is HBM, not the
+        # BF16 MFMA rate (the MFMA view is kept beside it)
+        "bf16": {"bound": "hbm", "achieved": round(bf16_bytes / hxb_best / 1e6, 1),
+                 "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+                 "frac": round(bf16_bytes / hxb_best / 1e6 / PEAK_HBM_GBPS, 4),
+                 "algorithmic_bytes": bf16_bytes, "traffic": bf16_traffic,
+                 "traffic_GBps": round(bf16_traffic / hxb_best / 1e6, 1) if bf16_traffic else None,
+                 "mfma_tflops": round(F / hxb_best / 1e9, 2), "mfma_peak": PEAK_TFLOPS["bf16"],
+                 "ms_variants": [round(x, 4) for x in hxb_ms], "variant": vb,
                  "speedup_vs_fp32": round(hx32_ms / hxb_best, 3)},
     }
 

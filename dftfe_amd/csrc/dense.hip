@@ -4,9 +4,10 @@
 //             Hamiltonian X^T (H~X)), lower-triangle tiles only when asked,
 //             split over M into chunks whose partial tiles are summed in a
 //             fixed order (deterministic);
-//   * rotate: X <- X R over row panels (R = Q, the Ritz rotation, or
-//             R = L^{-T}, the Cholesky-Gram-Schmidt one, whose zero blocks
-//             below the diagonal are skipped).
+//   * rotate: Y = X R over row panels (R = Q, the Ritz rotation, or a column
+//             block of R = L^{-T}, the Cholesky-Gram-Schmidt one, with the
+//             inner extent cut at the block's last row: the blocks below the
+//             diagonal are zero).
 // Reference steps replaced: fillParallelOverlapMatScalapack
 // (src/linAlg/linearAlgebraOperationsDevice.cc:3078-3240), XtHX
 // (src/dftOperator/kohnShamDFTOperatorDevice.cc:4000-4157),
@@ -168,15 +169,14 @@ __global__ void gram_reduce_kernel(const double *P, const int2 *tiles, int ntile
   C[gi + (int64_t)gj * ldc] = s;
 }
 
-// Y[k, J + j] = sum_i X[k, i] R[i, J + j] for the rows [r0, r0 + rows) of the
-// panel; R[i][j] = Rm[i + j ldr] (TR = 0) or Rm[j + i ldr] (TR = 1).
-// upper: R is upper triangular, slabs below the tile's last column are skipped.
-// LDS slabs: X as [128 rows][KS + 1] (odd stride: the 16 rows of an A-operand
+// Y[k, J + j] = sum_{i < K} X[k, i] R[i, J + j] for the rows [r0, r0 + rows)
+// of the panel and J + j < N; R[i][j] = Rm[i + j ldr] (TR = 0) or
+// Rm[j + i ldr] (TR = 1).  LDS slabs: X as [128 rows][KS + 1] (odd stride: the 16 rows of an A-operand
 // read fall in distinct banks); R as [128 columns][KS + 1] when its columns
 // are contiguous in memory (TR = 0), else [KS rows][128 + PADW].
 template <int TR>
 __global__ __launch_bounds__(NT, 2) void rotate_kernel(const double *X, int64_t ldx,
-                                                       const double *Rm, int64_t ldr, int upper,
+                                                       const double *Rm, int64_t ldr, int K,
                                                        int N, int64_t rows, double *Y,
                                                        int64_t ldy, int ntj, int per_xcd,
                                                        int nitems) {
@@ -191,15 +191,14 @@ __global__ __launch_bounds__(NT, 2) void rotate_kernel(const double *X, int64_t 
   const int J = (item % ntj) * TT;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wi = w & 1, wj = w >> 1;
-  const int iend = upper ? (J + TT < N ? J + TT : N) : N;
-  const int nslab = (iend + KS - 1) / KS;
+  const int nslab = (K + KS - 1) / KS;
 
   // X slab: thread t takes row t/2, HK consecutive columns from HK*(t%2)
   const int xr = tid >> 1, xc = (tid & 1) * HK;
   double rx[HK], rr[HK];
   // 16-byte loads when rows are aligned and every slab lies inside the matrix
   const bool vec = (ldx % 2 == 0) && (ldr % 2 == 0) && (((uintptr_t)X & 15) == 0) &&
-                   (((uintptr_t)Rm & 15) == 0) && (N % KS == 0);
+                   (((uintptr_t)Rm & 15) == 0) && (K % KS == 0);
   auto load = [&](int i0) {
     const int64_t k = R0 + xr;
     const bool kok = k < rows;
@@ -217,16 +216,16 @@ __global__ __launch_bounds__(NT, 2) void rotate_kernel(const double *X, int64_t 
 #pragma unroll
       for (int e = 0; e < HK; ++e) {
         const int i = i0 + xc + e;
-        const double v = xp[i < N ? i : 0];
-        rx[e] = (kok && i < N) ? v : 0.0;
+        const double v = xp[i < K ? i : 0];
+        rx[e] = (kok && i < K) ? v : 0.0;
       }
     }
     if (TR) {
       // (i, j) at Rm[j + i ldr]: thread t takes row i0 + t/16, 8 consecutive j
       const int ii = i0 + (tid >> 4), jj = J + (tid & 15) * 8;
       if (vec && jj + 8 <= N) {
-        const double z = ii < N ? 1.0 : 0.0;
-        const double2 *p = reinterpret_cast<const double2 *>(Rm + (int64_t)(ii < N ? ii : 0) * ldr + jj);
+        const double z = ii < K ? 1.0 : 0.0;
+        const double2 *p = reinterpret_cast<const double2 *>(Rm + (int64_t)(ii < K ? ii : 0) * ldr + jj);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const double2 v = p[e];
@@ -237,8 +236,8 @@ __global__ __launch_bounds__(NT, 2) void rotate_kernel(const double *X, int64_t 
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
           const int j = jj + e;
-          const double v = Rm[(int64_t)(ii < N ? ii : 0) * ldr + (j < N ? j : 0)];
-          rr[e] = (ii < N && j < N) ? v : 0.0;
+          const double v = Rm[(int64_t)(ii < K ? ii : 0) * ldr + (j < N ? j : 0)];
+          rr[e] = (ii < K && j < N) ? v : 0.0;
         }
       }
     } else {
@@ -257,8 +256,8 @@ __global__ __launch_bounds__(NT, 2) void rotate_kernel(const double *X, int64_t 
 #pragma unroll
         for (int e = 0; e < HK; ++e) {
           const int i = ib + e;
-          const double v = rp[i < N ? i : 0];
-          rr[e] = (i < N && j < N) ? v : 0.0;
+          const double v = rp[i < K ? i : 0];
+          rr[e] = (i < K && j < N) ? v : 0.0;
         }
       }
     }
@@ -379,8 +378,8 @@ hipError_t launch_gram(const double *A, int64_t lda, const double *B, int64_t ld
 }
 
 hipError_t launch_rotate(const double *X, int64_t ldx, const double *Rm, int64_t ldr, int trans,
-                         int upper, int N, int64_t rows, double *Y, int64_t ldy, hipStream_t s) {
-  if (rows <= 0 || N <= 0) return hipSuccess;
+                         int K, int N, int64_t rows, double *Y, int64_t ldy, hipStream_t s) {
+  if (rows <= 0 || N <= 0 || K <= 0) return hipSuccess;
   const int ntj = (N + TT - 1) / TT;
   const int64_t nti = (rows + TT - 1) / TT;
   const int64_t nitems64 = nti * ntj;
@@ -388,10 +387,10 @@ hipError_t launch_rotate(const double *X, int64_t ldx, const double *Rm, int64_t
   const int nitems = (int)nitems64;
   const int per_xcd = (nitems + 7) / 8;
   if (trans)
-    hipLaunchKernelGGL(rotate_kernel<1>, dim3(8 * per_xcd), dim3(NT), 0, s, X, ldx, Rm, ldr, upper,
+    hipLaunchKernelGGL(rotate_kernel<1>, dim3(8 * per_xcd), dim3(NT), 0, s, X, ldx, Rm, ldr, K,
                        N, rows, Y, ldy, ntj, per_xcd, nitems);
   else
-    hipLaunchKernelGGL(rotate_kernel<0>, dim3(8 * per_xcd), dim3(NT), 0, s, X, ldx, Rm, ldr, upper,
+    hipLaunchKernelGGL(rotate_kernel<0>, dim3(8 * per_xcd), dim3(NT), 0, s, X, ldx, Rm, ldr, K,
                        N, rows, Y, ldy, ntj, per_xcd, nitems);
   return hipGetLastError();
 }

@@ -23,11 +23,10 @@ hipError_t launch_gram(const double *A, int64_t lda, const double *B, int64_t ld
                        int2 *tiles_dev, hipStream_t s);
 size_t gram_partial_bytes(int64_t K, int m, int n, int diag_off);
 
-// Y[k ldy + j] = sum_i X[k ldx + i] R(i, j) for k < rows, i, j < N, with
-// R(i, j) = Rm[i + j ldr] (trans = 0) or Rm[j + i ldr] (trans = 1);
-// upper = 1 when R is upper triangular (the blocks below the diagonal are
-// skipped).  Y must not overlap X.
+// Y[k ldy + j] = sum_{i < K} X[k ldx + i] R(i, j) for k < rows, j < N, with
+// R(i, j) = Rm[i + j ldr] (trans = 0) or Rm[j + i ldr] (trans = 1).  Y must
+// not overlap X.
 hipError_t launch_rotate(const double *X, int64_t ldx, const double *Rm, int64_t ldr, int trans,
-                         int upper, int N, int64_t rows, double *Y, int64_t ldy, hipStream_t s);
+                         int K, int N, int64_t rows, double *Y, int64_t ldy, hipStream_t s);
 
 }  // namespace chfsi

@@ -452,7 +452,10 @@ chfsi_status gram_fp64(chfsi_solver *s, const double *A, int64_t lda, const doub
 // Panel p is rotated into buffer p % 2 on st; its copy back into X runs on the
 // solver's copy stream while panel p + 1 is rotated (the rotation reads only
 // its own panel of X).  Before panel p + 2 reuses the buffer, st waits for
-// panel p's copy; at the end st waits for every copy.
+// panel p's copy; at the end st waits for every copy.  upper: R is upper
+// triangular with its lower part stored as zeros; the panel is rotated by
+// column blocks of 256, block [c0, c0 + 256) over the inner extent
+// c0 + 256 only, so every launch has work items of one length.
 chfsi_status rotate_fp64(chfsi_solver *s, double *X, int64_t M, int N, const double *Rm,
                          int64_t ldr, int trans, int upper, hipStream_t st) {
   const int64_t chunk = panel_rows(M, N);
@@ -471,7 +474,17 @@ chfsi_status rotate_fp64(chfsi_solver *s, double *X, int64_t M, int N, const dou
     const int64_t n0 = p * chunk, nc = std::min(chunk, M - n0);
     double *tmp = sptr<double>(s->rot) + (int64_t)b * chunk * N;
     if (p >= 2) CHFSI_HIP(hipStreamWaitEvent(st, s->copy_done[b], 0));
-    CHFSI_HIP(launch_rotate(X + n0 * N, N, Rm, ldr, trans, upper, N, nc, tmp, N, st));
+    if (upper) {
+      constexpr int CB = 256;
+      for (int c0 = 0; c0 < N; c0 += CB) {
+        const int bw = std::min(CB, N - c0);
+        // R(i, c0 + j): Rm[c0 + j + i ldr] (trans) or Rm[i + (c0 + j) ldr]
+        const double *Rc = trans ? Rm + c0 : Rm + (int64_t)c0 * ldr;
+        CHFSI_HIP(launch_rotate(X + n0 * N, N, Rc, ldr, trans, c0 + bw, bw, nc, tmp + c0, N, st));
+      }
+    } else {
+      CHFSI_HIP(launch_rotate(X + n0 * N, N, Rm, ldr, trans, N, N, nc, tmp, N, st));
+    }
     CHFSI_HIP(hipEventRecord(s->rot_done[b], st));
     CHFSI_HIP(hipStreamWaitEvent(s->copy_stream, s->rot_done[b], 0));
     CHFSI_HIP(hipMemcpyAsync(X + n0 * N, tmp, sizeof(double) * nc * N, hipMemcpyDeviceToDevice,
