@@ -770,6 +770,27 @@ def cpu_baseline(prob, B, log):
     PF = pass_flops(small, swl.N, swl.B, swl.m, swl.lanczos_steps)
     log(f"cpu FP64 pass (20 bohr, p=4, 64 states): {pass_s * 1e3:.0f} ms "
         f"= {PF / pass_s / 1e9:.2f} GFLOP/s on {threads} threads")
+    lam_o, _, _ = O.chfsi_pass(ref_s, X0, a0, bLow, bUp, swl.m, swl.B)
+
+    # the same pass on the GPU (FP64, same start and bounds), median of 5
+    import torch
+    from dftfe_amd.chfsi import ChebyshevSolverDevice, KohnShamOperatorDevice
+
+    op_s = KohnShamOperatorDevice(small)
+    gt, lam_g = [], None
+    for _ in range(6):
+        solver = ChebyshevSolverDevice(a0, bLow, bUp, cheby_block_size=swl.B,
+                                       cheby_degree=swl.m, reuse_lanczos_upper=True)
+        Xd = torch.tensor(X0, device="cuda")
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        lam_g, _, _ = solver.solve(op_s, Xd)
+        torch.cuda.synchronize()
+        gt.append(time.perf_counter() - t)
+    gpu_s = float(np.median(gt[1:]))
+    gpu_err = float(np.abs(lam_g - lam_o).max())
+    log(f"gpu FP64 pass on the same workload: {gpu_s * 1e3:.2f} ms; max|dlambda| vs the CPU "
+        f"pass {gpu_err:.1e} Ha")
 
     t = time.time()
     ref = O.OracleOperator(prob)
@@ -785,6 +806,8 @@ def cpu_baseline(prob, B, log):
     return {"value": round(PF / pass_s / 1e9, 3), "unit": "GFLOP/s (FP64 ChFSI pass)",
             "cores": threads, "kind": "port",
             "pass_ms": round(pass_s * 1e3, 1), "pass_gflop": round(PF / 1e9, 3),
+            "gpu_fp64_pass_ms_same_workload": round(gpu_s * 1e3, 2),
+            "gpu_vs_cpu_max_eig_diff_Ha": gpu_err,
             "cpu_model": cpu, "blas": f"{blas.get('internal_api', '?')} {blas.get('version', '')} "
                                       f"({blas.get('architecture', '')})".strip(),
             "hx_benchmark_workload_gflops": round(gf, 2), "hx_seconds": round(dt, 3),
