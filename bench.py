@@ -208,7 +208,7 @@ def main():
     full_margin = args.lanczos_margin == "full"
     head_defect = args.mixed_form == "defect"
 
-    def make_solver(dense=False, defect=False, top=0, rocblas=False, shep=True):
+    def make_solver(dense=False, defect=False, top=0, rocblas=0, shep=True):
         s_ = ChebyshevSolverDevice(cheby_block_size=B, cheby_degree=m,
                                    lanczos_steps=wl.lanczos_steps, mixed_dense=dense,
                                    mixed_prec_defect_filter=defect,
@@ -219,7 +219,7 @@ def main():
         return s_
 
     def run_passes(mixed: bool, npass: int, dense: bool = False, defect: bool = False,
-                   label=None, top: int = 0, rocblas: bool = False, shep: bool = True):
+                   label=None, top: int = 0, rocblas: int = 0, shep: bool = True):
         """npass passes from the same start with the reference's bookkeeping
         (a0 = lowest, bLow = top Ritz value, kohnShamEigenSolve.cc:483-498).
         With a label, the last Ritz vectors and values are kept for the error
@@ -264,8 +264,10 @@ def main():
     # generalised Rayleigh-Ritz (SUBSPACE PROJ SHEP GPU = false: no CholGS,
     # one rotation by L^-T Q)
     rb_ms, lam_rb, _, ph_rb = run_passes(True, 2, defect=head_defect, top=args.fp64_top,
-                                         rocblas=True)
+                                         rocblas=1)
     log(f"mixed, dense steps on rocBLAS: {rb_ms:.1f} ms/pass; phases {ph_rb[-1]}")
+    hy_ms, _, _, ph_hy = run_passes(True, 2, defect=head_defect, top=args.fp64_top, rocblas=2)
+    log(f"mixed, library grams + rocBLAS rotations: {hy_ms:.1f} ms/pass; phases {ph_hy[-1]}")
     gep_ms, lam_gep, _, ph_gep = run_passes(True, 2, defect=head_defect, top=args.fp64_top,
                                             shep=False)
     log(f"mixed, generalised Rayleigh-Ritz: {gep_ms:.1f} ms/pass; phases {ph_gep[-1]}")
@@ -362,6 +364,8 @@ def main():
         "phase_ms_dense_rocblas_last": ph_rb[-1],
         "max_eig_err_dense_rocblas_vs_mixed_Ha": [float(np.abs(a - b).max())
                                                   for a, b in zip(lam_rb, lam_mx)],
+        "dense_hybrid_pass_ms": round(hy_ms, 2),
+        "phase_ms_dense_hybrid_last": ph_hy[-1],
         "gep_pass_ms": round(gep_ms, 2),
         "phase_ms_gep_last": ph_gep[-1],
         "max_eig_err_gep_vs_mixed_Ha": [float(np.abs(a - b).max())

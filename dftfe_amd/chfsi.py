@@ -482,7 +482,7 @@ class ChebyshevSolverDevice:
                  mixed_prec_defect_filter: bool = False, use_shep: bool = True,
                  lanczos_full_residual_margin: bool = False,
                  is_pseudopotential: bool = False, mixed_prec_fp64_top_columns: int = 0,
-                 dense_rocblas: bool = False):
+                 dense_rocblas: int = 0):
         """mixed_dense: FP32 off-diagonal blocks in the overlap, CholGS rotation,
         X^T H X and Rayleigh-Ritz rotation when a pass runs with
         use_mixed_prec_overall (the reference's USE MIXED PREC CGS O / CGS SR /
@@ -501,8 +501,9 @@ class ChebyshevSolverDevice:
         of a looked-up degree for all-electron runs.
         mixed_prec_fp64_top_columns: with a lower filter precision, the last
         k columns (highest Ritz values) are filtered in FP64.
-        dense_rocblas: FP64 overlap, X^T H X and rotations through rocBLAS
-        instead of the library's MFMA kernels."""
+        dense_rocblas: FP64 dense steps on the library's MFMA kernels (0),
+        on rocBLAS (1 or True), or the library's gram with rocBLAS rotations
+        (2)."""
         self.lib = L.load()
         p = L.Params()
         self.lib.chfsi_default_params(ct.byref(p))

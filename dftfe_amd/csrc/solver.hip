@@ -497,7 +497,7 @@ chfsi_status rotate_fp64(chfsi_solver *s, double *X, int64_t M, int N, const dou
 
 chfsi_status rotate_dense(chfsi_solver *s, double *X, const float *Xf, int64_t M, int N,
                           const double *Q, int bs, hipStream_t st) {
-  if (!Xf && !s->params.dense_rocblas) return rotate_fp64(s, X, M, N, Q, N, 0, 0, st);
+  if (!Xf && s->params.dense_rocblas == 0) return rotate_fp64(s, X, M, N, Q, N, 0, 0, st);
   const int64_t chunk = panel_rows(M, N);
   CHFSI_TRY(s->rot.ensure(sizeof(double) * chunk * N));
   double *tmp = sptr<double>(s->rot);
@@ -540,7 +540,7 @@ chfsi_status rotate_dense(chfsi_solver *s, double *X, const float *Xf, int64_t M
 chfsi_status rotate_lower(chfsi_solver *s, double *X, const float *Xf, int64_t M, int N,
                           const double *Linv, int bs, hipStream_t st) {
   // X L^{-T}: R(i, j) = Linv[j + i N], upper triangular
-  if (!Xf && !s->params.dense_rocblas) return rotate_fp64(s, X, M, N, Linv, N, 1, 1, st);
+  if (!Xf && s->params.dense_rocblas == 0) return rotate_fp64(s, X, M, N, Linv, N, 1, 1, st);
   const int64_t chunk = panel_rows(M, N);
   CHFSI_TRY(s->rot.ensure(sizeof(double) * chunk * N));
   double *tmp = sptr<double>(s->rot);
@@ -585,7 +585,7 @@ chfsi_status rotate_lower(chfsi_solver *s, double *X, const float *Xf, int64_t M
 // in FP64, the blocks below them from the FP32 copy.
 chfsi_status overlap_lower(chfsi_solver *s, const double *X, const float *Xf, int64_t M, int N,
                            int bs, double *S, hipStream_t st) {
-  if (!Xf && !s->params.dense_rocblas) return gram_fp64(s, X, N, X, N, N, N, M, 0, S, N, st);
+  if (!Xf && s->params.dense_rocblas != 1) return gram_fp64(s, X, N, X, N, N, N, M, 0, S, N, st);
   for (int j = 0; j < N; j += bs) {
     const int bw = std::min(bs, N - j);
     double *Sj = S + j + (int64_t)j * N;
@@ -643,7 +643,7 @@ chfsi_status xthx_lower(chfsi_solver *s, chfsi_operator *op, const double *X, in
     HXf = sptr<float>(s->HXf);
   }
   int g0 = 0, gw = 0;  // columns [g0, g0 + gw) of H~X are in HXg (ld gw)
-  if (!Xf && !s->params.dense_rocblas) {
+  if (!Xf && s->params.dense_rocblas != 1) {
     // per group of columns: Hp[:, g0:g0+gw] = X^T (H~X)[:, g0:g0+gw], the
     // tiles that reach the lower triangle
     for (g0 = 0; g0 < N; g0 += G) {

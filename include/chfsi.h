@@ -213,11 +213,13 @@ typedef struct {
                                       forms their rounding, relative to the
                                       column, is the largest.  No reference
                                       counterpart; 0: none                         */
-  int32_t dense_rocblas;           /* 1: the FP64 overlap, X^T H~ X and rotations
-                                      through rocBLAS dgemm; 0: the library's own
-                                      FP64 MFMA kernels (split-K lower-triangle
-                                      gram, row-panel rotation).  The mixed-
-                                      precision dense options use rocBLAS either way */
+  int32_t dense_rocblas;           /* FP64 dense steps: 0: the library's own FP64
+                                      MFMA kernels (split-K lower-triangle gram for
+                                      the overlap and X^T H~ X, row-panel
+                                      rotations); 1: rocBLAS dgemm for all of them;
+                                      2: the library's gram with rocBLAS dgemm for
+                                      the rotations.  The mixed-precision dense
+                                      options use rocBLAS either way             */
 } chfsi_params;
 
 const char *chfsi_last_error(void);

@@ -54,8 +54,9 @@ def test_cholgs_and_rayleigh_ritz(small, N):
 
 
 def test_pass_library_kernels_and_rocblas(small):
-    """One FP64 pass of 200 states in blocks of 64 on each dense route, both
-    within 1e-10 Ha of the oracle."""
+    """One FP64 pass of 200 states in blocks of 64 on each dense route (library
+    kernels, rocBLAS, library grams with rocBLAS rotations), all within 1e-10 Ha
+    of the oracle."""
     from dftfe_amd.chfsi import ChebyshevSolverDevice
 
     prob, ref, op = small
@@ -64,7 +65,7 @@ def test_pass_library_kernels_and_rocblas(small):
     bounds = O.first_call_bounds(lo, up, N, prob.M)
     X = prob.initial_guess(N=N, seed_offset=7)
     lam_o, res_o, _ = O.chfsi_pass(ref, X, *bounds, m, B)
-    for rb in (False, True):
+    for rb in (0, 1, 2):
         solver = ChebyshevSolverDevice(*bounds, cheby_block_size=B, cheby_degree=m,
                                        reuse_lanczos_upper=True, filter_precision=0,
                                        dense_rocblas=rb)
