@@ -219,12 +219,15 @@ def main():
         return s_
 
     def run_passes(mixed: bool, npass: int, dense: bool = False, defect: bool = False,
-                   label=None, top: int = 0, rocblas: int = 0, shep: bool = True):
+                   label=None, top: int = 0, rocblas: int = 0, shep: bool = True,
+                   precision=None):
         """npass passes from the same start with the reference's bookkeeping
         (a0 = lowest, bLow = top Ritz value, kohnShamEigenSolve.cc:483-498).
         With a label, the last Ritz vectors and values are kept for the error
-        checks."""
+        checks; precision: the filter precision of the mixed passes."""
         solver = make_solver(dense, defect, top, rocblas, shep)
+        if precision is not None:
+            solver.set_filter_precision(precision)
         X = X0.clone()
         lams, ress, phases = [], [], []
         torch.cuda.synchronize()
@@ -259,6 +262,10 @@ def main():
     if not args.no_fp64:
         fp64_ms, lam_64, res_64, ph_64 = run_passes(False, args.steps, label="fp64")
         log(f"fp64: {fp64_ms:.1f} ms/pass; last max residual {res_64[-1].max():.3e}")
+    # the mixed form with the filter bulk in BF16 (the schedule's first passes)
+    bf_ms, _, _, ph_bf = run_passes(True, 2, defect=head_defect, top=args.fp64_top,
+                                    precision=L.BF16)
+    log(f"mixed form with BF16 filter: {bf_ms:.1f} ms/pass; phases {ph_bf[-1]}")
     # the headline mixed passes with the dense FP64 steps on rocBLAS (Tensile
     # DGEMM per block) instead of the library's MFMA kernels, and with the
     # generalised Rayleigh-Ritz (SUBSPACE PROJ SHEP GPU = false: no CholGS,
@@ -360,6 +367,9 @@ def main():
                        "lanczos_margin": args.lanczos_margin},
         "mixed_pass_ms": round(mixed_ms, 2),
         "mixed_dense_pass_ms": round(md_ms, 2),
+        "bf16_pass_ms": round(bf_ms, 2),
+        "phase_ms_bf16_last": ph_bf[-1],
+        "bf16_over_fp64_speedup": round(fp64_ms / bf_ms, 3) if fp64_ms else None,
         "dense_rocblas_pass_ms": round(rb_ms, 2),
         "phase_ms_dense_rocblas_last": ph_rb[-1],
         "max_eig_err_dense_rocblas_vs_mixed_Ha": [float(np.abs(a - b).max())
@@ -455,8 +465,9 @@ def main():
 
 
 def convergence_record(prob, op, X0, make_solver, P, head_defect, fp64_top, log):
-    """P passes of FP64, of the timed mixed form and of the plain FP32 filter,
-    side by side from the same start with the same bookkeeping.  Per pass and
+    """P passes of FP64, of the timed mixed form, of the plain FP32 filter and of
+    the mixed form with BF16 filters in passes 1-2, side by side from the same
+    start with the same bookkeeping.  Per pass and
     mixed form: max |dlambda| over all states and over the lowest 90%, the
     quadrature-weighted L2 norm of the density difference and the band-energy
     difference (Fermi-Dirac occupations of 0.8 N electron pairs at kT = 0.01 Ha
@@ -464,20 +475,28 @@ def convergence_record(prob, op, X0, make_solver, P, head_defect, fp64_top, log)
     import torch
     from dftfe_amd.autograd import fermi_occupations
 
+    from dftfe_amd import lib as L
+
     N = X0.shape[1]
     nl = int(0.9 * N)
+    # bf16_schedule: the mixed form with the bulk of the filter in BF16 in
+    # passes 1-2 (BF16 cell matrices and inputs, FP32 accumulation), FP32 after
     forms = {"fp64": (False, make_solver()),
              "mixed": (True, make_solver(defect=head_defect, top=fp64_top)),
-             "plain_fp32": (True, make_solver())}
+             "plain_fp32": (True, make_solver()),
+             "bf16_schedule": (True, make_solver(defect=head_defect, top=fp64_top))}
+    others = ("mixed", "plain_fp32", "bf16_schedule")
     Xs = {k: X0.clone() for k in forms}
     jxw = torch.tensor(prob.JxW, device="cuda")[None, :]
     rec = {"passes": P, "lowest": nl, "fp64_max_residual": [], "fp64_max_residual_lowest": []}
-    for k in ("mixed", "plain_fp32"):
+    for k in others:
         rec[k] = {"max_eig_diff_all_Ha": [], "max_eig_diff_lowest_Ha": [], "density_l2": [],
                   "band_energy_Ha": []}
     for p in range(P):
         lam = {}
         for k, (mixed, solver) in forms.items():
+            if k == "bf16_schedule":
+                solver.set_filter_precision(L.BF16 if p < 2 else L.FP32)
             l, r, _ = solver.solve(op, Xs[k], is_first_filtering_call=(p == 0),
                                    use_mixed_prec_overall=mixed)
             _, _, bUp = solver.bounds()
@@ -489,14 +508,14 @@ def convergence_record(prob, op, X0, make_solver, P, head_defect, fp64_top, log)
         occ, _ = fermi_occupations(torch.tensor(lam["fp64"], device="cuda"), 0.8 * N, 0.01)
         w = (2.0 * occ).cpu()
         rho64 = op.density(Xs["fp64"], w)
-        for k in ("mixed", "plain_fp32"):
+        for k in others:
             d = op.density(Xs[k], w) - rho64
             r_ = rec[k]
             r_["max_eig_diff_all_Ha"].append(float(np.abs(lam[k] - lam["fp64"]).max()))
             r_["max_eig_diff_lowest_Ha"].append(float(np.abs(lam[k][:nl] - lam["fp64"][:nl]).max()))
             r_["density_l2"].append(float(torch.sqrt((d * d * jxw).sum())))
             r_["band_energy_Ha"].append(abs(float((w.numpy() * (lam[k] - lam["fp64"])).sum())))
-    for k in ("mixed", "plain_fp32"):
+    for k in others:
         e = rec[k]["max_eig_diff_all_Ha"]
         rho = rec[k]["density_l2"]
         ok = [i + 1 for i in range(P) if all(x <= 1e-8 for x in e[i:])]
