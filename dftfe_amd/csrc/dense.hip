@@ -171,9 +171,10 @@ __global__ void gram_reduce_kernel(const double *P, const int2 *tiles, int ntile
 
 // Y[k, J + j] = sum_{i < K} X[k, i] R[i, J + j] for the rows [r0, r0 + rows)
 // of the panel and J + j < N; R[i][j] = Rm[i + j ldr] (TR = 0) or
-// Rm[j + i ldr] (TR = 1).  LDS slabs: X as [128 rows][KS + 1] (odd stride: the 16 rows of an A-operand
-// read fall in distinct banks); R as [128 columns][KS + 1] when its columns
-// are contiguous in memory (TR = 0), else [KS rows][128 + PADW].
+// Rm[j + i ldr] (TR = 1).  LDS slabs: X as [128 rows][KS + 1] (odd stride:
+// the 16 rows of an A-operand read fall in distinct banks); R as
+// [128 columns][KS + 1] when its columns are contiguous in memory (TR = 0),
+// else [KS rows][128 + PADW].
 template <int TR>
 __global__ __launch_bounds__(NT, 2) void rotate_kernel(const double *X, int64_t ldx,
                                                        const double *Rm, int64_t ldr, int K,

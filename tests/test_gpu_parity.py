@@ -83,6 +83,24 @@ def test_hx_tile_variants(small, precision, variant):
     assert _rel(Yd.double().cpu().numpy(), want) < (1e-13 if precision == 0 else 2e-6)
 
 
+@pytest.mark.parametrize("B", [2, 71, 130])
+@pytest.mark.parametrize("variant", [6, 7])
+def test_hx_fp64_ring_widths(small, variant, B):
+    """The FP64 ring (variants 6, 7) at the narrowest even width, at an odd
+    width (it takes the register-staged kernel) and across a partial
+    64-column tile: the oracle's H~X to 1e-13."""
+    prob, ref, op = small
+    X = prob.initial_guess(N=B, seed_offset=B + 31)
+    Xd = torch.tensor(X, device="cuda")
+    Yd = torch.zeros_like(Xd)
+    op.set_hx_variant(0, variant)
+    try:
+        op.HX(Xd, False, 1.0, Yd)
+    finally:
+        op.set_hx_variant(0, 0)
+    assert _rel(Yd.cpu().numpy(), ref.apply(X)) < 1e-13
+
+
 @pytest.mark.parametrize("B", [72, 256, 70])
 @pytest.mark.parametrize("variant", [5, 6, 11])
 def test_hx_ring_variants_bitwise(small, variant, B):
