@@ -311,8 +311,8 @@ def main():
         return (time.perf_counter() - t) / npass * 1e3, lams, phases, n_core
 
     # the mixed passes with the plain FP32 filter (no defect form, no FP64 columns)
-    df_ms, lam_df, res_df, ph_df = run_passes(True, args.steps, defect=not head_defect,
-                                              label="other")
+    df_ms, lam_df, _, ph_df = run_passes(True, args.steps, defect=not head_defect,
+                                         label="other")
     errors = density_band_errors(prob, op, finals, log) if "fp64" in finals else None
     finals.clear()
     log(f"mixed, {'plain' if head_defect else 'defect-form'} filter: {df_ms:.1f} ms/pass")
