@@ -162,14 +162,14 @@ class KohnShamOperatorDevice:
     def autotune(self, B: int) -> dict:
         """Time every HX tile variant on a B-column block and keep the fastest."""
         b64, b32 = ct.c_int32(), ct.c_int32()
-        t = (ct.c_float * 40)()
+        t = (ct.c_float * 48)()
         L.check(self.lib.chfsi_operator_autotune(self.handle, B, ct.byref(b64), ct.byref(b32), t,
                                                  _stream()))
         return {"fp64": b64.value, "fp32": b32.value,
                 "fp64_ms": [x for x in t[0:16] if x >= 0],
-                "fp32_ms": [x for x in t[20:36] if x >= 0],
-                "fp64_proj_ms": [x for x in t[16:20] if x >= 0],
-                "fp32_proj_ms": [x for x in t[36:40] if x >= 0]}
+                "fp32_ms": [x for x in t[24:40] if x >= 0],
+                "fp64_proj_ms": [x for x in t[16:24] if x >= 0],
+                "fp32_proj_ms": [x for x in t[40:48] if x >= 0]}
 
     def getSqrtMassVec(self) -> int:
         return self.lib.chfsi_operator_sqrt_mass(self.handle)

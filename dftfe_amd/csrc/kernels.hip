@@ -843,7 +843,9 @@ __global__ __launch_bounds__(64 * NW) void hx_bf16_kernel(HxLaunch<float> a, con
 // ---------------------------------------------------------------------------
 // ST node tiles of 16 per stage (one barrier per stage); accumulators for
 // MAXSRB slot blocks of 16 (the largest slot count of any cell, rounded up).
-template <typename T, int NW, int TN, int MAXSRB, int NRB, int ST>
+// OWN: the rows are the cell's owned rows (own_map, own_n) instead of its
+// nodes, and only their tiles are visited.
+template <typename T, int NW, int TN, int MAXSRB, int NRB, int ST, bool OWN = false>
 __global__ __launch_bounds__(64 * NW) void proj_kernel(ProjLaunch<T> a) {
   using v4 = typename V4<T>::t;
   constexpr int NT = 64 * NW;
@@ -877,12 +879,16 @@ __global__ __launch_bounds__(64 * NW) void proj_kernel(ProjLaunch<T> a) {
   const int nsrb = nslot >> 4;
   const int nchunk = nslot * 4;
   const T *Acell = a.nl_Ct + (int64_t)s0 * ROWS;
-  const uint32_t *cmap = a.map + (int64_t)cell * a.ne;
+  const uint32_t *cmap = OWN ? a.own_map + (int64_t)cell * ROWS : a.map + (int64_t)cell * a.ne;
+  const int nrow = OWN ? a.own_n[cell] : a.ne;
+  // stages holding rows (the tiles past the last row of an owned-row cell
+  // are zero in nl_Ct and read row 0 with scale 0)
+  const int nstage = OWN ? (((nrow + 15) >> 4) + ST - 1) / ST : NSTAGE;
 
   for (int i = tid; i < ROWS; i += NT) {
     uint32_t w = 0;
     T sc = T(0);
-    if (i < a.ne) {
+    if (i < nrow) {
       w = cmap[i];
       sc = a.s_in ? a.s_in[w & 0x7fffffffu] : T(1);
     }
@@ -940,9 +946,9 @@ __global__ __launch_bounds__(64 * NW) void proj_kernel(ProjLaunch<T> a) {
   load_stage(0);
   store_stage(0);
   __syncthreads();
-  for (int sg = 0; sg < NSTAGE; ++sg) {
+  for (int sg = 0; sg < nstage; ++sg) {
     const int buf = sg & 1;
-    if (sg + 1 < NSTAGE) load_stage(sg + 1);
+    if (sg + 1 < nstage) load_stage(sg + 1);
 #pragma unroll
     for (int u = 0; u < ST; ++u) {
       v4 b4[CBW];
@@ -962,7 +968,7 @@ __global__ __launch_bounds__(64 * NW) void proj_kernel(ProjLaunch<T> a) {
         }
       }
     }
-    if (sg + 1 < NSTAGE) store_stage(buf ^ 1);
+    if (sg + 1 < nstage) store_stage(buf ^ 1);
     __syncthreads();
   }
 #pragma unroll
@@ -2171,8 +2177,9 @@ hipError_t launch_f64_to_bf16(const double *x, int64_t n, uint16_t *y, hipStream
 // matrices) and the 16 gathered source rows ([k][column] groups).  All waves
 // split the columns and hold every slot block of the cell.  The products and
 // their order are proj_kernel's (B operand X*s, four 16x16x4 MFMAs per slice),
-// so the partials agree bit for bit.  Requirements as hxp_kernel's.
-template <int NW, int TN, int MAXSRB, int NRB, int S>
+// so the partials agree bit for bit.  Requirements as hxp_kernel's.  OWN:
+// owned rows instead of the cell's nodes, as in proj_kernel.
+template <int NW, int TN, int MAXSRB, int NRB, int S, bool OWN = false>
 __global__ __launch_bounds__(64 * NW) void projp_kernel(ProjLaunch<float> a) {
   using T = float;
   using v4 = f32x4;
@@ -2215,12 +2222,14 @@ __global__ __launch_bounds__(64 * NW) void projp_kernel(ProjLaunch<float> a) {
   const int nsrb = nslot >> 4;
   const int nops = nsrb + NX;
   const T *Acell = a.nl_Ct + (int64_t)s0 * ROWS;
-  const uint32_t *cmap = a.map + (int64_t)cell * a.ne;
+  const uint32_t *cmap = OWN ? a.own_map + (int64_t)cell * ROWS : a.map + (int64_t)cell * a.ne;
+  const int nrow = OWN ? a.own_n[cell] : a.ne;
+  const int nt = OWN ? (nrow > 0 ? (nrow + 15) >> 4 : 1) : NRB;  // K slices visited
 
   for (int i = tid; i < ROWS; i += NT) {
     int64_t off = 0;
     T sc = T(0);
-    if (i < a.ne) {
+    if (i < nrow) {
       const uint32_t r = cmap[i] & 0x7fffffffu;
       off = (int64_t)r * a.ld_src;
       sc = a.s_in ? a.s_in[r] : T(1);
@@ -2290,12 +2299,12 @@ __global__ __launch_bounds__(64 * NW) void projp_kernel(ProjLaunch<float> a) {
   };
 
 #pragma unroll
-  for (int st = 0; st < S - 1; ++st) issue(st < NRB ? st : NRB - 1, st);
-  for (int t = 0; t < NRB; ++t) {
+  for (int st = 0; st < S - 1; ++st) issue(st < nt ? st : nt - 1, st);
+  for (int t = 0; t < nt; ++t) {
     wait_vmcnt<(S - 2) * D>();
     __syncthreads();
     const int tn = t + S - 1;
-    issue(tn < NRB ? tn : NRB - 1, tn % S);
+    issue(tn < nt ? tn : nt - 1, tn % S);
     compute(t % S, t);
   }
   wait_vmcnt<0>();  // no copy may still target LDS when the workgroup ends
@@ -2316,25 +2325,190 @@ __global__ __launch_bounds__(64 * NW) void projp_kernel(ProjLaunch<float> a) {
   }
 }
 
-template <int NW, int TN, int MAXSRB, int NRB, int S>
+// FP64 projection on the direct-to-LDS ring of hxd_kernel.  A stage holds
+// half of a 16-wide K slice: the cell's slot rows of nl_Ct (8 doubles = 64
+// bytes of each, one 1 KB copy per 16 slots, 16-byte units swizzled as for
+// the cell matrices) and the 8 gathered source rows of TN = 128 columns (one
+// 1 KB copy per row, the copies XG bytes apart so that the B-operand reads of
+// the four lane groups fall in two 128-byte bank windows).  Four waves split
+// the columns (32 each) and hold every slot block of the cell.  In the r-th
+// of the two MFMAs of a half slice lane l supplies K row 2 (l/16) + r.  OWN:
+// owned rows instead of the cell's nodes, as in proj_kernel.  Requirements
+// (checked by the launcher): B even, 16-byte aligned src rows (ld_src even)
+// and nl_Ct.
+template <int MAXSRB, int NRB, int S, bool OWN>
+__global__ __launch_bounds__(256) void projd_kernel(ProjLaunch<double> a) {
+  using T = double;
+  using v4 = f64x4;
+  constexpr int NW = 4;
+  constexpr int NT = 64 * NW;
+  constexpr int TN = 128;
+  constexpr int CBW = TN / 16 / NW;
+  constexpr int ROWS = NRB * 16;
+  constexpr int XG = 1024 + 64;  // bytes per gathered copy incl. spacing
+  constexpr int NX = 8;          // gathered copies (K rows) per half slice
+  constexpr int AB = MAXSRB * 1024;
+  constexpr int SB = AB + NX * XG;
+  constexpr int D = (MAXSRB + NX + NW - 1) / NW;  // copies per wave per stage
+  static_assert(S >= 2 && (S - 2) * D < 64, "ring depth");
+
+  __shared__ __align__(16) unsigned char ring[S * SB];
+  __shared__ __align__(16) int64_t soff[ROWS];
+  __shared__ __align__(16) T sscl[ROWS];
+
+  // XCD-aware: consecutive cells run on one XCD, the column tiles of a cell
+  // next to each other (the cell's slot rows of nl_Ct stay in its L2)
+  const int wid = blockIdx.x;
+  const int local = wid >> 3;
+  if (local >= a.per_xcd) return;
+  const int item = (wid & 7) * a.per_xcd + local;
+  if (item >= a.ncells * a.ntiles) return;
+  const int cell = a.cells[item / a.ntiles];
+  const int v0 = (item % a.ntiles) * TN;
+  const int tid = threadIdx.x;
+  __builtin_assume(tid < NT);
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wave_s = __builtin_amdgcn_readfirstlane(wave);
+  const int s0 = a.nl_off[cell];
+  const int nslot = a.nl_off[cell + 1] - s0;
+  const int nsrb = nslot >> 4;
+  const int nops = nsrb + NX;
+  const T *Acell = a.nl_Ct + (int64_t)s0 * ROWS;
+  const uint32_t *cmap = OWN ? a.own_map + (int64_t)cell * ROWS : a.map + (int64_t)cell * a.ne;
+  const int nrow = OWN ? a.own_n[cell] : a.ne;
+  const int nt = OWN ? (nrow > 0 ? (nrow + 15) >> 4 : 1) : NRB;  // K slices visited
+  const int nh = 2 * nt;                                         // half slices
+
+  for (int i = tid; i < ROWS; i += NT) {
+    int64_t off = 0;
+    T sc = T(0);
+    if (i < nrow) {
+      const uint32_t r = cmap[i] & 0x7fffffffu;
+      off = (int64_t)r * a.ld_src;
+      sc = a.s_in ? a.s_in[r] : T(1);
+    }
+    soff[i] = off;
+    sscl[i] = sc;
+  }
+  __syncthreads();
+
+  const uint32_t ring0 = lds_address(ring);
+  const int hr = lane >> 2;  // slot row within a copy's 16 rows
+  const int hoff = hr * 16 + 2 * ((lane & 3) ^ (((hr >> 1) ^ (hr >> 3)) & 3));
+  const int xc = v0 + 2 * lane;
+  const int xcl = xc < a.B - 2 ? xc : a.B - 2;  // columns >= B are never stored
+  const T *xb = a.src + xcl;
+
+  auto issue = [&](int h, int stage) {
+    const int t = h >> 1, half = h & 1;
+    const T *tile = Acell + (int64_t)t * nslot * 16 + half * 8;
+    const uint32_t sbase = ring0 + stage * SB;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const int o = (wave_s + NW * j) % nops;  // wave-uniform
+      if (o < nsrb) {
+        copy16_to_lds(tile + o * 256 + hoff, sbase + o * 1024);
+      } else {
+        const int x = o - nsrb;
+        copy16_to_lds(xb + soff[t * 16 + half * 8 + x], sbase + AB + x * XG);
+      }
+    }
+  };
+
+  v4 acc[MAXSRB][CBW];
+#pragma unroll
+  for (int j = 0; j < MAXSRB; ++j)
+#pragma unroll
+    for (int cb = 0; cb < CBW; ++cb) acc[j][cb] = v4{0, 0, 0, 0};
+
+  const int arow = lane & 15;
+  const int g = lane >> 4;  // K rows 2g, 2g + 1 of the half slice
+  const int aoff = arow * 64 + ((g ^ (((arow >> 1) ^ (arow >> 3)) & 3)) * 16);  // bytes
+  const int xo0 = 2 * g * XG, xo1 = xo0 + XG;  // copies of K rows 2g, 2g + 1
+
+  auto compute = [&](int stage, int h) {
+    const unsigned char *As = ring + stage * SB;
+    const T *X0 = reinterpret_cast<const T *>(ring + stage * SB + AB + xo0);
+    const T *X1 = reinterpret_cast<const T *>(ring + stage * SB + AB + xo1);
+    const int k0 = (h >> 1) * 16 + (h & 1) * 8 + 2 * g;
+    const T sc0 = sscl[k0], sc1 = sscl[k0 + 1];
+    T b0[CBW], b1[CBW];
+#pragma unroll
+    for (int cb = 0; cb < CBW; ++cb) {
+      const int col = (wave * CBW + cb) * 16 + (lane & 15);
+      b0[cb] = X0[col] * sc0;
+      b1[cb] = X1[col] * sc1;
+    }
+#pragma unroll
+    for (int j = 0; j < MAXSRB; ++j) {
+      if (j < nsrb) {
+        const double2 a2 = *reinterpret_cast<const double2 *>(As + j * 1024 + aoff);
+#pragma unroll
+        for (int cb = 0; cb < CBW; ++cb) acc[j][cb] = mfma16(a2.x, b0[cb], acc[j][cb]);
+#pragma unroll
+        for (int cb = 0; cb < CBW; ++cb) acc[j][cb] = mfma16(a2.y, b1[cb], acc[j][cb]);
+      }
+    }
+  };
+
+#pragma unroll
+  for (int st = 0; st < S - 1; ++st) issue(st < nh ? st : nh - 1, st);
+  for (int h = 0; h < nh; ++h) {
+    wait_vmcnt<(S - 2) * D>();
+    __syncthreads();
+    const int hn = h + S - 1;
+    issue(hn < nh ? hn : nh - 1, hn % S);
+    compute(h % S, h);
+  }
+  wait_vmcnt<0>();  // no copy may still target LDS when the workgroup ends
+
+#pragma unroll
+  for (int j = 0; j < MAXSRB; ++j) {
+    if (j >= nsrb) continue;
+#pragma unroll
+    for (int cb = 0; cb < CBW; ++cb) {
+      const int v = v0 + (wave * CBW + cb) * 16 + (lane & 15);
+      if (v >= a.B) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int srow = j * 16 + acc_row<T>(lane, r);
+        a.partial[(int64_t)(s0 + srow) * a.ldP + v] = acc[j][cb][r];
+      }
+    }
+  }
+}
+
+template <int MAXSRB, int NRB, int S, bool OWN>
+static hipError_t projd_go(ProjLaunch<double> a, hipStream_t s) {
+  a.ntiles = (a.B + 127) / 128;
+  const int items = a.ncells * a.ntiles;
+  if (items == 0) return hipSuccess;
+  a.per_xcd = (items + 7) / 8;
+  hipLaunchKernelGGL((projd_kernel<MAXSRB, NRB, S, OWN>), dim3(8 * a.per_xcd), dim3(256), 0, s,
+                     a);
+  return hipGetLastError();
+}
+
+template <int NW, int TN, int MAXSRB, int NRB, int S, bool OWN = false>
 static hipError_t projp_go(ProjLaunch<float> a, hipStream_t s) {
   a.ntiles = (a.B + TN - 1) / TN;
   const int items = a.ncells * a.ntiles;
   if (items == 0) return hipSuccess;
   a.per_xcd = (items + 7) / 8;
-  hipLaunchKernelGGL((projp_kernel<NW, TN, MAXSRB, NRB, S>), dim3(8 * a.per_xcd), dim3(64 * NW),
-                     0, s, a);
+  hipLaunchKernelGGL((projp_kernel<NW, TN, MAXSRB, NRB, S, OWN>), dim3(8 * a.per_xcd),
+                     dim3(64 * NW), 0, s, a);
   return hipGetLastError();
 }
 
-template <typename T, int NW, int TN, int MAXSRB, int NRB, int ST>
+template <typename T, int NW, int TN, int MAXSRB, int NRB, int ST, bool OWN = false>
 static hipError_t proj_go(ProjLaunch<T> a, hipStream_t s) {
   a.ntiles = (a.B + TN - 1) / TN;
   const int items = a.ncells * a.ntiles;
   if (items == 0) return hipSuccess;
   a.per_xcd = (items + 7) / 8;
-  hipLaunchKernelGGL((proj_kernel<T, NW, TN, MAXSRB, NRB, ST>), dim3(8 * a.per_xcd), dim3(64 * NW),
-                     0, s, a);
+  hipLaunchKernelGGL((proj_kernel<T, NW, TN, MAXSRB, NRB, ST, OWN>), dim3(8 * a.per_xcd),
+                     dim3(64 * NW), 0, s, a);
   return hipGetLastError();
 }
 
@@ -2343,11 +2517,30 @@ static hipError_t proj_go(ProjLaunch<T> a, hipStream_t s) {
 // accumulators sized to the largest cell; 2: 2 waves x half-width tile, two
 // node tiles per stage, sized accumulators; 3 (FP32; FP64 and unaligned
 // blocks take 0): 8 waves x 256 columns (128 for blocks of at most 256),
-// direct-to-LDS ring of 3 slices.
+// direct-to-LDS ring of 3 slices; 4: the owned-row form (each DOF row read by
+// as few cells as cover its projectors) on the direct-to-LDS rings (FP32:
+// variant 3's kernel; FP64: projd_kernel, 128 columns, 3 half slices;
+// unaligned blocks: variant 5's kernel); 5: the owned-row form on variant 1's
+// register-staged kernel.
 template <typename T, int NRB, int MAXSRB>
 static hipError_t proj_variant(const ProjLaunch<T> &a, hipStream_t s) {
   constexpr int TNW = sizeof(T) == 4 ? 128 : 64;
   constexpr int ST = NRB % 2 == 0 ? 2 : 1;
+  if (a.variant >= 4) {
+    if (!a.own_map || !a.own_n) return hipErrorInvalidValue;
+    auto al = [](const void *p) { return ((uintptr_t)p & 15) == 0; };
+    if (a.variant == 4) {
+      if constexpr (sizeof(T) == 4) {
+        if (a.B >= 4 && a.B % 4 == 0 && a.ld_src % 4 == 0 && al(a.src) && al(a.nl_Ct))
+          return a.B > 256 ? projp_go<8, 256, MAXSRB, NRB, 3, true>(a, s)
+                           : projp_go<8, 128, MAXSRB, NRB, 3, true>(a, s);
+      } else {
+        if (a.B >= 2 && a.B % 2 == 0 && a.ld_src % 2 == 0 && al(a.src) && al(a.nl_Ct))
+          return projd_go<MAXSRB, NRB, 3, true>(a, s);
+      }
+    }
+    return proj_go<T, 4, TNW, MAXSRB, NRB, ST, true>(a, s);
+  }
   if constexpr (sizeof(T) == 4) {
     auto al = [](const void *p) { return ((uintptr_t)p & 15) == 0; };
     if (a.variant == 3 && a.B >= 4 && a.B % 4 == 0 && a.ld_src % 4 == 0 && al(a.src) &&
@@ -2369,7 +2562,7 @@ static hipError_t proj_fit(const ProjLaunch<T> &a, hipStream_t s) {
   return proj_variant<T, NRB, 8>(a, s);
 }
 
-int proj_variants() { return 4; }
+int proj_variants() { return 6; }
 
 template <typename T>
 hipError_t launch_proj(const ProjLaunch<T> &a, hipStream_t s) {

@@ -74,6 +74,10 @@ struct ProjLaunch {
   int B;
   int max_srb;             // largest slot count of any cell / 16 (1..8)
   int variant;             // projection variant (see proj_variants)
+  // owned-row form (variant 4): rows own_map[c * ne_pad + r], r < own_n[c],
+  // replace the cell's nodes, and nl_Ct holds the assembled values on them
+  const uint32_t *own_map;
+  const int *own_n;
 };
 
 template <typename T>

@@ -79,7 +79,7 @@ chfsi_operator::~chfsi_operator() {
                   d_nl_cells, d_c_rows, d_c_ptr, d_c_cols, d_c_vals, d_m_rows, d_m_ptr,
                   d_m_slaves, d_m_vals, d_Mc, d_Gd, d_Hk64, d_Hk32, d_nlCk64, d_nlCtk64,
                   d_nlCk32, d_nlCtk32, d_invJ, d_s_gather64, d_ones_cons64, d_s_gather32,
-                  d_ones_cons32};
+                  d_ones_cons32, d_own_map, d_own_n, d_nlCto64, d_nlCto32};
   for (void *p : ptrs)
     if (p) hipFree(p);
   ztab_dev.release();
@@ -330,6 +330,98 @@ extern "C" chfsi_status chfsi_operator_create(const chfsi_operator_desc *d, chfs
       proj_ptr[a + 1] = (int)proj_slots.size();
     }
     op->n_nl_cells = (int)nl_cells.size();
+    // ---- owned-row form.  C^T x = sum_g c[g] x[g] with c[g] the projector
+    // value assembled over every cell node mapped to DOF g; the cell form
+    // above reads x[g] once per cell node (343 nodes per cell at p = 6 for
+    // 216 DOFs).  Here each DOF g of the nonlocal cells is read by as few of
+    // its cells as cover its projectors: the cell whose projectors cover most
+    // of the ones left takes row g and, in its existing slots, their assembled
+    // values (zero in slots whose projector another cell took).  Slots, their
+    // reduction and the apply are unchanged.
+    {
+      std::vector<int64_t> gptr(M + 1, 0);
+      for (int c : nl_cells)
+        for (int i = 0; i < ne; ++i) ++gptr[(map[size_t(c) * ne + i] & 0x7fffffffu) + 1];
+      for (int64_t g = 0; g < M; ++g) gptr[g + 1] += gptr[g];
+      std::vector<int> inc_cell(gptr[M]), inc_node(gptr[M]);
+      {
+        std::vector<int64_t> fill(gptr.begin(), gptr.end() - 1);
+        for (int c : nl_cells)
+          for (int i = 0; i < ne; ++i) {
+            const int64_t k = fill[map[size_t(c) * ne + i] & 0x7fffffffu]++;
+            inc_cell[k] = c;
+            inc_node[k] = i;
+          }
+      }
+      std::vector<uint32_t> own_map(size_t(C) * ne_pad, 0u);
+      std::vector<int> own_n(C, 0);
+      std::vector<double> nlCto(nlCt.size(), 0.0);
+      std::vector<int> keys, cands;
+      std::vector<double> vals;
+      std::vector<char> done;
+      for (int64_t g = 0; g < M; ++g) {
+        if (gptr[g] == gptr[g + 1]) continue;
+        keys.clear();
+        vals.clear();
+        cands.clear();
+        for (int64_t k = gptr[g]; k < gptr[g + 1]; ++k) {
+          const int c = inc_cell[k], i = inc_node[k];
+          if (std::find(cands.begin(), cands.end(), c) == cands.end()) cands.push_back(c);
+          for (int p : pairs_of[c]) {
+            const int key = d->pair_proj0[p];
+            size_t q = std::find(keys.begin(), keys.end(), key) - keys.begin();
+            if (q == keys.size()) {
+              keys.push_back(key);
+              vals.resize(vals.size() + P, 0.0);
+            }
+            for (int al = 0; al < P; ++al) vals[q * P + al] += d->pair_C[(size_t(p) * ne + i) * P + al];
+          }
+        }
+        std::sort(cands.begin(), cands.end());
+        done.assign(keys.size(), 1);
+        int left = 0;
+        for (size_t q = 0; q < keys.size(); ++q)
+          for (int al = 0; al < P; ++al)
+            if (vals[q * P + al] != 0.0 && done[q]) {
+              done[q] = 0;
+              ++left;
+            }
+        while (left > 0) {
+          int best = -1, best_cnt = 0;
+          for (int c : cands) {
+            int cnt = 0;
+            for (int p : pairs_of[c]) {
+              const size_t q = std::find(keys.begin(), keys.end(), d->pair_proj0[p]) - keys.begin();
+              cnt += !done[q];
+            }
+            if (cnt > best_cnt) {
+              best_cnt = cnt;
+              best = c;
+            }
+          }
+          const int r = own_n[best]++;
+          own_map[size_t(best) * ne_pad + r] = (uint32_t)g;
+          const int s0 = nl_off[best], ns = nl_off[best + 1] - s0;
+          int local = 0;
+          for (int p : pairs_of[best]) {
+            const size_t q = std::find(keys.begin(), keys.end(), d->pair_proj0[p]) - keys.begin();
+            if (!done[q]) {
+              for (int al = 0; al < P; ++al)
+                nlCto[size_t(s0) * ne_pad + size_t(r / 16) * ns * 16 + size_t(local + al) * 16 +
+                      (r % 16)] = vals[q * P + al];
+              done[q] = 1;
+              --left;
+            }
+            local += P;
+          }
+        }
+      }
+      CHFSI_TRY(dev_upload(&op->d_own_map, own_map.data(), own_map.size()));
+      CHFSI_TRY(dev_upload(&op->d_own_n, own_n.data(), own_n.size()));
+      CHFSI_TRY(dev_upload(&op->d_nlCto64, nlCto.data(), nlCto.size()));
+      std::vector<float> o32(nlCto.begin(), nlCto.end());
+      CHFSI_TRY(dev_upload(&op->d_nlCto32, o32.data(), o32.size()));
+    }
     std::vector<float> tmp32(nlC.size());
     CHFSI_TRY(dev_upload(&op->d_nl_off, nl_off.data(), nl_off.size()));
     CHFSI_TRY(dev_upload(&op->d_slot_proj, slot_proj.data(), slot_proj.size()));
@@ -483,6 +575,7 @@ struct OpData<double> {
   static const double *Ht(chfsi_operator *op) { return op->d_Ht64; }
   static const double *nlC(chfsi_operator *op) { return op->d_nlC64; }
   static const double *nlCt(chfsi_operator *op) { return op->d_nlCt64; }
+  static const double *nlCto(chfsi_operator *op) { return op->d_nlCto64; }
   static const double *s(chfsi_operator *op) { return op->d_inv_sqrt_mass; }
   static Scratch &partial(chfsi_operator *op) { return op->partial64; }
   static Scratch &W(chfsi_operator *op) { return op->W64; }
@@ -494,6 +587,7 @@ struct OpData<float> {
   static const float *Ht(chfsi_operator *op) { return op->d_Ht32; }
   static const float *nlC(chfsi_operator *op) { return op->d_nlC32; }
   static const float *nlCt(chfsi_operator *op) { return op->d_nlCt32; }
+  static const float *nlCto(chfsi_operator *op) { return op->d_nlCto32; }
   static const float *s(chfsi_operator *op) { return op->d_inv_sqrt_mass32; }
   static Scratch &partial(chfsi_operator *op) { return op->partial32; }
   static Scratch &W(chfsi_operator *op) { return op->W32; }
@@ -564,6 +658,11 @@ chfsi_status hx_fused(chfsi_operator *op, const T *src, int64_t ld_src, T *dst, 
     pl.B = B;
     pl.max_srb = op->max_srb;
     pl.variant = sizeof(T) == 8 ? op->proj_variant64 : op->proj_variant32;
+    if (pl.variant >= 4) {
+      pl.nl_Ct = OpData<T>::nlCto(op);
+      pl.own_map = op->d_own_map;
+      pl.own_n = op->d_own_n;
+    }
     CHFSI_HIP(launch_proj<T>(pl, s));
     CHFSI_HIP(launch_proj_reduce<T>(op->d_proj_ptr, op->d_proj_slots, op->d_proj_V, op->n_proj,
                                     Pp, B, W, B, B, s));
@@ -744,7 +843,7 @@ extern "C" chfsi_status chfsi_operator_set_proj_variant(chfsi_operator *op, int3
 namespace {
 template <typename T>
 chfsi_status time_variants(chfsi_operator *op, int B, int *best, float *times, hipStream_t st) {
-  // times[0..15]: tile variants, times[16..19]: projection variants
+  // times[0..15]: tile variants, times[16..23]: projection variants
   const int nv = hx_variants(sizeof(T) == 8) < 16 ? hx_variants(sizeof(T) == 8) : 16;
   Scratch x, y;
   CHFSI_TRY(x.ensure(sizeof(T) * op->M * B));
@@ -782,7 +881,7 @@ chfsi_status time_variants(chfsi_operator *op, int B, int *best, float *times, h
     int &pv = sizeof(T) == 8 ? op->proj_variant64 : op->proj_variant32;
     float best_p = 1e30f;
     int bp = 0;
-    for (int v = 0; v < proj_variants() && v < 4; ++v) {
+    for (int v = 0; v < proj_variants() && v < 8; ++v) {
       pv = v;
       const T *xs = static_cast<const T *>(x.ptr);
       T *ys = static_cast<T *>(y.ptr);
@@ -816,15 +915,15 @@ extern "C" chfsi_status chfsi_operator_autotune(chfsi_operator *op, int32_t B, i
                                                 int32_t *best32, float *times_ms, void *stream) {
   CHFSI_ARG(op && B > 0, "bad argument");
   hipStream_t st = as_stream(stream);
-  float t[40];
+  float t[48];
   for (float &v : t) v = -1.0f;
   int b64 = 0, b32 = 0;
   CHFSI_TRY(time_variants<double>(op, B, &b64, t, st));
-  CHFSI_TRY(time_variants<float>(op, B, &b32, t + 20, st));
+  CHFSI_TRY(time_variants<float>(op, B, &b32, t + 24, st));
   if (best64) *best64 = b64;
   if (best32) *best32 = b32;
   if (times_ms)
-    for (int i = 0; i < 40; ++i) times_ms[i] = t[i];
+    for (int i = 0; i < 48; ++i) times_ms[i] = t[i];
   return CHFSI_OK;
 }
 

@@ -79,6 +79,14 @@ struct chfsi_operator {
       *d_proj_slots = nullptr, *d_nl_cells = nullptr;
   double *d_nlC64 = nullptr, *d_nlCt64 = nullptr, *d_proj_V = nullptr;
   float *d_nlC32 = nullptr, *d_nlCt32 = nullptr;
+  // owned-row form of the projection (projection variant 4): per nonlocal
+  // cell the DOFs whose (DOF, projector) terms that cell forms (own_n[c] of
+  // them, own_map[c * ne_pad + r]) and the assembled projector values on
+  // those rows, in the slot layout of nlCt
+  uint32_t *d_own_map = nullptr;
+  int *d_own_n = nullptr;
+  double *d_nlCto64 = nullptr;
+  float *d_nlCto32 = nullptr;
   chfsi::Scratch partial64, partial32, W64, W32, tmp;
   // HX tile and projection variants per precision (chosen by chfsi_operator_autotune)
   int hx_variant64 = 0, hx_variant32 = 0;

@@ -263,9 +263,12 @@ chfsi_status chfsi_operator_get_cell_matrices(chfsi_operator *op, double *H_dev,
  * reference calls hipBLAS gemmStridedBatched).  chfsi_operator_autotune times
  * every tile variant of both precisions on a B-column block, keeps the fastest,
  * then times the projection-kernel variants with it and keeps the fastest.
- * times_ms (40 entries) [0..15]: FP64 tile variants, [16..19]: FP64 projection
- * variants, [20..35]: FP32 tile variants, [36..39]: FP32 projection variants
- * (ms per HX; -1 where a variant does not exist).  precision CHFSI_BF16 selects
+ * times_ms (48 entries) [0..15]: FP64 tile variants, [16..23]: FP64 projection
+ * variants, [24..39]: FP32 tile variants, [40..47]: FP32 projection variants
+ * (ms per HX; -1 where a variant does not exist).  Projection variants 4
+ * (direct-to-LDS rings) and 5 (register-staged) read each DOF row once per
+ * covering cell (owned rows with assembled projector values) instead of once
+ * per cell node.  precision CHFSI_BF16 selects
  * the BF16 apply's kernel: K = 32 MFMA on the direct-to-LDS ring with 0
  * (default) 8 waves and 2 stages (two workgroups per CU), 2: 8 waves and 3
  * stages, 3: 16 waves and 3 stages; 1: the register-staged K = 16 form. */

@@ -88,7 +88,7 @@ def test_hx_variants_flags_and_cheby(gm):
     D = prob.initial_guess(N=64, seed_offset=22)
     want = {0: ref.apply(X), 1: ref.apply(X.astype(np.float32).astype(np.float64))}
     bad = []
-    for prec, nvar, kind in ((0, 5, "hx"), (1, 12, "hx"), (0, 4, "proj"), (1, 4, "proj")):
+    for prec, nvar, kind in ((0, 5, "hx"), (1, 12, "hx"), (0, 6, "proj"), (1, 6, "proj")):
         setter = op.set_hx_variant if kind == "hx" else op.set_proj_variant
         dt = torch.float64 if prec == 0 else torch.float32
         for v in range(nvar):

@@ -154,7 +154,7 @@ def test_projection_ring_bitwise(small, B):
     assert torch.equal(out[0], out[3])
 
 
-@pytest.mark.parametrize("precision,variant", [(p, v) for p in (0, 1) for v in range(4)])
+@pytest.mark.parametrize("precision,variant", [(p, v) for p in (0, 1) for v in range(6)])
 def test_projection_variants(small, precision, variant):
     """Every projection-kernel variant gives the oracle's H~X."""
     prob, ref, op = small
@@ -169,6 +169,39 @@ def test_projection_variants(small, precision, variant):
         op.set_proj_variant(precision, 0)
     want = ref.apply(X if precision == 0 else X.astype(np.float32).astype(np.float64))
     assert _rel(Yd.double().cpu().numpy(), want) < (1e-13 if precision == 0 else 2e-6)
+
+
+@pytest.mark.parametrize("name,mode", [("tiny", "unique"), ("brute", "unique"),
+                                       ("cube20_p4_n64", "dealii")])
+def test_projection_owned_rows(name, mode):
+    """Projection variants 4 (direct-to-LDS rings) and 5 (register-staged) form
+    C^T x over DOF rows with the projector values assembled over the cell
+    nodes, each row read by the cells that cover its projectors.  They give the
+    oracle's H~X on a mesh of 2 cells per side (a cell's opposite faces map to
+    the same DOFs), on p = 3, and with the periodic rows kept as constraint
+    rows (dealii layout), in both precisions, for one column (B = 1, the
+    register-staged fallback), B = 6 (FP64 ring, FP32 fallback) and B = 40."""
+    _dev()
+    from dftfe_amd.chfsi import KohnShamOperatorDevice
+
+    prob = fe.build_workload(name, mode=mode)
+    assert len(prob.pair_cell) > 0
+    ref = O.OracleOperator(prob)
+    op = KohnShamOperatorDevice(prob)
+    for B in (1, 6, 40):
+        X = prob.initial_guess(N=B, seed_offset=70 + B)
+        for prec, dt, tol in ((0, torch.float64, 1e-13), (1, torch.float32, 2e-6)):
+            want = ref.apply(X if prec == 0 else X.astype(np.float32).astype(np.float64))
+            Xd = torch.tensor(X, device="cuda", dtype=dt)
+            for v in (4, 5):
+                Yd = torch.zeros_like(Xd)
+                op.set_proj_variant(prec, v)
+                try:
+                    op.HX(Xd, False, 1.0, Yd)
+                finally:
+                    op.set_proj_variant(prec, 0)
+                err = _rel(Yd.double().cpu().numpy(), want)
+                assert err < tol, (name, mode, B, prec, v, err)
 
 
 def test_hx_strided_block_and_flags(small):

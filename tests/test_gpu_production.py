@@ -86,14 +86,14 @@ def test_hx_fp64_256_columns(big):
 
 
 def test_hx_variants(big):
-    """Every HX tile variant (FP64 0-7, FP32 0-11) and projection variant (0-3
+    """Every HX tile variant (FP64 0-7, FP32 0-11) and projection variant (0-5
     per precision) on 64 columns."""
     prob, ref, op, gold = big
     X = prob.initial_guess(N=64, seed_offset=62)
     want = {0: ref.apply(X), 1: ref.apply(X.astype(np.float32).astype(np.float64))}
     tol = {0: 1e-13, 1: 2e-6}
     errs, bad = {}, []
-    for prec, nvar, kind in ((0, 8, "hx"), (1, 12, "hx"), (0, 4, "proj"), (1, 4, "proj")):
+    for prec, nvar, kind in ((0, 8, "hx"), (1, 12, "hx"), (0, 6, "proj"), (1, 6, "proj")):
         setter = op.set_hx_variant if kind == "hx" else op.set_proj_variant
         dt = torch.float64 if prec == 0 else torch.float32
         Xd = torch.tensor(X, device="cuda", dtype=dt)
