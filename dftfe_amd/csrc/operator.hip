@@ -186,6 +186,41 @@ extern "C" chfsi_status chfsi_operator_create(const chfsi_operator_desc *d, chfs
     CHFSI_TRY(dev_upload(&op->d_m_vals, mvals.data(), mvals.size()));
   }
 
+  // ---- projector pairs per cell.  The fused kernels carry at most kCellSlots
+  // projector slots per cell; a cell with more keeps its first pairs and hands
+  // each further group to an extra cell id (C, C + 1, ...) that shares its
+  // nodes and has a zero cell matrix, in a colour after all real colours, so
+  // the apply adds the group's C_e W term on a later, non-first touch.
+  constexpr int kCellSlots = 128;
+  const int P = d->n_pairs > 0 ? d->proj_per_pair : 0;
+  std::vector<std::vector<int>> pairs_of(C);
+  std::vector<int> src_cell(C);  // the real cell of every cell id
+  for (int c = 0; c < C; ++c) src_cell[c] = c;
+  if (d->n_pairs > 0) {
+    CHFSI_ARG(P > 0 && P <= kCellSlots && d->pair_cell && d->pair_proj0 && d->pair_C && d->proj_V,
+              "missing nonlocal arrays (or more than %d projectors per pair)", kCellSlots);
+    for (int p = 0; p < d->n_pairs; ++p) {
+      CHFSI_ARG(d->pair_cell[p] >= 0 && d->pair_cell[p] < C, "pair_cell out of range");
+      CHFSI_ARG(d->pair_proj0[p] >= 0 && d->pair_proj0[p] + P <= d->n_proj,
+                "pair projector out of range");
+      pairs_of[d->pair_cell[p]].push_back(p);
+    }
+    const size_t per = kCellSlots / P;  // pairs one cell id carries
+    for (int c = 0; c < C; ++c) {
+      if (pairs_of[c].size() <= per) continue;
+      const std::vector<int> all = pairs_of[c];
+      pairs_of[c].resize(per);
+      for (size_t b = per; b < all.size(); b += per) {
+        pairs_of.emplace_back(all.begin() + b, all.begin() + std::min(all.size(), b + per));
+        src_cell.push_back(c);
+      }
+    }
+  }
+  const int Ctot = (int)pairs_of.size();  // real and extra cell ids
+  op->n_extra_cells = Ctot - C;
+  CHFSI_ARG(op->n_extra_cells == 0 || d->n_kpoints == 0,
+            "with k-points a cell overlaps at most %d projector slots", kCellSlots);
+
   // ---- colour classes and first-touch flags
   constexpr int KC = chfsi_operator::kMaxColors;
   std::vector<std::vector<int>> by_color(KC);
@@ -195,19 +230,33 @@ extern "C" chfsi_status chfsi_operator_create(const chfsi_operator_desc *d, chfs
     by_color[col].push_back(c);
     op->ncolors = std::max(op->ncolors, col + 1);
   }
+  {
+    // group j of a cell of colour k goes to colour j * ncolors_real + k: cells
+    // of one real colour share no DOF, so neither do their extra cells
+    const int nreal = op->ncolors;
+    std::vector<int> group(C, 0);
+    for (int c = C; c < Ctot; ++c) {
+      const int rc = src_cell[c];
+      const int col = ++group[rc] * nreal + d->cell_color[rc];
+      CHFSI_ARG(col < KC, "cell %d overlaps too many projectors (%d colours needed, %d supported)",
+                rc, col + 1, KC);
+      by_color[col].push_back(c);
+      op->ncolors = std::max(op->ncolors, col + 1);
+    }
+  }
   std::vector<int> cells_sorted;
   op->color_off[0] = 0;
   for (int k = 0; k < op->ncolors; ++k) {
     cells_sorted.insert(cells_sorted.end(), by_color[k].begin(), by_color[k].end());
     op->color_off[k + 1] = (int)cells_sorted.size();
   }
-  std::vector<uint32_t> map(size_t(C) * ne);
+  std::vector<uint32_t> map(size_t(Ctot) * ne);
   std::vector<int> touched_color(M, -1);
   std::vector<int> touched_cell(M, -1);
   for (int k = 0; k < op->ncolors; ++k) {
     for (int c : by_color[k]) {
       for (int i = 0; i < ne; ++i) {
-        const int64_t g0 = d->cell_map[size_t(c) * ne + i];
+        const int64_t g0 = d->cell_map[size_t(src_cell[c]) * ne + i];
         CHFSI_ARG(g0 >= 0 && g0 < M, "cell_map entry out of range");
         const int64_t g = fold[g0];
         uint32_t w = (uint32_t)g;
@@ -271,38 +320,34 @@ extern "C" chfsi_status chfsi_operator_create(const chfsi_operator_desc *d, chfs
     CHFSI_TRY(dev_alloc(&op->d_Hk64, per_k * op->nk));
     CHFSI_TRY(dev_alloc(&op->d_Hk32, per_k * op->nk));
   }
-  CHFSI_TRY(dev_alloc(&op->d_Ht64, size_t(C) * ne_pad * ne_pad));
-  CHFSI_TRY(dev_alloc(&op->d_Ht32, size_t(C) * ne_pad * ne_pad));
-  CHFSI_TRY(dev_alloc(&op->d_Ht16, size_t(C) * ne_pad * ne_pad));
+  CHFSI_TRY(dev_alloc(&op->d_Ht64, size_t(Ctot) * ne_pad * ne_pad));
+  CHFSI_TRY(dev_alloc(&op->d_Ht32, size_t(Ctot) * ne_pad * ne_pad));
+  CHFSI_TRY(dev_alloc(&op->d_Ht16, size_t(Ctot) * ne_pad * ne_pad));
+  if (Ctot > C) {
+    // the extra cells' matrices stay zero (the builds write cells 0..C-1)
+    const size_t off = size_t(C) * ne_pad * ne_pad, n = size_t(Ctot - C) * ne_pad * ne_pad;
+    CHFSI_HIP(hipMemset(op->d_Ht64 + off, 0, n * sizeof(double)));
+    CHFSI_HIP(hipMemset(op->d_Ht32 + off, 0, n * sizeof(float)));
+    CHFSI_HIP(hipMemset(op->d_Ht16 + off, 0, n * sizeof(uint16_t)));
+  }
 
   // ---- nonlocal slots: per cell the (pair, alpha) columns, padded to 16
   op->n_proj = d->n_proj;
   if (d->n_pairs > 0) {
-    const int P = d->proj_per_pair;
-    CHFSI_ARG(P > 0 && d->pair_cell && d->pair_proj0 && d->pair_C && d->proj_V,
-              "missing nonlocal arrays");
-    std::vector<std::vector<int>> pairs_of(C);
-    for (int p = 0; p < d->n_pairs; ++p) {
-      CHFSI_ARG(d->pair_cell[p] >= 0 && d->pair_cell[p] < C, "pair_cell out of range");
-      CHFSI_ARG(d->pair_proj0[p] >= 0 && d->pair_proj0[p] + P <= d->n_proj,
-                "pair projector out of range");
-      pairs_of[d->pair_cell[p]].push_back(p);
-    }
-    std::vector<int> nl_off(C + 1, 0);
-    for (int c = 0; c < C; ++c) {
+    std::vector<int> nl_off(Ctot + 1, 0);
+    for (int c = 0; c < Ctot; ++c) {
       const int cnt = (int)pairs_of[c].size() * P;
-      const int pad = (cnt + 15) / 16 * 16;
-      CHFSI_ARG(pad <= 128, "cell %d overlaps %d projectors; at most 128 supported", c, cnt);
-      nl_off[c + 1] = nl_off[c] + pad;
+      nl_off[c + 1] = nl_off[c] + (cnt + 15) / 16 * 16;  // at most kCellSlots
     }
-    const int S = nl_off[C];
+    const int S = nl_off[Ctot];
     op->n_slots = S;
-    for (int c = 0; c < C; ++c) op->max_srb = std::max(op->max_srb, (nl_off[c + 1] - nl_off[c]) / 16);
+    for (int c = 0; c < Ctot; ++c)
+      op->max_srb = std::max(op->max_srb, (nl_off[c + 1] - nl_off[c]) / 16);
     std::vector<int> slot_proj(S, -1);
     std::vector<double> nlC(size_t(S) * ne_pad, 0.0), nlCt(size_t(S) * ne_pad, 0.0);
     std::vector<std::vector<int>> slots_of(d->n_proj);
     std::vector<int> nl_cells;
-    for (int c = 0; c < C; ++c) {
+    for (int c = 0; c < Ctot; ++c) {
       const int s0 = nl_off[c], ns = nl_off[c + 1] - s0;
       if (ns == 0) continue;
       nl_cells.push_back(c);
@@ -353,8 +398,8 @@ extern "C" chfsi_status chfsi_operator_create(const chfsi_operator_desc *d, chfs
             inc_node[k] = i;
           }
       }
-      std::vector<uint32_t> own_map(size_t(C) * ne_pad, 0u);
-      std::vector<int> own_n(C, 0);
+      std::vector<uint32_t> own_map(size_t(Ctot) * ne_pad, 0u);
+      std::vector<int> own_n(Ctot, 0);
       std::vector<double> nlCto(nlCt.size(), 0.0);
       std::vector<int> keys, cands;
       std::vector<double> vals;

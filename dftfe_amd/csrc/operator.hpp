@@ -56,7 +56,7 @@ struct chfsi_operator {
   // cell -> DOF map with first-touch flag (bit 31), cells sorted by colour
   uint32_t *d_map = nullptr;
   int *d_cells_by_color = nullptr;
-  static constexpr int kMaxColors = 32;
+  static constexpr int kMaxColors = 128;
   int ncolors = 0;
   int color_off[kMaxColors + 1] = {0};
   // Loewdin scalings
@@ -93,6 +93,10 @@ struct chfsi_operator {
   int hx_variant16 = 0;  // BF16 apply: 0 K = 32 ring, 1 register-staged K = 16
   int proj_variant64 = 0, proj_variant32 = 0;
   int max_srb = 1;  // largest slot count of any cell / 16
+  // extra cell ids C .. C + n_extra_cells - 1: further projector groups of
+  // cells with more than 128 slots (zero cell matrix, own colours, the nodes
+  // of their real cell)
+  int n_extra_cells = 0;
   // constraints (slave-major and master-major CSR)
   int64_t nc = 0, nm = 0;
   int64_t *d_c_rows = nullptr, *d_c_ptr = nullptr, *d_c_cols = nullptr;

@@ -65,7 +65,10 @@ typedef struct {
   int32_t quad_per_cell;     /* n_q                                            */
   const int64_t *cell_map;   /* C*n_e: DOF of local node i of cell c
                                 (vectorUtilities.cc:398-500)                   */
-  const int8_t *cell_color;  /* C: colour 0..31; cells of one colour share no DOF */
+  const int8_t *cell_color;  /* C: colour >= 0; cells of one colour share no DOF.  A cell
+                                overlapping more than 128 projector slots takes further
+                                colours (j * n_colours + colour for its j-th extra group of
+                                slots); all colours together must stay below 128 */
   const double *sqrt_mass;   /* M: M^{1/2}, 0 on constrained rows              */
   const double *inv_sqrt_mass; /* M: M^{-1/2}, 0 on constrained rows           */
   const double *shape_values;  /* n_q*n_e: N_i(q), row q                        */

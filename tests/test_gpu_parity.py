@@ -204,6 +204,55 @@ def test_projection_owned_rows(name, mode):
                 assert err < tol, (name, mode, B, prec, v, err)
 
 
+@pytest.mark.parametrize("n,atoms", [(2, 70), (4, 40)])
+def test_crowded_cells(n, atoms):
+    """Cells overlapping more than 128 projector slots (70 wells on 2^3 cells:
+    276-280 slots per cell, three groups; 40 wells on 4^3 cells: 128-160): the
+    operator hands each further group of 32 pairs to an extra cell id with a
+    zero cell matrix in a later colour.  Every FP64 / FP32 tile variant and
+    projection variant, the one-column kernel and the BF16 apply give the
+    oracle's H~X."""
+    _dev()
+    from dftfe_amd.chfsi import KohnShamOperatorDevice
+
+    wl = fe.Workload(f"crowd{n}", 8.0, n, 2, 3, atoms, 16, 16, 8, seed=3030 + n)
+    prob = fe.build_problem(wl)
+    slots = np.bincount(prob.pair_cell, minlength=prob.C) * 4
+    assert slots.max() > 128
+    ref = O.OracleOperator(prob)
+    op = KohnShamOperatorDevice(prob)
+    bad = []
+    for N in (24, 1):
+        X = prob.initial_guess(N=N, seed_offset=5 + N)
+        want = {0: ref.apply(X), 1: ref.apply(X.astype(np.float32).astype(np.float64))}
+        for prec, nvar, kind in ((0, 8, "hx"), (1, 12, "hx"), (0, 6, "proj"), (1, 6, "proj")):
+            setter = op.set_hx_variant if kind == "hx" else op.set_proj_variant
+            dt = torch.float64 if prec == 0 else torch.float32
+            Xd = torch.tensor(X, device="cuda", dtype=dt)
+            for v in range(nvar):
+                Yd = torch.zeros_like(Xd)
+                setter(prec, v)
+                try:
+                    op.HX(Xd, False, 1.0, Yd)
+                finally:
+                    setter(prec, 0)
+                err = _rel(Yd.double().cpu().numpy(), want[prec])
+                if not err < (1e-13 if prec == 0 else 2e-6):
+                    bad.append((N, kind, prec, v, err))
+        Xd = torch.tensor(X, device="cuda", dtype=torch.float32)
+        for v in range(4):
+            Yd = torch.zeros_like(Xd)
+            op.set_hx_variant(2, v)
+            try:
+                op.HX(Xd, False, 1.0, Yd, bf16=True)
+            finally:
+                op.set_hx_variant(2, 0)
+            err = _rel(Yd.double().cpu().numpy(), want[1])
+            if not err < 3e-2:
+                bad.append((N, "bf16", 2, v, err))
+    assert not bad, bad
+
+
 def test_hx_strided_block_and_flags(small):
     """HX on a column block of a wider array, with scale_flag / scalar semantics."""
     prob, ref, op = small
