@@ -177,10 +177,10 @@ def main():
         return e0.elapsed_time(e1) / args.hx_reps
 
     def time_hx_bf16():
-        """BF16 apply on W columns: the LDS-ring kernels (variants 0, 2, 3) and
+        """BF16 apply on W columns: the LDS-ring kernels (variants 0, 2-5) and
         the register-staged one (variant 1); the fastest stays selected."""
         t = []
-        for v in range(4):
+        for v in range(6):
             op.set_hx_variant(L.BF16, v)
             t.append(time_hx(torch.float32, W, bf16=True))
         op.set_hx_variant(L.BF16, int(np.argmin(t)))

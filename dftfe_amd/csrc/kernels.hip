@@ -1912,7 +1912,7 @@ __attribute__((amdgpu_waves_per_eu(NW == 8 && S == 2 ? 4 : 1))) void hxb_kernel(
   constexpr int RBW = (NRB + WR - 1) / WR;
   constexpr int ROWS_MAX = NRB * 16;
   constexpr int CBW = TN / 16 / WC;
-  static_assert(TN == 64 && CBW >= 1 && TN % (16 * WC) == 0, "bad tile");
+  static_assert((TN == 64 || TN == 128) && CBW >= 1 && TN % (16 * WC) == 0, "bad tile");
   static_assert(NRB % 2 == 0, "even row-block count");
   static_assert(S >= 2, "at least two stages");
   constexpr int NLMAX = 128;
@@ -1921,7 +1921,9 @@ __attribute__((amdgpu_waves_per_eu(NW == 8 && S == 2 ? 4 : 1))) void hxb_kernel(
   constexpr int HCP = HSB / 1024;              // 1 KB copies per slice
   constexpr int RPO = 256 / TN;                // gathered rows per copy
   constexpr int LPR = TN / 4;                  // lanes per gathered row
-  constexpr int XG = 1024 + 32;                // bytes per gathered copy incl. spacing
+  // bytes per gathered copy incl. spacing: the four K groups of a B-operand
+  // read (copies 2g or 4g apart) fall in four distinct 64-byte bank windows
+  constexpr int XG = 1024 + 8 * RPO;
   constexpr int NXC = 32 / RPO;                // gathered copies per stage
   constexpr int HB = 2 * HSB;
   constexpr int SB = HB + NXC * XG;
@@ -2089,10 +2091,9 @@ __attribute__((amdgpu_waves_per_eu(NW == 8 && S == 2 ? 4 : 1))) void hxb_kernel(
   hx_scatter<float, NRB, WR, RBW, CBW, 8>(a, a.ne, acc, smap, sout, wr, wc, lane, v0);
 }
 
-template <int NRB, int NW, int WC, int S>
+template <int NRB, int NW, int WC, int S, int TN = 64>
 static hipError_t hxb_go(HxLaunch<float> a, const uint16_t *Ht16, const uint16_t *nlC16,
                          hipStream_t s) {
-  constexpr int TN = 64;
   a.ntiles = (a.B + TN - 1) / TN;
   const int items = a.ncells * a.ntiles;
   if (items == 0) return hipSuccess;
@@ -2115,20 +2116,20 @@ static hipError_t hx_bf16_go(HxLaunch<float> a, const uint16_t *Ht16, const uint
   return hipGetLastError();
 }
 
-template <int NW, int WC, int S>
+template <int NW, int WC, int S, int TN = 64>
 static hipError_t hxb_dispatch(const HxLaunch<float> &a, const uint16_t *Ht16,
                                const uint16_t *nlC16, hipStream_t s) {
   switch (a.ne_pad >> 4) {
-    case 2: return hxb_go<2, NW, WC, S>(a, Ht16, nlC16, s);
-    case 4: return hxb_go<4, NW, WC, S>(a, Ht16, nlC16, s);
-    case 8: return hxb_go<8, NW, WC, S>(a, Ht16, nlC16, s);
-    case 14: return hxb_go<14, NW, WC, S>(a, Ht16, nlC16, s);
-    case 22: return hxb_go<22, NW, WC, S>(a, Ht16, nlC16, s);
+    case 2: return hxb_go<2, NW, WC, S, TN>(a, Ht16, nlC16, s);
+    case 4: return hxb_go<4, NW, WC, S, TN>(a, Ht16, nlC16, s);
+    case 8: return hxb_go<8, NW, WC, S, TN>(a, Ht16, nlC16, s);
+    case 14: return hxb_go<14, NW, WC, S, TN>(a, Ht16, nlC16, s);
+    case 22: return hxb_go<22, NW, WC, S, TN>(a, Ht16, nlC16, s);
     default: return hipErrorInvalidValue;
   }
 }
 
-int hx_bf16_variants() { return 4; }
+int hx_bf16_variants() { return 6; }
 
 hipError_t launch_hx_bf16(const HxLaunch<float> &a, const uint16_t *Ht16, const uint16_t *nlC16,
                           hipStream_t s) {
@@ -2145,6 +2146,8 @@ hipError_t launch_hx_bf16(const HxLaunch<float> &a, const uint16_t *Ht16, const 
     switch (a.variant) {
       case 2: return hxb_dispatch<8, 4, 3>(a, Ht16, nlC16, s);
       case 3: return hxb_dispatch<16, 4, 3>(a, Ht16, nlC16, s);
+      case 4: return hxb_dispatch<16, 8, 3, 128>(a, Ht16, nlC16, s);
+      case 5: return hxb_dispatch<16, 8, 2, 128>(a, Ht16, nlC16, s);
       default: return hxb_dispatch<8, 4, 2>(a, Ht16, nlC16, s);
     }
   }

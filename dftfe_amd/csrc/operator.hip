@@ -863,7 +863,7 @@ extern "C" chfsi_status chfsi_operator_set_hx_variant(chfsi_operator *op, int32_
   CHFSI_ARG(op, "null operator");
   if (precision == CHFSI_BF16) {
     CHFSI_ARG(variant >= 0 && variant < hx_bf16_variants(),
-              "bad BF16 variant (0, 2, 3: K = 32 ring, 1: K = 16)");
+              "bad BF16 variant (0, 2-5: K = 32 ring, 1: K = 16)");
     op->hx_variant16 = variant;
     return CHFSI_OK;
   }

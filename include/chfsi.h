@@ -274,7 +274,8 @@ chfsi_status chfsi_operator_get_cell_matrices(chfsi_operator *op, double *H_dev,
  * per cell node.  precision CHFSI_BF16 selects
  * the BF16 apply's kernel: K = 32 MFMA on the direct-to-LDS ring with 0
  * (default) 8 waves and 2 stages (two workgroups per CU), 2: 8 waves and 3
- * stages, 3: 16 waves and 3 stages; 1: the register-staged K = 16 form. */
+ * stages, 3: 16 waves and 3 stages, 4 / 5: 16 waves, 128-column tiles, 3 / 2
+ * stages; 1: the register-staged K = 16 form. */
 chfsi_status chfsi_operator_set_hx_variant(chfsi_operator *op, int32_t precision, int32_t variant);
 chfsi_status chfsi_operator_set_proj_variant(chfsi_operator *op, int32_t precision,
                                              int32_t variant);

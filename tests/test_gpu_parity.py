@@ -240,7 +240,7 @@ def test_crowded_cells(n, atoms):
                 if not err < (1e-13 if prec == 0 else 2e-6):
                     bad.append((N, kind, prec, v, err))
         Xd = torch.tensor(X, device="cuda", dtype=torch.float32)
-        for v in range(4):
+        for v in range(6):
             Yd = torch.zeros_like(Xd)
             op.set_hx_variant(2, v)
             try:
@@ -313,7 +313,7 @@ def test_hx_bf16(small):
 
 @pytest.mark.parametrize("B", [64, 72, 8])
 def test_hx_bf16_ring_matches_register_kernel(small, B):
-    """K = 32 BF16 ring (variants 0, 2, 3: stage and wave counts) against the
+    """K = 32 BF16 ring (variants 0, 2-5: stage, wave and column counts) against the
     register-staged K = 16 form (variant 1): FP32 accumulation rounding apart.
     Cells here hold 16 projector slots (one 16-wide slice), so every stage
     count is odd and the last stage's second half is the zero half."""
@@ -323,7 +323,7 @@ def test_hx_bf16_ring_matches_register_kernel(small, B):
     X = prob.initial_guess(N=B, seed_offset=B + 11)
     Xd = torch.tensor(X, device="cuda", dtype=torch.float32)
     out = {}
-    for v in (0, 1, 2, 3):
+    for v in (0, 1, 2, 3, 4, 5):
         Yd = torch.zeros_like(Xd)
         op.set_hx_variant(L.BF16, v)
         try:
@@ -331,7 +331,7 @@ def test_hx_bf16_ring_matches_register_kernel(small, B):
         finally:
             op.set_hx_variant(L.BF16, 0)
         out[v] = Yd.double().cpu().numpy()
-    for v in (0, 2, 3):
+    for v in (0, 2, 3, 4, 5):
         assert np.isfinite(out[v]).all()
         assert _rel(out[v], out[1]) < 1e-5, v
 
