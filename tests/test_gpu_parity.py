@@ -251,6 +251,12 @@ def test_crowded_cells(n, atoms):
             if not err < 3e-2:
                 bad.append((N, "bf16", 2, v, err))
     assert not bad, bad
+    # the k-point kernels carry at most 128 slots per cell: refused, not wrong
+    from dftfe_amd.lib import ChfsiError
+
+    kpts, _ = fe.kpoint_set(prob, n=2)
+    with pytest.raises(ChfsiError, match="k-points"):
+        KohnShamOperatorDevice(prob, kpoints=kpts)
 
 
 def test_hx_strided_block_and_flags(small):
