@@ -221,6 +221,10 @@ class KohnShamOperatorDevice:
     def set_hx_variant_complex(self, precision: int, variant: int) -> None:
         L.check(self.lib.chfsi_operator_set_hx_variant_complex(self.handle, precision, variant))
 
+    def set_proj_variant_complex(self, variant: int) -> None:
+        """Complex projection: 0 the cell form, 1 owned DOF rows (both precisions)."""
+        L.check(self.lib.chfsi_operator_set_proj_variant_complex(self.handle, variant))
+
     def autotune_complex(self, Xs, kidx, bounds, m: int = 2) -> dict:
         """Time the complex tile variants on a batch and keep the fastest per precision."""
         kb, keep = _kbatch(Xs, kidx, bounds, m, L.FP64)

@@ -52,7 +52,8 @@ struct ZOp<double> {
     return op->d_nlCk64 ? op->d_nlCk64 + size_t(2) * op->n_slots * op->ne_pad * k : nullptr;
   }
   static const double *nlCt(chfsi_operator *op, int k) {
-    return op->d_nlCtk64 ? op->d_nlCtk64 + size_t(2) * op->n_slots * op->ne_pad * k : nullptr;
+    const double *t = op->zproj_variant == 1 ? op->d_nlCtko64 : op->d_nlCtk64;
+    return t ? t + size_t(2) * op->n_slots * op->ne_pad * k : nullptr;
   }
   static const double *s(chfsi_operator *op) { return op->d_inv_sqrt_mass; }
   static int &variant(chfsi_operator *op) { return op->zhx_variant64; }
@@ -66,7 +67,8 @@ struct ZOp<float> {
     return op->d_nlCk32 ? op->d_nlCk32 + size_t(2) * op->n_slots * op->ne_pad * k : nullptr;
   }
   static const float *nlCt(chfsi_operator *op, int k) {
-    return op->d_nlCtk32 ? op->d_nlCtk32 + size_t(2) * op->n_slots * op->ne_pad * k : nullptr;
+    const float *t = op->zproj_variant == 1 ? op->d_nlCtko32 : op->d_nlCtk32;
+    return t ? t + size_t(2) * op->n_slots * op->ne_pad * k : nullptr;
   }
   static const float *s(chfsi_operator *op) { return op->d_inv_sqrt_mass32; }
   static int &variant(chfsi_operator *op) { return op->zhx_variant32; }
@@ -161,6 +163,10 @@ chfsi_status zapply_round(chfsi_operator *op, const ZHxItem<T> *dev_items,
     pl.ne_pad = op->ne_pad;
     pl.s_in = sc;
     pl.nl_off = op->d_nl_off;
+    if (op->zproj_variant == 1) {
+      pl.own_map = op->d_own_map;
+      pl.own_n = op->d_own_n;
+    }
     CHFSI_HIP(launch_zproj<T>(pl, host_items, s));
     CHFSI_HIP(launch_zproj_reduce<T>(dev_items, n, maxB, op->d_proj_ptr, op->d_proj_slots,
                                      op->d_proj_V, op->n_proj, s));
@@ -420,6 +426,17 @@ extern "C" chfsi_status chfsi_operator_set_hx_variant_complex(chfsi_operator *op
   CHFSI_ARG(precision == CHFSI_FP64 || precision == CHFSI_FP32, "bad precision");
   CHFSI_ARG(variant >= 0 && variant < zhx_variants(precision == CHFSI_FP64), "bad variant");
   (precision == CHFSI_FP64 ? op->zhx_variant64 : op->zhx_variant32) = variant;
+  return CHFSI_OK;
+}
+
+extern "C" chfsi_status chfsi_operator_set_proj_variant_complex(chfsi_operator *op,
+                                                               int32_t variant) {
+  CHFSI_ARG(op, "null operator");
+  CHFSI_ARG(variant == 0 || variant == 1, "bad variant (0: cell form, 1: owned rows)");
+  CHFSI_ARG(variant == 0 || op->n_slots == 0 || op->d_nlCtko64,
+            "operator was created without k-points");
+  CHFSI_ARG(op->nk > 0, "operator was created without k-points");
+  op->zproj_variant = variant;
   return CHFSI_OK;
 }
 

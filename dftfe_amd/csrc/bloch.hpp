@@ -65,6 +65,10 @@ struct ZProjLaunch {
   int ne, ne_pad;
   const T *s_in;
   const int *nl_off;
+  // owned-row form (null: the cell form): rows own_map[c * ne_pad + r],
+  // r < own_n[c], with the items' nlCt holding the assembled values
+  const uint32_t *own_map;
+  const int *own_n;
 };
 
 // host copies of the items give the per-entry widths for the work split

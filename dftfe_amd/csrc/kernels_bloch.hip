@@ -224,7 +224,10 @@ __global__ __launch_bounds__(64 * NW) void zhx_kernel(ZHxLaunch<T> a) {
 // ---------------------------------------------------------------------------
 // Tile 2t: Re C against node slice t; tile 2t+1: Im C against the same slice
 // with the column pair swapped and the imaginary column negated.
-template <typename T, int NW, int TN, int MAXSRB, int NRB>
+// OWN: the rows are the cell's owned DOF rows (own_map, own_n) with the
+// projector values assembled over the cell nodes, and only their tiles are
+// visited (as proj_kernel's owned-row form).
+template <typename T, int NW, int TN, int MAXSRB, int NRB, bool OWN = false>
 __global__ __launch_bounds__(64 * NW) void zproj_kernel(ZProjLaunch<T> a) {
   using v4 = typename V4<T>::t;
   constexpr int NT = 64 * NW;
@@ -258,12 +261,16 @@ __global__ __launch_bounds__(64 * NW) void zproj_kernel(ZProjLaunch<T> a) {
   const int nsrb = nslot >> 4;
   const int nchunk = nslot * 4;
   const T *Acell = it.nlCt + (int64_t)s0 * ROWS * 2;
-  const uint32_t *cmap = a.map + (int64_t)cell * a.ne;
+  const uint32_t *cmap = OWN ? a.own_map + (int64_t)cell * ROWS : a.map + (int64_t)cell * a.ne;
+  const int nrow = OWN ? a.own_n[cell] : a.ne;
+  // (Re, Im) tile pairs holding rows; the tiles past an owned-row cell's last
+  // row are zero and read row 0 with scale 0
+  const int ntot = OWN ? 2 * (nrow > 0 ? (nrow + 15) >> 4 : 1) : 2 * NRB;
 
   for (int i = tid; i < ROWS; i += NT) {
     uint32_t w = 0;
     T sc = T(0);
-    if (i < a.ne) {
+    if (i < nrow) {
       w = cmap[i];
       sc = a.s_in ? a.s_in[w & 0x7fffffffu] : T(1);
     }
@@ -319,7 +326,6 @@ __global__ __launch_bounds__(64 * NW) void zproj_kernel(ZProjLaunch<T> a) {
   store_tile(0);
   __syncthreads();
   const T sgn = (lane & 1) ? T(-1) : T(1);
-  constexpr int ntot = 2 * NRB;
   for (int t = 0; t < ntot; ++t) {
     const int ab = t & 1;
     const int xb = (t >> 1) & 1;
@@ -501,7 +507,7 @@ hipError_t launch_zhx<double>(ZHxLaunch<double> a, const ZHxItem<double> *h, int
   }
 }
 
-template <typename T, int NRB, int MAXSRB>
+template <typename T, int NRB, int MAXSRB, bool OWN>
 static hipError_t zproj_go(ZProjLaunch<T> a, const ZHxItem<T> *h, hipStream_t s) {
   constexpr int TN = sizeof(T) == 4 ? 128 : 64;
   int total = 0;
@@ -512,17 +518,23 @@ static hipError_t zproj_go(ZProjLaunch<T> a, const ZHxItem<T> *h, hipStream_t s)
   a.kbegin[a.nk] = total;
   if (total == 0) return hipSuccess;
   a.per_xcd = (total + 7) / 8;
-  hipLaunchKernelGGL((zproj_kernel<T, 4, TN, MAXSRB, NRB>), dim3(8 * a.per_xcd), dim3(256), 0, s,
-                     a);
+  hipLaunchKernelGGL((zproj_kernel<T, 4, TN, MAXSRB, NRB, OWN>), dim3(8 * a.per_xcd), dim3(256), 0,
+                     s, a);
   return hipGetLastError();
 }
 
 // accumulators sized to the largest slot count of any cell
+template <typename T, int NRB, bool OWN>
+static hipError_t zproj_msrb(ZProjLaunch<T> a, const ZHxItem<T> *h, hipStream_t s) {
+  if (a.max_srb <= 2) return zproj_go<T, NRB, 2, OWN>(a, h, s);
+  if (a.max_srb <= 4) return zproj_go<T, NRB, 4, OWN>(a, h, s);
+  return zproj_go<T, NRB, 8, OWN>(a, h, s);
+}
+
 template <typename T, int NRB>
 static hipError_t zproj_fit(ZProjLaunch<T> a, const ZHxItem<T> *h, hipStream_t s) {
-  if (a.max_srb <= 2) return zproj_go<T, NRB, 2>(a, h, s);
-  if (a.max_srb <= 4) return zproj_go<T, NRB, 4>(a, h, s);
-  return zproj_go<T, NRB, 8>(a, h, s);
+  if (a.own_map && a.own_n) return zproj_msrb<T, NRB, true>(a, h, s);
+  return zproj_msrb<T, NRB, false>(a, h, s);
 }
 
 template <typename T>

@@ -79,7 +79,8 @@ chfsi_operator::~chfsi_operator() {
                   d_nl_cells, d_c_rows, d_c_ptr, d_c_cols, d_c_vals, d_m_rows, d_m_ptr,
                   d_m_slaves, d_m_vals, d_Mc, d_Gd, d_Hk64, d_Hk32, d_nlCk64, d_nlCtk64,
                   d_nlCk32, d_nlCtk32, d_invJ, d_s_gather64, d_ones_cons64, d_s_gather32,
-                  d_ones_cons32, d_own_map, d_own_n, d_nlCto64, d_nlCto32};
+                  d_ones_cons32, d_own_map, d_own_n, d_nlCto64, d_nlCto32, d_nlCtko64,
+                  d_nlCtko32};
   for (void *p : ptrs)
     if (p) hipFree(p);
   ztab_dev.release();
@@ -401,13 +402,19 @@ extern "C" chfsi_status chfsi_operator_create(const chfsi_operator_desc *d, chfs
       std::vector<uint32_t> own_map(size_t(Ctot) * ne_pad, 0u);
       std::vector<int> own_n(Ctot, 0);
       std::vector<double> nlCto(nlCt.size(), 0.0);
+      // with k-points the same rows carry each k-point's assembled complex
+      // values in the paired (Re, Im) tile layout of the complex projection
+      const int nk = op->nk;
+      const size_t per_k = size_t(2) * S * ne_pad;
+      std::vector<double> zcto(per_k * nk, 0.0);
       std::vector<int> keys, cands;
-      std::vector<double> vals;
+      std::vector<double> vals, valsk;  // valsk: [key][k-point][alpha][Re, Im]
       std::vector<char> done;
       for (int64_t g = 0; g < M; ++g) {
         if (gptr[g] == gptr[g + 1]) continue;
         keys.clear();
         vals.clear();
+        valsk.clear();
         cands.clear();
         for (int64_t k = gptr[g]; k < gptr[g + 1]; ++k) {
           const int c = inc_cell[k], i = inc_node[k];
@@ -418,19 +425,28 @@ extern "C" chfsi_status chfsi_operator_create(const chfsi_operator_desc *d, chfs
             if (q == keys.size()) {
               keys.push_back(key);
               vals.resize(vals.size() + P, 0.0);
+              valsk.resize(valsk.size() + size_t(nk) * P * 2, 0.0);
             }
             for (int al = 0; al < P; ++al) vals[q * P + al] += d->pair_C[(size_t(p) * ne + i) * P + al];
+            for (int kp = 0; kp < nk; ++kp)
+              for (int al = 0; al < P; ++al)
+                for (int h = 0; h < 2; ++h)
+                  valsk[((q * nk + kp) * P + al) * 2 + h] +=
+                      d->pair_C_kpoints[2 * (((size_t(kp) * d->n_pairs + p) * ne + i) * P + al) + h];
           }
         }
         std::sort(cands.begin(), cands.end());
         done.assign(keys.size(), 1);
         int left = 0;
-        for (size_t q = 0; q < keys.size(); ++q)
-          for (int al = 0; al < P; ++al)
-            if (vals[q * P + al] != 0.0 && done[q]) {
-              done[q] = 0;
-              ++left;
-            }
+        for (size_t q = 0; q < keys.size(); ++q) {
+          bool nz = false;
+          for (int al = 0; al < P; ++al) nz = nz || vals[q * P + al] != 0.0;
+          for (size_t e = 0; e < size_t(nk) * P * 2; ++e) nz = nz || valsk[q * nk * P * 2 + e] != 0.0;
+          if (nz) {
+            done[q] = 0;
+            ++left;
+          }
+        }
         while (left > 0) {
           int best = -1, best_cnt = 0;
           for (int c : cands) {
@@ -454,6 +470,12 @@ extern "C" chfsi_status chfsi_operator_create(const chfsi_operator_desc *d, chfs
               for (int al = 0; al < P; ++al)
                 nlCto[size_t(s0) * ne_pad + size_t(r / 16) * ns * 16 + size_t(local + al) * 16 +
                       (r % 16)] = vals[q * P + al];
+              for (int kp = 0; kp < nk; ++kp)
+                for (int al = 0; al < P; ++al)
+                  for (int h = 0; h < 2; ++h)
+                    zcto[per_k * kp + size_t(s0) * 2 * ne_pad +
+                         size_t(2 * (r / 16) + h) * ns * 16 + size_t(local + al) * 16 + (r % 16)] =
+                        valsk[((q * nk + kp) * P + al) * 2 + h];
               done[q] = 1;
               --left;
             }
@@ -466,6 +488,11 @@ extern "C" chfsi_status chfsi_operator_create(const chfsi_operator_desc *d, chfs
       CHFSI_TRY(dev_upload(&op->d_nlCto64, nlCto.data(), nlCto.size()));
       std::vector<float> o32(nlCto.begin(), nlCto.end());
       CHFSI_TRY(dev_upload(&op->d_nlCto32, o32.data(), o32.size()));
+      if (nk > 0) {
+        CHFSI_TRY(dev_upload(&op->d_nlCtko64, zcto.data(), zcto.size()));
+        std::vector<float> z32(zcto.begin(), zcto.end());
+        CHFSI_TRY(dev_upload(&op->d_nlCtko32, z32.data(), z32.size()));
+      }
     }
     std::vector<float> tmp32(nlC.size());
     CHFSI_TRY(dev_upload(&op->d_nl_off, nl_off.data(), nl_off.size()));

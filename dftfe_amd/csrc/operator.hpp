@@ -120,6 +120,11 @@ struct chfsi_operator {
   float *d_Hk32 = nullptr;
   double *d_nlCk64 = nullptr, *d_nlCtk64 = nullptr;
   float *d_nlCk32 = nullptr, *d_nlCtk32 = nullptr;
+  // owned-row form of the complex projector tiles (the rows of d_own_map),
+  // per k-point; zproj_variant 0: cell form, 1: owned rows (the default)
+  double *d_nlCtko64 = nullptr;
+  float *d_nlCtko32 = nullptr;
+  int zproj_variant = 1;
   int zhx_variant64 = 0, zhx_variant32 = 0;
   chfsi::Scratch ztab_dev, ztab_W, ztab_P;  // tables and scratch of single complex applies
 

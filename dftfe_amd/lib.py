@@ -164,6 +164,7 @@ SIGNATURES = {
     "chfsi_batch_filter_launch": (ct.c_int, [_VP, _VP]),
     "chfsi_batch_filter_destroy": (None, [_VP]),
     "chfsi_operator_set_hx_variant_complex": (ct.c_int, [_VP, ct.c_int32, ct.c_int32]),
+    "chfsi_operator_set_proj_variant_complex": (ct.c_int, [_VP, ct.c_int32]),
     "chfsi_operator_autotune_complex": (ct.c_int, [_VP, ct.POINTER(KBatch), c_int32_p, c_int32_p,
                                                    ct.POINTER(ct.c_float), _VP]),
     "chfsi_lanczos_bounds_complex": (ct.c_int, [_VP, _VP, ct.c_int32, ct.c_int32, c_double_p,

@@ -512,6 +512,14 @@ void chfsi_batch_filter_destroy(chfsi_batch *bt);
 chfsi_status chfsi_operator_set_hx_variant_complex(chfsi_operator *op, int32_t precision,
                                                    int32_t variant);
 
+/* Complex projection (no reference counterpart; the reference's projection is
+ * hipBLAS gemmBatched + a reduction gemm, computeNonLocalHamiltonianTimesX-
+ * MemoryOptBatchGEMMDevice.cc:48-82): 1 (default) each DOF row once per
+ * covering cell, with each k-point's projector values assembled over the cell
+ * nodes; 0 the cell form, once per cell node.  Both precisions, every complex
+ * call and batch made after the switch. */
+chfsi_status chfsi_operator_set_proj_variant_complex(chfsi_operator *op, int32_t variant);
+
 /* Tile-variant timing of the complex HX on a batch (as chfsi_operator_autotune):
  * keeps the fastest variant per precision; times_ms[0..7] FP64, [8..15] FP32. */
 chfsi_status chfsi_operator_autotune_complex(chfsi_operator *op, const chfsi_kbatch *batch,

@@ -96,6 +96,58 @@ def test_complex_hx_fp32_and_variants(kw):
     op.set_hx_variant_complex(L.FP32, 0)
 
 
+@pytest.mark.parametrize("B", [48, 13, 1])
+def test_complex_projection_forms(kw, B):
+    """Both complex projection forms: owned DOF rows (variant 1, the default:
+    each row read by the cells that cover its projectors, each k-point's
+    Bloch-phased projector values assembled over the cell nodes) and the cell
+    form (variant 0).  Each gives the oracle's H~X per k-point in both
+    precisions and every tile variant; under the cell form a batched filter
+    over uneven band counts equals per-k filters bit for bit (the default form
+    is covered by the batch tests below)."""
+    from dftfe_amd import lib as L
+    from dftfe_amd.chfsi import batch_filter, chebyshev_filter_complex
+
+    prob, kpts, op, ref = kw
+    try:
+        for pv in (1, 0):
+            op.set_proj_variant_complex(pv)
+            for k in (0, 4):
+                op.reinitkPointSpinIndex(k)
+                X = fe.complex_guess(prob, B, seed_offset=30 + k)
+                want = ref(k).apply(X)
+                for prec, nvar, dt, tol in ((L.FP64, 3, torch.complex128, 1e-13),
+                                            (L.FP32, 5, torch.complex64, 2e-5)):
+                    for v in range(nvar):
+                        op.set_hx_variant_complex(prec, v)
+                        Y = torch.zeros(X.shape, dtype=dt, device="cuda")
+                        op.HX_complex(_dev(X).to(dt), False, 1.0, Y)
+                        err = _rel(Y.to(torch.complex128).cpu().numpy(), want)
+                        assert err < tol, (pv, k, prec, v, err)
+                op.set_hx_variant_complex(L.FP64, 0)
+                op.set_hx_variant_complex(L.FP32, 0)
+        if B != 48:
+            return
+        counts = [24, 9, 40]
+        bounds = [(4.0, 60.0, -1.0), (5.0, 55.0, -1.5), (3.5, 65.0, -0.5)]
+        X0 = [prob.sqrt_mass[:, None] * fe.complex_guess(prob, n, seed_offset=40 + i)
+              for i, n in enumerate(counts)]
+        seq = []
+        for k, X in enumerate(X0):
+            op.reinitkPointSpinIndex(k)
+            Xd = _dev(X)
+            chebyshev_filter_complex(op, Xd, 5, *bounds[k])
+            seq.append(Xd)
+        bat = [_dev(X) for X in X0]
+        batch_filter(op, bat, list(range(len(counts))), bounds, 5)
+        for s_, b_ in zip(seq, bat):
+            assert torch.equal(s_, b_)
+        want = O.chebyshev_filter(ref(1).HX, X0[1], 5, *bounds[1])
+        assert _rel(bat[1].cpu().numpy(), want) < 1e-11
+    finally:
+        op.set_proj_variant_complex(1)
+
+
 def test_complex_filter_matches_oracle(kw):
     from dftfe_amd.chfsi import chebyshev_filter_complex
 
