@@ -126,6 +126,10 @@ def main():
     ap.add_argument("--hx-only", action="store_true",
                     help="setup, tile autotune and the HX timing only (for counter runs), "
                          "plus a 1 GiB device copy that calibrates the traffic counters")
+    ap.add_argument("--pass-only", action="store_true",
+                    help="setup, tile autotune and three passes of the timed mixed form only, "
+                         "the last between two 1-element fill kernels that mark it in a "
+                         "kernel trace")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
@@ -247,6 +251,27 @@ def main():
         if label:
             finals[label] = (X, lams[-1])
         return ms, lams, ress, phases
+
+    if args.pass_only:
+        solver = make_solver(False, head_defect, args.fp64_top)
+        X = X0.clone()
+        marker = torch.zeros(1, device=dev, dtype=torch.int32)
+        for k in range(3):
+            if k == 2:
+                torch.cuda.synchronize()
+                marker.fill_(1)
+                t = time.perf_counter()
+            lam, _, _ = solver.solve(op, X, is_first_filtering_call=(k == 0),
+                                     use_mixed_prec_overall=True)
+            _, _, bUp = solver.bounds()
+            solver.reinitSpectrumBounds(float(lam[0]), float(lam[-1]), bUp)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t) * 1e3
+        marker.fill_(2)
+        torch.cuda.synchronize()
+        print(json.dumps({"mixed_pass_ms": round(ms, 2), "phase_ms": solver.last_timings_ms}),
+              flush=True)
+        return
 
     for _ in range(args.warmup):
         run_passes(True, 1, defect=head_defect, top=args.fp64_top)
