@@ -1648,8 +1648,12 @@ static hipError_t hxd_go(HxLaunch<double> a, hipStream_t s) {
 // load covers RPW rows (1 KB contiguous) and the workgroup RPI rows; the
 // partial sums of a row are reduced over its LPR lanes at the end.  Gather
 // tables, nonlocal slices (rows of W) and the first-touch combine are those
-// of hx_kernel.
-template <typename T, int NRB>
+// of hx_kernel.  SYM (cell matrices exactly symmetric): slice t of the cell
+// matrix is read only for the row blocks rb >= t (52% of the bytes at p = 6);
+// a row block below the diagonal also gives its transposed product to the
+// rows of block t, summed over a wave's lanes by shuffles and over the waves
+// in LDS, in a fixed order (deterministic).
+template <typename T, int NRB, bool SYM = false>
 __global__ __launch_bounds__(256) void hx1_kernel(HxLaunch<T> a) {
   constexpr int ROWS = NRB * 16;
   constexpr int KMAX = ROWS + 128;
@@ -1663,6 +1667,7 @@ __global__ __launch_bounds__(256) void hx1_kernel(HxLaunch<T> a) {
   __shared__ uint32_t smap[ROWS];
   __shared__ T sout[ROWS];
   __shared__ T xs[KMAX];
+  __shared__ T tpart[SYM ? 4 * ROWS : 1];  // [wave][row]: transposed products
 
   const int wid = blockIdx.x;
   const int local = wid >> 3;
@@ -1705,6 +1710,7 @@ __global__ __launch_bounds__(256) void hx1_kernel(HxLaunch<T> a) {
   __syncthreads();
 
   const int lr = lane / LPR, lk = lane % LPR;
+  static_assert(16 % RPW == 0 || RPW % 16 == 0, "a wave load stays in one row block");
   T acc[NG];
 #pragma unroll
   for (int g = 0; g < NG; ++g) acc[g] = T(0);
@@ -1718,7 +1724,14 @@ __global__ __launch_bounds__(256) void hx1_kernel(HxLaunch<T> a) {
     for (int g = 0; g < NG; ++g) {
       const int row = g * RPI + wave * RPW + lr;
       const int rc = row < ROWS ? row : 0;
-      v[g] = *reinterpret_cast<const L16 *>(tile + (int64_t)rc * 16 + lk * EPL);
+      // SYM: the row blocks above the diagonal of slice t are not read
+      // (wave-uniform: a wave load lies in one row block)
+      const bool skip = SYM && t < NRB && (g * RPI + wave * RPW) / 16 < t;
+      if (skip) {
+        v[g] = L16{};
+      } else {
+        v[g] = *reinterpret_cast<const L16 *>(tile + (int64_t)rc * 16 + lk * EPL);
+      }
     }
     // one common first use of every load of the slice, so that all of them
     // are issued before the first wait (left alone, the compiler sinks each
@@ -1734,11 +1747,47 @@ __global__ __launch_bounds__(256) void hx1_kernel(HxLaunch<T> a) {
 #pragma unroll
       for (int q = 0; q < EPL; ++q) acc[g] = fmadd(e[q], xk[q], acc[g]);
     }
+    if constexpr (SYM) {
+      if (t < NRB) {
+        // transposed products of the row blocks below the diagonal: column
+        // t*16 + lk*EPL + q of the slice collects H[row][col] x[row]
+        T ts[EPL];
+#pragma unroll
+        for (int q = 0; q < EPL; ++q) ts[q] = T(0);
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+          const int row = g * RPI + wave * RPW + lr;
+          if ((g * RPI + wave * RPW) / 16 > t && row < ROWS) {
+            const T *e = reinterpret_cast<const T *>(&v[g]);
+            const T xr = xs[row];
+#pragma unroll
+            for (int q = 0; q < EPL; ++q) ts[q] = fmadd(e[q], xr, ts[q]);
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < EPL; ++q)
+#pragma unroll
+          for (int o = LPR; o < 64; o <<= 1) ts[q] += __shfl_xor(ts[q], o);
+        if (lr == 0) {
+#pragma unroll
+          for (int q = 0; q < EPL; ++q) tpart[wave * ROWS + t * 16 + lk * EPL + q] = ts[q];
+        }
+      }
+    }
   }
 #pragma unroll
   for (int g = 0; g < NG; ++g)
 #pragma unroll
     for (int o = 1; o < LPR; o <<= 1) acc[g] += __shfl_xor(acc[g], o);
+  if constexpr (SYM) {
+    __syncthreads();  // every wave's transposed products are in LDS
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+      const int row = g * RPI + wave * RPW + lr;
+      if (row < ROWS)
+        acc[g] += ((tpart[row] + tpart[ROWS + row]) + tpart[2 * ROWS + row]) + tpart[3 * ROWS + row];
+    }
+  }
   if (lk != 0) return;
 #pragma unroll
   for (int g = 0; g < NG; ++g) {
@@ -1763,7 +1812,10 @@ static hipError_t hx1_go(HxLaunch<T> a, hipStream_t s) {
   a.ntiles = 1;
   if (a.ncells == 0) return hipSuccess;
   a.per_xcd = (a.ncells + 7) / 8;
-  hipLaunchKernelGGL((hx1_kernel<T, NRB>), dim3(8 * a.per_xcd), dim3(256), 0, s, a);
+  if (a.h_sym)
+    hipLaunchKernelGGL((hx1_kernel<T, NRB, true>), dim3(8 * a.per_xcd), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((hx1_kernel<T, NRB>), dim3(8 * a.per_xcd), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

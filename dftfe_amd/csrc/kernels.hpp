@@ -53,6 +53,9 @@ struct HxLaunch {
   const T *R;
   int64_t ldR;
   const T *rcoef;
+  // 1: the cell matrices are exactly symmetric (built on the device); the
+  // one-column kernel then reads the slices on and below the diagonal only
+  int h_sym;
 };
 
 // Arguments of the projection C_e^T (s_in * src)_e for all cells with slots.

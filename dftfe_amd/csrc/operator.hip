@@ -558,6 +558,7 @@ extern "C" chfsi_status chfsi_operator_set_potential(chfsi_operator *op, const d
                            op->nq, op->d_Ht64, op->d_Ht32, op->d_Ht16, as_stream(stream),
                            op->K_per_cell));
   op->has_H = true;
+  op->h_sym = true;  // the build writes each off-diagonal pair block from one sum
   return rebuild_kpoint_matrices(op, as_stream(stream));
 }
 
@@ -570,6 +571,7 @@ extern "C" chfsi_status chfsi_operator_set_potential_gga(chfsi_operator *op, con
                            op->nq, op->d_Ht64, op->d_Ht32, op->d_Ht16, as_stream(stream),
                            op->K_per_cell, op->d_invJ, op->jac));
   op->has_H = true;
+  op->h_sym = true;
   return rebuild_kpoint_matrices(op, as_stream(stream));
 }
 
@@ -581,6 +583,7 @@ extern "C" chfsi_status chfsi_operator_set_cell_matrices(chfsi_operator *op, con
   CHFSI_HIP(launch_f64_to_bf16(op->d_Ht64, int64_t(op->C) * op->ne_pad * op->ne_pad, op->d_Ht16,
                                as_stream(stream)));
   op->has_H = true;
+  op->h_sym = false;  // caller's matrices: not assumed symmetric
   return rebuild_kpoint_matrices(op, as_stream(stream));
 }
 
@@ -759,6 +762,7 @@ chfsi_status hx_fused(chfsi_operator *op, const T *src, int64_t ld_src, T *dst, 
   a.R = R;
   a.ldR = ldR;
   a.rcoef = rcoef;
+  a.h_sym = op->h_sym ? 1 : 0;
   if (op->n_slots > 0) {
     a.nl_off = op->d_nl_off;
     a.nl_C = OpData<T>::nlC(op);

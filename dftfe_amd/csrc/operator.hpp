@@ -73,6 +73,7 @@ struct chfsi_operator {
   float *d_Ht32 = nullptr;
   uint16_t *d_Ht16 = nullptr, *d_nlC16 = nullptr;  // BF16 copies
   bool has_H = false;
+  bool h_sym = false;  // cell matrices exactly symmetric (built on the device)
   // nonlocal projector slots
   int n_proj = 0, n_slots = 0, n_nl_cells = 0;
   int *d_nl_off = nullptr, *d_slot_proj = nullptr, *d_proj_ptr = nullptr,

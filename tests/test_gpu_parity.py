@@ -388,6 +388,36 @@ def test_lanczos_bounds_match_oracle(small):
     assert (lo, up) == (olo, oup)
 
 
+@pytest.mark.parametrize("name", ["cube20_p4_n64", "cube18_p6_n64"])
+def test_single_column_symmetric_read(name):
+    """The one-column kernel reads only the slices on and below the diagonal of
+    device-built (exactly symmetric) cell matrices; after set_cell_matrices
+    (caller's matrices, not assumed symmetric) it reads them whole.  Both give
+    the oracle's H~x (FP64 1e-13, FP32 2e-6) and agree to rounding, at p = 4
+    and p = 6."""
+    _dev()
+    from dftfe_amd.chfsi import KohnShamOperatorDevice
+
+    prob = fe.build_workload(name)
+    ref = O.OracleOperator(prob)
+    op = KohnShamOperatorDevice(prob)
+    X = prob.initial_guess(N=1, seed_offset=91)
+    out = {}
+    for form in ("sym", "full"):
+        if form == "full":
+            op.set_cell_matrices(op.cell_matrices())
+        for dt, tol in ((torch.float64, 1e-13), (torch.float32, 2e-6)):
+            Xd = torch.tensor(X, device="cuda", dtype=dt)
+            Yd = torch.zeros_like(Xd)
+            op.HX(Xd, False, 1.0, Yd)
+            want = ref.apply(X if dt == torch.float64 else X.astype(np.float32).astype(np.float64))
+            y = Yd.double().cpu().numpy()
+            assert _rel(y, want) < tol, (form, dt)
+            out[(form, dt)] = y
+    assert _rel(out[("sym", torch.float64)], out[("full", torch.float64)]) < 1e-14
+    assert _rel(out[("sym", torch.float32)], out[("full", torch.float32)]) < 1e-6
+
+
 def test_hx_fp32_single_column_matches_oracle(small):
     """One column takes the single-column kernel (cell matrices streamed as a
     matrix-vector product) in FP32 too."""
