@@ -1247,11 +1247,14 @@ __global__ void axpby_kernel(int64_t n, T a, const T *x, T b, T *y) {
     y[i] = a * x[i] + b * y[i];
 }
 
+// With rescale (nullable) each element of X is multiplied by rescale[g] after
+// it is read (the pass's return to the FE basis folded into this read of X).
 __global__ __launch_bounds__(256) void residual_partial_kernel(const double *HX, int64_t ldH,
-                                                               const double *X, int64_t ldX,
+                                                               double *X, int64_t ldX,
                                                                const double *lam, int64_t M,
                                                                int B, double *partial,
-                                                               int nchunks) {
+                                                               int nchunks,
+                                                               const double *rescale) {
   __shared__ double red[4][64];
   const int tv = threadIdx.x & 63, tg = threadIdx.x >> 6;
   const int v = blockIdx.y * 64 + tv;
@@ -1262,8 +1265,10 @@ __global__ __launch_bounds__(256) void residual_partial_kernel(const double *HX,
   if (v < B) {
     const double l = lam[v];
     for (int64_t g = g0 + tg; g < g1; g += 4) {
-      const double r = HX[g * ldH + v] - l * X[g * ldX + v];
+      const double x = X[g * ldX + v];
+      const double r = HX[g * ldH + v] - l * x;
       s += r * r;
+      if (rescale) X[g * ldX + v] = x * rescale[g];
     }
   }
   red[tg][tv] = s;
@@ -2727,9 +2732,13 @@ template hipError_t launch_axpby<float>(int64_t, float, const float *, float, fl
 
 hipError_t launch_residual_partial(const double *HX, int64_t ldH, const double *X, int64_t ldX,
                                    const double *lam, int64_t M, int B, double *partial,
-                                   int nchunks, hipStream_t st) {
+                                   int nchunks, hipStream_t st, double *X_rescaled,
+                                   const double *rescale) {
+  // X_rescaled, when given, is X itself (writable): it is scaled in place
+  if ((X_rescaled != nullptr) != (rescale != nullptr) || (X_rescaled && X_rescaled != X))
+    return hipErrorInvalidValue;
   hipLaunchKernelGGL(residual_partial_kernel, dim3(nchunks, (B + 63) / 64), dim3(256), 0, st, HX,
-                     ldH, X, ldX, lam, M, B, partial, nchunks);
+                     ldH, const_cast<double *>(X), ldX, lam, M, B, partial, nchunks, rescale);
   return hipGetLastError();
 }
 

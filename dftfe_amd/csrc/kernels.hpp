@@ -127,10 +127,12 @@ hipError_t launch_copy_block(const TS *src, int64_t ld_src, TD *dst, int64_t ld_
 // y = a*x + b*y on contiguous arrays
 template <typename T>
 hipError_t launch_axpby(int64_t n, T a, const T *x, T b, T *y, hipStream_t st);
-// column sums of squares of (Y - X*diag(lam)) into partial[chunk][B]
+// column sums of squares of (Y - X*diag(lam)) into partial[chunk][B]; with
+// X_rescaled (== X) and rescale, X[g, v] *= rescale[g] once it has been read
 hipError_t launch_residual_partial(const double *HX, int64_t ldH, const double *X, int64_t ldX,
                                    const double *lam, int64_t M, int B, double *partial,
-                                   int nchunks, hipStream_t st);
+                                   int nchunks, hipStream_t st, double *X_rescaled = nullptr,
+                                   const double *rescale = nullptr);
 // column sums of X*HX and X*X into partial[chunk][2][B] (Rayleigh quotients)
 hipError_t launch_colstats_partial(const double *HX, int64_t ldH, const double *X, int64_t ldX,
                                    int64_t M, int B, double *partial, int nchunks, hipStream_t st);

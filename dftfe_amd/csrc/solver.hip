@@ -757,9 +757,12 @@ chfsi_status gep_impl(chfsi_solver *s, chfsi_operator *op, double *X, int64_t M,
   return rotate_dense(s, X, mixed_rot ? Xf : nullptr, M, N, W, B, st);
 }
 
+// With rescale (nullable) each column block of X is multiplied by rescale[g]
+// row-wise in place once its residuals are formed (X must then be writable;
+// the blocks not yet applied stay untouched).
 chfsi_status residual_impl(chfsi_solver *s, chfsi_operator *op, const double *X, int64_t ldX,
                            int64_t M, int N, int B, const double *eig_host, double *res_host,
-                           hipStream_t st) {
+                           hipStream_t st, const double *rescale = nullptr) {
   const int G = filter_group_columns(s->params, M, N, B);
   CHFSI_TRY(s->HXb.ensure(sizeof(double) * M * G));
   CHFSI_TRY(s->lam.ensure(sizeof(double) * N));
@@ -778,7 +781,8 @@ chfsi_status residual_impl(chfsi_solver *s, chfsi_operator *op, const double *X,
       if (op->nc) CHFSI_HIP(launch_zero_rows(op->d_c_rows, op->nc, HX, gw, gw, st));
     }
     CHFSI_HIP(launch_residual_partial(HX + (j - g0), gw, X + j, ldX, lam + j, M, bw, part, nch,
-                                      st));
+                                      st, rescale ? const_cast<double *>(X) + j : nullptr,
+                                      rescale));
     CHFSI_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * nch * bw, hipMemcpyDeviceToHost, st));
     CHFSI_HIP(hipStreamSynchronize(st));
     for (int v = 0; v < bw; ++v) {
@@ -1264,14 +1268,15 @@ extern "C" chfsi_status chfsi_solve(chfsi_solver *s, chfsi_operator *op, double 
   }
   CHFSI_HIP(hipEventRecord(pe.ev[4], st));
 
+  // back to the FE basis: folded into the residuals' read of X when they are
+  // computed
   if (compute_residual) {
-    CHFSI_TRY(residual_impl(s, op, X, N, M, N, B, eig_host, res_host, st));
+    CHFSI_TRY(residual_impl(s, op, X, N, M, N, B, eig_host, res_host, st, op->d_inv_sqrt_mass));
     const int nlow = std::max(1, (9 * N) / 10);
     s->last_res_lowest = *std::max_element(res_host, res_host + nlow);
+  } else {
+    CHFSI_HIP(launch_scale_rows<double>(X, N, M, N, 1.0, op->d_inv_sqrt_mass, st));
   }
-
-  // back to the FE basis
-  CHFSI_HIP(launch_scale_rows<double>(X, N, M, N, 1.0, op->d_inv_sqrt_mass, st));
   CHFSI_HIP(hipEventRecord(pe.ev[5], st));
   CHFSI_TRY(pe.report(timings_ms));
   if (bUp_out) *bUp_out = s->bUp;
