@@ -151,20 +151,22 @@ ChebState cheb_init(double a, double b, double a0) {
 // degree 1 from X into Y
 template <typename T>
 chfsi_status cheb_first(chfsi_operator *op, const T *X, int64_t ldX, T *Y, int64_t ldY, int B,
-                        const ChebState &cs, hipStream_t st, bool bf16) {
+                        const ChebState &cs, hipStream_t st, bool bf16, void *hx_ws = nullptr) {
   return hx_fused<T>(op, X, ldX, Y, ldY, B, T(cs.sigma / cs.e), T(0), T(-cs.c * cs.sigma / cs.e),
-                     1, true, st, bf16);
+                     1, true, st, bf16, nullptr, 0, nullptr, hx_ws);
 }
 
-// degrees d0..d1 (d0 >= 2); *cur holds the latest iterate, *prev the one before
+// degrees d0..d1 (d0 >= 2); *cur holds the latest iterate, *prev the one before;
+// hx_ws (nullable): projection scratch of the HX calls in place of the operator's
 template <typename T>
 chfsi_status cheb_steps(chfsi_operator *op, T **cur, int64_t *ldc, T **prev, int64_t *ldp, int B,
-                        int d0, int d1, ChebState &cs, hipStream_t st, bool bf16) {
+                        int d0, int d1, ChebState &cs, hipStream_t st, bool bf16,
+                        void *hx_ws = nullptr) {
   for (int degree = d0; degree <= d1; ++degree) {
     const double sigma2 = 1.0 / (cs.gamma - cs.sigma);
     const double alpha1 = 2.0 * sigma2 / cs.e, alpha2 = -(cs.sigma * sigma2);
     CHFSI_TRY(hx_fused<T>(op, *cur, *ldc, *prev, *ldp, B, T(alpha1), T(alpha2),
-                          T(-cs.c * alpha1), 0, true, st, bf16));
+                          T(-cs.c * alpha1), 0, true, st, bf16, nullptr, 0, nullptr, hx_ws));
     std::swap(*cur, *prev);
     std::swap(*ldc, *ldp);
     cs.sigma = sigma2;
@@ -267,20 +269,22 @@ chfsi_status filter_block_defect(chfsi_solver *s, chfsi_operator *op, double *X,
 // In-place filter of the B-column block X (leading dimension ld).  FP32 and
 // BF16 forms cast the block to FP32 once; with tail > 0 the last `tail`
 // degrees run in FP64 from the cast-back pair of iterates, so that the
-// rounding of the low-precision steps is itself filtered.
+// rounding of the low-precision steps is itself filtered.  hx_ws (nullable,
+// FP64 form only): projection scratch of its HX calls in place of the
+// operator's own.
 chfsi_status filter_block(chfsi_solver *s, chfsi_operator *op, double *X, int64_t ld, int B, int m,
                           double a, double b, double a0, int precision, int tail,
-                          hipStream_t st) {
+                          hipStream_t st, void *hx_ws = nullptr) {
   const int64_t M = op->M;
   ChebState cs = cheb_init(a, b, a0);
   CHFSI_TRY(s->Y64.ensure(sizeof(double) * M * B));
   double *Y = sptr<double>(s->Y64);
   if (precision == CHFSI_FP64) {
     CHFSI_HIP(hipMemsetAsync(Y, 0, sizeof(double) * M * B, st));
-    CHFSI_TRY(cheb_first<double>(op, X, ld, Y, B, B, cs, st, false));
+    CHFSI_TRY(cheb_first<double>(op, X, ld, Y, B, B, cs, st, false, hx_ws));
     double *cur = Y, *prev = X;
     int64_t ldc = B, ldp = ld;
-    CHFSI_TRY(cheb_steps<double>(op, &cur, &ldc, &prev, &ldp, B, 2, m, cs, st, false));
+    CHFSI_TRY(cheb_steps<double>(op, &cur, &ldc, &prev, &ldp, B, 2, m, cs, st, false, hx_ws));
     if (cur != X) CHFSI_HIP(launch_copy_block<double, double>(cur, ldc, X, ld, M, B, st));
     return CHFSI_OK;
   }
@@ -1166,10 +1170,45 @@ chfsi_status filter_all(chfsi_solver *s, chfsi_operator *op, double *X, int64_t 
   // the top columns in FP64 when the filter runs in a lower precision
   const int ktop = prec == CHFSI_FP64 ? 0 : std::max(0, std::min(P.mixed_prec_fp64_top_columns, N));
   const int nlow = N - ktop;
+  const int tail = use_mixed_prec_overall ? P.fp64_tail_degrees : 0;
+  // The FP64 top columns and the low-precision columns are independent
+  // recurrences: the top groups run on the solver's filter stream beside the
+  // low groups on st, so that the colour launches of each fill the other's
+  // ramp and drain.  The top groups' HX projections use their own scratch
+  // (top_hx), so the defect form's FP64 apply on st keeps the operator's
+  // buffers; the low groups never touch Y64 unless they run an FP64 tail, in
+  // which case both sets stay on st.
+  const bool side = ktop > 0 && nlow > 0 &&
+                    ((prec == CHFSI_FP32 && P.mixed_prec_defect_filter) || tail == 0);
+  hipStream_t top_st = st;
+  void *top_ws = nullptr;
+  if (side) {
+    if (!s->filter_stream)
+      CHFSI_HIP(hipStreamCreateWithFlags(&s->filter_stream, hipStreamNonBlocking));
+    for (hipEvent_t *e : {&s->filter_fork, &s->filter_join})
+      if (!*e) CHFSI_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    if (op->n_slots > 0) {
+      CHFSI_TRY(s->top_hx.ensure(
+          noalloc_scratch_bytes(sizeof(double), op->n_slots, op->n_proj, std::min(G, ktop))));
+      top_ws = s->top_hx.ptr;
+    }
+    top_st = s->filter_stream;
+    CHFSI_HIP(hipEventRecord(s->filter_fork, st));
+    CHFSI_HIP(hipStreamWaitEvent(top_st, s->filter_fork, 0));
+    for (int j = nlow; j < N; j += G) {
+      const int bw = std::min(G, N - j);
+      CHFSI_TRY(filter_block(s, op, X + j, N, bw, m, s->bLow, s->bUp, s->a0, CHFSI_FP64, 0, top_st,
+                             top_ws));
+    }
+    CHFSI_HIP(hipEventRecord(s->filter_join, top_st));
+  }
   for (int j = 0; j < nlow; j += G) {
     const int bw = std::min(G, nlow - j);
-    CHFSI_TRY(filter_block(s, op, X + j, N, bw, m, s->bLow, s->bUp, s->a0, prec,
-                           use_mixed_prec_overall ? P.fp64_tail_degrees : 0, st));
+    CHFSI_TRY(filter_block(s, op, X + j, N, bw, m, s->bLow, s->bUp, s->a0, prec, tail, st));
+  }
+  if (side) {
+    CHFSI_HIP(hipStreamWaitEvent(st, s->filter_join, 0));
+    return CHFSI_OK;
   }
   for (int j = nlow; j < N; j += G) {
     const int bw = std::min(G, N - j);

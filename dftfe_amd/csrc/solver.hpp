@@ -36,6 +36,11 @@ struct chfsi_solver {
   // stream, overlapped with the next panel's rotation (two panel buffers)
   hipStream_t copy_stream = nullptr;
   hipEvent_t rot_done[2] = {nullptr, nullptr}, copy_done[2] = {nullptr, nullptr};
+  // mixed filter: the FP64 top columns run on a second stream beside the
+  // low-precision columns, their HX projections in their own scratch
+  hipStream_t filter_stream = nullptr;
+  hipEvent_t filter_fork = nullptr, filter_join = nullptr;
+  chfsi::Scratch top_hx;
   bool have_v0 = false;
   // largest residual norm over the lowest 90% of states of the last pass that
   // computed residuals, and the filter precision the last pass used
@@ -43,12 +48,14 @@ struct chfsi_solver {
   int last_precision = CHFSI_FP64;
   ~chfsi_solver() {
     if (blas) rocblas_destroy_handle(blas);
-    for (hipEvent_t e : {rot_done[0], rot_done[1], copy_done[0], copy_done[1]})
+    for (hipEvent_t e : {rot_done[0], rot_done[1], copy_done[0], copy_done[1], filter_fork,
+                         filter_join})
       if (e) (void)hipEventDestroy(e);
     if (copy_stream) (void)hipStreamDestroy(copy_stream);
+    if (filter_stream) (void)hipStreamDestroy(filter_stream);
     for (chfsi::Scratch *s : {&Y64, &X32, &Y32, &HXb, &Hp, &rot, &evals, &info, &part, &lam, &v0,
                               &split, &Xf, &HXf, &Qf, &split32, &rot32, &R32, &dcoef, &gep,
-                              &gtiles})
+                              &gtiles, &top_hx})
       s->release();
   }
 };
