@@ -1349,6 +1349,27 @@ __global__ __launch_bounds__(256) void dot_partial_kernel(const double *x, const
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// *slot = partial[0] + partial[1] + ... in index order (the order of a host
+// loop over the same values, so the sum is the same to the bit)
+__global__ void sum_slot_kernel(const double *partial, int n, double *slot) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double s = 0.0;
+  for (int k = 0; k < n; ++k) s += partial[k];
+  *slot = s;
+}
+
+// y = a x + b y with a formed from the device scalar c = *slot: -c (form 0),
+// 1 / sqrt(c) (form 1) or -sqrt(c) (form 2); the element update is the
+// expression of axpby_kernel
+__global__ void axpby_slot_kernel(int64_t n, const double *slot, int form, const double *x,
+                                  double b, double *y) {
+  const double c = *slot;
+  const double a = form == 0 ? -c : (form == 1 ? 1.0 / sqrt(c) : -sqrt(c));
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = a * x[i] + b * y[i];
+}
+
 __global__ void sum_partials_kernel(const double *P, int nk, int m, int n, double *C,
                                     int64_t ldc) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2775,6 +2796,19 @@ hipError_t launch_dot_partial(const double *x, const double *y, int64_t n, doubl
                               int nchunks, hipStream_t st) {
   hipLaunchKernelGGL(dot_partial_kernel, dim3(nchunks), dim3(256), 0, st, x, y, n, partial,
                      nchunks);
+  return hipGetLastError();
+}
+
+hipError_t launch_sum_slot(const double *partial, int n, double *slot, hipStream_t st) {
+  hipLaunchKernelGGL(sum_slot_kernel, dim3(1), dim3(64), 0, st, partial, n, slot);
+  return hipGetLastError();
+}
+
+hipError_t launch_axpby_slot(int64_t n, const double *slot, int form, const double *x, double b,
+                             double *y, hipStream_t st) {
+  if (form < 0 || form > 2) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(axpby_slot_kernel, dim3(grid_for(n, 256, 65536)), dim3(256), 0, st, n, slot,
+                     form, x, b, y);
   return hipGetLastError();
 }
 

@@ -147,6 +147,12 @@ hipError_t launch_defect_finish(double *X, int64_t ldX, const double *D, const f
 // dot products x.y into partial[chunk]; then the host sums
 hipError_t launch_dot_partial(const double *x, const double *y, int64_t n, double *partial,
                               int nchunks, hipStream_t st);
+// *slot = sum of partial[0..n) in index order, on the device
+hipError_t launch_sum_slot(const double *partial, int n, double *slot, hipStream_t st);
+// y = a*x + b*y with a = -c (form 0), 1/sqrt(c) (1) or -sqrt(c) (2), c = *slot
+// read on the device
+hipError_t launch_axpby_slot(int64_t n, const double *slot, int form, const double *x, double b,
+                             double *y, hipStream_t st);
 
 // g[c*nqc + q] (+)= wscale JxW[c*nqc + q] sum_v w_{v >> wshift} (N(q).(s a_v)|_c)(N(q).(s b_v)|_c)
 // s, w and JxW nullable (== 1); wshift = 1 shares each weight between the two

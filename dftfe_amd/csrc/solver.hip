@@ -333,35 +333,52 @@ chfsi_status lanczos_impl(chfsi_solver *s, chfsi_operator *op, const double *v0,
   CHFSI_TRY(dot(s, v, v, M, &nrm2, st));
   CHFSI_ARG(nrm2 > 0, "lanczos start vector is zero");
   CHFSI_HIP(launch_axpby<double>(M, 0.0, v, 1.0 / std::sqrt(nrm2), v, st));
-  std::vector<double> T(size_t(steps) * steps, 0.0);
   auto apply = [&](const double *x, double *y) -> chfsi_status {
     CHFSI_TRY(hx_fused<double>(op, x, 1, y, 1, 1, 1.0, 0.0, 0.0, 1, true, st));
     if (op->nc) CHFSI_HIP(launch_zero_rows(op->d_c_rows, op->nc, y, 1, 1, st));
     return CHFSI_OK;
   };
+  // The recurrence's scalars stay on the device: each dot product's partials
+  // are summed into a slot (alpha_0 at 0; for step j, |f|^2 at 2j - 1 and
+  // alpha_j at 2j; the last |f|^2 at 2 steps - 1) that the vector updates read,
+  // and the host reads the slots once at the end.  The sums run in the order
+  // of a host loop and the updates form -alpha, 1 / beta and -beta as the
+  // host would, so the bounds equal those of a host-driven recurrence.
+  const int nch = 256;
+  CHFSI_TRY(s->part.ensure(nch * sizeof(double)));
+  CHFSI_TRY(s->lz.ensure(sizeof(double) * 2 * steps));
+  double *part = sptr<double>(s->part), *slot = sptr<double>(s->lz);
+  auto dev_dot = [&](const double *x, const double *y, int k) -> chfsi_status {
+    CHFSI_HIP(launch_dot_partial(x, y, M, part, nch, st));
+    CHFSI_HIP(launch_sum_slot(part, nch, slot + k, st));
+    return CHFSI_OK;
+  };
   CHFSI_TRY(apply(v, f));
-  double alpha;
-  CHFSI_TRY(dot(s, f, v, M, &alpha, st));
-  CHFSI_HIP(launch_axpby<double>(M, -alpha, v, 1.0, f, st));
-  T[0] = alpha;
+  CHFSI_TRY(dev_dot(f, v, 0));
+  CHFSI_HIP(launch_axpby_slot(M, slot, 0, v, 1.0, f, st));  // f <- f - alpha_0 v
   for (int j = 1; j < steps; ++j) {
-    double fn2;
-    CHFSI_TRY(dot(s, f, f, M, &fn2, st));
-    const double beta = std::sqrt(fn2);
-    std::swap(v, vold);                                        // vold <- v
-    CHFSI_HIP(launch_axpby<double>(M, 1.0 / beta, f, 0.0, v, st));  // v <- f / beta
+    CHFSI_TRY(dev_dot(f, f, 2 * j - 1));
+    std::swap(v, vold);                                                    // vold <- v
+    CHFSI_HIP(launch_axpby_slot(M, slot + 2 * j - 1, 1, f, 0.0, v, st));    // v <- f / beta
     CHFSI_TRY(apply(v, f));
-    CHFSI_HIP(launch_axpby<double>(M, -beta, vold, 1.0, f, st));
-    CHFSI_TRY(dot(s, f, v, M, &alpha, st));
-    CHFSI_HIP(launch_axpby<double>(M, -alpha, v, 1.0, f, st));
+    CHFSI_HIP(launch_axpby_slot(M, slot + 2 * j - 1, 2, vold, 1.0, f, st));  // f -= beta vold
+    CHFSI_TRY(dev_dot(f, v, 2 * j));
+    CHFSI_HIP(launch_axpby_slot(M, slot + 2 * j, 0, v, 1.0, f, st));        // f -= alpha v
+  }
+  CHFSI_TRY(dev_dot(f, f, 2 * steps - 1));
+  std::vector<double> hs(2 * size_t(steps));
+  CHFSI_HIP(hipMemcpyAsync(hs.data(), slot, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, st));
+  CHFSI_HIP(hipStreamSynchronize(st));
+  std::vector<double> T(size_t(steps) * steps, 0.0);
+  T[0] = hs[0];
+  for (int j = 1; j < steps; ++j) {
+    const double beta = std::sqrt(hs[2 * j - 1]);
     T[size_t(j) * steps + j - 1] = beta;
     T[size_t(j - 1) * steps + j] = beta;
-    T[size_t(j) * steps + j] = alpha;
+    T[size_t(j) * steps + j] = hs[2 * j];
   }
   std::vector<double> theta = small_sym_eigvals(T, steps);
-  double fn2;
-  CHFSI_TRY(dot(s, f, f, M, &fn2, st));
-  const double fnorm = std::sqrt(fn2);
+  const double fnorm = std::sqrt(hs[2 * steps - 1]);
   *lower = std::floor(theta.front());
   *upper = std::ceil(theta.back() + (reproducible ? fnorm : fnorm / 10.0));
   return CHFSI_OK;

@@ -28,6 +28,8 @@ struct chfsi_solver {
   chfsi::Scratch Xf, HXf, Qf, split32, rot32;
   // defect form of the FP32 filter: FP32 defect source, coefficient tables
   chfsi::Scratch R32, dcoef;
+  // Lanczos: dot products and norms kept on the device until the last step
+  chfsi::Scratch lz;
   // generalised Rayleigh-Ritz: L^{-1} and a product buffer (2 N x N)
   chfsi::Scratch gep;
   // tile table of the FP64 gram kernel
@@ -55,7 +57,7 @@ struct chfsi_solver {
     if (filter_stream) (void)hipStreamDestroy(filter_stream);
     for (chfsi::Scratch *s : {&Y64, &X32, &Y32, &HXb, &Hp, &rot, &evals, &info, &part, &lam, &v0,
                               &split, &Xf, &HXf, &Qf, &split32, &rot32, &R32, &dcoef, &gep,
-                              &gtiles, &top_hx})
+                              &gtiles, &top_hx, &lz})
       s->release();
   }
 };
