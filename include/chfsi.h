@@ -420,7 +420,9 @@ chfsi_status chfsi_solver_set_lanczos_start(chfsi_solver *solver, const double *
  * ascending eigenvalues and residual norms (host), returns the upper bound of
  * the unwanted spectrum in *bUp_out.  Per-phase device times (ms) go to
  * timings_ms[0..5] = {lanczos, filter, cholgs, rayleigh_ritz, residual, total}
- * when timings_ms is not NULL. */
+ * when timings_ms is not NULL.  Work is ordered on `stream`; the solver's own
+ * streams (the filter's FP64 top columns, rotation copy-backs) start after
+ * an event on it and are joined back into it before the call returns. */
 chfsi_status chfsi_solve(chfsi_solver *solver, chfsi_operator *op, double *X, int64_t M,
                          int32_t N, double *eig_host, double *res_host,
                          int32_t is_first_filtering_call, int32_t compute_residual,
