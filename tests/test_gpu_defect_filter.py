@@ -39,11 +39,12 @@ def converged():
     return prob, op, X, s.bounds(), float(res.max())
 
 
-def _pass(op, X, bounds, mixed, defect):
+def _pass(op, X, bounds, mixed, defect, top=0):
     from dftfe_amd.chfsi import ChebyshevSolverDevice
 
     s = ChebyshevSolverDevice(*bounds, cheby_block_size=16, cheby_degree=20, lanczos_steps=10,
-                              reuse_lanczos_upper=True, mixed_prec_defect_filter=defect)
+                              reuse_lanczos_upper=True, mixed_prec_defect_filter=defect,
+                              mixed_prec_fp64_top_columns=top)
     Xc = X.clone()
     lam, res, _ = s.solve(op, Xc, is_first_filtering_call=False, use_mixed_prec_overall=mixed)
     return lam, res, Xc
@@ -77,3 +78,21 @@ def test_defect_pass_is_deterministic(converged):
     la, ra, Xa = _pass(op, X, bounds, True, True)
     lb, rb, Xb = _pass(op, X, bounds, True, True)
     assert torch.equal(Xa, Xb) and np.array_equal(la, lb)
+
+
+def test_defect_pass_with_fp64_top_columns(converged):
+    """The top 16 columns are filtered in FP64 on the solver's second stream,
+    beside the FP32 defect form of the other 48 on the caller's stream: two
+    runs are bitwise equal, the converged states keep FP64 accuracy and every
+    eigenvalue stays within 1e-8 Ha of the FP64 pass."""
+    prob, op, X, bounds, _ = converged
+    l64, r64, _ = _pass(op, X, bounds, False, False)
+    la, ra, Xa = _pass(op, X, bounds, True, True, top=16)
+    lb, rb, Xb = _pass(op, X, bounds, True, True, top=16)
+    assert torch.equal(Xa, Xb) and np.array_equal(la, lb) and np.array_equal(ra, rb)
+    conv = r64 < 1e-6
+    e_conv = float(np.abs(la - l64)[conv].max())
+    e_all = float(np.abs(la - l64).max())
+    print(f"\ntop 16 in FP64: converged states vs FP64 {e_conv:.2e} Ha, all {e_all:.2e} Ha")
+    assert e_conv < 1e-14
+    assert e_all < 1e-8
