@@ -490,7 +490,9 @@ chfsi_status rotate_fp64(chfsi_solver *s, double *X, int64_t M, int N, const dou
     if (!s->copy_done[b])
       CHFSI_HIP(hipEventCreateWithFlags(&s->copy_done[b], hipEventDisableTiming));
   }
-  for (int64_t p = 0; p < npanel; ++p) {
+  // one panel; on a failure the copies already queued are waited for before
+  // the error returns, so that none of them writes X after the call
+  auto panel = [&](int64_t p) -> chfsi_status {
     const int b = (int)(p & 1);
     const int64_t n0 = p * chunk, nc = std::min(chunk, M - n0);
     double *tmp = sptr<double>(s->rot) + (int64_t)b * chunk * N;
@@ -511,6 +513,14 @@ chfsi_status rotate_fp64(chfsi_solver *s, double *X, int64_t M, int N, const dou
     CHFSI_HIP(hipMemcpyAsync(X + n0 * N, tmp, sizeof(double) * nc * N, hipMemcpyDeviceToDevice,
                              s->copy_stream));
     CHFSI_HIP(hipEventRecord(s->copy_done[b], s->copy_stream));
+    return CHFSI_OK;
+  };
+  for (int64_t p = 0; p < npanel; ++p) {
+    const chfsi_status stl = panel(p);
+    if (stl != CHFSI_OK) {
+      (void)hipStreamSynchronize(s->copy_stream);
+      return stl;
+    }
   }
   for (int b = 0; b < nbuf; ++b) CHFSI_HIP(hipStreamWaitEvent(st, s->copy_done[b], 0));
   return CHFSI_OK;
@@ -1212,20 +1222,33 @@ chfsi_status filter_all(chfsi_solver *s, chfsi_operator *op, double *X, int64_t 
     top_st = s->filter_stream;
     CHFSI_HIP(hipEventRecord(s->filter_fork, st));
     CHFSI_HIP(hipStreamWaitEvent(top_st, s->filter_fork, 0));
-    for (int j = nlow; j < N; j += G) {
+    // st waits for whatever the filter stream was given, also when a group
+    // fails to enqueue, so that no work on X outlives the call unordered
+    auto join = [&](chfsi_status stl) -> chfsi_status {
+      const hipError_t e1 = hipEventRecord(s->filter_join, top_st);
+      const hipError_t e2 = e1 == hipSuccess ? hipStreamWaitEvent(st, s->filter_join, 0) : e1;
+      if (stl == CHFSI_OK && e2 != hipSuccess) {
+        set_error("HIP error %s while joining the filter stream", hipGetErrorString(e2));
+        return CHFSI_ERR_HIP;
+      }
+      if (e2 != hipSuccess) (void)hipStreamSynchronize(top_st);
+      return stl;
+    };
+    chfsi_status stl = CHFSI_OK;
+    for (int j = nlow; j < N && stl == CHFSI_OK; j += G) {
       const int bw = std::min(G, N - j);
-      CHFSI_TRY(filter_block(s, op, X + j, N, bw, m, s->bLow, s->bUp, s->a0, CHFSI_FP64, 0, top_st,
-                             top_ws));
+      stl = filter_block(s, op, X + j, N, bw, m, s->bLow, s->bUp, s->a0, CHFSI_FP64, 0, top_st,
+                         top_ws);
     }
-    CHFSI_HIP(hipEventRecord(s->filter_join, top_st));
+    for (int j = 0; j < nlow && stl == CHFSI_OK; j += G) {
+      const int bw = std::min(G, nlow - j);
+      stl = filter_block(s, op, X + j, N, bw, m, s->bLow, s->bUp, s->a0, prec, tail, st);
+    }
+    return join(stl);
   }
   for (int j = 0; j < nlow; j += G) {
     const int bw = std::min(G, nlow - j);
     CHFSI_TRY(filter_block(s, op, X + j, N, bw, m, s->bLow, s->bUp, s->a0, prec, tail, st));
-  }
-  if (side) {
-    CHFSI_HIP(hipStreamWaitEvent(st, s->filter_join, 0));
-    return CHFSI_OK;
   }
   for (int j = nlow; j < N; j += G) {
     const int bw = std::min(G, N - j);
